@@ -1,0 +1,185 @@
+"""Benchmark: 4K closed-form matte solve (assembly + Jacobi-PCG to 1e-6
+relative residual), mixed precision vs FP32, plus banded SpMV HBM GB/s.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+One step = one full solve of the 3840x2160 workload (three soft ellipses, 4 %
+constraints, eps 1e-7, lambda 100): GPU Laplacian assembly into the 25-band
+matrix, then Jacobi-PCG until the FP64-checked relative residual is <= 1e-6.
+Inputs (image, weights, targets) are resident in HBM before timing starts.
+`value` is the mixed-precision (FP16 diagonals, FP32 accumulate, FP64 residual
+replacement) ms per solve; the FP32 solve is timed the same way beside it.
+The roofline object is the FP32 banded SpMV (y = A x, 25 diagonals), timed
+with HIP events on its launch stream; the CPU baseline is the oracle's OpenMP
+restatement of the same assembly + Jacobi-PCG on a bounded sample.
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "4K matte solve ms to 1e-6 resid (mixed vs FP32) + banded SpMV HBM GB/s"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def cpu_baseline(p, iters_needed, sample_iters):
+    """Oracle OpenMP assembly + Jacobi-PCG on the host: times the full 4K
+    assembly and `sample_iters` PCG iterations, extrapolates to the iteration
+    count the solve needs."""
+    import numpy as np
+
+    from oracle import pyoracle as O
+    t0 = time.perf_counter()
+    a, b, _ = O.levin_gather_acc_f32(p.image, p.lam, p.target, p.eps)
+    t_asm = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.baseline_pcg_f32(a, O.bands25(p.W), b, np.zeros(p.N, np.float32), O.pad_for(p.W), sample_iters)
+    t_it = (time.perf_counter() - t0) / sample_iters
+    ms = 1e3 * (t_asm + t_it * iters_needed)
+    return {"value": round(ms, 1), "unit": "ms per 4K solve (extrapolated)", "cores": O.num_threads(),
+            "kind": "port",
+            "sample": f"full 4K assembly ({t_asm * 1e3:.0f} ms) + {sample_iters} Jacobi-PCG iterations "
+                      f"({t_it * 1e3:.1f} ms each) x {iters_needed} iterations; {cpu_model()}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-iters", type=int, default=40)
+    ap.add_argument("--spmv-reps", type=int, default=50)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: this build has no multi-GPU path (the solve would shard by image "
+              "strips with a 2-row halo; see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from gpumatting_amd import ASSEMBLY_ACCURATE, METHOD_PCG, PREC_FP16, PREC_FP32, lib, matting, synth
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    lib()
+    log("generating the 3840x2160 workload")
+    p = synth.config_4k()
+    W, H, N = p.W, p.H, p.N
+    img = torch.from_numpy(p.image).cuda()
+    lam = torch.from_numpy(p.lam).cuda()
+    tgt = torch.from_numpy(p.target).cuda()
+    L = matting.DeviceMatrix(N, matting.levin_bands(W))
+    pad = matting.levin_pad(W)
+    x = matting.Padded(N, pad)
+
+    def step(prec):
+        s = matting.assemble(img, lam, tgt, p.eps, ASSEMBLY_ACCURATE, L=L)
+        x.buf.zero_()
+        _, info = matting.solve(s, x=x, method=METHOD_PCG, precision=prec, tol=1e-6, max_iter=20000)
+        return s, info
+
+    results = {}
+    for name, prec in (("mixed", PREC_FP16), ("fp32", PREC_FP32)):
+        for _ in range(args.warmup):
+            step(prec)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        infos = []
+        t0 = time.perf_counter()
+        e0.record()
+        for _ in range(args.steps):
+            s, info = step(prec)
+            infos.append((info.iterations, info.true_relres, info.status, info.replacements))
+        e1.record()
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) * 1e3 / args.steps
+        ms = e0.elapsed_time(e1) / args.steps
+        results[name] = {"ms": ms, "wall_ms": wall, "iterations": infos[-1][0], "true_relres": infos[-1][1],
+                         "status": infos[-1][2], "replacements": infos[-1][3]}
+        log(f"{name}: {ms:.2f} ms/solve (wall {wall:.2f}), iterations {infos[-1][0]}, "
+            f"true relres {infos[-1][1]:.3e}")
+
+    # banded SpMV roofline: y = A x on the assembled FP32 matrix
+    s = matting.assemble(img, lam, tgt, p.eps, ASSEMBLY_ACCURATE, L=L)
+    xs = matting.Padded(N, pad, init=torch.rand(N, device="cuda"))
+    for _ in range(3):
+        matting.bm_ax(L, xs)
+    y = torch.empty(N, device="cuda")
+    import ctypes as C
+    from gpumatting_amd._lib import call
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.spmv_reps):
+        call("gm_bm_ax", C.c_void_p(y.data_ptr()), L.ref, xs.ptr, stream)
+    e1.record()
+    torch.cuda.synchronize()
+    spmv_ms = e0.elapsed_time(e1) / args.spmv_reps
+    spmv_bytes = N * (25 * 4 + 8)
+    spmv_gbs = spmv_bytes / (spmv_ms * 1e-3) / 1e9
+    log(f"spmv: {spmv_ms * 1e3:.1f} us/launch, {spmv_gbs:.0f} GB/s algorithmic")
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline (oracle, OpenMP)")
+        cpu = cpu_baseline(p, results["fp32"]["iterations"], args.cpu_iters)
+
+    mixed, fp32 = results["mixed"], results["fp32"]
+    out = {
+        "metric": METRIC,
+        "value": round(mixed["ms"], 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mixed["ms"], 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp16 diagonals / f32 vectors / f64 residual",
+        "data": "synthetic (seeded generator gpumatting_amd/synth.py, config_4k seed 3000)",
+        "config": {"workload": "3840x2160 matte, 3 ellipses, 4% constraints, eps 1e-7, lambda 100, "
+                               "assembly + Jacobi-PCG to 1e-6 true relative residual",
+                   "width": W, "height": H, "unknowns": N, "tol": 1e-6},
+        "fp32_ms": round(fp32["ms"], 3),
+        "mixed_speedup_vs_fp32": round(fp32["ms"] / mixed["ms"], 3),
+        "iterations": {"mixed": mixed["iterations"], "fp32": fp32["iterations"]},
+        "true_relres": {"mixed": mixed["true_relres"], "fp32": fp32["true_relres"]},
+        "residual_replacements_mixed": mixed["replacements"],
+        "spmv_gbs": round(spmv_gbs, 1),
+        "spmv_us": round(spmv_ms * 1e3, 2),
+        "roofline": {"bound": "hbm", "kernel": "k_spmv25<4,F32,0> (banded SpMV, 25 FP32 diagonals)",
+                     "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes": spmv_bytes},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,385 @@
+// gm_internal.h -- shared device helpers and host plumbing of libgpumatting.
+//
+// Layout conventions (all kernels):
+//   * banded matrix: band k at a + k*apitch, row i at +i (diagonal-major)
+//   * vectors: FP32, row i at v[i]; kernels that stage a tile in LDS never read
+//     outside [0, N) (out-of-range neighbours are zero), generic kernels read
+//     v[i + off] and rely on the caller's zero padding like the reference
+//   * Levin structure: 25 bands, band k = 5*(dy+2) + (dx+2) at offset dy*W + dx
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <mutex>
+#include <string>
+
+#include "gpumatting.h"
+
+namespace gm {
+
+constexpr int kBlock = 256;     // threads per workgroup (4 wave64)
+constexpr int kWaves = kBlock / 64;
+constexpr int kMaxBands = 64;
+constexpr int kNumXcd = 8;      // MI355X: 8 XCDs, workgroups dispatched round-robin over them
+constexpr int kMaxGrid = 4096;  // upper bound of persistent grids (partials arrays are sized by it)
+
+// ---------------------------------------------------------------------------
+// error reporting
+void set_error(const char* fmt, ...);
+gm_status fail(gm_status st, const char* fmt, ...);
+gm_status hip_fail(hipError_t e, const char* what, const char* file, int line);
+
+#define GM_HIP(call)                                                             \
+    do {                                                                         \
+        hipError_t e_ = (call);                                                  \
+        if (e_ != hipSuccess) return ::gm::hip_fail(e_, #call, __FILE__, __LINE__); \
+    } while (0)
+
+#define GM_LAUNCH_CHECK() GM_HIP(hipGetLastError())
+
+// ---------------------------------------------------------------------------
+// band offsets handed to kernels by value
+struct BandArgs {
+    int nb;
+    int center;          // index of the band with offset 0 (-1 if none)
+    int off[kMaxBands];
+};
+
+// Host-side description of a device matrix's band structure.
+struct BandPlan {
+    BandArgs args;
+    bool levin25 = false;  // offsets are dy*W+dx, dy,dx in [-2,2], in that order
+    int W = 0;             // image width when levin25
+    int maxoff = 0;        // max |offset|
+};
+
+// Looks up (or builds, with one synchronous D2H copy of the offsets) the plan
+// of a device matrix.
+gm_status get_plan(const BandedMatrix* A, BandPlan* out);
+void register_plan(const int* dev_bands, int nbands, const int* host_bands);
+void unregister_plan(const int* dev_bands);
+void make_plan(const int* host_bands, int nbands, BandPlan* p);
+
+// ---------------------------------------------------------------------------
+// device info
+struct DevInfo {
+    int device = -1;
+    int cus = 256;
+};
+gm_status dev_info(DevInfo* d);
+
+// Persistent grid size for a tiled kernel: a multiple of kNumXcd, at most
+// `per_cu` workgroups per CU and never more than the tile count rounded up.
+int grid_for_tiles(const DevInfo& d, long ntiles, int per_cu);
+
+// Per-(device, stream) reduction scratch: 4*kMaxGrid partial sums + counter.
+struct RedScratch {
+    double* partials = nullptr;
+    unsigned int* counter = nullptr;
+};
+gm_status red_scratch(gm_stream_t s, RedScratch* out);
+void release_red_scratch();
+// Grid for grid-stride elementwise kernels.
+int elementwise_grid(const DevInfo& d, long n, int per_thread);
+
+// ---------------------------------------------------------------------------
+// device scalar block driving the iterative solvers
+struct Scalars {
+    double rz;       // <r, z> of the current residual (z = M r)
+    double rz_prev;  // previous <r, z> (kept for residual replacement)
+    double rr;       // squared residual norm in the stopping norm
+    double pAp;      // <p, A p>
+    double bb;       // <b, b>
+    double tol2;     // stop when rr <= tol2 (tol^2 * bb)
+    double g0, g1;   // scratch scalars (gradient descent)
+    float alpha;     // step length
+    float beta;      // direction update factor
+    int done;        // 0 running, 1 converged, 2 breakdown, 3 non-finite, 4 max_iter
+    int verify;      // 1: recursive residual says converged, awaiting the FP64 check
+    int iters;       // iterations completed
+    int maxit;
+    int replacements;
+    int pad_;
+    unsigned int counter;  // arrival counter of the last-block reduction
+    unsigned int pad2_;
+};
+
+// ---------------------------------------------------------------------------
+// device helpers
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ double load_agent(const double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Two-stage deterministic grid reduction of NQ double partial sums.  Every
+// workgroup of the grid must call it exactly once.  Workgroup partials go to
+// partials[q * gridDim.x + blockIdx.x]; the last workgroup to arrive sums them
+// in index order (a fixed tree, independent of arrival order), calls
+// fin(totals) on thread 0 and re-arms the counter.
+template <int NQ, class Fin>
+__device__ __forceinline__ void grid_reduce(double (&v)[NQ], double* partials, unsigned int* counter,
+                                            Fin fin) {
+    __shared__ double red[NQ][kWaves];
+    __shared__ int is_last;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        double s = wave_sum(v[q]);
+        if (lane == 0) red[q][wid] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            double s = 0;
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) s += red[q][w];
+            partials[(size_t)q * gridDim.x + blockIdx.x] = s;
+        }
+        __threadfence();
+        unsigned int prev = atomicAdd(counter, 1u);
+        is_last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
+    double tot[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        double s = 0;
+        for (int i = threadIdx.x; i < (int)gridDim.x; i += kBlock)
+            s += load_agent(partials + (size_t)q * gridDim.x + i);
+        s = wave_sum(s);
+        if (lane == 0) red[q][wid] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            double s = 0;
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) s += red[q][w];
+            tot[q] = s;
+        }
+        fin(tot);
+        __threadfence();
+        *counter = 0u;
+    }
+}
+
+// Block-level sum reduction to thread 0 only (fixed order).
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double red[kWaves];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    v = wave_sum(v);
+    __syncthreads();
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    double s = 0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < kWaves; ++w) s += red[w];
+    return s;
+}
+
+// XCD-aware tile schedule for a persistent grid (gridDim.x a multiple of
+// kNumXcd).  Workgroup b runs on XCD b % 8; the tiles are cut into 8
+// contiguous chunks, one per XCD, so vertically neighbouring tiles (which share
+// x halo rows) are processed by the same XCD and meet in its L2.
+struct TileRange {
+    int begin, end, step;
+};
+__device__ __forceinline__ TileRange xcd_tiles(int ntiles) {
+    const int G = gridDim.x, b = blockIdx.x;
+    const int x = b % kNumXcd, l = b / kNumXcd, per = G / kNumXcd;
+    const int chunk = (ntiles + kNumXcd - 1) / kNumXcd;
+    TileRange t;
+    t.begin = x * chunk + l;
+    t.end = min(ntiles, (x + 1) * chunk);
+    t.step = per;
+    return t;
+}
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
+// Storage formats of matrix entries.
+enum Fmt { kF32 = 0, kF16 = 1, kBF16 = 2 };
+
+__device__ __forceinline__ float bf16_to_f32(unsigned short h) {
+    return __uint_as_float(((unsigned int)h) << 16);
+}
+__device__ __forceinline__ unsigned short f32_to_bf16_rne(float f) {
+    unsigned int u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40u);  // quiet NaN
+    unsigned int lsb = (u >> 16) & 1u;
+    u += 0x7fffu + lsb;
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ unsigned short f32_to_f16_rne(float f) {
+    _Float16 h = (_Float16)f;  // round to nearest even
+    return *reinterpret_cast<unsigned short*>(&h);
+}
+
+// R consecutive matrix entries of one band, kept packed in registers until use
+// (16-bit formats hold two entries per dword).
+template <int R, int FMT>
+struct Raw;
+template <int R>
+struct Raw<R, kF32> {
+    f32x4 d[R / 4];
+    __device__ __forceinline__ float get(int j) const { return d[j / 4][j % 4]; }
+};
+template <int R>
+struct Raw<R, kF16> {
+    unsigned int d[R / 2];
+    __device__ __forceinline__ float get(int j) const {
+        unsigned short h = (unsigned short)((j % 2) ? (d[j / 2] >> 16) : (d[j / 2] & 0xffffu));
+        return (float)__builtin_bit_cast(_Float16, h);
+    }
+};
+template <int R>
+struct Raw<R, kBF16> {
+    unsigned int d[R / 2];
+    __device__ __forceinline__ float get(int j) const {
+        return __uint_as_float((j % 2) ? (d[j / 2] & 0xffff0000u) : (d[j / 2] << 16));
+    }
+};
+
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// Load R consecutive entries of one band starting at element i (16-byte
+// aligned for FP32 / 16-bit R=8, 8-byte for 16-bit R=4) with a non-temporal
+// hint: bands are streamed once per product and must not evict the vectors
+// from L2 / MALL.
+template <int R, int FMT>
+__device__ __forceinline__ void load_band(const void* base, size_t i, Raw<R, FMT>& out) {
+    if constexpr (FMT == kF32) {
+        const f32x4* p = reinterpret_cast<const f32x4*>(reinterpret_cast<const float*>(base) + i);
+#pragma unroll
+        for (int q = 0; q < R / 4; ++q) out.d[q] = __builtin_nontemporal_load(p + q);
+    } else if constexpr (R % 8 == 0) {
+        const u32x4* p = reinterpret_cast<const u32x4*>(reinterpret_cast<const unsigned short*>(base) + i);
+#pragma unroll
+        for (int q = 0; q < R / 8; ++q) {
+            u32x4 t = __builtin_nontemporal_load(p + q);
+            out.d[4 * q] = t.x; out.d[4 * q + 1] = t.y; out.d[4 * q + 2] = t.z; out.d[4 * q + 3] = t.w;
+        }
+    } else {
+        const u32x2* p = reinterpret_cast<const u32x2*>(reinterpret_cast<const unsigned short*>(base) + i);
+#pragma unroll
+        for (int q = 0; q < R / 4; ++q) {
+            u32x2 t = __builtin_nontemporal_load(p + q);
+            out.d[2 * q] = t.x; out.d[2 * q + 1] = t.y;
+        }
+    }
+}
+
+// Store R consecutive floats at v + i (rows >= n are skipped); vectorised when
+// the whole chunk is in range and 16-byte aligned.
+template <int R>
+__device__ __forceinline__ void store_rows(float* v, int i, int n, const float (&y)[R]) {
+    float* p = v + i;
+    if (i + R <= n && ((reinterpret_cast<uintptr_t>(p) & 15u) == 0)) {
+#pragma unroll
+        for (int q = 0; q < R / 4; ++q) {
+            f32x4 t = {y[4 * q], y[4 * q + 1], y[4 * q + 2], y[4 * q + 3]};
+            *reinterpret_cast<f32x4*>(p + 4 * q) = t;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < R; ++j)
+            if (i + j < n) p[j] = y[j];
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void load_rows(const float* v, int i, int n, float (&y)[R]) {
+    const float* p = v + i;
+    if (i + R <= n && ((reinterpret_cast<uintptr_t>(p) & 15u) == 0)) {
+#pragma unroll
+        for (int q = 0; q < R / 4; ++q) {
+            f32x4 t = *reinterpret_cast<const f32x4*>(p + 4 * q);
+            y[4 * q] = t.x; y[4 * q + 1] = t.y; y[4 * q + 2] = t.z; y[4 * q + 3] = t.w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < R; ++j) y[j] = (i + j < n) ? p[j] : 0.f;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Levin 5x5-band tile machinery.
+//
+// A tile is ROWS = kBlock*R consecutive rows [i0, i0+ROWS).  Its x window is 5
+// segments (dy = -2..2) of SEG = ROWS + 4 floats staged in LDS:
+//   seg[s][e] = X(i0 + (s-2)*W - 2 + e),  X(g) = f(g) for 0 <= g < N, else 0.
+// Thread t owns rows i0 + R*t .. + R-1 and reads seg[s][R*t .. R*t+R+3].
+template <int R>
+struct Tile {
+    static constexpr int ROWS = kBlock * R;
+    static constexpr int SEG = ROWS + 4;
+    static constexpr int LDS_FLOATS = 5 * SEG;
+};
+
+template <int R, class F>
+__device__ __forceinline__ void stage5(float* __restrict__ lds, int i0, int W, int N, F f) {
+    constexpr int SEG = Tile<R>::SEG;
+#pragma unroll
+    for (int s = 0; s < 5; ++s) {
+        const long long gb = (long long)i0 + (long long)(s - 2) * W - 2;
+        for (int e = threadIdx.x; e < SEG; e += kBlock) {
+            const long long g = gb + e;
+            lds[s * SEG + e] = (g >= 0 && g < N) ? f((int)g) : 0.f;
+        }
+    }
+}
+
+// y[j] = sum_k A_k[i+j] * X(i+j+off_k), k = 0..24 in band order (FMA).
+// UNIT: band 12 is an implicit 1 (Jacobi-scaled storage) and is not loaded.
+template <int R, int FMT, bool UNIT>
+__device__ __forceinline__ void tile_rows(const void* __restrict__ A, size_t pitch, int ib,
+                                          const float* __restrict__ xs, float (&y)[R]) {
+    constexpr int SEG = Tile<R>::SEG;
+#pragma unroll
+    for (int j = 0; j < R; ++j) y[j] = 0.f;
+    Raw<R, FMT> av[25];
+#pragma unroll
+    for (int k = 0; k < 25; ++k) {
+        if (UNIT && k == 12) continue;
+        load_band<R, FMT>(A, (size_t)k * pitch + (size_t)ib, av[k]);
+    }
+#pragma unroll
+    for (int s = 0; s < 5; ++s) {
+        float xw[R + 4];
+#pragma unroll
+        for (int q = 0; q < (R + 4) / 4; ++q) {
+            f32x4 t = *reinterpret_cast<const f32x4*>(xs + s * SEG + 4 * q);
+            xw[4 * q] = t.x; xw[4 * q + 1] = t.y; xw[4 * q + 2] = t.z; xw[4 * q + 3] = t.w;
+        }
+#pragma unroll
+        for (int dx = 0; dx < 5; ++dx) {
+            const int k = 5 * s + dx;
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                if (UNIT && k == 12)
+                    y[j] += xw[j + dx];
+                else
+                    y[j] = fmaf(av[k].get(j), xw[j + dx], y[j]);
+            }
+        }
+    }
+}
+
+}  // namespace gm

@@ -1,0 +1,306 @@
+// gm_levin.hip -- Levin matting Laplacian assembly, gather form.
+//
+// Replaces the reference's single-threaded host scatter (hostLevinLaplacian,
+// Matting.cu:39-219) and its unfinished GPU scatter kernel (Matting.cu:240-382,
+// which raced on L).  Each thread owns one row i of L (one pixel) and
+// collects, for every interior 3x3 window containing the pixel (window centres
+// in raster order), that window's 9 contributions -(1 + w_i.(x_j - mu))/9 into
+// 25 register accumulators.  An entry L[i, band] therefore receives the same
+// float additions in the same order as the reference's scatter loop, which is
+// what makes GM_ASSEMBLY_REFERENCE bit-identical to it; no atomics, no races.
+//
+// Workgroup tile: 64 x 4 pixels.  LDS holds the 68 x 8 pixel halo and the
+// statistics (mean, FP64 inverse covariance) of the 66 x 6 windows touching
+// the tile; each window's statistics are computed once per workgroup.
+// Output per pixel: 25 diagonals (100 B), b, 1/L_ii -> HBM-bound on the writes.
+#include "gm_internal.h"
+
+namespace gm {
+
+constexpr int kTX = 64, kTY = 4;              // pixels per workgroup
+constexpr int kWX = kTX + 2, kWY = kTY + 2;   // windows touching the tile
+constexpr int kPX = kTX + 4, kPY = kTY + 4;   // pixels those windows read
+constexpr int kNW = kWX * kWY;
+
+// MODE: GM_ASSEMBLY_REFERENCE or GM_ASSEMBLY_ACCURATE
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_levin(const float* __restrict__ im, int nch, size_t pitch,
+                                                  const float* __restrict__ lam, const float* __restrict__ tgt,
+                                                  const unsigned char* __restrict__ scr, float gamma, float eps,
+                                                  int W, int H, float* __restrict__ A, size_t apitch,
+                                                  float* __restrict__ bvec, float* __restrict__ dinv) {
+#pragma clang fp contract(off)
+    __shared__ float px[3][kPY][kPX];
+    // per-window statistics, structure of arrays:
+    //   REFERENCE: ws[0..5] = adjugate d11 d12 d13 d22 d23 d33, ws[6] = det,
+    //              ws[7..9] = float mean (held exactly as double)
+    //   ACCURATE : ws[0..5] = inverse S11 S12 S13 S22 S23 S33, ws[7..9] = mean
+    __shared__ double ws[10][kNW];
+    __shared__ unsigned char wvalid[kNW];
+
+    const int x0 = blockIdx.x * kTX, y0 = blockIdx.y * kTY;
+
+    for (int e = threadIdx.x; e < kPX * kPY; e += kBlock) {
+        const int ly = e / kPX, lx = e % kPX;
+        const int gx = x0 - 2 + lx, gy = y0 - 2 + ly;
+        float c0 = 0.f, c1 = 0.f, c2 = 0.f;
+        if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
+            const float* p = im + ((size_t)gy * pitch + (size_t)gx) * (size_t)nch;
+            c0 = p[0]; c1 = p[1]; c2 = p[2];
+        }
+        px[0][ly][lx] = c0; px[1][ly][lx] = c1; px[2][ly][lx] = c2;
+    }
+    __syncthreads();
+
+    for (int e = threadIdx.x; e < kNW; e += kBlock) {
+        const int wy = e / kWX, wx = e % kWX;
+        const int cx = x0 - 1 + wx, cy = y0 - 1 + wy;
+        const bool ok = cx >= 1 && cx <= W - 2 && cy >= 1 && cy <= H - 2;
+        wvalid[e] = ok;
+        if (!ok) continue;
+        if constexpr (MODE == GM_ASSEMBLY_REFERENCE) {
+            // Matting.cu:95-137: float mean, double sums of float products,
+            // eps/9 in float, double determinant and adjugate.
+            float mx = 0.f, my = 0.f, mz = 0.f;
+            double c11 = 0, c12 = 0, c13 = 0, c22 = 0, c23 = 0, c33 = 0;
+#pragma unroll
+            for (int v1 = 0; v1 < 3; ++v1)
+#pragma unroll
+                for (int u1 = 0; u1 < 3; ++u1) {
+                    const float x = px[0][wy + v1][wx + u1], y = px[1][wy + v1][wx + u1], z = px[2][wy + v1][wx + u1];
+                    mx += x; my += y; mz += z;
+                    float t;
+                    t = x * x; c11 += (double)t;
+                    t = x * y; c12 += (double)t;
+                    t = x * z; c13 += (double)t;
+                    t = y * y; c22 += (double)t;
+                    t = y * z; c23 += (double)t;
+                    t = z * z; c33 += (double)t;
+                }
+            mx /= 9.0f; my /= 9.0f; mz /= 9.0f;
+            const float reg = eps / 9.0f;
+            float t;
+            t = mx * mx; c11 = c11 / 9.0 - (double)t + (double)reg;
+            t = mx * my; c12 = c12 / 9.0 - (double)t;
+            t = mx * mz; c13 = c13 / 9.0 - (double)t;
+            t = my * my; c22 = c22 / 9.0 - (double)t + (double)reg;
+            t = my * mz; c23 = c23 / 9.0 - (double)t;
+            t = mz * mz; c33 = c33 / 9.0 - (double)t + (double)reg;
+            ws[6][e] = c11 * c22 * c33 + c12 * c23 * c13 + c13 * c12 * c23 - c13 * c22 * c13 - c12 * c12 * c33 -
+                       c11 * c23 * c23;
+            ws[0][e] = c33 * c22 - c23 * c23;
+            ws[1][e] = c13 * c23 - c33 * c12;
+            ws[2][e] = c23 * c12 - c22 * c13;
+            ws[3][e] = c33 * c11 - c13 * c13;
+            ws[4][e] = c12 * c13 - c23 * c11;
+            ws[5][e] = c22 * c11 - c12 * c12;
+            ws[7][e] = (double)mx; ws[8][e] = (double)my; ws[9][e] = (double)mz;
+        } else {
+            // exact double mean and covariance sums, FP64 adjugate inverse
+            double m0 = 0, m1 = 0, m2 = 0, s00 = 0, s01 = 0, s02 = 0, s11 = 0, s12 = 0, s22 = 0;
+#pragma unroll
+            for (int v1 = 0; v1 < 3; ++v1)
+#pragma unroll
+                for (int u1 = 0; u1 < 3; ++u1) {
+                    const double x = px[0][wy + v1][wx + u1], y = px[1][wy + v1][wx + u1], z = px[2][wy + v1][wx + u1];
+                    m0 += x; m1 += y; m2 += z;
+                    s00 += x * x; s01 += x * y; s02 += x * z;
+                    s11 += y * y; s12 += y * z; s22 += z * z;
+                }
+            m0 /= 9.0; m1 /= 9.0; m2 /= 9.0;
+            const double reg = (double)eps / 9.0;
+            const double c11 = s00 / 9.0 - m0 * m0 + reg;
+            const double c12 = s01 / 9.0 - m0 * m1;
+            const double c13 = s02 / 9.0 - m0 * m2;
+            const double c22 = s11 / 9.0 - m1 * m1 + reg;
+            const double c23 = s12 / 9.0 - m1 * m2;
+            const double c33 = s22 / 9.0 - m2 * m2 + reg;
+            const double d11 = c22 * c33 - c23 * c23;
+            const double d12 = c13 * c23 - c12 * c33;
+            const double d13 = c12 * c23 - c13 * c22;
+            const double d22 = c11 * c33 - c13 * c13;
+            const double d23 = c12 * c13 - c11 * c23;
+            const double d33 = c11 * c22 - c12 * c12;
+            const double det = c11 * d11 + c12 * d12 + c13 * d13;
+            ws[0][e] = d11 / det; ws[1][e] = d12 / det; ws[2][e] = d13 / det;
+            ws[3][e] = d22 / det; ws[4][e] = d23 / det; ws[5][e] = d33 / det;
+            ws[7][e] = m0; ws[8][e] = m1; ws[9][e] = m2;
+        }
+    }
+    __syncthreads();
+
+    const int tx = threadIdx.x % kTX, ty = threadIdx.x / kTX;
+    const int gx = x0 + tx, gy = y0 + ty;
+    if (gx >= W || gy >= H) return;
+
+    float acc[25];
+#pragma unroll
+    for (int k = 0; k < 25; ++k) acc[k] = 0.f;
+    const float xi0 = px[0][ty + 2][tx + 2], xi1 = px[1][ty + 2][tx + 2], xi2 = px[2][ty + 2][tx + 2];
+
+#pragma unroll
+    for (int wy2 = 0; wy2 < 3; ++wy2)
+#pragma unroll
+        for (int wx2 = 0; wx2 < 3; ++wx2) {
+            const int e = (ty + wy2) * kWX + (tx + wx2);
+            if (!wvalid[e]) continue;
+            float wx, wy, wz;
+            if constexpr (MODE == GM_ASSEMBLY_REFERENCE) {
+                // Matting.cu:146-153
+                const float mx = (float)ws[7][e], my = (float)ws[8][e], mz = (float)ws[9][e];
+                const float rx = xi0 - mx, ry = xi1 - my, rz = xi2 - mz;
+                const double det = ws[6][e];
+                wx = (float)((ws[0][e] * (double)rx + ws[1][e] * (double)ry + ws[2][e] * (double)rz) / det);
+                wy = (float)((ws[1][e] * (double)rx + ws[3][e] * (double)ry + ws[4][e] * (double)rz) / det);
+                wz = (float)((ws[2][e] * (double)rx + ws[4][e] * (double)ry + ws[5][e] * (double)rz) / det);
+            } else {
+                const double rx = (double)xi0 - ws[7][e], ry = (double)xi1 - ws[8][e], rz = (double)xi2 - ws[9][e];
+                wx = (float)(ws[0][e] * rx + ws[1][e] * ry + ws[2][e] * rz);
+                wy = (float)(ws[1][e] * rx + ws[3][e] * ry + ws[4][e] * rz);
+                wz = (float)(ws[2][e] * rx + ws[4][e] * ry + ws[5][e] * rz);
+            }
+#pragma unroll
+            for (int vy = 0; vy < 3; ++vy)
+#pragma unroll
+                for (int vx = 0; vx < 3; ++vx) {
+                    const int lx = tx + wx2 + vx, ly = ty + wy2 + vy;  // pixel j in the halo
+                    float rx, ry, rz;
+                    if constexpr (MODE == GM_ASSEMBLY_REFERENCE) {
+                        rx = px[0][ly][lx] - (float)ws[7][e];
+                        ry = px[1][ly][lx] - (float)ws[8][e];
+                        rz = px[2][ly][lx] - (float)ws[9][e];
+                    } else {
+                        rx = (float)((double)px[0][ly][lx] - ws[7][e]);
+                        ry = (float)((double)px[1][ly][lx] - ws[8][e]);
+                        rz = (float)((double)px[2][ly][lx] - ws[9][e]);
+                    }
+                    // Matting.cu:177
+                    float s = 1.0f + wx * rx;
+                    s = s + wy * ry;
+                    s = s + wz * rz;
+                    acc[12 + (wx2 - 1 + vx - 1) + 5 * (wy2 - 1 + vy - 1)] += -s / 9.0f;
+                }
+        }
+
+    const size_t i = (size_t)gy * (size_t)W + (size_t)gx;
+    float l = 0.f, tg = 0.f;
+    if (scr) {
+        const unsigned char sv = scr[(size_t)gy * pitch + (size_t)gx];
+        if (sv == 255) { l = gamma; tg = 1.f; }
+        else if (sv == 0) { l = gamma; tg = 0.f; }
+    } else if (lam) {
+        l = lam[i];
+        tg = tgt ? tgt[i] : 0.f;
+    }
+    if constexpr (MODE == GM_ASSEMBLY_REFERENCE) {
+        // Matting.cu:191-218: L_ii -= rowsum (double), then the data term
+        double rs = 0;
+#pragma unroll
+        for (int k = 0; k < 25; ++k) rs += (double)acc[k];
+        acc[12] = (float)((double)acc[12] - rs);
+    } else {
+        double off = 0;
+#pragma unroll
+        for (int k = 0; k < 25; ++k)
+            if (k != 12) off += (double)acc[k];
+        acc[12] = (float)(-off);
+    }
+    float bi = 0.f;
+    if (l != 0.f) {
+        acc[12] += l;
+        bi = l * tg;
+    }
+#pragma unroll
+    for (int k = 0; k < 25; ++k) A[(size_t)k * apitch + i] = acc[k];
+    bvec[i] = bi;
+    if (dinv) dinv[i] = acc[12] != 0.f ? 1.f / acc[12] : 0.f;
+}
+
+static gm_status levin_launch(BandedMatrix* L, float* b, float* dinv, const float* im, int nch, size_t pitch,
+                              const float* lam, const float* tgt, const unsigned char* scr, float gamma, float eps,
+                              int mode, int W, int H, gm_stream_t s) {
+    if (!L || !b || !im) return fail(GM_ERR_INVALID, "levin: null pointer");
+    if (W < 1 || H < 1) return fail(GM_ERR_INVALID, "levin: image %dx%d", W, H);
+    if (nch != 3 && nch != 4) return fail(GM_ERR_INVALID, "levin: nch must be 3 or 4");
+    if (pitch < (size_t)W) return fail(GM_ERR_INVALID, "levin: pitch %zu < W %d", pitch, W);
+    if ((long long)W * H != (long long)L->rows) return fail(GM_ERR_INVALID, "levin: L->rows != W*H");
+    if (mode != GM_ASSEMBLY_REFERENCE && mode != GM_ASSEMBLY_ACCURATE) return fail(GM_ERR_INVALID, "levin: mode");
+    if (!(eps >= 0.f)) return fail(GM_ERR_INVALID, "levin: eps must be >= 0");
+    BandPlan p;
+    gm_status st = get_plan(L, &p);
+    if (st != GM_OK) return st;
+    if (L->nbands != 25) return fail(GM_ERR_INVALID, "levin: L needs 25 bands");
+    // the band order must be the 5x5 window order; for W < 5 the offsets alias
+    // but the layout is still defined by main.cu:161-190
+    for (int dy = -2; dy <= 2; ++dy)
+        for (int dx = -2; dx <= 2; ++dx)
+            if (p.args.off[5 * (dy + 2) + (dx + 2)] != dy * W + dx)
+                return fail(GM_ERR_INVALID, "levin: L bands are not the 5x5 window offsets for W=%d", W);
+    dim3 grid((W + kTX - 1) / kTX, (H + kTY - 1) / kTY);
+    if (grid.y > 65535u) return fail(GM_ERR_UNSUPPORTED, "levin: image too tall");
+    if (mode == GM_ASSEMBLY_REFERENCE)
+        hipLaunchKernelGGL(k_levin<GM_ASSEMBLY_REFERENCE>, grid, dim3(kBlock), 0, (hipStream_t)s, im, nch, pitch, lam,
+                           tgt, scr, gamma, eps, W, H, L->a, L->apitch, b, dinv);
+    else
+        hipLaunchKernelGGL(k_levin<GM_ASSEMBLY_ACCURATE>, grid, dim3(kBlock), 0, (hipStream_t)s, im, nch, pitch, lam,
+                           tgt, scr, gamma, eps, W, H, L->a, L->apitch, b, dinv);
+    GM_LAUNCH_CHECK();
+    return GM_OK;
+}
+
+}  // namespace gm
+
+using namespace gm;
+
+extern "C" {
+
+gm_status gm_levin_laplacian(BandedMatrix* L, float* b, float* dinv, const float* im, int nch, size_t im_pitch,
+                             const float* lam, const float* tgt, float eps, int mode, int W, int H, gm_stream_t s) {
+    if (lam && !tgt) return fail(GM_ERR_INVALID, "levin: lam without tgt");
+    return levin_launch(L, b, dinv, im, nch, im_pitch, lam, tgt, nullptr, 0.f, eps, mode, W, H, s);
+}
+
+gm_status gm_levin_laplacian_u8(BandedMatrix* L, float* b, float* dinv, const float* im, int nch, size_t im_pitch,
+                                const unsigned char* scribs, float eps, float gamma, int mode, int W, int H,
+                                gm_stream_t s) {
+    if (!scribs) return fail(GM_ERR_INVALID, "levin: null scribbles");
+    return levin_launch(L, b, dinv, im, nch, im_pitch, nullptr, nullptr, scribs, gamma, eps, mode, W, H, s);
+}
+
+gm_status gm_host_levin_laplacian(BandedMatrix* L, float* b, float eps, const float* im4,
+                                  const unsigned char* scribs, int W, int H, size_t im_pitch) {
+    if (!L || !L->a || !L->bands || !b || !im4 || !scribs) return fail(GM_ERR_INVALID, "hostLevinLaplacian: null");
+    if (L->nbands != 25 || (long long)L->rows != (long long)W * H || (size_t)L->rows > L->apitch)
+        return fail(GM_ERR_INVALID, "hostLevinLaplacian: L must have 25 bands and W*H rows");
+    if (im_pitch < (size_t)W) return fail(GM_ERR_INVALID, "hostLevinLaplacian: pitch < W");
+    BandedMatrix d{};
+    float *db = nullptr, *dim = nullptr;
+    unsigned char* dsc = nullptr;
+    gm_status st = gm_bm_device_alloc(&d, L->rows, 25, L->bands);
+    if (st != GM_OK) return st;
+    const size_t npx = im_pitch * (size_t)(H > 0 ? H : 1);
+    hipError_t e = hipMalloc(&db, sizeof(float) * (size_t)(L->rows > 0 ? L->rows : 1));
+    if (e == hipSuccess) e = hipMalloc(&dim, sizeof(float) * 4 * npx);
+    if (e == hipSuccess) e = hipMalloc(&dsc, npx);
+    if (e == hipSuccess) e = hipMemcpy(dim, im4, sizeof(float) * 4 * npx, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dsc, scribs, npx, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        st = hip_fail(e, "hostLevinLaplacian staging", __FILE__, __LINE__);
+    } else {
+        // gamma = 1 as the reference hard-codes (Matting.cu:48)
+        st = levin_launch(&d, db, nullptr, dim, 4, im_pitch, nullptr, nullptr, dsc, 1.0f, eps, GM_ASSEMBLY_REFERENCE,
+                          W, H, nullptr);
+        if (st == GM_OK) st = gm_bm_copy_to_host(L, &d);
+        if (st == GM_OK && L->rows > 0) {
+            e = hipMemcpy(b, db, sizeof(float) * L->rows, hipMemcpyDeviceToHost);
+            if (e != hipSuccess) st = hip_fail(e, "hipMemcpy(b)", __FILE__, __LINE__);
+        }
+    }
+    (void)hipFree(db);
+    (void)hipFree(dim);
+    (void)hipFree(dsc);
+    gm_bm_device_free(&d);
+    return st;
+}
+
+}  // extern "C"

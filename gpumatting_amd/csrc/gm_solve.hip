@@ -1,0 +1,1038 @@
+// gm_solve.hip -- CG, Jacobi-PCG, steepest descent and the mixed-precision
+// (16-bit diagonals + FP64 residual replacement) solver on banded systems.
+//
+// Reference roles: cgSolve main.cu:458-526, gradSolve main.cu:406-456, the
+// scalar kernels main.cu:376-404 (folded into the reductions' finalisers).
+//
+// One CG iteration is two kernels, all scalars stay on the device:
+//   K1 k_cgstep : p_new = -z + beta*p_old is formed while the tile's x window
+//                 is staged in LDS (z = r, D^-1 r, or r of the scaled system),
+//                 Ap = A p_new, <p_new, Ap>   -> alpha = <r,z>/<p,Ap>
+//   K2 k_cgupdate: x += alpha p, r += alpha Ap, z, <r,z>, ||r||^2
+//                                            -> beta, convergence flag
+// Sign convention as the reference: r = A x - b, p = -z + beta p.
+// Every reduction is a deterministic two-stage sum (grid_reduce), so a solve
+// is bitwise repeatable run to run.
+//
+// The host enqueues `check_every` iterations at a time and reads the scalar
+// block of the previous batch through an event (one batch in flight), so the
+// GPU never waits on the host; kernels that find the solve finished return at
+// once.
+#include <string.h>
+
+#include <map>
+#include <utility>
+
+#include "gm_internal.h"
+
+namespace gm {
+
+gm_status spmv_launch(float* out, const BandedMatrix* A, const BandPlan& p, const float* x, const float* z, int mode,
+                      gm_stream_t s);
+bool tiled_ok(const BandPlan& p, const void* a, size_t pitch);
+
+enum PMode { kPlain = 0, kJacobi = 1, kScaled = 2 };
+
+// done codes
+enum { kRun = 0, kConverged = 1, kBreakdown = 2, kNonFinite = 3, kMaxIt = 4 };
+
+__device__ __forceinline__ bool stopped(const Scalars* sc) { return sc->done != kRun || sc->verify != 0; }
+
+// Finaliser shared by the residual-producing kernels: record <r,z>, ||r||^2
+// and test convergence.  start: beta = 0 (next step is p = -z).
+__device__ __forceinline__ void fin_residual(Scalars* sc, double rz, double rr, bool start, bool replace) {
+    if (!isfinite(rz) || !isfinite(rr)) {
+        sc->done = kNonFinite;
+        return;
+    }
+    if (replace && !start) {
+        sc->beta = sc->rz_prev > 0 ? (float)(rz / sc->rz_prev) : 0.f;
+        sc->replacements += 1;
+    } else {
+        sc->beta = 0.f;
+    }
+    sc->rz = rz;
+    sc->rr = rr;
+    sc->verify = 0;
+    if (sc->tol2 >= 0 && rr <= sc->tol2) sc->done = kConverged;
+    else if (sc->iters >= sc->maxit) sc->done = kMaxIt;
+}
+
+// ---------------------------------------------------------------------------
+// setup kernels
+
+__global__ void k_init_scalars(Scalars* sc, int maxit) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        Scalars z = {};
+        z.maxit = maxit;
+        z.tol2 = -1.0;
+        z.counter = sc->counter;  // the reduction counter lives in the scratch block; keep it
+        *sc = z;
+    }
+}
+
+// <b,b> -> bb, tol2 = tol^2 * bb (tol <= 0: -1, never converged)
+__global__ __launch_bounds__(kBlock) void k_bnorm(const float* __restrict__ b, int n, double tol, Scalars* sc,
+                                                  double* partials, unsigned int* counter) {
+    double v[1] = {0.0};
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) v[0] += (double)b[i] * (double)b[i];
+    grid_reduce<1>(v, partials, counter, [&](double* t) {
+        sc->bb = t[0];
+        const double bb = t[0] > 0 ? t[0] : 1.0;
+        sc->tol2 = tol > 0 ? tol * tol * bb : -1.0;
+    });
+}
+
+// ---------------------------------------------------------------------------
+// tiled kernels (Levin 5x5 structure)
+
+template <int PM>
+__device__ __forceinline__ float zval(const float* __restrict__ r, const float* __restrict__ dinv, int g) {
+    if constexpr (PM == kJacobi) return dinv[g] * r[g];
+    else return r[g];
+}
+
+// r = A x - b (FP32), z, <r,z>, ||r||^2.  start: also the first step (beta=0).
+template <int R, int PM>
+__global__ __launch_bounds__(kBlock) void k_resid25(const float* __restrict__ A, size_t pitch, int W, int N,
+                                                    const float* __restrict__ x, const float* __restrict__ b,
+                                                    const float* __restrict__ dinv, float* __restrict__ r,
+                                                    Scalars* sc, double* partials, unsigned int* counter, int start) {
+    if (!start && sc->done != kRun) return;
+    __shared__ __attribute__((aligned(16))) float lds[Tile<R>::LDS_FLOATS];
+    const int ntiles = (N + Tile<R>::ROWS - 1) / Tile<R>::ROWS;
+    const TileRange tr = xcd_tiles(ntiles);
+    double v[2] = {0.0, 0.0};
+    for (int t = tr.begin; t < tr.end; t += tr.step) {
+        const int i0 = t * Tile<R>::ROWS;
+        __syncthreads();
+        stage5<R>(lds, i0, W, N, [&](int g) { return x[g]; });
+        __syncthreads();
+        const int ib = i0 + R * threadIdx.x;
+        if (ib < N) {
+            float y[R], bb[R], dd[R];
+            tile_rows<R, kF32, false>(A, pitch, ib, lds + R * threadIdx.x, y);
+            load_rows<R>(b, ib, N, bb);
+            if (PM == kJacobi) load_rows<R>(dinv, ib, N, dd);
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                y[j] -= bb[j];
+                if (ib + j < N) {
+                    const float z = PM == kJacobi ? dd[j] * y[j] : y[j];
+                    v[0] += (double)(y[j] * z);
+                    v[1] += (double)(y[j] * y[j]);
+                }
+            }
+            store_rows<R>(r, ib, N, y);
+        }
+    }
+    grid_reduce<2>(v, partials, counter, [&](double* tt) { fin_residual(sc, tt[0], tt[1], true, false); });
+}
+
+// K1: p_new = -z + beta p_old (staged), Ap = A p_new, <p_new, Ap>.
+template <int R, int FMT, int PM>
+__global__ __launch_bounds__(kBlock) void k_cgstep25(const void* __restrict__ A, size_t pitch, int W, int N,
+                                                     const float* __restrict__ r, const float* __restrict__ dinv,
+                                                     float* __restrict__ p0, float* __restrict__ p1,
+                                                     float* __restrict__ Ap, Scalars* sc, double* partials,
+                                                     unsigned int* counter) {
+    if (stopped(sc)) return;
+    const float beta = sc->beta;
+    const int par = sc->iters & 1;
+    const float* __restrict__ pold = par ? p1 : p0;
+    float* __restrict__ pnew = par ? p0 : p1;
+    __shared__ __attribute__((aligned(16))) float lds[Tile<R>::LDS_FLOATS];
+    const int ntiles = (N + Tile<R>::ROWS - 1) / Tile<R>::ROWS;
+    const TileRange tr = xcd_tiles(ntiles);
+    double v[1] = {0.0};
+    for (int t = tr.begin; t < tr.end; t += tr.step) {
+        const int i0 = t * Tile<R>::ROWS;
+        __syncthreads();
+        if (beta != 0.f)
+            stage5<R>(lds, i0, W, N, [&](int g) { return fmaf(beta, pold[g], -zval<PM>(r, dinv, g)); });
+        else
+            stage5<R>(lds, i0, W, N, [&](int g) { return -zval<PM>(r, dinv, g); });
+        __syncthreads();
+        const int ib = i0 + R * threadIdx.x;
+        if (ib < N) {
+            float y[R], pc[R];
+            tile_rows<R, FMT, PM == kScaled>(A, pitch, ib, lds + R * threadIdx.x, y);
+#pragma unroll
+            for (int j = 0; j < R; ++j) pc[j] = lds[2 * Tile<R>::SEG + R * threadIdx.x + 2 + j];
+            store_rows<R>(pnew, ib, N, pc);
+            store_rows<R>(Ap, ib, N, y);
+#pragma unroll
+            for (int j = 0; j < R; ++j)
+                if (ib + j < N) v[0] += (double)(pc[j] * y[j]);
+        }
+    }
+    grid_reduce<1>(v, partials, counter, [&](double* tt) {
+        const double pAp = tt[0];
+        sc->pAp = pAp;
+        if (!isfinite(pAp)) sc->done = kNonFinite;
+        else if (!(pAp > 0)) sc->done = kBreakdown;
+        else sc->alpha = (float)(sc->rz / pAp);
+    });
+}
+
+// K2: x += alpha p (scaled: alpha s p), r += alpha Ap, <r,z>, ||r||^2 (scaled:
+// the unscaled residual (r/s)^2).
+template <int PM>
+__global__ __launch_bounds__(kBlock) void k_cgupdate(float* __restrict__ x, float* __restrict__ r,
+                                                     const float* __restrict__ dinv, const float* __restrict__ p0,
+                                                     const float* __restrict__ p1, const float* __restrict__ Ap,
+                                                     int N, int verify, Scalars* sc, double* partials,
+                                                     unsigned int* counter) {
+    if (stopped(sc)) return;
+    const float alpha = sc->alpha;
+    const float* __restrict__ p = (sc->iters & 1) ? p0 : p1;  // the p_new K1 wrote
+    double v[2] = {0.0, 0.0};
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < N; i += gridDim.x * kBlock) {
+        const float pi = p[i];
+        if (PM == kScaled) x[i] = fmaf(alpha * dinv[i], pi, x[i]);
+        else x[i] = fmaf(alpha, pi, x[i]);
+        const float rn = fmaf(alpha, Ap[i], r[i]);
+        r[i] = rn;
+        if (PM == kJacobi) {
+            v[0] += (double)(rn * (dinv[i] * rn));
+            v[1] += (double)(rn * rn);
+        } else if (PM == kScaled) {
+            const float u = rn / dinv[i];  // unscaled residual
+            v[0] += (double)(rn * rn);
+            v[1] += (double)(u * u);
+        } else {
+            v[0] += (double)(rn * rn);
+            v[1] = v[0];
+        }
+    }
+    grid_reduce<2>(v, partials, counter, [&](double* tt) {
+        const double rz = tt[0], rr = tt[1];
+        sc->iters += 1;
+        if (!isfinite(rz) || !isfinite(rr)) {
+            sc->done = kNonFinite;
+            return;
+        }
+        sc->rz_prev = sc->rz;
+        sc->beta = (float)(rz / sc->rz);
+        sc->rz = rz;
+        sc->rr = rr;
+        if (sc->tol2 >= 0 && rr <= sc->tol2) {
+            if (verify) sc->verify = 1;
+            else sc->done = kConverged;
+        } else if (sc->iters >= sc->maxit) {
+            sc->done = kMaxIt;
+        }
+    });
+}
+
+// FP64-accumulated true residual t = A x - b.  op 0: measure only (g0 = ||t||^2);
+// op 1: replace (r = t, or s*t for the scaled system; <r,z>, ||t||^2);
+// op 2: start the scaled system (as 1 with beta = 0).
+template <int R, int PM>
+__global__ __launch_bounds__(kBlock) void k_trueres25(const float* __restrict__ A, size_t pitch, int W, int N,
+                                                      const float* __restrict__ x, const float* __restrict__ b,
+                                                      const float* __restrict__ dinv, float* __restrict__ r, int op,
+                                                      Scalars* sc, double* partials, unsigned int* counter) {
+    if (op == 1 && sc->done != kRun) return;
+    __shared__ __attribute__((aligned(16))) float lds[Tile<R>::LDS_FLOATS];
+    const int ntiles = (N + Tile<R>::ROWS - 1) / Tile<R>::ROWS;
+    const TileRange tr = xcd_tiles(ntiles);
+    constexpr int SEG = Tile<R>::SEG;
+    double v[2] = {0.0, 0.0};
+    for (int t = tr.begin; t < tr.end; t += tr.step) {
+        const int i0 = t * Tile<R>::ROWS;
+        __syncthreads();
+        stage5<R>(lds, i0, W, N, [&](int g) { return x[g]; });
+        __syncthreads();
+        const int ib = i0 + R * threadIdx.x;
+        if (ib < N) {
+            const float* xs = lds + R * threadIdx.x;
+            double acc[R];
+#pragma unroll
+            for (int j = 0; j < R; ++j) acc[j] = 0.0;
+#pragma unroll
+            for (int k = 0; k < 25; ++k) {
+                Raw<R, kF32> a;
+                load_band<R, kF32>(A, (size_t)k * pitch + (size_t)ib, a);
+                const int s = k / 5, dx = k % 5;
+#pragma unroll
+                for (int j = 0; j < R; ++j) acc[j] = fma((double)a.get(j), (double)xs[s * SEG + j + dx], acc[j]);
+            }
+            float bb[R], dd[R], out[R];
+            load_rows<R>(b, ib, N, bb);
+            if (PM != kPlain) load_rows<R>(dinv, ib, N, dd);
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                const double tj = acc[j] - (double)bb[j];
+                float rj = (float)tj;
+                if (PM == kScaled) rj = (float)(tj * (double)dd[j]);
+                out[j] = rj;
+                if (ib + j < N) {
+                    v[1] += tj * tj;
+                    if (PM == kJacobi) v[0] += (double)(rj * (dd[j] * rj));
+                    else v[0] += (double)(rj * rj);
+                }
+            }
+            if (op != 0) store_rows<R>(r, ib, N, out);
+        }
+    }
+    grid_reduce<2>(v, partials, counter, [&](double* tt) {
+        sc->g0 = tt[1];
+        if (op == 0) return;
+        fin_residual(sc, tt[0], tt[1], op == 2, true);
+    });
+}
+
+// Mixed-precision operator: As = S A S with S = diag(1/sqrt(a_ii)) (unit
+// diagonal, band 12 not stored), rounded to FP16 or BF16; s written to svec.
+template <int R, int FMT>
+__global__ __launch_bounds__(kBlock) void k_scale16(const float* __restrict__ A, size_t pitch, int W, int N,
+                                                    unsigned short* __restrict__ A16, size_t pitch16,
+                                                    float* __restrict__ svec) {
+    __shared__ __attribute__((aligned(16))) float lds[Tile<R>::LDS_FLOATS];
+    constexpr int SEG = Tile<R>::SEG;
+    const int ntiles = (N + Tile<R>::ROWS - 1) / Tile<R>::ROWS;
+    const TileRange tr = xcd_tiles(ntiles);
+    for (int t = tr.begin; t < tr.end; t += tr.step) {
+        const int i0 = t * Tile<R>::ROWS;
+        __syncthreads();
+        stage5<R>(lds, i0, W, N, [&](int g) {
+            const float d = A[(size_t)12 * pitch + g];
+            return d > 0.f ? 1.0f / sqrtf(d) : 1.0f;
+        });
+        __syncthreads();
+        const int ib = i0 + R * threadIdx.x;
+        if (ib >= N) continue;
+        const float* xs = lds + R * threadIdx.x;
+#pragma unroll
+        for (int k = 0; k < 25; ++k) {
+            if (k == 12) continue;
+            const int s = k / 5, dx = k % 5;
+            Raw<R, kF32> a;
+            load_band<R, kF32>(A, (size_t)k * pitch + (size_t)ib, a);
+            unsigned short h[R];
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                const float v = (a.get(j) * xs[2 * SEG + j + 2]) * xs[s * SEG + j + dx];
+                h[j] = FMT == kF16 ? f32_to_f16_rne(v) : f32_to_bf16_rne(v);
+            }
+            unsigned short* dst = A16 + (size_t)k * pitch16 + ib;
+#pragma unroll
+            for (int j = 0; j < R; ++j) dst[j] = h[j];
+        }
+        float sc[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) sc[j] = xs[2 * SEG + j + 2];
+        store_rows<R>(svec, ib, N, sc);
+    }
+}
+
+// Gradient descent (main.cu:424-449): d = A x - b, <d,b>, ||d||^2
+template <int R>
+__global__ __launch_bounds__(kBlock) void k_gd1_25(const float* __restrict__ A, size_t pitch, int W, int N,
+                                                   const float* __restrict__ x, const float* __restrict__ b,
+                                                   float* __restrict__ d, Scalars* sc, double* partials,
+                                                   unsigned int* counter) {
+    if (sc->done != kRun) return;
+    __shared__ __attribute__((aligned(16))) float lds[Tile<R>::LDS_FLOATS];
+    const int ntiles = (N + Tile<R>::ROWS - 1) / Tile<R>::ROWS;
+    const TileRange tr = xcd_tiles(ntiles);
+    double v[2] = {0.0, 0.0};
+    for (int t = tr.begin; t < tr.end; t += tr.step) {
+        const int i0 = t * Tile<R>::ROWS;
+        __syncthreads();
+        stage5<R>(lds, i0, W, N, [&](int g) { return x[g]; });
+        __syncthreads();
+        const int ib = i0 + R * threadIdx.x;
+        if (ib < N) {
+            float y[R], bb[R];
+            tile_rows<R, kF32, false>(A, pitch, ib, lds + R * threadIdx.x, y);
+            load_rows<R>(b, ib, N, bb);
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                y[j] -= bb[j];
+                if (ib + j < N) {
+                    v[0] += (double)(y[j] * bb[j]);
+                    v[1] += (double)(y[j] * y[j]);
+                }
+            }
+            store_rows<R>(d, ib, N, y);
+        }
+    }
+    grid_reduce<2>(v, partials, counter, [&](double* tt) {
+        sc->g0 = tt[0];
+        sc->rr = tt[1];
+        if (!isfinite(tt[0]) || !isfinite(tt[1])) sc->done = kNonFinite;
+        else if (sc->tol2 >= 0 && tt[1] <= sc->tol2) sc->done = kConverged;
+    });
+}
+
+// e = A d (not stored), <e,x>, <e,d> -> alpha = (<d,b> - <e,x>)/<e,d>
+template <int R>
+__global__ __launch_bounds__(kBlock) void k_gd2_25(const float* __restrict__ A, size_t pitch, int W, int N,
+                                                   const float* __restrict__ x, const float* __restrict__ d,
+                                                   Scalars* sc, double* partials, unsigned int* counter) {
+    if (sc->done != kRun) return;
+    __shared__ __attribute__((aligned(16))) float lds[Tile<R>::LDS_FLOATS];
+    const int ntiles = (N + Tile<R>::ROWS - 1) / Tile<R>::ROWS;
+    const TileRange tr = xcd_tiles(ntiles);
+    double v[2] = {0.0, 0.0};
+    for (int t = tr.begin; t < tr.end; t += tr.step) {
+        const int i0 = t * Tile<R>::ROWS;
+        __syncthreads();
+        stage5<R>(lds, i0, W, N, [&](int g) { return d[g]; });
+        __syncthreads();
+        const int ib = i0 + R * threadIdx.x;
+        if (ib < N) {
+            float y[R], xx[R];
+            tile_rows<R, kF32, false>(A, pitch, ib, lds + R * threadIdx.x, y);
+            load_rows<R>(x, ib, N, xx);
+#pragma unroll
+            for (int j = 0; j < R; ++j)
+                if (ib + j < N) {
+                    v[0] += (double)(y[j] * xx[j]);
+                    v[1] += (double)(y[j] * lds[2 * Tile<R>::SEG + R * threadIdx.x + 2 + j]);
+                }
+        }
+    }
+    grid_reduce<2>(v, partials, counter, [&](double* tt) {
+        float k = (float)sc->g0;
+        k = k - (float)tt[0];
+        k = k / (float)tt[1];
+        if (!isfinite(k)) {
+            sc->done = tt[1] == 0 ? kConverged : kNonFinite;
+            return;
+        }
+        sc->alpha = k;
+    });
+}
+
+// x += alpha d; counts the iteration
+__global__ __launch_bounds__(kBlock) void k_gd3(float* __restrict__ x, const float* __restrict__ d, int N, Scalars* sc,
+                                                double* partials, unsigned int* counter) {
+    if (sc->done != kRun) return;
+    const float a = sc->alpha;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < N; i += gridDim.x * kBlock) x[i] = fmaf(a, d[i], x[i]);
+    double v[1] = {0.0};
+    grid_reduce<1>(v, partials, counter, [&](double*) {
+        sc->iters += 1;
+        if (sc->iters >= sc->maxit) sc->done = kMaxIt;
+    });
+}
+
+// ---------------------------------------------------------------------------
+// generic kernels (any band set, one row per thread, padded vectors)
+
+template <int PM>
+__global__ __launch_bounds__(kBlock) void k_resid_g(const float* __restrict__ a, size_t pitch, BandArgs ba, int N,
+                                                    const float* __restrict__ x, const float* __restrict__ b,
+                                                    const float* __restrict__ dinv, float* __restrict__ r, Scalars* sc,
+                                                    double* partials, unsigned int* counter, int start) {
+    if (!start && sc->done != kRun) return;
+    double v[2] = {0.0, 0.0};
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < N; i += gridDim.x * kBlock) {
+        float s = 0.f;
+        for (int k = 0; k < ba.nb; ++k) s = fmaf(a[(size_t)i + (size_t)k * pitch], x[(long long)i + ba.off[k]], s);
+        s -= b[i];
+        r[i] = s;
+        const float z = PM == kJacobi ? dinv[i] * s : s;
+        v[0] += (double)(s * z);
+        v[1] += (double)(s * s);
+    }
+    grid_reduce<2>(v, partials, counter, [&](double* tt) { fin_residual(sc, tt[0], tt[1], true, false); });
+}
+
+template <int PM>
+__global__ __launch_bounds__(kBlock) void k_cgstep_g(const float* __restrict__ a, size_t pitch, BandArgs ba, int N,
+                                                     const float* __restrict__ r, const float* __restrict__ dinv,
+                                                     float* __restrict__ p0, float* __restrict__ p1,
+                                                     float* __restrict__ Ap, Scalars* sc, double* partials,
+                                                     unsigned int* counter) {
+    if (stopped(sc)) return;
+    const float beta = sc->beta;
+    const int par = sc->iters & 1;
+    const float* __restrict__ pold = par ? p1 : p0;
+    float* __restrict__ pnew = par ? p0 : p1;
+    double v[1] = {0.0};
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < N; i += gridDim.x * kBlock) {
+        float s = 0.f;
+        for (int k = 0; k < ba.nb; ++k) {
+            const long long g = (long long)i + ba.off[k];
+            const float z = PM == kJacobi ? dinv[g] * r[g] : r[g];
+            const float pg = beta != 0.f ? fmaf(beta, pold[g], -z) : -z;
+            s = fmaf(a[(size_t)i + (size_t)k * pitch], pg, s);
+        }
+        const float zi = PM == kJacobi ? dinv[i] * r[i] : r[i];
+        const float pi = beta != 0.f ? fmaf(beta, pold[i], -zi) : -zi;
+        pnew[i] = pi;
+        Ap[i] = s;
+        v[0] += (double)(pi * s);
+    }
+    grid_reduce<1>(v, partials, counter, [&](double* tt) {
+        const double pAp = tt[0];
+        sc->pAp = pAp;
+        if (!isfinite(pAp)) sc->done = kNonFinite;
+        else if (!(pAp > 0)) sc->done = kBreakdown;
+        else sc->alpha = (float)(sc->rz / pAp);
+    });
+}
+
+template <int PM>
+__global__ __launch_bounds__(kBlock) void k_trueres_g(const float* __restrict__ a, size_t pitch, BandArgs ba, int N,
+                                                      const float* __restrict__ x, const float* __restrict__ b,
+                                                      const float* __restrict__ dinv, float* __restrict__ r, int op,
+                                                      Scalars* sc, double* partials, unsigned int* counter) {
+    if (op == 1 && sc->done != kRun) return;
+    double v[2] = {0.0, 0.0};
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < N; i += gridDim.x * kBlock) {
+        double s = 0.0;
+        for (int k = 0; k < ba.nb; ++k)
+            s = fma((double)a[(size_t)i + (size_t)k * pitch], (double)x[(long long)i + ba.off[k]], s);
+        const double t = s - (double)b[i];
+        const float rj = (float)t;
+        if (op != 0) r[i] = rj;
+        v[1] += t * t;
+        v[0] += PM == kJacobi ? (double)(rj * (dinv[i] * rj)) : (double)(rj * rj);
+    }
+    grid_reduce<2>(v, partials, counter, [&](double* tt) {
+        sc->g0 = tt[1];
+        if (op == 0) return;
+        fin_residual(sc, tt[0], tt[1], op == 2, true);
+    });
+}
+
+__global__ __launch_bounds__(kBlock) void k_gd1_g(const float* __restrict__ a, size_t pitch, BandArgs ba, int N,
+                                                  const float* __restrict__ x, const float* __restrict__ b,
+                                                  float* __restrict__ d, Scalars* sc, double* partials,
+                                                  unsigned int* counter) {
+    if (sc->done != kRun) return;
+    double v[2] = {0.0, 0.0};
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < N; i += gridDim.x * kBlock) {
+        float s = 0.f;
+        for (int k = 0; k < ba.nb; ++k) s = fmaf(a[(size_t)i + (size_t)k * pitch], x[(long long)i + ba.off[k]], s);
+        s -= b[i];
+        d[i] = s;
+        v[0] += (double)(s * b[i]);
+        v[1] += (double)(s * s);
+    }
+    grid_reduce<2>(v, partials, counter, [&](double* tt) {
+        sc->g0 = tt[0];
+        sc->rr = tt[1];
+        if (!isfinite(tt[0]) || !isfinite(tt[1])) sc->done = kNonFinite;
+        else if (sc->tol2 >= 0 && tt[1] <= sc->tol2) sc->done = kConverged;
+    });
+}
+
+__global__ __launch_bounds__(kBlock) void k_gd2_g(const float* __restrict__ a, size_t pitch, BandArgs ba, int N,
+                                                  const float* __restrict__ x, const float* __restrict__ d,
+                                                  Scalars* sc, double* partials, unsigned int* counter) {
+    if (sc->done != kRun) return;
+    double v[2] = {0.0, 0.0};
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < N; i += gridDim.x * kBlock) {
+        float s = 0.f;
+        for (int k = 0; k < ba.nb; ++k) s = fmaf(a[(size_t)i + (size_t)k * pitch], d[(long long)i + ba.off[k]], s);
+        v[0] += (double)(s * x[i]);
+        v[1] += (double)(s * d[i]);
+    }
+    grid_reduce<2>(v, partials, counter, [&](double* tt) {
+        float k = (float)sc->g0;
+        k = k - (float)tt[0];
+        k = k / (float)tt[1];
+        if (!isfinite(k)) {
+            sc->done = tt[1] == 0 ? kConverged : kNonFinite;
+            return;
+        }
+        sc->alpha = k;
+    });
+}
+
+__global__ __launch_bounds__(kBlock) void k_diag_inv(const float* __restrict__ diag, float* __restrict__ dinv, int N) {
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < N; i += gridDim.x * kBlock) {
+        const float d = diag[i];
+        dinv[i] = d != 0.f ? 1.f / d : 0.f;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// workspaces: vectors padded by `pad` zeros (generic kernels read p, r and
+// dinv at i + off), one allocation, each vector 256-byte aligned.
+
+struct Workspace {
+    int dev = -1;
+    long n = 0;
+    int pad = 0;
+    float* base = nullptr;
+    float *r = nullptr, *p0 = nullptr, *p1 = nullptr, *Ap = nullptr, *dinv = nullptr, *d = nullptr;
+    unsigned short* A16 = nullptr;
+    size_t pitch16 = 0;
+    long cap16 = 0;
+    Scalars* sc = nullptr;
+    Scalars* hsc = nullptr;  // pinned, 2 slots
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+static std::mutex g_ws_mu;
+static std::map<std::pair<int, gm_stream_t>, Workspace*> g_ws;
+
+static void ws_free_vectors(Workspace* w) {
+    if (w->base) (void)hipFree(w->base);
+    w->base = nullptr;
+    w->n = 0;
+}
+
+static gm_status ws_get(gm_stream_t s, long n, int pad, bool need16, Workspace** out) {
+    int dev = 0;
+    GM_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> g(g_ws_mu);
+    Workspace*& w = g_ws[std::make_pair(dev, s)];
+    if (!w) {
+        w = new Workspace();
+        w->dev = dev;
+        GM_HIP(hipMalloc(&w->sc, sizeof(Scalars)));
+        GM_HIP(hipMemset(w->sc, 0, sizeof(Scalars)));
+        GM_HIP(hipHostMalloc(&w->hsc, 2 * sizeof(Scalars), hipHostMallocDefault));
+        GM_HIP(hipEventCreateWithFlags(&w->ev[0], hipEventDisableTiming));
+        GM_HIP(hipEventCreateWithFlags(&w->ev[1], hipEventDisableTiming));
+    }
+    pad = (pad + 63) / 64 * 64;
+    if (w->n < n || w->pad < pad) {
+        ws_free_vectors(w);
+        const long stride = (n + 2L * pad + 63) / 64 * 64;
+        float* b = nullptr;
+        GM_HIP(hipMalloc(&b, sizeof(float) * stride * 6));
+        GM_HIP(hipMemset(b, 0, sizeof(float) * stride * 6));
+        w->base = b;
+        w->n = n;
+        w->pad = pad;
+        w->r = b + 0 * stride + pad;
+        w->p0 = b + 1 * stride + pad;
+        w->p1 = b + 2 * stride + pad;
+        w->Ap = b + 3 * stride + pad;
+        w->dinv = b + 4 * stride + pad;
+        w->d = b + 5 * stride + pad;
+    }
+    if (need16 && w->cap16 < n) {
+        if (w->A16) (void)hipFree(w->A16);
+        w->A16 = nullptr;
+        w->pitch16 = (size_t)((n + 63) / 64 * 64);
+        GM_HIP(hipMalloc(&w->A16, sizeof(unsigned short) * w->pitch16 * 25 + 256));
+        GM_HIP(hipMemset(w->A16, 0, sizeof(unsigned short) * w->pitch16 * 25 + 256));
+        w->cap16 = n;
+    }
+    *out = w;
+    return GM_OK;
+}
+
+void release_workspaces() {
+    std::lock_guard<std::mutex> g(g_ws_mu);
+    for (auto& kv : g_ws) {
+        Workspace* w = kv.second;
+        ws_free_vectors(w);
+        if (w->A16) (void)hipFree(w->A16);
+        if (w->sc) (void)hipFree(w->sc);
+        if (w->hsc) (void)hipHostFree(w->hsc);
+        for (int i = 0; i < 2; ++i)
+            if (w->ev[i]) (void)hipEventDestroy(w->ev[i]);
+        delete w;
+    }
+    g_ws.clear();
+}
+
+// ---------------------------------------------------------------------------
+// host driver
+
+struct Solver {
+    const BandedMatrix* A;
+    BandPlan plan;
+    bool tiled;
+    int N;
+    const float* b;
+    float* x;
+    Workspace* w;
+    RedScratch rs;
+    DevInfo dv;
+    hipStream_t hs;
+    int pm;        // PMode
+    int fmt;       // kF32 / kF16 / kBF16
+    int g_tile4, g_tile8, g_elem, g_gen;
+};
+
+static gm_status launch_resid(Solver& S, int start) {
+    if (S.tiled) {
+        if (S.pm == kJacobi)
+            hipLaunchKernelGGL((k_resid25<4, kJacobi>), dim3(S.g_tile4), dim3(kBlock), 0, S.hs, S.A->a, S.A->apitch,
+                               S.plan.W, S.N, S.x, S.b, S.w->dinv, S.w->r, S.w->sc, S.rs.partials, S.rs.counter, start);
+        else
+            hipLaunchKernelGGL((k_resid25<4, kPlain>), dim3(S.g_tile4), dim3(kBlock), 0, S.hs, S.A->a, S.A->apitch,
+                               S.plan.W, S.N, S.x, S.b, S.w->dinv, S.w->r, S.w->sc, S.rs.partials, S.rs.counter, start);
+    } else {
+        if (S.pm == kJacobi)
+            hipLaunchKernelGGL(k_resid_g<kJacobi>, dim3(S.g_gen), dim3(kBlock), 0, S.hs, S.A->a, S.A->apitch,
+                               S.plan.args, S.N, S.x, S.b, S.w->dinv, S.w->r, S.w->sc, S.rs.partials, S.rs.counter,
+                               start);
+        else
+            hipLaunchKernelGGL(k_resid_g<kPlain>, dim3(S.g_gen), dim3(kBlock), 0, S.hs, S.A->a, S.A->apitch,
+                               S.plan.args, S.N, S.x, S.b, S.w->dinv, S.w->r, S.w->sc, S.rs.partials, S.rs.counter,
+                               start);
+    }
+    GM_LAUNCH_CHECK();
+    return GM_OK;
+}
+
+static gm_status launch_trueres(Solver& S, int op) {
+    if (S.tiled) {
+        if (S.pm == kJacobi)
+            hipLaunchKernelGGL((k_trueres25<4, kJacobi>), dim3(S.g_tile4), dim3(kBlock), 0, S.hs, S.A->a,
+                               S.A->apitch, S.plan.W, S.N, S.x, S.b, S.w->dinv, S.w->r, op, S.w->sc, S.rs.partials,
+                               S.rs.counter);
+        else if (S.pm == kScaled)
+            hipLaunchKernelGGL((k_trueres25<4, kScaled>), dim3(S.g_tile4), dim3(kBlock), 0, S.hs, S.A->a,
+                               S.A->apitch, S.plan.W, S.N, S.x, S.b, S.w->dinv, S.w->r, op, S.w->sc, S.rs.partials,
+                               S.rs.counter);
+        else
+            hipLaunchKernelGGL((k_trueres25<4, kPlain>), dim3(S.g_tile4), dim3(kBlock), 0, S.hs, S.A->a,
+                               S.A->apitch, S.plan.W, S.N, S.x, S.b, S.w->dinv, S.w->r, op, S.w->sc, S.rs.partials,
+                               S.rs.counter);
+    } else {
+        if (S.pm == kJacobi)
+            hipLaunchKernelGGL(k_trueres_g<kJacobi>, dim3(S.g_gen), dim3(kBlock), 0, S.hs, S.A->a, S.A->apitch,
+                               S.plan.args, S.N, S.x, S.b, S.w->dinv, S.w->r, op, S.w->sc, S.rs.partials,
+                               S.rs.counter);
+        else
+            hipLaunchKernelGGL(k_trueres_g<kPlain>, dim3(S.g_gen), dim3(kBlock), 0, S.hs, S.A->a, S.A->apitch,
+                               S.plan.args, S.N, S.x, S.b, S.w->dinv, S.w->r, op, S.w->sc, S.rs.partials,
+                               S.rs.counter);
+    }
+    GM_LAUNCH_CHECK();
+    return GM_OK;
+}
+
+static gm_status launch_cg_iter(Solver& S, int verify) {
+    Workspace* w = S.w;
+    if (S.tiled) {
+        if (S.pm == kScaled) {
+            if (S.fmt == kF16)
+                hipLaunchKernelGGL((k_cgstep25<8, kF16, kScaled>), dim3(S.g_tile8), dim3(kBlock), 0, S.hs, w->A16,
+                                   w->pitch16, S.plan.W, S.N, w->r, w->dinv, w->p0, w->p1, w->Ap, w->sc, S.rs.partials,
+                                   S.rs.counter);
+            else
+                hipLaunchKernelGGL((k_cgstep25<8, kBF16, kScaled>), dim3(S.g_tile8), dim3(kBlock), 0, S.hs, w->A16,
+                                   w->pitch16, S.plan.W, S.N, w->r, w->dinv, w->p0, w->p1, w->Ap, w->sc, S.rs.partials,
+                                   S.rs.counter);
+        } else if (S.pm == kJacobi) {
+            hipLaunchKernelGGL((k_cgstep25<4, kF32, kJacobi>), dim3(S.g_tile4), dim3(kBlock), 0, S.hs, S.A->a,
+                               S.A->apitch, S.plan.W, S.N, w->r, w->dinv, w->p0, w->p1, w->Ap, w->sc, S.rs.partials,
+                               S.rs.counter);
+        } else {
+            hipLaunchKernelGGL((k_cgstep25<4, kF32, kPlain>), dim3(S.g_tile4), dim3(kBlock), 0, S.hs, S.A->a,
+                               S.A->apitch, S.plan.W, S.N, w->r, w->dinv, w->p0, w->p1, w->Ap, w->sc, S.rs.partials,
+                               S.rs.counter);
+        }
+    } else {
+        if (S.pm == kJacobi)
+            hipLaunchKernelGGL(k_cgstep_g<kJacobi>, dim3(S.g_gen), dim3(kBlock), 0, S.hs, S.A->a, S.A->apitch,
+                               S.plan.args, S.N, w->r, w->dinv, w->p0, w->p1, w->Ap, w->sc, S.rs.partials,
+                               S.rs.counter);
+        else
+            hipLaunchKernelGGL(k_cgstep_g<kPlain>, dim3(S.g_gen), dim3(kBlock), 0, S.hs, S.A->a, S.A->apitch,
+                               S.plan.args, S.N, w->r, w->dinv, w->p0, w->p1, w->Ap, w->sc, S.rs.partials,
+                               S.rs.counter);
+    }
+    GM_LAUNCH_CHECK();
+    if (S.pm == kJacobi)
+        hipLaunchKernelGGL(k_cgupdate<kJacobi>, dim3(S.g_elem), dim3(kBlock), 0, S.hs, S.x, w->r, w->dinv, w->p0,
+                           w->p1, w->Ap, S.N, verify, w->sc, S.rs.partials, S.rs.counter);
+    else if (S.pm == kScaled)
+        hipLaunchKernelGGL(k_cgupdate<kScaled>, dim3(S.g_elem), dim3(kBlock), 0, S.hs, S.x, w->r, w->dinv, w->p0,
+                           w->p1, w->Ap, S.N, verify, w->sc, S.rs.partials, S.rs.counter);
+    else
+        hipLaunchKernelGGL(k_cgupdate<kPlain>, dim3(S.g_elem), dim3(kBlock), 0, S.hs, S.x, w->r, w->dinv, w->p0,
+                           w->p1, w->Ap, S.N, verify, w->sc, S.rs.partials, S.rs.counter);
+    GM_LAUNCH_CHECK();
+    return GM_OK;
+}
+
+static gm_status launch_gd_iter(Solver& S) {
+    Workspace* w = S.w;
+    if (S.tiled) {
+        hipLaunchKernelGGL((k_gd1_25<4>), dim3(S.g_tile4), dim3(kBlock), 0, S.hs, S.A->a, S.A->apitch, S.plan.W, S.N,
+                           S.x, S.b, w->d, w->sc, S.rs.partials, S.rs.counter);
+        GM_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_gd2_25<4>), dim3(S.g_tile4), dim3(kBlock), 0, S.hs, S.A->a, S.A->apitch, S.plan.W, S.N,
+                           S.x, w->d, w->sc, S.rs.partials, S.rs.counter);
+    } else {
+        hipLaunchKernelGGL(k_gd1_g, dim3(S.g_gen), dim3(kBlock), 0, S.hs, S.A->a, S.A->apitch, S.plan.args, S.N, S.x,
+                           S.b, w->d, w->sc, S.rs.partials, S.rs.counter);
+        GM_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_gd2_g, dim3(S.g_gen), dim3(kBlock), 0, S.hs, S.A->a, S.A->apitch, S.plan.args, S.N, S.x,
+                           w->d, w->sc, S.rs.partials, S.rs.counter);
+    }
+    GM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_gd3, dim3(S.g_elem), dim3(kBlock), 0, S.hs, S.x, w->d, S.N, w->sc, S.rs.partials,
+                       S.rs.counter);
+    GM_LAUNCH_CHECK();
+    return GM_OK;
+}
+
+static gm_status status_of_done(int done) {
+    switch (done) {
+        case kConverged: return GM_OK;
+        case kBreakdown: return GM_ERR_BREAKDOWN;
+        case kNonFinite: return GM_ERR_NONFINITE;
+        case kMaxIt: return GM_ERR_NOT_CONVERGED;
+        default: return GM_OK;
+    }
+}
+
+gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, const gm_solve_opts* o,
+                     gm_solve_info* info, gm_stream_t s) {
+    Solver S{};
+    gm_status st = get_plan(A, &S.plan);
+    if (st != GM_OK) return st;
+    if (!x || !b) return fail(GM_ERR_INVALID, "solve: null vector");
+    if (o->max_iter <= 0) return fail(GM_ERR_INVALID, "solve: max_iter must be > 0");
+    if (o->method < GM_METHOD_CG || o->method > GM_METHOD_GD) return fail(GM_ERR_INVALID, "solve: method");
+    if (o->precision < GM_PREC_FP32 || o->precision > GM_PREC_BF16) return fail(GM_ERR_INVALID, "solve: precision");
+    S.A = A;
+    S.N = A->rows;
+    S.b = b;
+    S.x = x;
+    S.hs = (hipStream_t)s;
+    S.tiled = tiled_ok(S.plan, A->a, A->apitch);
+    if (!S.tiled && pad < S.plan.maxoff)
+        return fail(GM_ERR_INVALID, "solve: pad %d < largest band offset %d", pad, S.plan.maxoff);
+    const bool mixed = o->precision != GM_PREC_FP32;
+    if (mixed && (!S.tiled || o->method == GM_METHOD_GD))
+        return fail(GM_ERR_UNSUPPORTED, "16-bit forms need the 5x5 Levin band structure and CG/PCG");
+    if ((o->method == GM_METHOD_PCG || mixed) && S.plan.args.center < 0)
+        return fail(GM_ERR_INVALID, "solve: Jacobi scaling needs a band with offset 0");
+    S.pm = mixed ? kScaled : (o->method == GM_METHOD_PCG ? kJacobi : kPlain);
+    S.fmt = o->precision == GM_PREC_FP16 ? kF16 : (o->precision == GM_PREC_BF16 ? kBF16 : kF32);
+    if ((st = dev_info(&S.dv)) != GM_OK) return st;
+    if ((st = red_scratch(s, &S.rs)) != GM_OK) return st;
+    if ((st = ws_get(s, S.N > 0 ? S.N : 1, S.plan.maxoff, mixed, &S.w)) != GM_OK) return st;
+    if (info) {
+        memset(info, 0, sizeof(*info));
+        info->true_relres = -1;
+    }
+    if (S.N == 0) return GM_OK;
+    S.g_tile4 = grid_for_tiles(S.dv, (S.N + Tile<4>::ROWS - 1) / Tile<4>::ROWS, 4);
+    S.g_tile8 = grid_for_tiles(S.dv, (S.N + Tile<8>::ROWS - 1) / Tile<8>::ROWS, 3);
+    S.g_elem = elementwise_grid(S.dv, S.N, 4);
+    S.g_gen = elementwise_grid(S.dv, S.N, 1);
+    Workspace* w = S.w;
+
+    hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(64), 0, S.hs, w->sc, o->max_iter);
+    GM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_bnorm, dim3(S.g_elem), dim3(kBlock), 0, S.hs, b, S.N, o->tol, w->sc, S.rs.partials,
+                       S.rs.counter);
+    GM_LAUNCH_CHECK();
+
+    const int verify = (o->tol > 0 && o->verify) ? 1 : 0;
+    if (S.pm == kJacobi) {
+        hipLaunchKernelGGL(k_diag_inv, dim3(S.g_elem), dim3(kBlock), 0, S.hs, A->a + (size_t)S.plan.args.center * A->apitch,
+                           w->dinv, S.N);
+        GM_LAUNCH_CHECK();
+    } else if (S.pm == kScaled) {
+        if (S.fmt == kF16)
+            hipLaunchKernelGGL((k_scale16<8, kF16>), dim3(S.g_tile8), dim3(kBlock), 0, S.hs, A->a, A->apitch, S.plan.W,
+                               S.N, w->A16, w->pitch16, w->dinv);
+        else
+            hipLaunchKernelGGL((k_scale16<8, kBF16>), dim3(S.g_tile8), dim3(kBlock), 0, S.hs, A->a, A->apitch,
+                               S.plan.W, S.N, w->A16, w->pitch16, w->dinv);
+        GM_LAUNCH_CHECK();
+    }
+
+    const int chunk = o->check_every > 0 ? o->check_every : 16;
+    const int refine = mixed ? (o->refine_every > 0 ? o->refine_every : 50) : 0;
+    long enq = 0;  // iterations enqueued
+    int polls = 0;
+    bool finished = false;
+
+    if (o->method == GM_METHOD_GD) {
+        while (!finished) {
+            for (int j = 0; j < chunk && enq < o->max_iter; ++j, ++enq)
+                if ((st = launch_gd_iter(S)) != GM_OK) return st;
+            GM_HIP(hipMemcpyAsync(&w->hsc[polls & 1], w->sc, sizeof(Scalars), hipMemcpyDeviceToHost, S.hs));
+            GM_HIP(hipEventRecord(w->ev[polls & 1], S.hs));
+            if (polls > 0) {
+                GM_HIP(hipEventSynchronize(w->ev[(polls - 1) & 1]));
+                if (w->hsc[(polls - 1) & 1].done != kRun) finished = true;
+            }
+            ++polls;
+            if (enq >= o->max_iter) finished = true;
+        }
+    } else {
+        if (S.pm == kScaled) {
+            if ((st = launch_trueres(S, 2)) != GM_OK) return st;
+        } else {
+            if ((st = launch_resid(S, 1)) != GM_OK) return st;
+        }
+        long since_restart = 0;
+        while (!finished) {
+            for (int j = 0; j < chunk && enq < o->max_iter; ++j) {
+                if ((st = launch_cg_iter(S, verify)) != GM_OK) return st;
+                ++enq;
+                ++since_restart;
+                if (o->restart > 0 && since_restart >= o->restart && enq < o->max_iter) {
+                    // cgSolve's periodic restart (main.cu:483-493): r from scratch, p = -z
+                    if ((st = launch_resid(S, 0)) != GM_OK) return st;
+                    since_restart = 0;
+                }
+                if (refine > 0 && enq % refine == 0) {
+                    if ((st = launch_trueres(S, 1)) != GM_OK) return st;
+                }
+            }
+            GM_HIP(hipMemcpyAsync(&w->hsc[polls & 1], w->sc, sizeof(Scalars), hipMemcpyDeviceToHost, S.hs));
+            GM_HIP(hipEventRecord(w->ev[polls & 1], S.hs));
+            if (polls > 0) {
+                GM_HIP(hipEventSynchronize(w->ev[(polls - 1) & 1]));
+                const Scalars& h = w->hsc[(polls - 1) & 1];
+                if (h.done != kRun) finished = true;
+                else if (h.verify) {
+                    // recursive residual reached the tolerance: check the true one
+                    if ((st = launch_trueres(S, 1)) != GM_OK) return st;
+                }
+            }
+            ++polls;
+            if (enq >= o->max_iter) {
+                // drain: a pending verification may still need the final check
+                GM_HIP(hipMemcpyAsync(&w->hsc[polls & 1], w->sc, sizeof(Scalars), hipMemcpyDeviceToHost, S.hs));
+                GM_HIP(hipStreamSynchronize(S.hs));
+                const Scalars& h = w->hsc[polls & 1];
+                if (h.done == kRun && h.verify) {
+                    if ((st = launch_trueres(S, 1)) != GM_OK) return st;
+                }
+                finished = true;
+            }
+        }
+    }
+    GM_HIP(hipStreamSynchronize(S.hs));
+    Scalars fin;
+    GM_HIP(hipMemcpy(&fin, w->sc, sizeof(Scalars), hipMemcpyDeviceToHost));
+    const double bb = fin.bb > 0 ? fin.bb : 1.0;
+    gm_status result = status_of_done(fin.done);
+    if (fin.done == kRun) result = (o->tol > 0) ? GM_ERR_NOT_CONVERGED : GM_OK;
+    if (o->tol <= 0 && fin.done == kMaxIt) result = GM_OK;  // fixed-iteration run
+    if (info) {
+        info->iterations = fin.iters;
+        info->status = result;
+        info->relres = sqrt(fin.rr / bb);
+        info->replacements = fin.replacements;
+        info->true_relres = -1;
+    }
+    if (o->verify && info && o->method != GM_METHOD_GD) {
+        // FP64-accumulated true residual of the returned x
+        if ((st = launch_trueres(S, 0)) != GM_OK) return st;
+        GM_HIP(hipStreamSynchronize(S.hs));
+        GM_HIP(hipMemcpy(&fin, w->sc, sizeof(Scalars), hipMemcpyDeviceToHost));
+        info->true_relres = sqrt(fin.g0 / bb);
+    }
+    if (result != GM_OK && result != GM_ERR_NOT_CONVERGED) {
+        set_error("solve stopped: %s after %d iterations",
+                  result == GM_ERR_BREAKDOWN ? "breakdown (p'Ap <= 0)" : "non-finite value", fin.iters);
+    } else if (result == GM_ERR_NOT_CONVERGED) {
+        set_error("solve: tolerance not reached in %d iterations (relres %.3e)", fin.iters, sqrt(fin.rr / bb));
+    }
+    return result;
+}
+
+}  // namespace gm
+
+using namespace gm;
+
+extern "C" {
+
+void gm_solve_opts_default(gm_solve_opts* o) {
+    if (!o) return;
+    o->method = GM_METHOD_PCG;
+    o->precision = GM_PREC_FP32;
+    o->max_iter = 10000;
+    o->tol = 1e-6;
+    o->restart = 0;
+    o->refine_every = 50;
+    o->check_every = 16;
+    o->verify = 1;
+}
+
+gm_status gm_solve(float* alpha, const BandedMatrix* L, const float* b, int pad, const gm_solve_opts* opts,
+                   gm_solve_info* info, gm_stream_t stream) {
+    gm_solve_opts o;
+    if (opts) o = *opts;
+    else gm_solve_opts_default(&o);
+    return solve_impl(alpha, L, b, pad, &o, info, stream);
+}
+
+gm_status gm_cg_solve(float* alpha, const BandedMatrix* L, const float* b, int pad, int iterations,
+                      int restart_interval, gm_stream_t stream) {
+    if (iterations <= 0) return GM_OK;
+    gm_solve_opts o;
+    gm_solve_opts_default(&o);
+    o.method = GM_METHOD_CG;
+    o.max_iter = iterations;
+    o.tol = 0;  // fixed iteration count (main.cu:481)
+    o.restart = restart_interval > 0 ? restart_interval : 0;
+    o.verify = 0;
+    o.check_every = 32;
+    gm_status st = solve_impl(alpha, L, b, pad, &o, nullptr, stream);
+    return st == GM_ERR_NOT_CONVERGED ? GM_OK : st;
+}
+
+gm_status gm_grad_solve(float* alpha, const BandedMatrix* L, const float* b, int iterations, int pad,
+                        gm_stream_t stream) {
+    if (iterations <= 0) return GM_OK;
+    gm_solve_opts o;
+    gm_solve_opts_default(&o);
+    o.method = GM_METHOD_GD;
+    o.max_iter = iterations;
+    o.tol = 0;
+    o.verify = 0;
+    o.check_every = 32;
+    gm_status st = solve_impl(alpha, L, b, pad, &o, nullptr, stream);
+    return st == GM_ERR_NOT_CONVERGED ? GM_OK : st;
+}
+
+gm_status gm_true_relres(const BandedMatrix* A, const float* x, const float* b, double* relres, gm_stream_t s) {
+    if (!relres) return fail(GM_ERR_INVALID, "null relres");
+    Solver S{};
+    gm_status st = get_plan(A, &S.plan);
+    if (st != GM_OK) return st;
+    if (!x || !b) return fail(GM_ERR_INVALID, "null vector");
+    S.A = A;
+    S.N = A->rows;
+    S.b = b;
+    S.x = const_cast<float*>(x);
+    S.hs = (hipStream_t)s;
+    S.tiled = tiled_ok(S.plan, A->a, A->apitch);
+    S.pm = kPlain;
+    if ((st = dev_info(&S.dv)) != GM_OK) return st;
+    if ((st = red_scratch(s, &S.rs)) != GM_OK) return st;
+    if ((st = ws_get(s, S.N > 0 ? S.N : 1, S.plan.maxoff, false, &S.w)) != GM_OK) return st;
+    if (S.N == 0) {
+        *relres = 0;
+        return GM_OK;
+    }
+    S.g_tile4 = grid_for_tiles(S.dv, (S.N + Tile<4>::ROWS - 1) / Tile<4>::ROWS, 4);
+    S.g_elem = elementwise_grid(S.dv, S.N, 4);
+    S.g_gen = elementwise_grid(S.dv, S.N, 1);
+    hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(64), 0, S.hs, S.w->sc, 1);
+    GM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_bnorm, dim3(S.g_elem), dim3(kBlock), 0, S.hs, b, S.N, 0.0, S.w->sc, S.rs.partials,
+                       S.rs.counter);
+    GM_LAUNCH_CHECK();
+    if ((st = launch_trueres(S, 0)) != GM_OK) return st;
+    Scalars h;
+    GM_HIP(hipMemcpyAsync(&h, S.w->sc, sizeof(Scalars), hipMemcpyDeviceToHost, S.hs));
+    GM_HIP(hipStreamSynchronize(S.hs));
+    const double bb = h.bb > 0 ? h.bb : 1.0;
+    *relres = sqrt(h.g0 / bb);
+    return GM_OK;
+}
+
+gm_status gm_release_workspaces(void) {
+    release_workspaces();
+    release_red_scratch();
+    return GM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,201 @@
+/* gpumatting.h -- C ABI of libgpumatting.so (MI355X / gfx950, HIP).
+ *
+ * Plain C: pointers, sizes and POD structs only.  Every entry point returns a
+ * gm_status (0 = success); gm_last_error() gives the message of the last
+ * failure on the calling thread.  Device pointers are HIP device memory;
+ * "stream" is a hipStream_t (NULL = the null stream).  Nothing inside an
+ * iteration synchronises with the host except where stated.
+ *
+ * Each entry point names the reference interface (rocketman768/gpumatting,
+ * file:line) whose role it takes.  The C++ drop-in headers BandedMatrix.h,
+ * Vector.h, Solve.h and Matting.h wrap these calls under the reference's own
+ * function names and parameter orders.
+ */
+#ifndef GPUMATTING_H
+#define GPUMATTING_H
+
+#include <stddef.h>
+#include "BandedMatrix.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* hipStream_t without pulling in the HIP headers */
+typedef struct ihipStream_t* gm_stream_t;
+
+typedef enum gm_status_e {
+   GM_OK = 0,
+   GM_ERR_INVALID = 1,       /* bad argument (null pointer, size, pitch, pad) */
+   GM_ERR_OOM = 2,           /* device or pinned allocation failed */
+   GM_ERR_HIP = 3,           /* a HIP runtime call failed */
+   GM_ERR_BREAKDOWN = 4,     /* solver breakdown: p'Ap <= 0 (matrix not SPD) */
+   GM_ERR_NONFINITE = 5,     /* NaN/Inf met in the iteration */
+   GM_ERR_NOT_CONVERGED = 6, /* tolerance not reached within max_iter */
+   GM_ERR_UNSUPPORTED = 7    /* valid request this build does not implement */
+} gm_status;
+
+const char* gm_last_error(void);
+const char* gm_version(void);
+/* Number of HIP devices the runtime sees (0 when none); no device context is created. */
+int gm_device_count(void);
+
+/* ---------------------------------------------------------------------- */
+/* Matrices and vectors (BandedMatrix.cu, Vector.cu)                        */
+
+/* bmCopyToDevice, BandedMatrix.cu:23-41: allocate dA (pitched, zero padded)
+ * and copy hA's bands and offsets.  dA->apitch is in elements. */
+gm_status gm_bm_copy_to_device(BandedMatrix* dA, const BandedMatrix* hA);
+/* bmDeviceFree, BandedMatrix.cu:43-47 */
+gm_status gm_bm_device_free(BandedMatrix* dA);
+/* Allocate a zeroed device matrix with the given host band offsets (no
+ * reference counterpart: the reference only ever copied a host matrix). */
+gm_status gm_bm_device_alloc(BandedMatrix* dA, int rows, int nbands, const int* host_bands);
+/* Copy a device matrix back into a host matrix with the same rows/nbands
+ * (hA->a holds nbands*hA->apitch floats). */
+gm_status gm_bm_copy_to_host(BandedMatrix* hA, const BandedMatrix* dA);
+
+/* vecDeviceMalloc, Vector.cu:23-30: zeroed, *dx points past the left pad. */
+gm_status gm_vec_device_malloc(float** dx, int length, int left_pad, int right_pad);
+/* vecCopyToDevice, Vector.cu:37-41 */
+gm_status gm_vec_copy_to_device(float** dx, const float* hx, int length, int left_pad, int right_pad);
+/* vecDeviceFree, Vector.cu:43-46 */
+gm_status gm_vec_device_free(float* dx, int left_pad);
+
+/* ---------------------------------------------------------------------- */
+/* Kernels as host launchers (BandedMatrix.cu, Vector.cu)                   */
+
+/* bmAx_k, BandedMatrix.cu:107-141,194-198: b = A x.  x readable at
+ * i + bands[k] (zero padded, BandedMatrix.cu:145-146). */
+gm_status gm_bm_ax(float* b, const BandedMatrix* A, const float* x, gm_stream_t stream);
+/* bmAxpy_k, BandedMatrix.cu:152-189,203-207: b = A x + y (add) or A x - y. */
+gm_status gm_bm_axpy(float* b, const BandedMatrix* A, const float* x, const float* y, int add,
+                     gm_stream_t stream);
+/* jacobi_k, BandedMatrix.cu:61-98 (diagonal taken from the band with offset 0,
+ * for any band count): xx = omega*(x + (b - A x)/a_ii) + (1-omega)*x. */
+gm_status gm_bm_jacobi_sweep(float* xx, const float* x, const BandedMatrix* A, const float* b,
+                             float omega, gm_stream_t stream);
+
+/* vecAdd_k / vecSub_k, Vector.cu:142-150 */
+gm_status gm_vec_add(float* r, const float* x, const float* y, int len, gm_stream_t stream);
+gm_status gm_vec_sub(float* r, const float* x, const float* y, int len, gm_stream_t stream);
+/* vecScaleConst_k, Vector.cu:175 */
+gm_status gm_vec_scale_const(float* r, const float* x, float s, int len, gm_stream_t stream);
+/* vecScale_k, Vector.cu:180 (s is a device scalar) */
+gm_status gm_vec_scale(float* r, const float* x, const float* s, int len, gm_stream_t stream);
+/* innerProd_k, Vector.cu:195-265: *result (device) = <x,y>; deterministic
+ * two-stage sum (float products, double accumulation), no atomics on data. */
+gm_status gm_inner_prod(float* result, const float* x, const float* y, int len, gm_stream_t stream);
+
+/* ---------------------------------------------------------------------- */
+/* Solvers                                                                 */
+
+/* gradSolve, main.cu:406-456: steepest descent, fixed iteration count,
+ * alpha (device, padded by pad) updated in place. */
+gm_status gm_grad_solve(float* alpha, const BandedMatrix* L, const float* b, int iterations, int pad,
+                        gm_stream_t stream);
+/* cgSolve, main.cu:458-526: CG with r = L alpha - b, restart every
+ * restart_interval iterations, fixed iteration count (stops early only on an
+ * exactly zero residual or a breakdown). */
+gm_status gm_cg_solve(float* alpha, const BandedMatrix* L, const float* b, int pad, int iterations,
+                      int restart_interval, gm_stream_t stream);
+/* jacobi, main.cu:528-556: damped Jacobi sweeps, result in x. */
+gm_status gm_jacobi(float* x, const BandedMatrix* A, const float* b, float omega, int pad,
+                    int iterations, gm_stream_t stream);
+/* gaussSeidel_host, Solve.cpp:3-33 / Solve.h:17: host pointers in and out
+ * (x padded by the matrix's largest |offset|); the sweep runs on the GPU in
+ * wavefront order and gives the lexicographic Gauss-Seidel result. */
+gm_status gm_gauss_seidel_host(float* x, const BandedMatrix* A, const float* b, size_t iterations);
+/* jacobi_host, Solve.cpp:35-85 / Solve.h:31: host pointers; result always in x. */
+gm_status gm_jacobi_host(float* x, const BandedMatrix* A, const float* b, size_t iterations, int xpad,
+                         float omega);
+
+typedef enum gm_method_e {
+   GM_METHOD_CG = 0,    /* conjugate gradient */
+   GM_METHOD_PCG = 1,   /* Jacobi-preconditioned CG */
+   GM_METHOD_GD = 2     /* steepest descent, exact line search */
+} gm_method;
+
+typedef enum gm_precision_e {
+   GM_PREC_FP32 = 0,    /* FP32 diagonals, FP32 vectors */
+   GM_PREC_FP16 = 1,    /* Jacobi-scaled FP16 diagonals, FP32 accumulate, FP64 residual replacement */
+   GM_PREC_BF16 = 2     /* as FP16 with BF16 diagonals */
+} gm_precision;
+
+typedef struct gm_solve_opts_s {
+   int method;          /* gm_method */
+   int precision;       /* gm_precision (16-bit forms: PCG on the scaled system) */
+   int max_iter;        /* iteration cap (> 0) */
+   double tol;          /* stop when ||b - A x|| / ||b|| <= tol; <= 0: run max_iter */
+   int restart;         /* > 0: recompute r = A x - b every `restart` iterations */
+   int refine_every;    /* 16-bit forms: FP64 residual replacement period (0 -> 50) */
+   int check_every;     /* iterations enqueued between host status polls (0 -> 16) */
+   int verify;          /* 1: confirm convergence with the FP64 true residual */
+} gm_solve_opts;
+
+typedef struct gm_solve_info_s {
+   int iterations;      /* iterations performed */
+   int status;          /* gm_status of the solve */
+   double relres;       /* last residual norm / ||b|| the iteration saw */
+   double true_relres;  /* FP64-accumulated ||b - A x|| / ||b|| at exit (-1 if not computed) */
+   int replacements;    /* FP64 residual replacements performed */
+} gm_solve_info;
+
+void gm_solve_opts_default(gm_solve_opts* o);
+
+/* North-star solver entry: (L + lambda D) alpha = lambda D b as assembled by
+ * gm_levin_laplacian, or any SPD banded system.  alpha: device, padded by
+ * `pad` (>= max |band offset|), initial guess in, solution out. */
+gm_status gm_solve(float* alpha, const BandedMatrix* L, const float* b, int pad, const gm_solve_opts* opts,
+                   gm_solve_info* info, gm_stream_t stream);
+
+/* FP64-accumulated ||b - A x|| / ||b|| (synchronises; writes *relres). */
+gm_status gm_true_relres(const BandedMatrix* A, const float* x, const float* b, double* relres,
+                         gm_stream_t stream);
+
+/* ---------------------------------------------------------------------- */
+/* Levin matting Laplacian (Matting.cu)                                      */
+
+typedef enum gm_assembly_mode_e {
+   GM_ASSEMBLY_REFERENCE = 0, /* bit-identical to Matting.cu:39-219's float/double rounding */
+   GM_ASSEMBLY_ACCURATE = 1   /* exact double covariance sums (same definition) */
+} gm_assembly_mode;
+
+/* GPU assembly, gather form (replaces hostLevinLaplacian Matting.cu:39-219 and
+ * the unfinished levinLaplacian kernel Matting.cu:240-382).
+ * L: device matrix with 25 bands in the layout bands[5(dy+2)+(dx+2)] = dy*W+dx
+ * (main.cu:161-190) and rows = W*H; every entry is written (no pre-zeroing).
+ * im: device, nch (3 or 4) interleaved floats per pixel, pitch in pixels.
+ * Data term: per-pixel weights lam[i] and targets tgt[i] (device, W*H, may be
+ * NULL = none): L_ii += lam_i, b_i = lam_i*tgt_i.  dinv (nullable) receives
+ * 1/L_ii (0 where L_ii == 0). */
+gm_status gm_levin_laplacian(BandedMatrix* L, float* b, float* dinv, const float* im, int nch,
+                             size_t im_pitch, const float* lam, const float* tgt, float eps, int mode,
+                             int W, int H, gm_stream_t stream);
+/* Same with the reference's u8 scribble rule (Matting.cu:205-216): 255 ->
+ * L_ii += gamma, b_i = gamma; 0 -> L_ii += gamma, b_i = 0; else b_i = 0.
+ * scribs: device, indexed with im_pitch like the reference (:205). */
+gm_status gm_levin_laplacian_u8(BandedMatrix* L, float* b, float* dinv, const float* im, int nch,
+                                size_t im_pitch, const unsigned char* scribs, float eps, float gamma,
+                                int mode, int W, int H, gm_stream_t stream);
+/* hostLevinLaplacian, Matting.cu:39-46 signature: host L (pre-allocated,
+ * 25 bands, apitch >= rows), host b, host float4 RGBA image with pitch in
+ * float4 elements, host u8 scribbles, eps (the reference calls it lambda),
+ * gamma = 1.  Runs the reference-rounding GPU assembly and copies back
+ * (synchronous). */
+gm_status gm_host_levin_laplacian(BandedMatrix* L, float* b, float eps, const float* im4,
+                                  const unsigned char* scribs, int W, int H, size_t im_pitch);
+
+/* 5x5-window band offsets (main.cu:161-190) and the vector pad
+ * ((2W+2)/32)*32+32 (main.cu:228-229). */
+void gm_levin_bands(int W, int* bands25);
+int gm_levin_pad(int W);
+
+/* Release cached solver workspaces (device memory held between solves). */
+gm_status gm_release_workspaces(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPUMATTING_H */

@@ -1,0 +1,346 @@
+"""ctypes front end of the CPU oracle (oracle/oracle.c) -- TEST INFRASTRUCTURE.
+
+Only tests/, the smoke check and bench.py's cpu_baseline leg import this.
+Matrices are numpy arrays of shape (nbands, apitch) (diagonal-major, the
+reference's layout, BandedMatrix.h:25-32); vectors that a banded product reads
+are padded numpy arrays of length N + 2*pad whose element `pad` is x[0].
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_REF = None
+
+f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+vp = C.c_void_p
+sz = C.c_size_t
+ci = C.c_int
+cf = C.c_float
+cd = C.c_double
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        L.orc_bands25.argtypes = [ci, i32p]
+        L.orc_pad.argtypes = [ci]
+        L.orc_pad.restype = ci
+        L.orc_levin_scatter_f32.argtypes = [f32p, sz, f32p, f32p, ci, sz, vp, vp, cf, ci, ci]
+        L.orc_levin_scatter_u8.argtypes = [f32p, sz, f32p, f32p, ci, sz, u8p, cf, cf, ci, ci]
+        L.orc_levin_gather_f32.argtypes = [f32p, sz, f32p, vp, f32p, ci, sz, vp, vp, cf, ci, ci]
+        L.orc_levin_gather_acc_f32.argtypes = [f32p, sz, f32p, vp, f32p, ci, sz, vp, vp, cf, ci, ci]
+        L.orc_levin_f64.argtypes = [f64p, sz, f64p, f32p, ci, sz, vp, vp, cd, ci, ci]
+        L.orc_levin_dense_f64.argtypes = [f64p, f32p, ci, sz, cd, ci, ci]
+        for nm in ("orc_bm_ax_f32",):
+            getattr(L, nm).argtypes = [vp, vp, sz, i32p, ci, ci, vp]
+        L.orc_bm_axpy_f32.argtypes = [vp, vp, sz, i32p, ci, ci, vp, vp, ci]
+        L.orc_bm_ax_f64.argtypes = [vp, vp, sz, i32p, ci, ci, vp]
+        L.orc_dot_f32.argtypes = [f32p, f32p, ci]
+        L.orc_dot_f32.restype = cd
+        L.orc_cg_ref_f32.argtypes = [vp, vp, sz, i32p, ci, ci, vp, ci, ci, ci]
+        L.orc_gd_ref_f32.argtypes = [vp, vp, sz, i32p, ci, ci, vp, ci, ci]
+        L.orc_pcg_f32.argtypes = [vp, vp, sz, i32p, ci, ci, vp, ci, cd, ci, ci, ci, C.POINTER(cd)]
+        L.orc_pcg_f32.restype = ci
+        L.orc_pcg_f64.argtypes = [vp, vp, sz, i32p, ci, ci, vp, ci, cd, ci, C.POINTER(cd)]
+        L.orc_pcg_f64.restype = ci
+        L.orc_true_relres_f32.argtypes = [vp, sz, i32p, ci, ci, vp, vp]
+        L.orc_true_relres_f32.restype = cd
+        L.orc_gs_f32.argtypes = [vp, vp, sz, i32p, ci, ci, vp, sz]
+        L.orc_jacobi_f32.argtypes = [vp, vp, sz, i32p, ci, ci, vp, sz, ci, cf]
+        L.orc_mse_clamped.argtypes = [f32p, f32p, ci, ci]
+        L.orc_mse_clamped.restype = cd
+        L.orc_num_threads.restype = ci
+        L.orc_baseline_pcg_f32.argtypes = [vp, vp, sz, i32p, ci, ci, vp, ci, ci]
+        L.orc_baseline_pcg_f32.restype = cd
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a: np.ndarray, off: int = 0) -> int:
+    return a.ctypes.data + off * a.itemsize
+
+
+def bands25(W: int) -> np.ndarray:
+    b = np.zeros(25, np.int32)
+    lib().orc_bands25(W, b)
+    return b
+
+
+def pad_for(W: int) -> int:
+    return lib().orc_pad(W)
+
+
+def padded(x: np.ndarray, pad: int) -> np.ndarray:
+    out = np.zeros(x.size + 2 * pad, x.dtype)
+    out[pad:pad + x.size] = x.ravel()
+    return out
+
+
+def _img(image: np.ndarray):
+    im = np.ascontiguousarray(image, np.float32)
+    H, W, nch = im.shape
+    return im, W, H, nch
+
+
+def _opt(a, dt):
+    if a is None:
+        return None, None
+    arr = np.ascontiguousarray(a, dt).ravel()
+    return arr, arr.ctypes.data
+
+
+def levin_scatter_f32(image, lam=None, tgt=None, eps=1e-7, apitch=None):
+    """Reference-semantics FP32 Laplacian (Matting.cu:39-219): returns (a, b)."""
+    im, W, H, nch = _img(image)
+    N = W * H
+    ap = apitch or N
+    a = np.zeros((25, ap), np.float32)
+    b = np.zeros(N, np.float32)
+    la, lp = _opt(lam, np.float32)
+    ta, tp = _opt(tgt, np.float32)
+    lib().orc_levin_scatter_f32(a, ap, b, im, nch, W, lp, tp, eps, W, H)
+    return a, b
+
+
+def levin_scatter_u8(image, scribs, eps=1e-2, gamma=1.0):
+    im, W, H, nch = _img(image)
+    N = W * H
+    a = np.zeros((25, N), np.float32)
+    b = np.zeros(N, np.float32)
+    lib().orc_levin_scatter_u8(a, N, b, im, nch, W, np.ascontiguousarray(scribs, np.uint8), eps,
+                               gamma, W, H)
+    return a, b
+
+
+def levin_gather_f32(image, lam=None, tgt=None, eps=1e-7, apitch=None):
+    """Same numbers as levin_scatter_f32, row-parallel; returns (a, b, dinv)."""
+    im, W, H, nch = _img(image)
+    N = W * H
+    ap = apitch or N
+    a = np.zeros((25, ap), np.float32)
+    b = np.zeros(N, np.float32)
+    dinv = np.zeros(N, np.float32)
+    la, lp = _opt(lam, np.float32)
+    ta, tp = _opt(tgt, np.float32)
+    lib().orc_levin_gather_f32(a, ap, b, dinv.ctypes.data, im, nch, W, lp, tp, eps, W, H)
+    return a, b, dinv
+
+
+def levin_gather_acc_f32(image, lam=None, tgt=None, eps=1e-7, apitch=None):
+    """Accurate-rounding FP32 assembly (double covariance sums); returns (a, b, dinv)."""
+    im, W, H, nch = _img(image)
+    N = W * H
+    ap = apitch or N
+    a = np.zeros((25, ap), np.float32)
+    b = np.zeros(N, np.float32)
+    dinv = np.zeros(N, np.float32)
+    la, lp = _opt(lam, np.float32)
+    ta, tp = _opt(tgt, np.float32)
+    lib().orc_levin_gather_acc_f32(a, ap, b, dinv.ctypes.data, im, nch, W, lp, tp, eps, W, H)
+    return a, b, dinv
+
+
+def levin_f64(image, lam=None, tgt=None, eps=1e-7):
+    im, W, H, nch = _img(image)
+    N = W * H
+    a = np.zeros((25, N), np.float64)
+    b = np.zeros(N, np.float64)
+    la, lp = _opt(lam, np.float64)
+    ta, tp = _opt(tgt, np.float64)
+    lib().orc_levin_f64(a, N, b, im, nch, W, lp, tp, eps, W, H)
+    return a, b
+
+
+def levin_dense_f64(image, eps=1e-7):
+    im, W, H, nch = _img(image)
+    N = W * H
+    L = np.zeros((N, N), np.float64)
+    lib().orc_levin_dense_f64(L, im, nch, W, eps, W, H)
+    return L
+
+
+def bm_ax_f32(a, bands, xpad, pad):
+    rows = a.shape[1] if xpad.size - 2 * pad > a.shape[1] else xpad.size - 2 * pad
+    y = np.zeros(rows, np.float32)
+    lib().orc_bm_ax_f32(_ptr(y), _ptr(a), a.shape[1], np.ascontiguousarray(bands, np.int32),
+                        len(bands), rows, _ptr(xpad, pad))
+    return y
+
+
+def bm_ax_f64(a, bands, xpad, pad):
+    rows = xpad.size - 2 * pad
+    y = np.zeros(rows, np.float64)
+    lib().orc_bm_ax_f64(_ptr(y), _ptr(a), a.shape[1], np.ascontiguousarray(bands, np.int32),
+                        len(bands), rows, _ptr(xpad, pad))
+    return y
+
+
+def dot_f32(x, y) -> float:
+    return lib().orc_dot_f32(np.ascontiguousarray(x, np.float32), np.ascontiguousarray(y, np.float32),
+                             x.size)
+
+
+def cg_ref_f32(a, bands, b, x0, pad, iterations, restart=101):
+    """main.cu:458-526 restated; returns alpha (unpadded)."""
+    N = b.size
+    xp = padded(np.asarray(x0, np.float32), pad)
+    lib().orc_cg_ref_f32(_ptr(xp, pad), _ptr(a), a.shape[1], np.ascontiguousarray(bands, np.int32),
+                         len(bands), N, _ptr(b), pad, iterations, restart)
+    return xp[pad:pad + N].copy()
+
+
+def gd_ref_f32(a, bands, b, x0, pad, iterations):
+    N = b.size
+    xp = padded(np.asarray(x0, np.float32), pad)
+    lib().orc_gd_ref_f32(_ptr(xp, pad), _ptr(a), a.shape[1], np.ascontiguousarray(bands, np.int32),
+                         len(bands), N, _ptr(b), pad, iterations)
+    return xp[pad:pad + N].copy()
+
+
+def pcg_f32(a, bands, b, x0, pad, tol=1e-6, maxit=10000, restart=0, precond=True):
+    N = b.size
+    xp = padded(np.asarray(x0, np.float32), pad)
+    rr = cd(0)
+    it = lib().orc_pcg_f32(_ptr(xp, pad), _ptr(a), a.shape[1], np.ascontiguousarray(bands, np.int32),
+                           len(bands), N, _ptr(b), pad, tol, maxit, restart, int(precond), C.byref(rr))
+    return xp[pad:pad + N].copy(), it, rr.value
+
+
+def pcg_f64(a64, bands, b64, x0, pad, tol=1e-12, maxit=100000):
+    N = b64.size
+    xp = padded(np.asarray(x0, np.float64), pad)
+    rr = cd(0)
+    it = lib().orc_pcg_f64(_ptr(xp, pad), _ptr(a64), a64.shape[1], np.ascontiguousarray(bands, np.int32),
+                           len(bands), N, _ptr(b64), pad, tol, maxit, C.byref(rr))
+    return xp[pad:pad + N].copy(), it, rr.value
+
+
+def true_relres_f32(a, bands, b, x, pad):
+    xp = padded(np.asarray(x, np.float32), pad)
+    b = np.ascontiguousarray(b, np.float32)
+    return lib().orc_true_relres_f32(_ptr(a), a.shape[1], np.ascontiguousarray(bands, np.int32),
+                                     len(bands), b.size, _ptr(b), _ptr(xp, pad))
+
+
+def gs_f32(a, bands, b, x0, pad, iterations):
+    N = b.size
+    xp = padded(np.asarray(x0, np.float32), pad)
+    lib().orc_gs_f32(_ptr(xp, pad), _ptr(a), a.shape[1], np.ascontiguousarray(bands, np.int32),
+                     len(bands), N, _ptr(b), iterations)
+    return xp[pad:pad + N].copy()
+
+
+def jacobi_f32(a, bands, b, x0, pad, iterations, omega=2.0 / 3.0):
+    N = b.size
+    xp = padded(np.asarray(x0, np.float32), pad)
+    lib().orc_jacobi_f32(_ptr(xp, pad), _ptr(a), a.shape[1], np.ascontiguousarray(bands, np.int32),
+                         len(bands), N, _ptr(b), iterations, pad, omega)
+    return xp[pad:pad + N].copy()
+
+
+def mse_clamped(alpha, gt):
+    H, W = gt.shape
+    return lib().orc_mse_clamped(np.ascontiguousarray(alpha, np.float32).ravel(),
+                                 np.ascontiguousarray(gt, np.float32).ravel(), W, H)
+
+
+def baseline_pcg_f32(a, bands, b, x0, pad, iters):
+    """OpenMP Jacobi-PCG, exactly `iters` iterations (the CPU baseline)."""
+    N = b.size
+    xp = padded(np.asarray(x0, np.float32), pad)
+    rr = lib().orc_baseline_pcg_f32(_ptr(xp, pad), _ptr(a), a.shape[1], np.ascontiguousarray(bands, np.int32),
+                                    len(bands), N, _ptr(b), pad, iters)
+    return xp[pad:pad + N].copy(), rr
+
+
+def num_threads() -> int:
+    return lib().orc_num_threads()
+
+
+def solve_f64(image, lam, tgt, eps=1e-7, tol=1e-12, maxit=200000):
+    """The FP64 accuracy oracle: FP64 Laplacian + FP64 Jacobi-PCG.  Returns alpha (N,)."""
+    H, W = image.shape[:2]
+    a64, b64 = levin_f64(image, lam, tgt, eps)
+    bands = bands25(W)
+    pad = pad_for(W)
+    x, it, rr = pcg_f64(a64, bands, b64, np.zeros(W * H), pad, tol, maxit)
+    return x, it, rr
+
+
+# --------------------------------------------------------------------------
+# The reference's own host solvers, compiled from /root/reference/Solve.cpp
+# into oracle/_ref (see oracle/Makefile).  C++ symbols are looked up by name.
+
+class _RefBM(C.Structure):
+    # BandedMatrix.h:23-38
+    _fields_ = [("a", C.c_void_p), ("apitch", C.c_size_t), ("bands", C.c_void_p),
+                ("nbands", C.c_int), ("rows", C.c_int), ("cols", C.c_int)]
+
+
+def ref_lib():
+    """The reference Solve.cpp build, or None when it was not built."""
+    global _REF
+    if _REF is None:
+        path = os.path.join(_HERE, "_ref", "libref_solve.so")
+        if not os.path.exists(path):
+            return None
+        _REF = C.CDLL(path)
+    return _REF
+
+
+def _ref_sym(name):
+    import re
+    L = ref_lib()
+    out = subprocess.run(["nm", "-D", os.path.join(_HERE, "_ref", "libref_solve.so")],
+                         capture_output=True, text=True, check=True).stdout
+    for line in out.splitlines():
+        m = re.search(r"\s(_Z\d+" + name + r"\S*)$", line)
+        if m:
+            return getattr(L, m.group(1))
+    raise KeyError(name)
+
+
+def ref_gauss_seidel(a, bands, b, x0, pad, iterations):
+    """Solve.cpp:3-33 (reference code) on a padded copy of x0."""
+    N = b.size
+    xp = padded(np.asarray(x0, np.float32), pad)
+    bd = np.ascontiguousarray(bands, np.int32)
+    M = _RefBM(_ptr(a), a.shape[1], bd.ctypes.data, len(bands), N, N)
+    f = _ref_sym("gaussSeidel_host")
+    f.argtypes = [C.c_void_p, C.POINTER(_RefBM), C.c_void_p, C.c_size_t]
+    f(_ptr(xp, pad), C.byref(M), _ptr(np.ascontiguousarray(b, np.float32)), iterations)
+    return xp[pad:pad + N].copy()
+
+
+def ref_jacobi(a, bands, b, x0, pad, iterations, omega=2.0 / 3.0):
+    """Solve.cpp:35-85 (reference code).  Only even iteration counts: after an odd
+    count the reference frees the caller's buffer (Solve.cpp:53-63)."""
+    if iterations % 2:
+        raise ValueError("reference jacobi_host is only safe for even iteration counts")
+    N = b.size
+    # the reference deletes its own buffer with delete[]; the caller's buffer
+    # must stay ours, which holds for even counts
+    xp = padded(np.asarray(x0, np.float32), pad)
+    bd = np.ascontiguousarray(bands, np.int32)
+    M = _RefBM(_ptr(a), a.shape[1], bd.ctypes.data, len(bands), N, N)
+    f = _ref_sym("jacobi_host")
+    f.argtypes = [C.c_void_p, C.POINTER(_RefBM), C.c_void_p, C.c_size_t, C.c_int, C.c_float]
+    f(_ptr(xp, pad), C.byref(M), _ptr(np.ascontiguousarray(b, np.float32)), iterations, pad, omega)
+    return xp[pad:pad + N].copy()

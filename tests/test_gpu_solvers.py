@@ -1,0 +1,231 @@
+"""GPU solver parity through the C ABI: cgSolve / gradSolve at equal iteration
+count vs the oracle's restatement (main.cu:406-526), Jacobi-PCG and the
+16-bit mixed forms vs the FP64 oracle, smoothers vs the reference's
+Solve.cpp semantics, failure statuses, determinism, and a 4K solve."""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+from gpumatting_amd import (ASSEMBLY_ACCURATE, ASSEMBLY_REFERENCE, METHOD_CG, METHOD_GD, METHOD_PCG, PREC_BF16,
+                            PREC_FP16, PREC_FP32, _lib, matting, synth)
+from gpumatting_amd._lib import BandedMatrix, call, lib
+from gpumatting_amd.matting import DeviceMatrix, Padded
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+S0 = C.c_void_p(0)
+
+
+def dptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+@pytest.fixture(scope="module")
+def small():
+    p = synth.config_small()
+    a, b, d = O.levin_gather_acc_f32(p.image, p.lam, p.target, 1e-7)
+    bands = O.bands25(p.W)
+    pad = O.pad_for(p.W)
+    x_exact, it, rr = O.pcg_f64(a.astype(np.float64), bands, b.astype(np.float64), np.zeros(p.N), pad, 1e-13)
+    return p, a, b, bands, pad, x_exact
+
+
+def _gpu_system(p, mode=ASSEMBLY_ACCURATE):
+    return matting.assemble(p.image, p.lam, p.target, 1e-7, mode)
+
+
+@pytest.mark.parametrize("iters,restart", [(50, 101), (200, 101), (150, 40)])
+def test_cgsolve_matches_reference_recurrence(gpu, small, iters, restart):
+    p, a, b, bands, pad, _ = small
+    s = _gpu_system(p)
+    x = Padded(p.N, pad, init=np.full(p.N, 0.5, np.float32))
+    call("gm_cg_solve", x.ptr, s.L.ref, dptr(s.b), pad, iters, restart, S0)
+    torch.cuda.synchronize()
+    xr = O.cg_ref_f32(a, bands, b, np.full(p.N, 0.5, np.float32), pad, iters, restart)
+    assert np.abs(x.numpy() - xr).max() <= 1e-3
+
+
+def test_gradsolve_matches_reference_recurrence(gpu, small):
+    p, a, b, bands, pad, _ = small
+    s = _gpu_system(p)
+    x = Padded(p.N, pad, init=np.full(p.N, 0.5, np.float32))
+    call("gm_grad_solve", x.ptr, s.L.ref, dptr(s.b), 60, pad, S0)
+    torch.cuda.synchronize()
+    xr = O.gd_ref_f32(a, bands, b, np.full(p.N, 0.5, np.float32), pad, 60)
+    assert np.abs(x.numpy() - xr).max() <= 1e-3
+
+
+@pytest.mark.parametrize("method", [METHOD_CG, METHOD_PCG])
+def test_tolerance_solve_vs_fp64(gpu, small, method):
+    p, a, b, bands, pad, x_exact = small
+    s = _gpu_system(p)
+    x, info = matting.solve(s, method=method, tol=1e-6, max_iter=5000)
+    assert info.status == 0 and info.true_relres <= 1e-6
+    xo, it, _ = O.pcg_f32(a, bands, b, np.zeros(p.N, np.float32), pad, 1e-6, 5000, precond=method == METHOD_PCG)
+    assert abs(info.iterations - it) <= max(5, 0.05 * it)
+    assert np.abs(x.numpy() - x_exact).max() <= 5e-3
+    x2, info2 = matting.solve(s, method=method, tol=5e-8, max_iter=5000)
+    assert np.abs(x2.numpy() - x_exact).max() <= 1e-4
+
+
+@pytest.mark.parametrize("prec", [PREC_FP16, PREC_BF16])
+def test_mixed_precision_vs_fp64(gpu, small, prec):
+    p, a, b, bands, pad, x_exact = small
+    s = _gpu_system(p)
+    _, info32 = matting.solve(s, method=METHOD_PCG, tol=1e-6, max_iter=5000)
+    x, info = matting.solve(s, method=METHOD_PCG, precision=prec, tol=1e-6, max_iter=5000)
+    assert info.status == 0 and info.true_relres <= 1e-6
+    assert info.replacements >= 1
+    assert info.iterations <= 1.5 * info32.iterations + 10
+    assert np.abs(x.numpy() - x_exact).max() <= 5e-3
+    if prec == PREC_FP16:
+        x2, info2 = matting.solve(s, method=METHOD_PCG, precision=prec, tol=5e-8, max_iter=5000)
+        assert np.abs(x2.numpy() - x_exact).max() <= 1e-4
+
+
+def test_solve_is_bitwise_deterministic(gpu, small):
+    p = small[0]
+    s = _gpu_system(p)
+    outs = [matting.solve(s, method=METHOD_PCG, tol=1e-6)[0].numpy() for _ in range(2)]
+    outs += [matting.solve(s, method=METHOD_PCG, precision=PREC_FP16, tol=1e-6)[0].numpy() for _ in range(2)]
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[2], outs[3])
+
+
+def test_true_relres_matches_oracle(gpu, small):
+    p, a, b, bands, pad, _ = small
+    s = _gpu_system(p)
+    x = np.random.default_rng(0).random(p.N).astype(np.float32)
+    got = matting.true_relres(s.L, Padded(p.N, pad, init=x), s.b)
+    assert got == pytest.approx(O.true_relres_f32(a, bands, b, x, pad), rel=1e-9)
+
+
+def _tridiag(N, diag, off):
+    bands = np.array([-1, 0, 1], np.int32)
+    a = np.zeros((3, N), np.float32)
+    a[0, 1:] = off
+    a[1] = diag
+    a[2, :-1] = off
+    return bands, a
+
+
+def test_breakdown_and_nonfinite_statuses(gpu):
+    N = 1000
+    bands, a = _tridiag(N, -2.0, 0.5)  # negative definite
+    M = DeviceMatrix(N, bands, a)
+    b = torch.ones(N, device="cuda")
+    x, info = matting.solve(M, b, pad=1, method=METHOD_CG, raise_on_error=False)
+    assert info.status == 4  # GM_ERR_BREAKDOWN
+    bands, a = _tridiag(N, 4.0, -1.0)
+    M = DeviceMatrix(N, bands, a)
+    b[17] = float("nan")
+    x, info = matting.solve(M, b, pad=1, method=METHOD_PCG, raise_on_error=False)
+    assert info.status == 5  # GM_ERR_NONFINITE
+    b[17] = 1.0
+    x, info = matting.solve(M, b, pad=1, method=METHOD_PCG, tol=1e-30, max_iter=3, raise_on_error=False)
+    assert info.status == 6 and info.iterations == 3  # GM_ERR_NOT_CONVERGED
+    x, info = matting.solve(M, b, pad=1, method=METHOD_PCG, tol=1e-6)
+    xd = np.linalg.solve(np.diag(np.full(N, 4.0)) + np.diag(np.full(N - 1, -1.0), 1) + np.diag(np.full(N - 1, -1.0), -1),
+                         np.ones(N))
+    assert np.abs(x.numpy() - xd).max() <= 1e-5
+
+
+def test_generic_band_structure_cg(gpu):
+    """Non-Levin band sets take the generic kernels; same answers."""
+    N = 5000
+    bands = np.array([-37, -1, 0, 1, 37], np.int32)
+    a = np.zeros((5, N), np.float32)
+    i = np.arange(N)
+    a[0, i >= 37] = -0.5
+    a[1, i >= 1] = -1
+    a[2] = 4.5
+    a[3, i + 1 < N] = -1
+    a[4, i + 37 < N] = -0.5
+    M = DeviceMatrix(N, bands, a)
+    rhs = np.random.default_rng(3).random(N).astype(np.float32)
+    b = torch.from_numpy(rhs).cuda()
+    for method in (METHOD_CG, METHOD_PCG):
+        x, info = matting.solve(M, b, pad=37, method=method, tol=1e-6)
+        xo, it, _ = O.pcg_f32(a, bands, rhs, np.zeros(N, np.float32), 37, 1e-6, 5000, precond=method == METHOD_PCG)
+        assert np.abs(x.numpy() - xo).max() <= 1e-4
+    with pytest.raises(_lib.GMError):
+        matting.solve(M, b, pad=37, method=METHOD_PCG, precision=PREC_FP16)
+
+
+def test_tiny_images(gpu):
+    for (W, H) in [(1, 1), (2, 5), (3, 3)]:
+        p = synth.make_problem(W, H, 1, 1, 0.0)
+        lam = np.full((H, W), 100.0, np.float32)
+        tgt = np.random.default_rng(W).random((H, W)).astype(np.float32)
+        s = matting.assemble(p.image, lam, tgt, 1e-7)
+        x, info = matting.solve(s, method=METHOD_PCG, tol=1e-7)
+        a, b, _ = O.levin_gather_acc_f32(p.image, lam, tgt, 1e-7)
+        xe, _, _ = O.pcg_f64(a.astype(np.float64), O.bands25(W), b.astype(np.float64), np.zeros(W * H),
+                             O.pad_for(W), 1e-13)
+        assert np.abs(x.numpy() - xe).max() <= 1e-5
+
+
+def _dd_levin(W=37, H=21):
+    p = synth.make_problem(W, H, 8, 1, 0.5)
+    a, b, _ = O.levin_gather_acc_f32(p.image, p.lam, p.target, 1e-2)
+    a[12] += 2.0
+    return p, a, b
+
+
+@pytest.mark.parametrize("iters", [1, 2, 7])
+def test_jacobi_host_bitwise(gpu, iters):
+    p, a, b = _dd_levin()
+    bands, pad = O.bands25(p.W), O.pad_for(p.W)
+    x0 = np.random.default_rng(2).random(p.N).astype(np.float32)
+    xp = O.padded(x0, pad)
+    h = BandedMatrix(a.ctypes.data, a.shape[1], bands.ctypes.data, 25, p.N, p.N)
+    call("gm_jacobi_host", xp.ctypes.data + 4 * pad, C.byref(h), b.ctypes.data, iters, pad, C.c_float(2.0 / 3.0))
+    assert np.array_equal(xp[pad:pad + p.N], O.jacobi_f32(a, bands, b, x0, pad, iters))
+
+
+@pytest.mark.parametrize("levin", [True, False])
+def test_gauss_seidel_host_bitwise(gpu, levin):
+    if levin:
+        p, a, b = _dd_levin()
+        bands, pad, N = O.bands25(p.W), O.pad_for(p.W), p.N
+    else:
+        N = 3001
+        bands = np.array([-10, 0, 10], np.int32)
+        a = np.zeros((3, N), np.float32)
+        i = np.arange(N)
+        rng = np.random.default_rng(4)
+        a[0, i >= 10] = rng.random((i >= 10).sum()) - 0.5
+        a[1] = 3.0
+        a[2, i + 10 < N] = rng.random((i + 10 < N).sum()) - 0.5
+        b = rng.random(N).astype(np.float32)
+        pad = 10
+    x0 = np.random.default_rng(5).random(N).astype(np.float32)
+    xp = O.padded(x0, pad)
+    h = BandedMatrix(a.ctypes.data, a.shape[1], bands.ctypes.data, len(bands), N, N)
+    call("gm_gauss_seidel_host", xp.ctypes.data + 4 * pad, C.byref(h), b.ctypes.data, 3)
+    assert np.array_equal(xp[pad:pad + N], O.gs_f32(a, bands, b, x0, pad, 3))
+
+
+def test_gpu_jacobi_driver(gpu):
+    p, a, b = _dd_levin()
+    bands, pad = O.bands25(p.W), O.pad_for(p.W)
+    M = DeviceMatrix(p.N, bands, a)
+    x0 = np.random.default_rng(6).random(p.N).astype(np.float32)
+    for iters in (1, 4, 9):
+        x = Padded(p.N, pad, init=x0)
+        bt = torch.from_numpy(b).cuda()
+        call("gm_jacobi", x.ptr, M.ref, dptr(bt), C.c_float(2.0 / 3.0), pad, iters, S0)
+        torch.cuda.synchronize()
+        ref = O.jacobi_f32(a, bands, b, x0, pad, iters)
+        assert np.abs(x.numpy() - ref).max() <= 1e-5
+
+
+@pytest.mark.parametrize("prec", [PREC_FP32, PREC_FP16])
+def test_4k_solve_converges(gpu, prec):
+    p = synth.config_4k()
+    s = matting.assemble(p.image, p.lam, p.target, 1e-7, ASSEMBLY_ACCURATE)
+    x, info = matting.solve(s, method=METHOD_PCG, precision=prec, tol=1e-6, max_iter=20000)
+    print(f"4K prec={prec}: iterations={info.iterations} true_relres={info.true_relres:.3e}")
+    assert info.status == 0 and info.true_relres <= 1e-6
+    assert np.isfinite(x.numpy()).all()

@@ -1,0 +1,333 @@
+"""CPU tests that pin the oracle (oracle/oracle.c) before it is trusted as the
+GPU checker: the reference's own known-answer tests, brute-force dense Levin
+Laplacians built by a different inverse, closed forms, invariants, dense
+solves, and the reference's own Solve.cpp compiled into oracle/_ref."""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+from gpumatting_amd import synth
+from oracle import pyoracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def banded_to_dense(a, bands, N):
+    D = np.zeros((N, N))
+    for k, o in enumerate(bands):
+        for i in range(N):
+            j = i + o
+            if 0 <= j < N:
+                D[i, j] += a[k, i]
+            else:
+                assert a[k, i] == 0, "entry outside the matrix must be zero"
+    return D
+
+
+def rand_image(W, H, seed, flat=False):
+    rng = np.random.default_rng(seed)
+    if flat:
+        return np.full((H, W, 3), 0.3, np.float32)
+    return rng.random((H, W, 3)).astype(np.float32)
+
+
+# --------------------------------------------------------------------------
+# reference known-answer tests
+
+def test_bandedtest_known_answer():
+    """BandedMatrixTest.cu:25-114: bands {-10,0,10}, valid entries 1, x_i = i."""
+    N = 1_000_000
+    bands = np.array([-10, 0, 10], np.int32)
+    a = np.zeros((3, N), np.float32)
+    i = np.arange(N)
+    a[0, i >= 10] = 1
+    a[1, :] = 1
+    a[2, i + 10 < N] = 1
+    x = O.padded(np.arange(N, dtype=np.float32), 10)
+    y = O.bm_ax_f32(a, bands, x, 10)
+    exp = np.where(i < 10, 10.0 + 2 * i, np.where(i < N - 10, 3.0 * i, 1999970.0 + 2 * (i - (N - 10))))
+    assert np.array_equal(y.astype(np.float64), exp)
+
+
+def test_vectortest_known_answer():
+    """VectorTest.cu:24-65: <1, 2> over 1e6 elements = 2e6."""
+    n = 1_000_000
+    assert O.dot_f32(np.ones(n, np.float32), np.full(n, 2, np.float32)) == 2e6
+
+
+# --------------------------------------------------------------------------
+# Laplacian assembly
+
+SIZES = [(1, 1), (2, 2), (3, 3), (4, 3), (3, 5), (5, 4), (17, 13), (64, 3), (3, 40), (33, 9)]
+
+
+@pytest.mark.parametrize("W,H", SIZES)
+def test_scatter_equals_gather_bitwise(W, H):
+    im = rand_image(W, H, W * 100 + H)
+    rng = np.random.default_rng(W + H)
+    lam = (rng.random((H, W)) < 0.3).astype(np.float32) * 100
+    tgt = rng.random((H, W)).astype(np.float32)
+    a1, b1 = O.levin_scatter_f32(im, lam, tgt, 1e-7)
+    a2, b2, d2 = O.levin_gather_f32(im, lam, tgt, 1e-7)
+    assert np.array_equal(a1.view(np.uint32), a2.view(np.uint32))
+    assert np.array_equal(b1.view(np.uint32), b2.view(np.uint32))
+
+
+@pytest.mark.parametrize("W,H", [(3, 3), (5, 4), (7, 6), (9, 9), (4, 7)])
+def test_f64_matches_dense_bruteforce(W, H):
+    """orc_levin_f64 (adjugate) vs getLevinLapRaster.m restated with a
+    Gauss-Jordan inverse."""
+    im = rand_image(W, H, 3 + W * H)
+    L = O.levin_dense_f64(im, 1e-5)
+    a, _ = O.levin_f64(im, None, None, 1e-5)
+    D = banded_to_dense(a, O.bands25(W), W * H)
+    assert np.abs(D - L).max() <= 1e-9 * max(1.0, np.abs(L).max())
+
+
+@pytest.mark.parametrize("eps", [1e-2, 1e-4])
+def test_reference_rounding_close_to_dense(eps):
+    """The reference's float/double mix is within 1e-5 of the exact Laplacian
+    (relative to the row's diagonal) at the reference's eps = 1e-2."""
+    W, H = 9, 8
+    im = rand_image(W, H, 11)
+    L = O.levin_dense_f64(im, eps)
+    a, _ = O.levin_scatter_f32(im, None, None, eps)
+    D = banded_to_dense(a.astype(np.float64), O.bands25(W), W * H)
+    rel = np.abs(D - L) / np.abs(np.diag(L))[:, None]
+    assert rel.max() <= (1e-5 if eps >= 1e-2 else 1e-3)
+
+
+def test_accurate_rounding_close_to_dense_small_eps():
+    p = synth.make_problem(12, 10, 4, 1, 0.0)
+    L = O.levin_dense_f64(p.image, 1e-7)
+    a, _, _ = O.levin_gather_acc_f32(p.image, None, None, 1e-7)
+    D = banded_to_dense(a.astype(np.float64), O.bands25(12), 120)
+    rel = np.abs(D - L) / np.abs(np.diag(L))[:, None]
+    assert rel.max() <= 1e-5
+
+
+def test_invariants_symmetry_rowsum_zero_outside():
+    W, H = 13, 11
+    im = rand_image(W, H, 5)
+    a, _ = O.levin_f64(im, None, None, 1e-4)
+    bands = O.bands25(W)
+    N = W * H
+    # zero row sums (no data term)
+    assert np.abs(a.sum(axis=0)).max() <= 1e-9 * np.abs(a).max()
+    # symmetry L[i, k] == L[i + o_k, 24 - k]; zero where the 5x5 neighbour leaves the image
+    for k, o in enumerate(bands):
+        dy, dx = k // 5 - 2, k % 5 - 2
+        for i in range(N):
+            u, v = i % W, i // W
+            if not (0 <= u + dx < W and 0 <= v + dy < H):
+                assert a[k, i] == 0.0
+                continue
+            assert abs(a[k, i] - a[24 - k, i + o]) <= 1e-9 * abs(a[12, i])
+
+
+def test_constant_image_closed_form():
+    """Constant colour: Sigma' = (eps/9) I, x - mu = 0, so L_ij = -n_ij/9 with
+    n_ij = number of interior windows containing i and j."""
+    W, H = 7, 6
+    im = rand_image(W, H, 0, flat=True)
+    a, _ = O.levin_scatter_f32(im, None, None, 1e-2)
+    bands = O.bands25(W)
+    for i in range(W * H):
+        u, v = i % W, i // W
+        off = 0.0
+        for k in range(25):
+            dy, dx = k // 5 - 2, k % 5 - 2
+            if k == 12:
+                continue
+            n = sum(1 for cy in range(1, H - 1) for cx in range(1, W - 1)
+                    if abs(cx - u) <= 1 and abs(cy - v) <= 1 and abs(cx - u - dx) <= 1 and abs(cy - v - dy) <= 1)
+            assert a[k, i] == pytest.approx(-n / 9.0, abs=1e-6)
+            off += -n / 9.0
+        assert a[12, i] == pytest.approx(-off, abs=1e-5)
+
+
+def test_three_by_three_is_one_window():
+    im = rand_image(3, 3, 9)
+    a, _ = O.levin_f64(im, None, None, 1e-3)
+    X = im.reshape(9, 3).astype(np.float64)
+    mu = X.mean(0)
+    C = (X - mu).T @ (X - mu) / 9 + 1e-3 / 9 * np.eye(3)
+    G = (1 + (X - mu) @ np.linalg.inv(C) @ (X - mu).T) / 9
+    Lx = np.diag(G.sum(1)) - G
+    D = banded_to_dense(a, O.bands25(3), 9)
+    assert np.abs(D - Lx).max() <= 1e-10 * np.abs(Lx).max()
+
+
+def test_laplacian_is_psd():
+    W, H = 8, 7
+    L = O.levin_dense_f64(rand_image(W, H, 21), 1e-5)
+    ev = np.linalg.eigvalsh(L)
+    assert ev.min() >= -1e-9 * ev.max()
+
+
+def test_data_term_and_u8_rule():
+    W, H = 10, 8
+    im = rand_image(W, H, 2)
+    sc = np.full((H, W), 128, np.uint8)
+    sc[1, 2] = 255
+    sc[4, 6] = 0
+    a, b = O.levin_scatter_u8(im, sc, eps=1e-2, gamma=1.0)
+    a0, _ = O.levin_scatter_f32(im, None, None, 1e-2)
+    i_fg, i_bg = 1 * W + 2, 4 * W + 6
+    assert b[i_fg] == 1.0 and b[i_bg] == 0.0 and np.count_nonzero(b) == 1
+    d = a[12] - a0[12]
+    assert d[i_fg] == pytest.approx(1.0, abs=1e-6) and d[i_bg] == pytest.approx(1.0, abs=1e-6)
+    assert np.count_nonzero(d) == 2
+
+
+def _dense_levin_numpy(im, eps, mutate=None):
+    """Independent numpy Levin Laplacian (mutations plant known mistakes)."""
+    H, W, _ = im.shape
+    N = W * H
+    A = np.zeros((N, N))
+    for v in range(1, H - 1):
+        for u in range(1, W - 1):
+            idx = [(u + du) + (v + dv) * W for dv in (-1, 0, 1) for du in (-1, 0, 1)]
+            X = np.array([im[i // W, i % W] for i in idx], np.float64)
+            mu = X.mean(0)
+            C = X.T @ X / 9 - np.outer(mu, mu) + (0 if mutate == "no_eps" else eps / 9) * np.eye(3)
+            S = np.linalg.inv(C)
+            Xc = X - mu
+            G = (1 + Xc @ S @ Xc.T) / 9 if mutate != "sign" else (1 - Xc @ S @ Xc.T) / 9
+            if mutate == "transpose_index":
+                idx = [(u + dv) + (v + du) * W for dv in (-1, 0, 1) for du in (-1, 0, 1)]
+            for s in range(9):
+                for t in range(9):
+                    A[idx[s], idx[t]] += G[s, t]
+    return np.diag(A.sum(1)) - A
+
+
+@pytest.mark.parametrize("mutate", [None, "sign", "no_eps", "transpose_index"])
+def test_dense_check_catches_planted_mistakes(mutate):
+    """The dense cross-check that pins the assembly fails on each planted mistake."""
+    W, H = 6, 5
+    im = rand_image(W, H, 17) * 0.1 + 0.45
+    a, _ = O.levin_f64(im, None, None, 1e-2)
+    D = banded_to_dense(a, O.bands25(W), W * H)
+    ok = np.abs(D - _dense_levin_numpy(im, 1e-2, mutate)).max() <= 1e-8 * np.abs(D).max()
+    assert ok == (mutate is None)
+
+
+# --------------------------------------------------------------------------
+# solvers
+
+def small_system(W=24, H=20, seed=5):
+    p = synth.make_problem(W, H, seed, 1, 0.3)
+    a, b, d = O.levin_gather_acc_f32(p.image, p.lam, p.target, 1e-4)
+    return p, a, b
+
+
+def test_pcg_f64_matches_dense_solve():
+    p, a, b = small_system()
+    D = banded_to_dense(a.astype(np.float64), O.bands25(p.W), p.N)
+    xd = np.linalg.solve(D, b.astype(np.float64))
+    x, it, rr = O.pcg_f64(a.astype(np.float64), O.bands25(p.W), b.astype(np.float64), np.zeros(p.N),
+                          O.pad_for(p.W), 1e-13)
+    assert np.abs(x - xd).max() <= 1e-9 * max(1, np.abs(xd).max())
+
+
+def test_cg_reference_recurrence_converges_and_a_norm_decreases():
+    p, a, b = small_system()
+    D = banded_to_dense(a.astype(np.float64), O.bands25(p.W), p.N)
+    xd = np.linalg.solve(D, b.astype(np.float64))
+    pad = O.pad_for(p.W)
+    errs = []
+    for k in range(1, 40):
+        x = O.cg_ref_f32(a, O.bands25(p.W), b, np.full(p.N, 0.5, np.float32), pad, k, restart=1000)
+        e = x - xd
+        errs.append(e @ D @ e)
+    assert all(errs[i + 1] <= errs[i] * (1 + 1e-5) + 1e-12 for i in range(len(errs) - 1))
+    x = O.cg_ref_f32(a, O.bands25(p.W), b, np.full(p.N, 0.5, np.float32), pad, 400, restart=101)
+    assert np.abs(x - xd).max() <= 1e-3
+
+
+def test_gd_reference_exact_line_search():
+    """After one steepest-descent step with exact line search the new gradient
+    is orthogonal to the old one (main.cu:424-449)."""
+    p, a, b = small_system()
+    pad = O.pad_for(p.W)
+    bands = O.bands25(p.W)
+    x0 = np.full(p.N, 0.5, np.float32)
+    d0 = O.bm_ax_f32(a, bands, O.padded(x0, pad), pad) - b
+    x1 = O.gd_ref_f32(a, bands, b, x0, pad, 1)
+    d1 = O.bm_ax_f32(a, bands, O.padded(x1, pad), pad) - b
+    assert abs(float(d0.astype(np.float64) @ d1)) <= 1e-3 * float(d0.astype(np.float64) @ d0)
+
+
+def test_pcg_f32_reaches_tolerance():
+    p, a, b = small_system(24, 20, 5)
+    x, it, rr = O.pcg_f32(a, O.bands25(p.W), b, np.zeros(p.N, np.float32), O.pad_for(p.W), 1e-6, 5000)
+    assert rr <= 1e-6 and O.true_relres_f32(a, O.bands25(p.W), b, x, O.pad_for(p.W)) <= 2e-6
+
+
+# --------------------------------------------------------------------------
+# host smoothers vs the reference's own Solve.cpp
+
+def _dd_system(W=9, H=7, seed=8):
+    p = synth.make_problem(W, H, seed, 1, 0.5)
+    a, b, _ = O.levin_gather_acc_f32(p.image, p.lam, p.target, 1e-2)
+    a[12] += 2.0  # strictly diagonally dominant enough for Jacobi/GS to converge
+    return p, a, b
+
+
+def test_gauss_seidel_matches_dense_sweep():
+    p, a, b = _dd_system()
+    bands = O.bands25(p.W)
+    D = banded_to_dense(a.astype(np.float64), bands, p.N)
+    x = np.zeros(p.N)
+    for _ in range(3):
+        for i in range(p.N):
+            x[i] = (b[i] - (D[i] @ x - D[i, i] * x[i])) / D[i, i]
+    xg = O.gs_f32(a, bands, b, np.zeros(p.N, np.float32), O.pad_for(p.W), 3)
+    assert np.abs(xg - x).max() <= 1e-5
+
+
+@pytest.mark.skipif(O.ref_lib() is None, reason="reference Solve.cpp not built (no /root/reference)")
+def test_gauss_seidel_bitwise_vs_reference_solve_cpp():
+    p, a, b = _dd_system()
+    bands = O.bands25(p.W)
+    x0 = np.random.default_rng(1).random(p.N).astype(np.float32)
+    pad = O.pad_for(p.W)
+    assert np.array_equal(O.gs_f32(a, bands, b, x0, pad, 5), O.ref_gauss_seidel(a, bands, b, x0, pad, 5))
+
+
+@pytest.mark.skipif(O.ref_lib() is None, reason="reference Solve.cpp not built (no /root/reference)")
+@pytest.mark.parametrize("iters", [2, 4, 10])
+def test_jacobi_bitwise_vs_reference_solve_cpp(iters):
+    p, a, b = _dd_system()
+    bands = O.bands25(p.W)
+    x0 = np.random.default_rng(2).random(p.N).astype(np.float32)
+    pad = O.pad_for(p.W)
+    assert np.array_equal(O.jacobi_f32(a, bands, b, x0, pad, iters),
+                          O.ref_jacobi(a, bands, b, x0, pad, iters))
+
+
+def test_mse_clamped_special_cases():
+    gt = np.random.default_rng(0).random((4, 5)).astype(np.float32)
+    assert O.mse_clamped(np.full((4, 5), 2.0, np.float32), gt) == pytest.approx(np.mean((1 - gt.astype(np.float64)) ** 2))
+    assert O.mse_clamped(np.full((4, 5), -1.0, np.float32), gt) == pytest.approx(np.mean(gt.astype(np.float64) ** 2))
+    assert O.mse_clamped(gt, gt) == 0.0
+
+
+# --------------------------------------------------------------------------
+# golden fixture from the reference's sample data
+
+def load_crop():
+    z = np.load(os.path.join(HERE, "golden", "data05_crop.npz"))
+    f = z["rgb"].astype(np.float32) / np.float32(int(z["maxval"]))
+    im4 = np.concatenate([f, np.ones(f.shape[:2] + (1,), np.float32)], axis=2)
+    return z, im4
+
+
+def test_golden_crop_digest():
+    z, im4 = load_crop()
+    a, b = O.levin_scatter_u8(im4, z["scribs"], eps=1e-2, gamma=1.0)
+    assert hashlib.sha256(a.tobytes()).digest() == z["L_sha256"].tobytes()
+    assert hashlib.sha256(b.tobytes()).digest() == z["b_sha256"].tobytes()
