@@ -78,6 +78,7 @@ _SIGS = {
     "gm_true_relres": ([BMp, vp, vp, C.POINTER(cd), vp], ci),
     "gm_levin_laplacian": ([BMp, vp, vp, vp, ci, sz, vp, vp, cf, ci, ci, ci, vp], ci),
     "gm_levin_laplacian_u8": ([BMp, vp, vp, vp, ci, sz, vp, cf, cf, ci, ci, ci, vp], ci),
+    "gm_levin_backward": ([vp, vp, vp, vp, ci, sz, vp, vp, vp, vp, cf, ci, ci, vp], ci),
     "gm_host_levin_laplacian": ([BMp, vp, cf, vp, vp, ci, ci, sz], ci),
     "gm_levin_bands": ([ci, vp], None),
     "gm_levin_pad": ([ci], ci),

@@ -216,6 +216,156 @@ __global__ __launch_bounds__(kBlock) void k_levin(const float* __restrict__ im, 
     if (dinv) dinv[i] = acc[12] != 0.f ? 1.f / acc[12] : 0.f;
 }
 
+
+// ---------------------------------------------------------------------------
+// Assembly adjoint (differentiation path; no reference counterpart).
+//
+// For the system (L(I) + Lambda) alpha = Lambda t with adjoint mu = A^-1 dLoss/dalpha,
+// dLoss/dL_ij = -mu_i alpha_j; with L_ii = -sum_{j!=i} L_ij + lam_i the
+// effective weight of an off-diagonal entry is mu_i (alpha_i - alpha_j).  Per
+// interior window (centred colours X, S = (X^T X/9 + eps/9 I)^-1, Y = X S,
+// H_st = (mu_s - mu_t)(alpha_s - alpha_t), K = H Y):
+//   dLoss/dX_p = -(1/9) [K_p - (1/9) Y_p (X^T K) - (1/9) 1^T K]
+// summed over the windows containing pixel p (gather, window raster order).
+// Everything in FP64 (exact double covariance, as GM_ASSEMBLY_ACCURATE).
+// Also dLoss/dlam_i = mu_i (t_i - alpha_i) and dLoss/dt_i = mu_i lam_i.
+constexpr int kBX = 32, kBY = 8;
+constexpr int kBWX = kBX + 2, kBWY = kBY + 2, kBNW = kBWX * kBWY;
+constexpr int kBPX = kBX + 4, kBPY = kBY + 4;
+
+__global__ __launch_bounds__(kBlock) void k_levin_backward(const float* __restrict__ im, int nch, size_t pitch,
+                                                           const float* __restrict__ mu,
+                                                           const float* __restrict__ alpha,
+                                                           const float* __restrict__ lam,
+                                                           const float* __restrict__ tgt, float eps, int W, int H,
+                                                           float* __restrict__ gim, float* __restrict__ glam,
+                                                           float* __restrict__ gtgt) {
+    __shared__ float px[5][kBPY][kBPX];  // r, g, b, mu, alpha
+    __shared__ double wst[21][kBNW];     // S(6) m(3) XtK(9) 1tK(3)
+    __shared__ unsigned char wok[kBNW];
+    const int x0 = blockIdx.x * kBX, y0 = blockIdx.y * kBY;
+    for (int e = threadIdx.x; e < kBPX * kBPY; e += kBlock) {
+        const int ly = e / kBPX, lx = e % kBPX;
+        const int gx = x0 - 2 + lx, gy = y0 - 2 + ly;
+        float c0 = 0.f, c1 = 0.f, c2 = 0.f, m = 0.f, a = 0.f;
+        if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
+            const float* p = im + ((size_t)gy * pitch + (size_t)gx) * (size_t)nch;
+            c0 = p[0]; c1 = p[1]; c2 = p[2];
+            const size_t i = (size_t)gy * W + gx;
+            m = mu[i];
+            a = alpha[i];
+        }
+        px[0][ly][lx] = c0; px[1][ly][lx] = c1; px[2][ly][lx] = c2; px[3][ly][lx] = m; px[4][ly][lx] = a;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kBNW; e += kBlock) {
+        const int wy = e / kBWX, wx = e % kBWX;
+        const int cx = x0 - 1 + wx, cy = y0 - 1 + wy;
+        const bool ok = cx >= 1 && cx <= W - 2 && cy >= 1 && cy <= H - 2;
+        wok[e] = ok;
+        if (!ok) continue;
+        double X[9][3], m0 = 0, m1 = 0, m2 = 0;
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            const int ly = wy + t / 3, lx = wx + t % 3;
+            X[t][0] = px[0][ly][lx]; X[t][1] = px[1][ly][lx]; X[t][2] = px[2][ly][lx];
+            m0 += X[t][0]; m1 += X[t][1]; m2 += X[t][2];
+        }
+        m0 /= 9.0; m1 /= 9.0; m2 /= 9.0;
+        double c11 = 0, c12 = 0, c13 = 0, c22 = 0, c23 = 0, c33 = 0;
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            X[t][0] -= m0; X[t][1] -= m1; X[t][2] -= m2;
+            c11 += X[t][0] * X[t][0]; c12 += X[t][0] * X[t][1]; c13 += X[t][0] * X[t][2];
+            c22 += X[t][1] * X[t][1]; c23 += X[t][1] * X[t][2]; c33 += X[t][2] * X[t][2];
+        }
+        const double reg = (double)eps / 9.0;
+        c11 = c11 / 9.0 + reg; c12 /= 9.0; c13 /= 9.0; c22 = c22 / 9.0 + reg; c23 /= 9.0; c33 = c33 / 9.0 + reg;
+        const double d11 = c22 * c33 - c23 * c23, d12 = c13 * c23 - c12 * c33, d13 = c12 * c23 - c13 * c22;
+        const double d22 = c11 * c33 - c13 * c13, d23 = c12 * c13 - c11 * c23, d33 = c11 * c22 - c12 * c12;
+        const double det = c11 * d11 + c12 * d12 + c13 * d13;
+        const double S[6] = {d11 / det, d12 / det, d13 / det, d22 / det, d23 / det, d33 / det};
+        double Y[9][3];
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            Y[t][0] = S[0] * X[t][0] + S[1] * X[t][1] + S[2] * X[t][2];
+            Y[t][1] = S[1] * X[t][0] + S[3] * X[t][1] + S[4] * X[t][2];
+            Y[t][2] = S[2] * X[t][0] + S[4] * X[t][1] + S[5] * X[t][2];
+        }
+        double XK[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, kk[3] = {0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < 9; ++s) {
+            const double ms = px[3][wy + s / 3][wx + s % 3], as = px[4][wy + s / 3][wx + s % 3];
+            double K0 = 0, K1 = 0, K2 = 0;
+#pragma unroll
+            for (int t = 0; t < 9; ++t) {
+                const double h = (ms - (double)px[3][wy + t / 3][wx + t % 3]) * (as - (double)px[4][wy + t / 3][wx + t % 3]);
+                K0 += h * Y[t][0]; K1 += h * Y[t][1]; K2 += h * Y[t][2];
+            }
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                XK[3 * r + 0] += X[s][r] * K0;
+                XK[3 * r + 1] += X[s][r] * K1;
+                XK[3 * r + 2] += X[s][r] * K2;
+            }
+            kk[0] += K0; kk[1] += K1; kk[2] += K2;
+        }
+#pragma unroll
+        for (int q = 0; q < 6; ++q) wst[q][e] = S[q];
+        wst[6][e] = m0; wst[7][e] = m1; wst[8][e] = m2;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) wst[9 + q][e] = XK[q];
+        wst[18][e] = kk[0]; wst[19][e] = kk[1]; wst[20][e] = kk[2];
+    }
+    __syncthreads();
+    const int tx = threadIdx.x % kBX, ty = threadIdx.x / kBX;
+    const int gx = x0 + tx, gy = y0 + ty;
+    if (gx >= W || gy >= H) return;
+    double g0 = 0, g1 = 0, g2 = 0;
+    const double mp = px[3][ty + 2][tx + 2], ap = px[4][ty + 2][tx + 2];
+#pragma unroll
+    for (int wy2 = 0; wy2 < 3; ++wy2)
+#pragma unroll
+        for (int wx2 = 0; wx2 < 3; ++wx2) {
+            const int e = (ty + wy2) * kBWX + (tx + wx2);
+            if (!wok[e]) continue;
+            double S[6];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) S[q] = wst[q][e];
+            const double m0 = wst[6][e], m1 = wst[7][e], m2 = wst[8][e];
+            double K0 = 0, K1 = 0, K2 = 0;
+#pragma unroll
+            for (int vy = 0; vy < 3; ++vy)
+#pragma unroll
+                for (int vx = 0; vx < 3; ++vx) {
+                    const int ly = ty + wy2 + vy, lx = tx + wx2 + vx;
+                    const double x = (double)px[0][ly][lx] - m0, y = (double)px[1][ly][lx] - m1,
+                                 z = (double)px[2][ly][lx] - m2;
+                    const double h = (mp - (double)px[3][ly][lx]) * (ap - (double)px[4][ly][lx]);
+                    K0 += h * (S[0] * x + S[1] * y + S[2] * z);
+                    K1 += h * (S[1] * x + S[3] * y + S[4] * z);
+                    K2 += h * (S[2] * x + S[4] * y + S[5] * z);
+                }
+            const double x = (double)px[0][ty + 2][tx + 2] - m0, y = (double)px[1][ty + 2][tx + 2] - m1,
+                         z = (double)px[2][ty + 2][tx + 2] - m2;
+            const double Y0 = S[0] * x + S[1] * y + S[2] * z;
+            const double Y1 = S[1] * x + S[3] * y + S[4] * z;
+            const double Y2 = S[2] * x + S[4] * y + S[5] * z;
+            const double yk0 = Y0 * wst[9][e] + Y1 * wst[12][e] + Y2 * wst[15][e];
+            const double yk1 = Y0 * wst[10][e] + Y1 * wst[13][e] + Y2 * wst[16][e];
+            const double yk2 = Y0 * wst[11][e] + Y1 * wst[14][e] + Y2 * wst[17][e];
+            g0 -= (K0 - yk0 / 9.0 - wst[18][e] / 9.0) / 9.0;
+            g1 -= (K1 - yk1 / 9.0 - wst[19][e] / 9.0) / 9.0;
+            g2 -= (K2 - yk2 / 9.0 - wst[20][e] / 9.0) / 9.0;
+        }
+    const size_t i = (size_t)gy * W + gx;
+    gim[3 * i + 0] = (float)g0;
+    gim[3 * i + 1] = (float)g1;
+    gim[3 * i + 2] = (float)g2;
+    if (glam) glam[i] = (float)(mp * ((tgt ? (double)tgt[i] : 0.0) - ap));
+    if (gtgt) gtgt[i] = (float)(mp * (lam ? (double)lam[i] : 0.0));
+}
+
 static gm_status levin_launch(BandedMatrix* L, float* b, float* dinv, const float* im, int nch, size_t pitch,
                               const float* lam, const float* tgt, const unsigned char* scr, float gamma, float eps,
                               int mode, int W, int H, gm_stream_t s) {
@@ -265,6 +415,20 @@ gm_status gm_levin_laplacian_u8(BandedMatrix* L, float* b, float* dinv, const fl
                                 gm_stream_t s) {
     if (!scribs) return fail(GM_ERR_INVALID, "levin: null scribbles");
     return levin_launch(L, b, dinv, im, nch, im_pitch, nullptr, nullptr, scribs, gamma, eps, mode, W, H, s);
+}
+
+gm_status gm_levin_backward(float* grad_im, float* grad_lam, float* grad_tgt, const float* im, int nch,
+                             size_t im_pitch, const float* mu, const float* alpha, const float* lam, const float* tgt,
+                             float eps, int W, int H, gm_stream_t s) {
+    if (!grad_im || !im || !mu || !alpha) return fail(GM_ERR_INVALID, "levin_backward: null pointer");
+    if (W < 1 || H < 1 || (nch != 3 && nch != 4) || im_pitch < (size_t)W)
+        return fail(GM_ERR_INVALID, "levin_backward: bad image shape");
+    dim3 grid((W + kBX - 1) / kBX, (H + kBY - 1) / kBY);
+    if (grid.y > 65535u) return fail(GM_ERR_UNSUPPORTED, "levin_backward: image too tall");
+    hipLaunchKernelGGL(k_levin_backward, grid, dim3(kBlock), 0, (hipStream_t)s, im, nch, im_pitch, mu, alpha, lam, tgt,
+                       eps, W, H, grad_im, grad_lam, grad_tgt);
+    GM_LAUNCH_CHECK();
+    return GM_OK;
 }
 
 gm_status gm_host_levin_laplacian(BandedMatrix* L, float* b, float eps, const float* im4,

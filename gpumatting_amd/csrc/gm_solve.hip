@@ -844,7 +844,8 @@ gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, c
     }
 
     const int chunk = o->check_every > 0 ? o->check_every : 16;
-    const int refine = mixed ? (o->refine_every > 0 ? o->refine_every : 50) : 0;
+    // FP64 residual replacement period: 16-bit forms default to 50, FP32 to none
+    const int refine = o->refine_every > 0 ? o->refine_every : (o->refine_every == 0 && mixed ? 50 : 0);
     long enq = 0;  // iterations enqueued
     int polls = 0;
     bool finished = false;
@@ -950,7 +951,7 @@ void gm_solve_opts_default(gm_solve_opts* o) {
     o->max_iter = 10000;
     o->tol = 1e-6;
     o->restart = 0;
-    o->refine_every = 50;
+    o->refine_every = 0;
     o->check_every = 16;
     o->verify = 1;
 }

@@ -149,7 +149,7 @@ def assemble_u8(image, scribs, eps: float = 1e-2, gamma: float = 1.0, mode: int 
 
 
 def solve_opts(method=_lib.METHOD_PCG, precision=_lib.PREC_FP32, tol=1e-6, max_iter=10000, restart=0,
-               refine_every=50, check_every=16, verify=True) -> SolveOpts:
+               refine_every=0, check_every=16, verify=True) -> SolveOpts:
     o = SolveOpts()
     lib().gm_solve_opts_default(C.byref(o))
     o.method, o.precision, o.tol, o.max_iter = method, precision, tol, max_iter
@@ -189,3 +189,16 @@ def bm_ax(L: DeviceMatrix, x: Padded, y: torch.Tensor | None = None, add: bool |
     else:
         call("gm_bm_axpy", _p(out), L.ref, x.ptr, _p(y), int(bool(add)), _stream())
     return out
+
+
+def levin_backward(image, mu: torch.Tensor, alpha: torch.Tensor, lam=None, tgt=None, eps: float = 1e-7):
+    """Assembly adjoint (gm_levin_backward): returns (d image (H,W,3), d lam (N), d tgt (N))."""
+    im = _dev(image)
+    H, W, nch = im.shape
+    gim = torch.empty(H * W * 3, dtype=torch.float32, device="cuda")
+    glam = torch.empty(H * W, dtype=torch.float32, device="cuda")
+    gtg = torch.empty(H * W, dtype=torch.float32, device="cuda")
+    la, tg = _dev(lam), _dev(tgt)
+    call("gm_levin_backward", _p(gim), _p(glam), _p(gtg), _p(im), nch, W, _p(mu.contiguous()),
+         _p(alpha.contiguous()), _p(la), _p(tg), eps, W, H, _stream())
+    return gim.view(H, W, 3), glam, gtg

@@ -128,7 +128,8 @@ typedef struct gm_solve_opts_s {
    int max_iter;        /* iteration cap (> 0) */
    double tol;          /* stop when ||b - A x|| / ||b|| <= tol; <= 0: run max_iter */
    int restart;         /* > 0: recompute r = A x - b every `restart` iterations */
-   int refine_every;    /* 16-bit forms: FP64 residual replacement period (0 -> 50) */
+   int refine_every;    /* FP64 residual replacement period: > 0 that many iterations; 0 = default
+                           (50 for the 16-bit forms, none for FP32); < 0 = none */
    int check_every;     /* iterations enqueued between host status polls (0 -> 16) */
    int verify;          /* 1: confirm convergence with the FP64 true residual */
 } gm_solve_opts;
@@ -185,6 +186,16 @@ gm_status gm_levin_laplacian_u8(BandedMatrix* L, float* b, float* dinv, const fl
  * (synchronous). */
 gm_status gm_host_levin_laplacian(BandedMatrix* L, float* b, float eps, const float* im4,
                                   const unsigned char* scribs, int W, int H, size_t im_pitch);
+
+/* Adjoint of the assembly for (L(im) + diag(lam)) alpha = lam * tgt (no
+ * reference counterpart: the differentiation path).  mu = (L + diag(lam))^-1
+ * dLoss/dalpha (an adjoint solve with gm_solve), alpha the forward solution.
+ * Writes dLoss/dim (W*H*3 floats, RGB interleaved, dense), and if non-NULL
+ * dLoss/dlam_i = mu_i (tgt_i - alpha_i) and dLoss/dtgt_i = mu_i lam_i.
+ * The image derivative is that of GM_ASSEMBLY_ACCURATE's Laplacian, in FP64. */
+gm_status gm_levin_backward(float* grad_im, float* grad_lam, float* grad_tgt, const float* im, int nch,
+                            size_t im_pitch, const float* mu, const float* alpha, const float* lam, const float* tgt,
+                            float eps, int W, int H, gm_stream_t stream);
 
 /* 5x5-window band offsets (main.cu:161-190) and the vector pad
  * ((2W+2)/32)*32+32 (main.cu:228-229). */

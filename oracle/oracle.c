@@ -874,3 +874,78 @@ double orc_baseline_pcg_f32(float* x, const float* a, size_t apitch, const int* 
     free(rb); free(pb); free(Ap);
     return bb > 0 ? sqrt(rr / bb) : sqrt(rr);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Adjoint of the assembly (FP64; no reference counterpart -- the north star's
+ * differentiation path).  For a loss whose gradient with respect to the
+ * stored off-diagonal entries is G_ij (the diagonal being -sum of the
+ * off-diagonals + lam), and G_ij = -mu_i alpha_j for a linear solve
+ * (L + Lambda) alpha = Lambda t with adjoint mu, the effective weight of entry
+ * (i,j), i != j, is gbar_ij = mu_i (alpha_i - alpha_j).  Per interior window w
+ * with centred colours X (9x3), S = (X^T X/9 + eps/9 I)^-1, Y = X S and
+ * H_ij = (mu_i - mu_j)(alpha_i - alpha_j):
+ *   dLoss/dX = Z - 1 (1^T Z)/9,  Z = -(1/9) [ H Y - (1/9) Y (X^T H Y) ]
+ * and the rows of dLoss/dX are added to the window's 9 pixels.
+ * grad (W*H*3 doubles, channel-interleaved) is overwritten. */
+void orc_levin_backward_f64(double* grad, const float* im, int nch, size_t pitch, const double* mu,
+                            const double* alpha, double eps, int W, int H)
+{
+    memset(grad, 0, sizeof(double) * (size_t)W * H * 3);
+    for (int v = 1; v < H - 1; ++v)
+        for (int u = 1; u < W - 1; ++u) {
+            int idx[9];
+            double X[9][3], m[3] = {0, 0, 0};
+            int q = 0;
+            for (int dv = -1; dv <= 1; ++dv)
+                for (int du = -1; du <= 1; ++du) {
+                    const float* p = im + ((size_t)(v + dv) * pitch + (u + du)) * nch;
+                    idx[q] = (u + du) + (v + dv) * W;
+                    for (int c = 0; c < 3; ++c) { X[q][c] = p[c]; m[c] += p[c]; }
+                    ++q;
+                }
+            for (int c = 0; c < 3; ++c) m[c] /= 9.0;
+            for (int t = 0; t < 9; ++t)
+                for (int c = 0; c < 3; ++c) X[t][c] -= m[c];
+            double C[9], S[9];
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) {
+                    double s = 0;
+                    for (int t = 0; t < 9; ++t) s += X[t][r] * X[t][c];
+                    C[3 * r + c] = s / 9.0 + (r == c ? eps / 9.0 : 0.0);
+                }
+            orc_inv3_gj(C, S);
+            double Y[9][3], K[9][3], Hm[9][9];
+            for (int t = 0; t < 9; ++t)
+                for (int c = 0; c < 3; ++c) {
+                    double s = 0;
+                    for (int r = 0; r < 3; ++r) s += X[t][r] * S[3 * r + c];
+                    Y[t][c] = s;
+                }
+            for (int s = 0; s < 9; ++s)
+                for (int t = 0; t < 9; ++t)
+                    Hm[s][t] = (mu[idx[s]] - mu[idx[t]]) * (alpha[idx[s]] - alpha[idx[t]]);
+            for (int s = 0; s < 9; ++s)
+                for (int c = 0; c < 3; ++c) {
+                    double acc = 0;
+                    for (int t = 0; t < 9; ++t) acc += Hm[s][t] * Y[t][c];
+                    K[s][c] = acc;
+                }
+            double XK[3][3];
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) {
+                    double acc = 0;
+                    for (int t = 0; t < 9; ++t) acc += X[t][r] * K[t][c];
+                    XK[r][c] = acc;
+                }
+            double Z[9][3], colsum[3] = {0, 0, 0};
+            for (int t = 0; t < 9; ++t)
+                for (int c = 0; c < 3; ++c) {
+                    double yk = 0;
+                    for (int r = 0; r < 3; ++r) yk += Y[t][r] * XK[r][c];
+                    Z[t][c] = -(K[t][c] - yk / 9.0) / 9.0;
+                    colsum[c] += Z[t][c];
+                }
+            for (int t = 0; t < 9; ++t)
+                for (int c = 0; c < 3; ++c) grad[(size_t)idx[t] * 3 + c] += Z[t][c] - colsum[c] / 9.0;
+        }
+}

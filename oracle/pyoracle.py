@@ -67,6 +67,7 @@ def lib():
         L.orc_mse_clamped.argtypes = [f32p, f32p, ci, ci]
         L.orc_mse_clamped.restype = cd
         L.orc_num_threads.restype = ci
+        L.orc_levin_backward_f64.argtypes = [f64p, f32p, ci, sz, f64p, f64p, cd, ci, ci]
         L.orc_baseline_pcg_f32.argtypes = [vp, vp, sz, i32p, ci, ci, vp, ci, ci]
         L.orc_baseline_pcg_f32.restype = cd
         _LIB = L
@@ -270,6 +271,16 @@ def baseline_pcg_f32(a, bands, b, x0, pad, iters):
     return xp[pad:pad + N].copy(), rr
 
 
+def levin_backward_f64(image, mu, alpha, eps=1e-7):
+    """d/d image of sum_{i!=j} mu_i (alpha_i - alpha_j) L_ij (the assembly
+    adjoint); returns (H, W, 3) float64."""
+    im, W, H, nch = _img(image)
+    g = np.zeros(W * H * 3, np.float64)
+    lib().orc_levin_backward_f64(g, im, nch, W, np.ascontiguousarray(mu, np.float64).ravel(),
+                                 np.ascontiguousarray(alpha, np.float64).ravel(), eps, W, H)
+    return g.reshape(H, W, 3)
+
+
 def num_threads() -> int:
     return lib().orc_num_threads()
 
@@ -344,3 +355,29 @@ def ref_jacobi(a, bands, b, x0, pad, iterations, omega=2.0 / 3.0):
     f.argtypes = [C.c_void_p, C.POINTER(_RefBM), C.c_void_p, C.c_size_t, C.c_int, C.c_float]
     f(_ptr(xp, pad), C.byref(M), _ptr(np.ascontiguousarray(b, np.float32)), iterations, pad, omega)
     return xp[pad:pad + N].copy()
+
+
+def matte_loss_grad_f64(image, lam, tgt, alpha_gt, eps=1e-7, tol=1e-14):
+    """FP64 host reference of loss = ||alpha - alpha_gt||^2 and its gradients
+    with respect to the image, lam and tgt (forward solve, adjoint solve,
+    assembly adjoint).  Returns (loss, alpha, d_image (H,W,3), d_lam, d_tgt)."""
+    H, W = image.shape[:2]
+    a64, b64 = levin_f64(image, lam, tgt, eps)
+    bands = bands25(W)
+    pad = pad_for(W)
+    al, _, _ = pcg_f64(a64, bands, b64, np.zeros(W * H), pad, tol, 1000000)
+    r = al - np.asarray(alpha_gt, np.float64).ravel()
+    loss = float(r @ r)
+    mu, _, _ = pcg_f64(a64, bands, 2 * r, np.zeros(W * H), pad, tol, 1000000)
+    gim = levin_backward_f64(image, mu, al, eps)
+    lam64 = np.asarray(lam, np.float64).ravel()
+    tg64 = np.asarray(tgt, np.float64).ravel()
+    return loss, al, gim, mu * (tg64 - al), mu * lam64
+
+
+def matte_loss_f64(image, lam, tgt, alpha_gt, eps=1e-7, tol=1e-14):
+    H, W = image.shape[:2]
+    a64, b64 = levin_f64(image, lam, tgt, eps)
+    al, _, _ = pcg_f64(a64, bands25(W), b64, np.zeros(W * H), pad_for(W), tol, 1000000)
+    r = al - np.asarray(alpha_gt, np.float64).ravel()
+    return float(r @ r)

@@ -331,3 +331,67 @@ def test_golden_crop_digest():
     a, b = O.levin_scatter_u8(im4, z["scribs"], eps=1e-2, gamma=1.0)
     assert hashlib.sha256(a.tobytes()).digest() == z["L_sha256"].tobytes()
     assert hashlib.sha256(b.tobytes()).digest() == z["b_sha256"].tobytes()
+
+
+# --------------------------------------------------------------------------
+# differentiation: the assembly adjoint and the full loss gradient
+
+def test_assembly_adjoint_matches_finite_differences():
+    """d/dI of sum_{i!=j} mu_i (alpha_i - alpha_j) L_ij(I), central differences of orc_levin_f64."""
+    rng = np.random.default_rng(0)
+    W, H = 7, 6
+    im = (rng.random((H, W, 3)) * 0.5 + 0.25).astype(np.float32)
+    mu, al = rng.standard_normal(W * H), rng.standard_normal(W * H)
+    eps = 1e-3
+    bands = O.bands25(W)
+
+    def f(img):
+        a, _ = O.levin_f64(img, None, None, eps)
+        tot = 0.0
+        for k, o in enumerate(bands):
+            if k == 12:
+                continue
+            i = np.arange(W * H)
+            j = i + o
+            m = (j >= 0) & (j < W * H)
+            tot += np.sum(a[k][m] * mu[i[m]] * (al[i[m]] - al[j[m]]))
+        return tot
+
+    g = O.levin_backward_f64(im, mu, al, eps)
+    for (y, x, c) in [(0, 0, 0), (2, 3, 1), (5, 6, 2), (3, 3, 0), (1, 4, 2), (4, 1, 1)]:
+        a, b = im.copy(), im.copy()
+        a[y, x, c] += np.float32(1e-3)
+        b[y, x, c] -= np.float32(1e-3)
+        fd = (f(a) - f(b)) / (float(a[y, x, c]) - float(b[y, x, c]))
+        assert abs(fd - g[y, x, c]) <= 1e-4 * max(1.0, abs(fd))
+
+
+def test_full_gradient_matches_finite_differences():
+    """loss = ||alpha - alpha_gt||^2 through the FP64 solve: adjoint gradient vs
+    central differences (h = 1e-4 in FP64 on the image, lam, tgt)."""
+    p = synth.make_problem(16, 14, 11, 1, 0.3)
+    im = p.image.astype(np.float32)
+    eps = 1e-4
+    loss, al, gim, glam, gtg = O.matte_loss_grad_f64(im, p.lam, p.target, p.alpha_gt, eps)
+    rng = np.random.default_rng(1)
+    for _ in range(6):
+        y, x, c = rng.integers(1, 13), rng.integers(1, 15), rng.integers(0, 3)
+        h = 1e-3
+        a, b = im.copy(), im.copy()
+        a[y, x, c] += np.float32(h)
+        b[y, x, c] -= np.float32(h)
+        fd = (O.matte_loss_f64(a, p.lam, p.target, p.alpha_gt, eps) -
+              O.matte_loss_f64(b, p.lam, p.target, p.alpha_gt, eps)) / (float(a[y, x, c]) - float(b[y, x, c]))
+        assert abs(fd - gim[y, x, c]) <= 2e-3 * max(abs(fd), 1e-3 * np.abs(gim).max())
+    idx = np.flatnonzero(p.lam.ravel() > 0)[:4]
+    for i in idx:
+        for arr, g in ((p.lam, glam), (p.target, gtg)):
+            h = 1e-4 * max(1.0, abs(float(arr.ravel()[i])))
+            up, dn = arr.copy(), arr.copy()
+            up.ravel()[i] += h
+            dn.ravel()[i] -= h
+            args_up = (up, p.target) if arr is p.lam else (p.lam, up)
+            args_dn = (dn, p.target) if arr is p.lam else (p.lam, dn)
+            fd = (O.matte_loss_f64(im, *args_up, p.alpha_gt, eps) -
+                  O.matte_loss_f64(im, *args_dn, p.alpha_gt, eps)) / (float(up.ravel()[i]) - float(dn.ravel()[i]))
+            assert abs(fd - g[i]) <= 1e-3 * max(abs(fd), 1e-6)
