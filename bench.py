@@ -64,6 +64,57 @@ def cpu_baseline(p, iters_needed, sample_iters):
                       f"({t_it * 1e3:.1f} ms each) x {iters_needed} iterations; {cpu_model()}"}
 
 
+def bench_batched(args):
+    """64 images (W ~ U[240,1080], H ~ U[320,1920]) x 4 constraint sets, all
+    solved together to 1e-6 per system; aggregate unknowns/s."""
+    import torch
+
+    from gpumatting_amd import PREC_FP16, PREC_FP32, synth
+    from gpumatting_amd.batch import Batch
+    log("generating the batched workload (64 images x 4 right-hand sides)")
+    work = synth.config_batched()
+    sizes = [(p.W, p.H) for p, _ in work]
+    unknowns = sum(p.N for p, _ in work) * 4
+    res = {}
+    for name, prec in (("mixed", PREC_FP16), ("fp32", PREC_FP32)):
+        b = Batch(sizes, 4, prec)
+        imgs = [torch.from_numpy(p.image).cuda() for p, _ in work]
+        rhs = [[(torch.from_numpy(l).cuda(), torch.from_numpy(t).cuda()) for (l, t) in r] for _, r in work]
+
+        def step():
+            for i in range(len(work)):
+                b.assemble(i, imgs[i], 1e-7)
+                for j in range(4):
+                    b.set_rhs(i, j, *rhs[i][j])
+            b.solve(tol=1e-6, max_iter=20000)
+
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        its = [b.info(i, j).iterations for i in range(len(work)) for j in range(4)]
+        rr = max(b.info(i, j).true_relres for i in range(len(work)) for j in range(4))
+        res[name] = {"ms": ms, "max_iterations": max(its), "mean_iterations": sum(its) / len(its),
+                     "max_true_relres": rr}
+        log(f"batched {name}: {ms:.1f} ms, iterations max {max(its)} mean {sum(its) / len(its):.0f}, "
+            f"max true relres {rr:.2e}")
+        b.close()
+    out = {"metric": "batched matte solve aggregate unknowns/s (64 images x 4 RHS, 1e-6 per system)",
+           "value": round(unknowns / (res["mixed"]["ms"] * 1e-3), 1), "unit": "unknowns/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(res["mixed"]["ms"], 2),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "fp16 off-diagonals / f32 diagonals+vectors / f64 residual",
+           "data": "synthetic (gpumatting_amd/synth.py config_batched seed 4000)",
+           "config": {"workload": "64 images W~U[240,1080] H~U[320,1920], 4 constraint sets each (2-10 %), "
+                                  "assembly + batched Jacobi-PCG to 1e-6", "unknowns_total": unknowns},
+           "fp32_unknowns_per_s": round(unknowns / (res["fp32"]["ms"] * 1e-3), 1), "detail": res}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +123,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=40)
     ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default shapes)")
+    ap.add_argument("--workload", choices=["4k", "batched"], default="4k",
+                    help="4k: the headline 3840x2160 solve; batched: 64 uneven images x 4 right-hand sides")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run a 1 GiB device copy (reference kernel for FETCH_SIZE/WRITE_SIZE scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (the solve would shard by image "
@@ -85,6 +141,8 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     lib()
+    if args.workload == "batched":
+        return bench_batched(args)
     log("generating the 3840x2160 workload")
     p = synth.config_4k()
     W, H, N = p.W, p.H, p.N
@@ -102,8 +160,14 @@ def main():
         return s, info
 
     results = {}
+    tuned = {}
     for name, prec in (("mixed", PREC_FP16), ("fp32", PREC_FP32)):
         for _ in range(args.warmup):
+            step(prec)
+        if not args.no_tune:
+            R, pc, us = matting.autotune(L, prec, reps=3)
+            tuned[name] = {"rows_per_thread": R, "wg_per_cu": pc, "step_us": round(us, 2)}
+            log(f"autotune {name}: R={R} wg/CU={pc} step {us:.1f} us")
             step(prec)
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -143,6 +207,15 @@ def main():
     spmv_gbs = spmv_bytes / (spmv_ms * 1e-3) / 1e9
     log(f"spmv: {spmv_ms * 1e3:.1f} us/launch, {spmv_gbs:.0f} GB/s algorithmic")
 
+    if args.copy_calib:
+        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy calibration: 3 x 1 GiB device-to-device copies")
+        del src, dst
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle, OpenMP)")
@@ -170,6 +243,7 @@ def main():
         "iterations": {"mixed": mixed["iterations"], "fp32": fp32["iterations"]},
         "true_relres": {"mixed": mixed["true_relres"], "fp32": fp32["true_relres"]},
         "residual_replacements_mixed": mixed["replacements"],
+        "autotuned_step": tuned,
         "spmv_gbs": round(spmv_gbs, 1),
         "spmv_us": round(spmv_ms * 1e3, 2),
         "roofline": {"bound": "hbm", "kernel": "k_spmv25<4,F32,0> (banded SpMV, 25 FP32 diagonals)",

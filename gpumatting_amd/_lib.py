@@ -39,6 +39,10 @@ class SolveOpts(C.Structure):
                 ("restart", C.c_int), ("refine_every", C.c_int), ("check_every", C.c_int), ("verify", C.c_int)]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("rows_per_thread", C.c_int), ("wg_per_cu", C.c_int), ("step_us", C.c_double)]
+
+
 class SolveInfo(C.Structure):
     _fields_ = [("iterations", C.c_int), ("status", C.c_int), ("relres", C.c_double),
                 ("true_relres", C.c_double), ("replacements", C.c_int)]
@@ -83,6 +87,17 @@ _SIGS = {
     "gm_levin_bands": ([ci, vp], None),
     "gm_levin_pad": ([ci], ci),
     "gm_release_workspaces": ([], ci),
+    "gm_autotune": ([BMp, ci, ci, C.POINTER(TuneResult), vp], ci),
+    "gm_tune_set": ([ci, ci, ci, ci, ci], ci),
+    "gm_batch_create": ([C.POINTER(vp), ci, vp, vp, ci, ci], ci),
+    "gm_batch_destroy": ([vp], ci),
+    "gm_batch_assemble": ([vp, ci, vp, ci, sz, cf, vp], ci),
+    "gm_batch_set_rhs": ([vp, ci, ci, vp, vp, vp, vp], ci),
+    "gm_batch_solve": ([vp, C.POINTER(SolveOpts), vp], ci),
+    "gm_batch_alpha": ([vp, ci, ci, C.POINTER(vp)], ci),
+    "gm_batch_info": ([vp, ci, ci, C.POINTER(SolveInfo)], ci),
+    "gm_batch_set_grid": ([vp, ci], ci),
+    "gm_batch_tiles": ([vp], ci),
 }
 
 

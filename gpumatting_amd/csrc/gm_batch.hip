@@ -1,0 +1,721 @@
+// gm_batch.hip -- many images of uneven size, several right-hand sides each,
+// solved together (no reference counterpart: the north star's batched form).
+//
+// Each image i has one Levin Laplacian L_i (25 FP32 bands, no data term) and
+// nrhs independent constraint sets j, i.e. nrhs systems
+//     (L_i + diag(lam_ij)) x_ij = lam_ij * t_ij .
+// The systems of one image share L_i's off-diagonal bands, so one pass over a
+// tile loads those 24 bands into registers once and applies them to every
+// right-hand side; each system brings its own FP32 diagonal d_ij = L_ii +
+// lam_ij.  Jacobi-PCG per system, with the single-system kernels' sign and
+// reduction conventions.
+//
+// One launch per phase covers all images: a global tile list (tile -> image)
+// is walked by a persistent, XCD-aware grid.  Reductions are per image: every
+// tile writes its partial sums, and the last tile of an image to arrive sums
+// that image's tiles in index order and finalises its systems' scalars
+// (deterministic).  Systems that have converged are skipped; an image whose
+// systems are all done costs nothing.  Iterations are captured in a hipGraph
+// (kernels only, no memset nodes) and replayed.
+//
+// Mixed precision: the off-diagonal bands are also kept in FP16 (unscaled;
+// |L_ij| <= 9), the per-system diagonals stay FP32, and the residual is
+// replaced by the FP64-accumulated true residual (FP32 L) every 50 iterations.
+#include <string.h>
+
+#include <vector>
+
+#include "gm_internal.h"
+
+namespace gm {
+
+gm_status levin_assemble_raw(float* A, size_t apitch, float* b, float* dinv, const float* im, int nch, size_t pitch,
+                             const float* lam, const float* tgt, const unsigned char* scr, float gamma, float eps,
+                             int mode, int W, int H, gm_stream_t s);
+
+constexpr int kMaxRhs = 8;
+constexpr int kBR = 4;  // rows per thread of the batched kernels
+
+struct BImg {
+    int W, N, ntiles, tile0, sys0, nsys;
+    const float* L;
+    size_t pitch;
+    const unsigned short* L16;
+    size_t pitch16;
+};
+
+struct BSys {
+    float *x, *r, *p0, *p1, *Ap, *d, *dinv, *b;
+};
+
+enum { kRun = 0, kConverged = 1, kBreakdown = 2, kNonFinite = 3, kMaxIt = 4 };
+
+__device__ __forceinline__ bool b_stopped(const Scalars* s) { return s->done != kRun || s->verify != 0; }
+
+// Arrive at the image's counter; returns true (on every thread) for the last tile.
+__device__ __forceinline__ bool image_arrive(unsigned int* counter, int ntiles) {
+    __shared__ int last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned int prev = atomicAdd(counter, 1u);
+        last = (prev == (unsigned int)ntiles - 1u);
+    }
+    __syncthreads();
+    if (last) __threadfence();
+    return last != 0;
+}
+
+// Sum partials[(tile0 + k) * stride + off] over k < ntiles in a fixed order.
+__device__ __forceinline__ double image_sum(const double* partials, int tile0, int ntiles, int stride, int off) {
+    double s = 0;
+    for (int k = threadIdx.x; k < ntiles; k += kBlock) s += load_agent(partials + (size_t)(tile0 + k) * stride + off);
+    return block_sum(s);
+}
+
+// K1: p = -D^-1 r + beta p_old staged per system, Ap = (L_off + diag d) p, <p, Ap>.
+template <int FMT>
+__global__ __launch_bounds__(kBlock) void kb_cgstep(const BImg* __restrict__ imgs, const BSys* __restrict__ sys,
+                                                    Scalars* sc, const int* __restrict__ tile_img, int ntot,
+                                                    double* partials, unsigned int* counters) {
+    constexpr int R = kBR;
+    __shared__ __attribute__((aligned(16))) float lds[Tile<R>::LDS_FLOATS];
+    __shared__ double part[kMaxRhs];
+    const TileRange tr = xcd_tiles(ntot);
+    for (int t = tr.begin; t < tr.end; t += tr.step) {
+        const int im = tile_img[t];
+        const BImg I = imgs[im];
+        unsigned mask = 0;
+        for (int j = 0; j < I.nsys; ++j)
+            if (!b_stopped(&sc[I.sys0 + j])) mask |= 1u << j;
+        if (mask == 0) continue;
+        const int i0 = (t - I.tile0) * Tile<R>::ROWS;
+        const int ib = i0 + R * threadIdx.x;
+        Bands<R, FMT> av;
+        if (ib < I.N) {
+            if (FMT == kF32) load_bands<R, FMT, kDiagCaller>(I.L, I.pitch, ib, av);
+            else load_bands<R, FMT, kDiagCaller>(I.L16, I.pitch16, ib, av);
+        }
+        for (int j = 0; j < I.nsys; ++j) {
+            if (!((mask >> j) & 1u)) {
+                if (threadIdx.x == 0) part[j] = 0;
+                continue;
+            }
+            const BSys S = sys[I.sys0 + j];
+            const Scalars* s = &sc[I.sys0 + j];
+            const float beta = s->beta;
+            const int par = s->iters & 1;
+            const float* __restrict__ pold = par ? S.p1 : S.p0;
+            float* __restrict__ pnew = par ? S.p0 : S.p1;
+            __syncthreads();
+            if (beta != 0.f)
+                stage5<R>(lds, i0, I.W, I.N, [&](int g) { return fmaf(beta, pold[g], -(S.dinv[g] * S.r[g])); });
+            else
+                stage5<R>(lds, i0, I.W, I.N, [&](int g) { return -(S.dinv[g] * S.r[g]); });
+            __syncthreads();
+            double v = 0;
+            if (ib < I.N) {
+                float y[R], pc[R], dd[R];
+                apply_bands<R, FMT, kDiagCaller>(av, lds + R * threadIdx.x, y);
+                load_rows<R>(S.d, ib, I.N, dd);
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    pc[q] = lds[2 * Tile<R>::SEG + R * threadIdx.x + 2 + q];
+                    y[q] = fmaf(dd[q], pc[q], y[q]);
+                }
+                store_rows<R>(pnew, ib, I.N, pc);
+                store_rows<R>(S.Ap, ib, I.N, y);
+#pragma unroll
+                for (int q = 0; q < R; ++q)
+                    if (ib + q < I.N) v += (double)(pc[q] * y[q]);
+            }
+            const double bs = block_sum(v);
+            if (threadIdx.x == 0) part[j] = bs;
+        }
+        if (threadIdx.x == 0)
+            for (int j = 0; j < I.nsys; ++j) partials[(size_t)t * kMaxRhs + j] = part[j];
+        if (image_arrive(&counters[im], I.ntiles)) {
+            for (int j = 0; j < I.nsys; ++j) {
+                if (!((mask >> j) & 1u)) continue;
+                const double pAp = image_sum(partials, I.tile0, I.ntiles, kMaxRhs, j);
+                if (threadIdx.x == 0) {
+                    Scalars* s = &sc[I.sys0 + j];
+                    s->pAp = pAp;
+                    if (!isfinite(pAp)) s->done = kNonFinite;
+                    else if (!(pAp > 0)) s->done = kBreakdown;
+                    else s->alpha = (float)(s->rz / pAp);
+                }
+            }
+            if (threadIdx.x == 0) counters[im] = 0u;
+        }
+    }
+}
+
+// K2: x += alpha p, r += alpha Ap, <r, D^-1 r>, ||r||^2 per system.
+__global__ __launch_bounds__(kBlock) void kb_cgupdate(const BImg* __restrict__ imgs, const BSys* __restrict__ sys,
+                                                      Scalars* sc, const int* __restrict__ tile_img, int ntot,
+                                                      int verify, double* partials, unsigned int* counters) {
+    constexpr int R = kBR;
+    __shared__ double part[kMaxRhs][2];
+    const TileRange tr = xcd_tiles(ntot);
+    for (int t = tr.begin; t < tr.end; t += tr.step) {
+        const int im = tile_img[t];
+        const BImg I = imgs[im];
+        unsigned mask = 0;
+        for (int j = 0; j < I.nsys; ++j)
+            if (!b_stopped(&sc[I.sys0 + j])) mask |= 1u << j;
+        if (mask == 0) continue;
+        const int i0 = (t - I.tile0) * Tile<R>::ROWS;
+        for (int j = 0; j < I.nsys; ++j) {
+            if (!((mask >> j) & 1u)) {
+                if (threadIdx.x == 0) part[j][0] = part[j][1] = 0;
+                continue;
+            }
+            const BSys S = sys[I.sys0 + j];
+            const Scalars* s = &sc[I.sys0 + j];
+            const float alpha = s->alpha;
+            const float* __restrict__ p = (s->iters & 1) ? S.p0 : S.p1;
+            double v0 = 0, v1 = 0;
+            for (int q = 0; q < R; ++q) {
+                const int i = i0 + q * kBlock + threadIdx.x;  // coalesced across the workgroup
+                if (i >= I.N) break;
+                S.x[i] = fmaf(alpha, p[i], S.x[i]);
+                const float rn = fmaf(alpha, S.Ap[i], S.r[i]);
+                S.r[i] = rn;
+                v0 += (double)(rn * (S.dinv[i] * rn));
+                v1 += (double)(rn * rn);
+            }
+            const double b0 = block_sum(v0);
+            const double b1 = block_sum(v1);
+            if (threadIdx.x == 0) {
+                part[j][0] = b0;
+                part[j][1] = b1;
+            }
+        }
+        if (threadIdx.x == 0)
+            for (int j = 0; j < I.nsys; ++j) {
+                partials[((size_t)t * kMaxRhs + j) * 2 + 0] = part[j][0];
+                partials[((size_t)t * kMaxRhs + j) * 2 + 1] = part[j][1];
+            }
+        if (image_arrive(&counters[im], I.ntiles)) {
+            for (int j = 0; j < I.nsys; ++j) {
+                if (!((mask >> j) & 1u)) continue;
+                const double rz = image_sum(partials, I.tile0, I.ntiles, 2 * kMaxRhs, 2 * j);
+                const double rr = image_sum(partials, I.tile0, I.ntiles, 2 * kMaxRhs, 2 * j + 1);
+                if (threadIdx.x == 0) {
+                    Scalars* s = &sc[I.sys0 + j];
+                    s->iters += 1;
+                    if (!isfinite(rz) || !isfinite(rr)) {
+                        s->done = kNonFinite;
+                    } else {
+                        s->rz_prev = s->rz;
+                        s->beta = (float)(rz / s->rz);
+                        s->rz = rz;
+                        s->rr = rr;
+                        if (s->tol2 >= 0 && rr <= s->tol2) {
+                            if (verify) s->verify = 1;
+                            else s->done = kConverged;
+                        } else if (s->iters >= s->maxit) {
+                            s->done = kMaxIt;
+                        }
+                    }
+                }
+            }
+            if (threadIdx.x == 0) counters[im] = 0u;
+        }
+    }
+}
+
+// FP64-accumulated true residual t = (L_off + diag d) x - b per system.
+// op 0: measure (g0 = ||t||^2) for every system; 1: replace r for running
+// systems; 2: start (all systems, beta = 0); 3: replace r for systems whose
+// recursive residual asked for verification.
+__global__ __launch_bounds__(kBlock) void kb_trueres(const BImg* __restrict__ imgs, const BSys* __restrict__ sys,
+                                                     Scalars* sc, const int* __restrict__ tile_img, int ntot, int op,
+                                                     double* partials, unsigned int* counters) {
+    constexpr int R = kBR;
+    constexpr int SEG = Tile<R>::SEG;
+    __shared__ __attribute__((aligned(16))) float lds[Tile<R>::LDS_FLOATS];
+    __shared__ double part[kMaxRhs][2];
+    const TileRange tr = xcd_tiles(ntot);
+    for (int t = tr.begin; t < tr.end; t += tr.step) {
+        const int im = tile_img[t];
+        const BImg I = imgs[im];
+        unsigned mask = 0;
+        for (int j = 0; j < I.nsys; ++j) {
+            const Scalars* s = &sc[I.sys0 + j];
+            const bool sel = op == 0 || op == 2 || (op == 1 && s->done == kRun) ||
+                             (op == 3 && s->done == kRun && s->verify);
+            if (sel) mask |= 1u << j;
+        }
+        if (mask == 0) continue;
+        const int i0 = (t - I.tile0) * Tile<R>::ROWS;
+        const int ib = i0 + R * threadIdx.x;
+        Bands<R, kF32> av;
+        if (ib < I.N) load_bands<R, kF32, kDiagCaller>(I.L, I.pitch, ib, av);
+        for (int j = 0; j < I.nsys; ++j) {
+            if (!((mask >> j) & 1u)) {
+                if (threadIdx.x == 0) part[j][0] = part[j][1] = 0;
+                continue;
+            }
+            const BSys S = sys[I.sys0 + j];
+            __syncthreads();
+            stage5<R>(lds, i0, I.W, I.N, [&](int g) { return S.x[g]; });
+            __syncthreads();
+            double v0 = 0, v1 = 0;
+            if (ib < I.N) {
+                const float* xs = lds + R * threadIdx.x;
+                double acc[R];
+#pragma unroll
+                for (int q = 0; q < R; ++q) acc[q] = 0.0;
+#pragma unroll
+                for (int k = 0; k < 25; ++k) {
+                    if (k == 12) continue;
+                    const int s5 = k / 5, dx = k % 5;
+#pragma unroll
+                    for (int q = 0; q < R; ++q)
+                        acc[q] = fma((double)av.v[k].get(q), (double)xs[s5 * SEG + q + dx], acc[q]);
+                }
+                float dd[R], bb[R], di[R], out[R];
+                load_rows<R>(S.d, ib, I.N, dd);
+                load_rows<R>(S.b, ib, I.N, bb);
+                load_rows<R>(S.dinv, ib, I.N, di);
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    const double tq = fma((double)dd[q], (double)xs[2 * SEG + q + 2], acc[q]) - (double)bb[q];
+                    out[q] = (float)tq;
+                    if (ib + q < I.N) {
+                        v1 += tq * tq;
+                        v0 += (double)(out[q] * (di[q] * out[q]));
+                    }
+                }
+                if (op != 0) store_rows<R>(S.r, ib, I.N, out);
+            }
+            const double b0 = block_sum(v0);
+            const double b1 = block_sum(v1);
+            if (threadIdx.x == 0) {
+                part[j][0] = b0;
+                part[j][1] = b1;
+            }
+        }
+        if (threadIdx.x == 0)
+            for (int j = 0; j < I.nsys; ++j) {
+                partials[((size_t)t * kMaxRhs + j) * 2 + 0] = part[j][0];
+                partials[((size_t)t * kMaxRhs + j) * 2 + 1] = part[j][1];
+            }
+        if (image_arrive(&counters[im], I.ntiles)) {
+            for (int j = 0; j < I.nsys; ++j) {
+                if (!((mask >> j) & 1u)) continue;
+                const double rz = image_sum(partials, I.tile0, I.ntiles, 2 * kMaxRhs, 2 * j);
+                const double rr = image_sum(partials, I.tile0, I.ntiles, 2 * kMaxRhs, 2 * j + 1);
+                if (threadIdx.x == 0) {
+                    Scalars* s = &sc[I.sys0 + j];
+                    s->g0 = rr;
+                    if (op != 0) {
+                        if (!isfinite(rz) || !isfinite(rr)) {
+                            s->done = kNonFinite;
+                        } else {
+                            if (op == 2) {
+                                s->beta = 0.f;
+                            } else if (s->verify) {
+                                // drifted recursion: restart from the true residual
+                                s->beta = 0.f;
+                                s->replacements += 1;
+                                if (s->g1 > 0 && rr > 0.25 * s->g1) s->stall += 1;
+                                else s->stall = 0;
+                                s->g1 = rr;
+                            } else {
+                                s->beta = s->rz_prev > 0 ? (float)(rz / s->rz_prev) : 0.f;
+                                s->replacements += 1;
+                            }
+                            s->rz = rz;
+                            s->rr = rr;
+                            s->verify = 0;
+                            if (s->tol2 >= 0 && rr <= s->tol2) s->done = kConverged;
+                            else if (s->iters >= s->maxit || s->stall >= 3) s->done = kMaxIt;
+                        }
+                    }
+                }
+            }
+            if (threadIdx.x == 0) counters[im] = 0u;
+        }
+    }
+}
+
+// per system: d = L_ii + lam, dinv = 1/d (0 where d == 0), b = lam*tgt, x = x0 or 0
+__global__ __launch_bounds__(kBlock) void kb_set_rhs(const float* __restrict__ Ldiag, int N,
+                                                     const float* __restrict__ lam, const float* __restrict__ tgt,
+                                                     const float* __restrict__ x0, float* __restrict__ d,
+                                                     float* __restrict__ dinv, float* __restrict__ b,
+                                                     float* __restrict__ x) {
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < N; i += gridDim.x * kBlock) {
+        const float l = lam ? lam[i] : 0.f;
+        float di = Ldiag[i];
+        float bi = 0.f;
+        if (l != 0.f) {
+            di += l;
+            bi = l * (tgt ? tgt[i] : 0.f);
+        }
+        d[i] = di;
+        dinv[i] = di != 0.f ? 1.f / di : 0.f;
+        b[i] = bi;
+        x[i] = x0 ? x0[i] : 0.f;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void kb_to16(const float* __restrict__ L, size_t pitch, int N,
+                                                  unsigned short* __restrict__ L16, size_t pitch16) {
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < N; i += gridDim.x * kBlock)
+#pragma unroll
+        for (int k = 0; k < 25; ++k)
+            if (k != 12) L16[(size_t)k * pitch16 + i] = f32_to_f16_rne(L[(size_t)k * pitch + i]);
+}
+
+__global__ void kb_init(Scalars* sc, int nsys, int maxit, double tol) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nsys) return;
+    Scalars z = {};
+    z.maxit = maxit;
+    z.tol2 = -1.0;
+    sc[j] = z;
+    (void)tol;
+}
+
+// per system <b,b> -> tol2 (one workgroup per system)
+__global__ __launch_bounds__(kBlock) void kb_bnorm(const BSys* __restrict__ sys, const int* __restrict__ sysN,
+                                                   Scalars* sc, double tol) {
+    const int j = blockIdx.x;
+    const BSys S = sys[j];
+    const int N = sysN[j];
+    double v = 0;
+    for (int i = threadIdx.x; i < N; i += kBlock) v += (double)S.b[i] * (double)S.b[i];
+    const double bb = block_sum(v);
+    if (threadIdx.x == 0) {
+        sc[j].bb = bb;
+        sc[j].tol2 = tol > 0 ? tol * tol * (bb > 0 ? bb : 1.0) : -1.0;
+    }
+}
+
+}  // namespace gm
+
+using namespace gm;
+
+struct gm_batch_s {
+    int nimg = 0, nrhs = 0, nsys = 0, precision = 0, ntiles = 0;
+    std::vector<BImg> himg;
+    std::vector<BSys> hsys;
+    std::vector<int> hsysN;
+    BImg* dimg = nullptr;
+    BSys* dsys = nullptr;
+    int* dsysN = nullptr;
+    Scalars* dsc = nullptr;
+    Scalars* hsc = nullptr;  // pinned, nsys
+    int* dtile = nullptr;
+    double* dpart = nullptr;
+    unsigned int* dcnt = nullptr;
+    float* Lmem = nullptr;
+    unsigned short* L16mem = nullptr;
+    float* vmem = nullptr;
+    hipStream_t cap = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int graph_iters = 0, graph_grid = 0, graph_verify = -1;
+    hipEvent_t ev = nullptr;
+    int per_cu = 2;
+};
+
+static void batch_free(gm_batch* b) {
+    if (!b) return;
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->cap) (void)hipStreamDestroy(b->cap);
+    if (b->ev) (void)hipEventDestroy(b->ev);
+    for (void* p : {(void*)b->dimg, (void*)b->dsys, (void*)b->dsysN, (void*)b->dsc, (void*)b->dtile, (void*)b->dpart,
+                    (void*)b->Lmem, (void*)b->L16mem, (void*)b->vmem})
+        if (p) (void)hipFree(p);
+    if (b->hsc) (void)hipHostFree(b->hsc);
+    delete b;
+}
+
+extern "C" {
+
+gm_status gm_batch_create(gm_batch** out, int nimages, const int* widths, const int* heights, int nrhs, int precision) {
+    if (!out || nimages <= 0 || !widths || !heights || nrhs <= 0 || nrhs > kMaxRhs)
+        return fail(GM_ERR_INVALID, "gm_batch_create: bad arguments (nrhs must be 1..%d)", kMaxRhs);
+    if (precision != GM_PREC_FP32 && precision != GM_PREC_FP16)
+        return fail(GM_ERR_UNSUPPORTED, "gm_batch_create: batched forms are FP32 and FP16");
+    gm_batch* b = new gm_batch();
+    b->nimg = nimages;
+    b->nrhs = nrhs;
+    b->nsys = nimages * nrhs;
+    b->precision = precision;
+    size_t Lfloats = 0, L16n = 0, vfloats = 0;
+    int tiles = 0;
+    std::vector<size_t> loff(nimages), l16off(nimages), voff(nimages);
+    for (int i = 0; i < nimages; ++i) {
+        const int W = widths[i], H = heights[i];
+        if (W < 1 || H < 1 || (long long)W * H > (1LL << 30)) {
+            batch_free(b);
+            return fail(GM_ERR_INVALID, "gm_batch_create: image %d is %dx%d", i, W, H);
+        }
+        const int N = W * H;
+        const size_t pitch = (size_t)(N + 63) / 64 * 64;
+        BImg I{};
+        I.W = W;
+        I.N = N;
+        I.ntiles = (N + Tile<kBR>::ROWS - 1) / Tile<kBR>::ROWS;
+        I.tile0 = tiles;
+        I.sys0 = i * nrhs;
+        I.nsys = nrhs;
+        I.pitch = pitch;
+        I.pitch16 = pitch;
+        tiles += I.ntiles;
+        loff[i] = Lfloats;
+        Lfloats += 25 * pitch;
+        l16off[i] = L16n;
+        if (precision == GM_PREC_FP16) L16n += 25 * pitch;
+        voff[i] = vfloats;
+        vfloats += (size_t)nrhs * 8 * pitch;
+        b->himg.push_back(I);
+    }
+    b->ntiles = tiles;
+    hipError_t e = hipSuccess;
+#define BTRY(x)                                                        \
+    do {                                                               \
+        e = (x);                                                       \
+        if (e != hipSuccess) {                                         \
+            batch_free(b);                                             \
+            return hip_fail(e, #x, __FILE__, __LINE__);                \
+        }                                                              \
+    } while (0)
+    BTRY(hipMalloc(&b->Lmem, sizeof(float) * Lfloats + 256));
+    BTRY(hipMemset(b->Lmem, 0, sizeof(float) * Lfloats + 256));
+    if (L16n) {
+        BTRY(hipMalloc(&b->L16mem, sizeof(unsigned short) * L16n + 256));
+        BTRY(hipMemset(b->L16mem, 0, sizeof(unsigned short) * L16n + 256));
+    }
+    BTRY(hipMalloc(&b->vmem, sizeof(float) * vfloats + 256));
+    BTRY(hipMemset(b->vmem, 0, sizeof(float) * vfloats + 256));
+    for (int i = 0; i < nimages; ++i) {
+        BImg& I = b->himg[i];
+        I.L = b->Lmem + loff[i];
+        I.L16 = L16n ? b->L16mem + l16off[i] : nullptr;
+        for (int j = 0; j < nrhs; ++j) {
+            float* base = b->vmem + voff[i] + (size_t)j * 8 * I.pitch;
+            BSys S;
+            S.x = base;
+            S.r = base + 1 * I.pitch;
+            S.p0 = base + 2 * I.pitch;
+            S.p1 = base + 3 * I.pitch;
+            S.Ap = base + 4 * I.pitch;
+            S.d = base + 5 * I.pitch;
+            S.dinv = base + 6 * I.pitch;
+            S.b = base + 7 * I.pitch;
+            b->hsys.push_back(S);
+            b->hsysN.push_back(I.N);
+        }
+    }
+    std::vector<int> tile_img(tiles);
+    for (int i = 0; i < nimages; ++i)
+        for (int k = 0; k < b->himg[i].ntiles; ++k) tile_img[b->himg[i].tile0 + k] = i;
+    BTRY(hipMalloc(&b->dimg, sizeof(BImg) * nimages));
+    BTRY(hipMalloc(&b->dsys, sizeof(BSys) * b->nsys));
+    BTRY(hipMalloc(&b->dsysN, sizeof(int) * b->nsys));
+    BTRY(hipMalloc(&b->dsc, sizeof(Scalars) * b->nsys));
+    BTRY(hipMemset(b->dsc, 0, sizeof(Scalars) * b->nsys));
+    BTRY(hipMalloc(&b->dtile, sizeof(int) * tiles));
+    const size_t nparts = (size_t)tiles * kMaxRhs * 2;
+    BTRY(hipMalloc(&b->dpart, sizeof(double) * nparts + sizeof(unsigned int) * nimages + 256));
+    BTRY(hipMemset(b->dpart, 0, sizeof(double) * nparts + sizeof(unsigned int) * nimages + 256));
+    b->dcnt = reinterpret_cast<unsigned int*>(b->dpart + nparts);
+    BTRY(hipMemcpy(b->dimg, b->himg.data(), sizeof(BImg) * nimages, hipMemcpyHostToDevice));
+    BTRY(hipMemcpy(b->dsys, b->hsys.data(), sizeof(BSys) * b->nsys, hipMemcpyHostToDevice));
+    BTRY(hipMemcpy(b->dsysN, b->hsysN.data(), sizeof(int) * b->nsys, hipMemcpyHostToDevice));
+    BTRY(hipMemcpy(b->dtile, tile_img.data(), sizeof(int) * tiles, hipMemcpyHostToDevice));
+    BTRY(hipHostMalloc(&b->hsc, sizeof(Scalars) * b->nsys, hipHostMallocDefault));
+    BTRY(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
+    BTRY(hipEventCreateWithFlags(&b->ev, hipEventDisableTiming));
+#undef BTRY
+    *out = b;
+    return GM_OK;
+}
+
+gm_status gm_batch_destroy(gm_batch* b) {
+    batch_free(b);
+    return GM_OK;
+}
+
+gm_status gm_batch_assemble(gm_batch* b, int image, const float* im, int nch, size_t im_pitch, float eps,
+                            gm_stream_t s) {
+    if (!b || image < 0 || image >= b->nimg) return fail(GM_ERR_INVALID, "gm_batch_assemble: bad image index");
+    const BImg& I = b->himg[image];
+    const int H = I.N / I.W;
+    gm_status st = levin_assemble_raw(const_cast<float*>(I.L), I.pitch, nullptr, nullptr, im, nch, im_pitch, nullptr,
+                                      nullptr, nullptr, 0.f, eps, GM_ASSEMBLY_ACCURATE, I.W, H, s);
+    if (st != GM_OK) return st;
+    if (b->precision == GM_PREC_FP16) {
+        DevInfo d;
+        if ((st = dev_info(&d)) != GM_OK) return st;
+        hipLaunchKernelGGL(kb_to16, dim3(elementwise_grid(d, I.N, 1)), dim3(kBlock), 0, (hipStream_t)s, I.L, I.pitch,
+                           I.N, const_cast<unsigned short*>(I.L16), I.pitch16);
+        GM_LAUNCH_CHECK();
+    }
+    return GM_OK;
+}
+
+gm_status gm_batch_set_rhs(gm_batch* b, int image, int rhs, const float* lam, const float* tgt, const float* x0,
+                           gm_stream_t s) {
+    if (!b || image < 0 || image >= b->nimg || rhs < 0 || rhs >= b->nrhs)
+        return fail(GM_ERR_INVALID, "gm_batch_set_rhs: bad index");
+    const BImg& I = b->himg[image];
+    const BSys& S = b->hsys[(size_t)image * b->nrhs + rhs];
+    DevInfo d;
+    gm_status st = dev_info(&d);
+    if (st != GM_OK) return st;
+    hipLaunchKernelGGL(kb_set_rhs, dim3(elementwise_grid(d, I.N, 1)), dim3(kBlock), 0, (hipStream_t)s,
+                       I.L + 12 * I.pitch, I.N, lam, tgt, x0, S.d, S.dinv, S.b, S.x);
+    GM_LAUNCH_CHECK();
+    return GM_OK;
+}
+
+gm_status gm_batch_alpha(gm_batch* b, int image, int rhs, float** alpha) {
+    if (!b || !alpha || image < 0 || image >= b->nimg || rhs < 0 || rhs >= b->nrhs)
+        return fail(GM_ERR_INVALID, "gm_batch_alpha: bad index");
+    *alpha = b->hsys[(size_t)image * b->nrhs + rhs].x;
+    return GM_OK;
+}
+
+gm_status gm_batch_info(gm_batch* b, int image, int rhs, gm_solve_info* info) {
+    if (!b || !info || image < 0 || image >= b->nimg || rhs < 0 || rhs >= b->nrhs)
+        return fail(GM_ERR_INVALID, "gm_batch_info: bad index");
+    const Scalars& h = b->hsc[(size_t)image * b->nrhs + rhs];
+    const double bb = h.bb > 0 ? h.bb : 1.0;
+    info->iterations = h.iters;
+    info->status = h.done == kConverged ? GM_OK
+                 : h.done == kBreakdown ? GM_ERR_BREAKDOWN
+                 : h.done == kNonFinite ? GM_ERR_NONFINITE : GM_ERR_NOT_CONVERGED;
+    info->relres = sqrt(h.rr / bb);
+    info->true_relres = h.g0 >= 0 ? sqrt(h.g0 / bb) : -1;
+    info->replacements = h.replacements;
+    return GM_OK;
+}
+
+static gm_status batch_launch_iter(gm_batch* b, int grid, int verify, hipStream_t hs) {
+    if (b->precision == GM_PREC_FP16)
+        hipLaunchKernelGGL(kb_cgstep<kF16>, dim3(grid), dim3(kBlock), 0, hs, b->dimg, b->dsys, b->dsc, b->dtile,
+                           b->ntiles, b->dpart, b->dcnt);
+    else
+        hipLaunchKernelGGL(kb_cgstep<kF32>, dim3(grid), dim3(kBlock), 0, hs, b->dimg, b->dsys, b->dsc, b->dtile,
+                           b->ntiles, b->dpart, b->dcnt);
+    GM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(kb_cgupdate, dim3(grid), dim3(kBlock), 0, hs, b->dimg, b->dsys, b->dsc, b->dtile, b->ntiles,
+                       verify, b->dpart, b->dcnt);
+    GM_LAUNCH_CHECK();
+    return GM_OK;
+}
+
+static gm_status batch_trueres(gm_batch* b, int grid, int op, hipStream_t hs) {
+    hipLaunchKernelGGL(kb_trueres, dim3(grid), dim3(kBlock), 0, hs, b->dimg, b->dsys, b->dsc, b->dtile, b->ntiles, op,
+                       b->dpart, b->dcnt);
+    GM_LAUNCH_CHECK();
+    return GM_OK;
+}
+
+// Capture `iters` iterations (K1, K2 pairs) into a graph on the batch's own
+// stream (capture needs a non-null stream); replayed on the caller's stream.
+static gm_status batch_graph(gm_batch* b, int iters, int grid, int verify) {
+    if (b->gexec && b->graph_iters == iters && b->graph_grid == grid && b->graph_verify == verify) return GM_OK;
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+    GM_HIP(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+    gm_status st = GM_OK;
+    for (int k = 0; k < iters && st == GM_OK; ++k) st = batch_launch_iter(b, grid, verify, b->cap);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->cap, &g);
+    if (st != GM_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+    b->graph = g;
+    GM_HIP(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+    b->graph_iters = iters;
+    b->graph_grid = grid;
+    b->graph_verify = verify;
+    return GM_OK;
+}
+
+gm_status gm_batch_solve(gm_batch* b, const gm_solve_opts* opts, gm_stream_t s) {
+    if (!b) return fail(GM_ERR_INVALID, "gm_batch_solve: null batch");
+    gm_solve_opts o;
+    if (opts) o = *opts;
+    else gm_solve_opts_default(&o);
+    if (o.method != GM_METHOD_PCG) return fail(GM_ERR_UNSUPPORTED, "gm_batch_solve: batched solver is Jacobi-PCG");
+    if (o.max_iter <= 0) return fail(GM_ERR_INVALID, "gm_batch_solve: max_iter");
+    DevInfo d;
+    gm_status st = dev_info(&d);
+    if (st != GM_OK) return st;
+    hipStream_t hs = (hipStream_t)s;
+    const int grid = grid_for_tiles(d, b->ntiles, b->per_cu);
+    const int verify = (o.tol > 0 && o.verify) ? 1 : 0;
+    const int chunk = o.check_every > 0 ? o.check_every : 10;
+    const bool mixed = b->precision == GM_PREC_FP16;
+    int refine = o.refine_every > 0 ? o.refine_every : (o.refine_every == 0 && mixed ? 50 : 0);
+    if (refine > 0 && refine % chunk) refine = (refine / chunk + 1) * chunk;  // replacement at chunk boundaries
+
+    hipLaunchKernelGGL(kb_init, dim3((b->nsys + 63) / 64), dim3(64), 0, hs, b->dsc, b->nsys, o.max_iter, o.tol);
+    GM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(kb_bnorm, dim3(b->nsys), dim3(kBlock), 0, hs, b->dsys, b->dsysN, b->dsc, o.tol);
+    GM_LAUNCH_CHECK();
+    if ((st = batch_trueres(b, grid, 2, hs)) != GM_OK) return st;
+    if ((st = batch_graph(b, chunk, grid, verify)) != GM_OK) return st;
+
+    long enq = 0;
+    bool finished = false;
+    while (!finished) {
+        GM_HIP(hipGraphLaunch(b->gexec, hs));
+        enq += chunk;
+        if (refine > 0 && enq % refine == 0) {
+            if ((st = batch_trueres(b, grid, 1, hs)) != GM_OK) return st;
+        }
+        GM_HIP(hipMemcpyAsync(b->hsc, b->dsc, sizeof(Scalars) * b->nsys, hipMemcpyDeviceToHost, hs));
+        GM_HIP(hipEventRecord(b->ev, hs));
+        GM_HIP(hipEventSynchronize(b->ev));
+        int running = 0, verifying = 0;
+        for (int j = 0; j < b->nsys; ++j) {
+            if (b->hsc[j].done == kRun) {
+                ++running;
+                if (b->hsc[j].verify) ++verifying;
+            }
+        }
+        if (running == 0 || enq >= o.max_iter) finished = true;
+        if (verifying > 0) {
+            if ((st = batch_trueres(b, grid, 3, hs)) != GM_OK) return st;
+            if (finished && enq >= o.max_iter) {
+                // the last replacement may have converged them; nothing more to run
+            }
+        }
+    }
+    // final FP64 true residual of every system (g0), then a host copy of all scalars
+    if ((st = batch_trueres(b, grid, 0, hs)) != GM_OK) return st;
+    GM_HIP(hipMemcpyAsync(b->hsc, b->dsc, sizeof(Scalars) * b->nsys, hipMemcpyDeviceToHost, hs));
+    GM_HIP(hipStreamSynchronize(hs));
+    int bad = 0;
+    for (int j = 0; j < b->nsys; ++j)
+        if (b->hsc[j].done != kConverged) ++bad;
+    if (bad && o.tol > 0) return fail(GM_ERR_NOT_CONVERGED, "gm_batch_solve: %d of %d systems did not converge", bad,
+                                      b->nsys);
+    return GM_OK;
+}
+
+gm_status gm_batch_set_grid(gm_batch* b, int per_cu) {
+    if (!b || per_cu < 1 || per_cu > 16) return fail(GM_ERR_INVALID, "gm_batch_set_grid: per_cu 1..16");
+    b->per_cu = per_cu;
+    return GM_OK;
+}
+
+int gm_batch_tiles(gm_batch* b) { return b ? b->ntiles : 0; }
+
+}  // extern "C"

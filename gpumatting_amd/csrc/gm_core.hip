@@ -46,6 +46,10 @@ void make_plan(const int* hb, int nb, BandPlan* p) {
     p->args.nb = nb;
     p->args.center = -1;
     p->maxoff = 0;
+    // the diagonal band: the reference's convention nbands/2 (Solve.cpp:10)
+    // when its offset is 0, else the first band with offset 0 (offsets of the
+    // 5x5 layout alias for W < 5)
+    if (nb > 0 && nb <= kMaxBands && hb[nb / 2] == 0) p->args.center = nb / 2;
     for (int k = 0; k < nb && k < kMaxBands; ++k) {
         p->args.off[k] = hb[k];
         if (hb[k] == 0 && p->args.center < 0) p->args.center = k;

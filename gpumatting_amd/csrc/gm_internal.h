@@ -100,7 +100,7 @@ struct Scalars {
     int iters;       // iterations completed
     int maxit;
     int replacements;
-    int pad_;
+    int stall;       // verifications in a row that failed to reduce the true residual 4x
     unsigned int counter;  // arrival counter of the last-block reduction
     unsigned int pad2_;
 };
@@ -237,8 +237,10 @@ template <int R, int FMT>
 struct Raw;
 template <int R>
 struct Raw<R, kF32> {
-    f32x4 d[R / 4];
-    __device__ __forceinline__ float get(int j) const { return d[j / 4][j % 4]; }
+    static constexpr int V = R >= 4 ? 4 : R;  // floats per load
+    typedef float vec __attribute__((ext_vector_type(V)));
+    vec d[R / V];
+    __device__ __forceinline__ float get(int j) const { return d[j / V][j % V]; }
 };
 template <int R>
 struct Raw<R, kF16> {
@@ -266,9 +268,11 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 template <int R, int FMT>
 __device__ __forceinline__ void load_band(const void* base, size_t i, Raw<R, FMT>& out) {
     if constexpr (FMT == kF32) {
-        const f32x4* p = reinterpret_cast<const f32x4*>(reinterpret_cast<const float*>(base) + i);
+        typedef typename Raw<R, kF32>::vec vec;
+        constexpr int V = Raw<R, kF32>::V;
+        const vec* p = reinterpret_cast<const vec*>(reinterpret_cast<const float*>(base) + i);
 #pragma unroll
-        for (int q = 0; q < R / 4; ++q) out.d[q] = __builtin_nontemporal_load(p + q);
+        for (int q = 0; q < R / V; ++q) out.d[q] = __builtin_nontemporal_load(p + q);
     } else if constexpr (R % 8 == 0) {
         const u32x4* p = reinterpret_cast<const u32x4*>(reinterpret_cast<const unsigned short*>(base) + i);
 #pragma unroll
@@ -291,7 +295,7 @@ __device__ __forceinline__ void load_band(const void* base, size_t i, Raw<R, FMT
 template <int R>
 __device__ __forceinline__ void store_rows(float* v, int i, int n, const float (&y)[R]) {
     float* p = v + i;
-    if (i + R <= n && ((reinterpret_cast<uintptr_t>(p) & 15u) == 0)) {
+    if (R % 4 == 0 && i + R <= n && ((reinterpret_cast<uintptr_t>(p) & 15u) == 0)) {
 #pragma unroll
         for (int q = 0; q < R / 4; ++q) {
             f32x4 t = {y[4 * q], y[4 * q + 1], y[4 * q + 2], y[4 * q + 3]};
@@ -307,7 +311,7 @@ __device__ __forceinline__ void store_rows(float* v, int i, int n, const float (
 template <int R>
 __device__ __forceinline__ void load_rows(const float* v, int i, int n, float (&y)[R]) {
     const float* p = v + i;
-    if (i + R <= n && ((reinterpret_cast<uintptr_t>(p) & 15u) == 0)) {
+    if (R % 4 == 0 && i + R <= n && ((reinterpret_cast<uintptr_t>(p) & 15u) == 0)) {
 #pragma unroll
         for (int q = 0; q < R / 4; ++q) {
             f32x4 t = *reinterpret_cast<const f32x4*>(p + 4 * q);
@@ -346,40 +350,70 @@ __device__ __forceinline__ void stage5(float* __restrict__ lds, int i0, int W, i
     }
 }
 
-// y[j] = sum_k A_k[i+j] * X(i+j+off_k), k = 0..24 in band order (FMA).
-// UNIT: band 12 is an implicit 1 (Jacobi-scaled storage) and is not loaded.
-template <int R, int FMT, bool UNIT>
-__device__ __forceinline__ void tile_rows(const void* __restrict__ A, size_t pitch, int ib,
-                                          const float* __restrict__ xs, float (&y)[R]) {
+// The 25 (24 with UNIT) band chunks of rows ib .. ib+R-1, loaded before the x
+// window is staged so that their HBM latency overlaps the staging.
+template <int R, int FMT>
+struct Bands {
+    Raw<R, FMT> v[25];
+};
+
+// DIAG: kDiagStored (band 12 loaded), kDiagUnit (implicit 1), kDiagCaller
+// (band 12 skipped; the caller adds its own diagonal term).
+enum { kDiagStored = 0, kDiagUnit = 1, kDiagCaller = 2 };
+
+template <int R, int FMT, int DIAG>
+__device__ __forceinline__ void load_bands(const void* __restrict__ A, size_t pitch, int ib, Bands<R, FMT>& b) {
+#pragma unroll
+    for (int k = 0; k < 25; ++k) {
+        if (DIAG != kDiagStored && k == 12) continue;
+        load_band<R, FMT>(A, (size_t)k * pitch + (size_t)ib, b.v[k]);
+    }
+}
+
+// y[j] = sum_k A_k[i+j] * X(i+j+off_k), k = 0..24 in band order (FMA), with X
+// read from the staged LDS window (band 12 per DIAG).
+template <int R, int FMT, int DIAG>
+__device__ __forceinline__ void apply_bands(const Bands<R, FMT>& av, const float* __restrict__ xs, float (&y)[R]) {
     constexpr int SEG = Tile<R>::SEG;
 #pragma unroll
     for (int j = 0; j < R; ++j) y[j] = 0.f;
-    Raw<R, FMT> av[25];
-#pragma unroll
-    for (int k = 0; k < 25; ++k) {
-        if (UNIT && k == 12) continue;
-        load_band<R, FMT>(A, (size_t)k * pitch + (size_t)ib, av[k]);
-    }
 #pragma unroll
     for (int s = 0; s < 5; ++s) {
         float xw[R + 4];
+        if constexpr (R % 4 == 0) {
 #pragma unroll
-        for (int q = 0; q < (R + 4) / 4; ++q) {
-            f32x4 t = *reinterpret_cast<const f32x4*>(xs + s * SEG + 4 * q);
-            xw[4 * q] = t.x; xw[4 * q + 1] = t.y; xw[4 * q + 2] = t.z; xw[4 * q + 3] = t.w;
+            for (int q = 0; q < (R + 4) / 4; ++q) {
+                f32x4 t = *reinterpret_cast<const f32x4*>(xs + s * SEG + 4 * q);
+                xw[4 * q] = t.x; xw[4 * q + 1] = t.y; xw[4 * q + 2] = t.z; xw[4 * q + 3] = t.w;
+            }
+        } else {  // R = 2: the thread's window starts 8-byte aligned
+#pragma unroll
+            for (int q = 0; q < (R + 4) / 2; ++q) {
+                f32x2 t = *reinterpret_cast<const f32x2*>(xs + s * SEG + 2 * q);
+                xw[2 * q] = t.x; xw[2 * q + 1] = t.y;
+            }
         }
 #pragma unroll
         for (int dx = 0; dx < 5; ++dx) {
             const int k = 5 * s + dx;
 #pragma unroll
             for (int j = 0; j < R; ++j) {
-                if (UNIT && k == 12)
+                if (DIAG == kDiagUnit && k == 12)
                     y[j] += xw[j + dx];
-                else
-                    y[j] = fmaf(av[k].get(j), xw[j + dx], y[j]);
+                else if (!(DIAG == kDiagCaller && k == 12))
+                    y[j] = fmaf(av.v[k].get(j), xw[j + dx], y[j]);
             }
         }
     }
+}
+
+// Loads and applies in one go (callers with nothing to overlap).
+template <int R, int FMT, int DIAG>
+__device__ __forceinline__ void tile_rows(const void* __restrict__ A, size_t pitch, int ib,
+                                          const float* __restrict__ xs, float (&y)[R]) {
+    Bands<R, FMT> av;
+    load_bands<R, FMT, DIAG>(A, pitch, ib, av);
+    apply_bands<R, FMT, DIAG>(av, xs, y);
 }
 
 }  // namespace gm

@@ -212,7 +212,7 @@ __global__ __launch_bounds__(kBlock) void k_levin(const float* __restrict__ im, 
     }
 #pragma unroll
     for (int k = 0; k < 25; ++k) A[(size_t)k * apitch + i] = acc[k];
-    bvec[i] = bi;
+    if (bvec) bvec[i] = bi;
     if (dinv) dinv[i] = acc[12] != 0.f ? 1.f / acc[12] : 0.f;
 }
 
@@ -364,6 +364,30 @@ __global__ __launch_bounds__(kBlock) void k_levin_backward(const float* __restri
     gim[3 * i + 2] = (float)g2;
     if (glam) glam[i] = (float)(mp * ((tgt ? (double)tgt[i] : 0.0) - ap));
     if (gtgt) gtgt[i] = (float)(mp * (lam ? (double)lam[i] : 0.0));
+}
+
+// Assembly into a raw 25-band array (band k at A + k*apitch, rows W*H in raster
+// order).  b and dinv may be NULL.
+gm_status levin_assemble_raw(float* A, size_t apitch, float* b, float* dinv, const float* im, int nch, size_t pitch,
+                             const float* lam, const float* tgt, const unsigned char* scr, float gamma, float eps,
+                             int mode, int W, int H, gm_stream_t s) {
+    if (!A || !im) return fail(GM_ERR_INVALID, "levin: null pointer");
+    if (W < 1 || H < 1) return fail(GM_ERR_INVALID, "levin: image %dx%d", W, H);
+    if (nch != 3 && nch != 4) return fail(GM_ERR_INVALID, "levin: nch must be 3 or 4");
+    if (pitch < (size_t)W) return fail(GM_ERR_INVALID, "levin: pitch %zu < W %d", pitch, W);
+    if (apitch < (size_t)W * H) return fail(GM_ERR_INVALID, "levin: apitch < W*H");
+    if (mode != GM_ASSEMBLY_REFERENCE && mode != GM_ASSEMBLY_ACCURATE) return fail(GM_ERR_INVALID, "levin: mode");
+    if (!(eps >= 0.f)) return fail(GM_ERR_INVALID, "levin: eps must be >= 0");
+    dim3 grid((W + kTX - 1) / kTX, (H + kTY - 1) / kTY);
+    if (grid.y > 65535u) return fail(GM_ERR_UNSUPPORTED, "levin: image too tall");
+    if (mode == GM_ASSEMBLY_REFERENCE)
+        hipLaunchKernelGGL(k_levin<GM_ASSEMBLY_REFERENCE>, grid, dim3(kBlock), 0, (hipStream_t)s, im, nch, pitch, lam,
+                           tgt, scr, gamma, eps, W, H, A, apitch, b, dinv);
+    else
+        hipLaunchKernelGGL(k_levin<GM_ASSEMBLY_ACCURATE>, grid, dim3(kBlock), 0, (hipStream_t)s, im, nch, pitch, lam,
+                           tgt, scr, gamma, eps, W, H, A, apitch, b, dinv);
+    GM_LAUNCH_CHECK();
+    return GM_OK;
 }
 
 static gm_status levin_launch(BandedMatrix* L, float* b, float* dinv, const float* im, int nch, size_t pitch,

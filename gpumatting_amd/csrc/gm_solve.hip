@@ -46,8 +46,19 @@ __device__ __forceinline__ void fin_residual(Scalars* sc, double rz, double rr, 
         return;
     }
     if (replace && !start) {
-        sc->beta = sc->rz_prev > 0 ? (float)(rz / sc->rz_prev) : 0.f;
         sc->replacements += 1;
+        if (sc->verify) {
+            // The recursive residual reached the tolerance but the true one did
+            // not: the recursion has drifted, so restart from the true residual
+            // (beta = 0) rather than keep a direction built on the drifted one.
+            sc->beta = 0.f;
+            if (sc->g1 > 0 && rr > 0.25 * sc->g1) sc->stall += 1;
+            else sc->stall = 0;
+            sc->g1 = rr;
+        } else {
+            // periodic replacement: keep the search direction
+            sc->beta = sc->rz_prev > 0 ? (float)(rz / sc->rz_prev) : 0.f;
+        }
     } else {
         sc->beta = 0.f;
     }
@@ -55,7 +66,7 @@ __device__ __forceinline__ void fin_residual(Scalars* sc, double rz, double rr, 
     sc->rr = rr;
     sc->verify = 0;
     if (sc->tol2 >= 0 && rr <= sc->tol2) sc->done = kConverged;
-    else if (sc->iters >= sc->maxit) sc->done = kMaxIt;
+    else if (sc->iters >= sc->maxit || sc->stall >= 3) sc->done = kMaxIt;  // max_iter or stagnation
 }
 
 // ---------------------------------------------------------------------------
@@ -111,7 +122,7 @@ __global__ __launch_bounds__(kBlock) void k_resid25(const float* __restrict__ A,
         const int ib = i0 + R * threadIdx.x;
         if (ib < N) {
             float y[R], bb[R], dd[R];
-            tile_rows<R, kF32, false>(A, pitch, ib, lds + R * threadIdx.x, y);
+            tile_rows<R, kF32, kDiagStored>(A, pitch, ib, lds + R * threadIdx.x, y);
             load_rows<R>(b, ib, N, bb);
             if (PM == kJacobi) load_rows<R>(dinv, ib, N, dd);
 #pragma unroll
@@ -147,16 +158,18 @@ __global__ __launch_bounds__(kBlock) void k_cgstep25(const void* __restrict__ A,
     double v[1] = {0.0};
     for (int t = tr.begin; t < tr.end; t += tr.step) {
         const int i0 = t * Tile<R>::ROWS;
+        const int ib = i0 + R * threadIdx.x;
+        Bands<R, FMT> av;
+        if (ib < N) load_bands<R, FMT, PM == kScaled ? kDiagUnit : kDiagStored>(A, pitch, ib, av);
         __syncthreads();
         if (beta != 0.f)
             stage5<R>(lds, i0, W, N, [&](int g) { return fmaf(beta, pold[g], -zval<PM>(r, dinv, g)); });
         else
             stage5<R>(lds, i0, W, N, [&](int g) { return -zval<PM>(r, dinv, g); });
         __syncthreads();
-        const int ib = i0 + R * threadIdx.x;
         if (ib < N) {
             float y[R], pc[R];
-            tile_rows<R, FMT, PM == kScaled>(A, pitch, ib, lds + R * threadIdx.x, y);
+            apply_bands<R, FMT, PM == kScaled ? kDiagUnit : kDiagStored>(av, lds + R * threadIdx.x, y);
 #pragma unroll
             for (int j = 0; j < R; ++j) pc[j] = lds[2 * Tile<R>::SEG + R * threadIdx.x + 2 + j];
             store_rows<R>(pnew, ib, N, pc);
@@ -346,7 +359,7 @@ __global__ __launch_bounds__(kBlock) void k_gd1_25(const float* __restrict__ A, 
         const int ib = i0 + R * threadIdx.x;
         if (ib < N) {
             float y[R], bb[R];
-            tile_rows<R, kF32, false>(A, pitch, ib, lds + R * threadIdx.x, y);
+            tile_rows<R, kF32, kDiagStored>(A, pitch, ib, lds + R * threadIdx.x, y);
             load_rows<R>(b, ib, N, bb);
 #pragma unroll
             for (int j = 0; j < R; ++j) {
@@ -385,7 +398,7 @@ __global__ __launch_bounds__(kBlock) void k_gd2_25(const float* __restrict__ A, 
         const int ib = i0 + R * threadIdx.x;
         if (ib < N) {
             float y[R], xx[R];
-            tile_rows<R, kF32, false>(A, pitch, ib, lds + R * threadIdx.x, y);
+            tile_rows<R, kF32, kDiagStored>(A, pitch, ib, lds + R * threadIdx.x, y);
             load_rows<R>(x, ib, N, xx);
 #pragma unroll
             for (int j = 0; j < R; ++j)
@@ -546,6 +559,20 @@ __global__ __launch_bounds__(kBlock) void k_gd2_g(const float* __restrict__ a, s
     });
 }
 
+__global__ void k_tune_reset(Scalars* sc) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        sc->done = 0;
+        sc->verify = 0;
+        sc->iters = 0;
+        sc->beta = 0.5f;
+        sc->rz = 1.0;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill(float* v, int n, float val) {
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) v[i] = val;
+}
+
 __global__ __launch_bounds__(kBlock) void k_diag_inv(const float* __restrict__ diag, float* __restrict__ dinv, int N) {
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < N; i += gridDim.x * kBlock) {
         const float d = diag[i];
@@ -641,6 +668,30 @@ void release_workspaces() {
 // ---------------------------------------------------------------------------
 // host driver
 
+// ---------------------------------------------------------------------------
+// tile / grid choices of the fused CG step (K1) per format, filled by
+// gm_autotune and consulted by every solve; defaults when untuned.
+struct TuneKey {
+    int fmt, W;
+    long N;
+    bool operator<(const TuneKey& o) const {
+        return fmt != o.fmt ? fmt < o.fmt : (W != o.W ? W < o.W : N < o.N);
+    }
+};
+struct TuneVal {
+    int R, per_cu;
+    double us;
+};
+static std::mutex g_tune_mu;
+static std::map<TuneKey, TuneVal> g_tune;
+
+static TuneVal tune_lookup(int fmt, int W, long N) {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    auto it = g_tune.find(TuneKey{fmt, W, N});
+    if (it != g_tune.end()) return it->second;
+    return fmt == kF32 ? TuneVal{4, 4, 0} : TuneVal{8, 3, 0};
+}
+
 struct Solver {
     const BandedMatrix* A;
     BandPlan plan;
@@ -655,6 +706,7 @@ struct Solver {
     int pm;        // PMode
     int fmt;       // kF32 / kF16 / kBF16
     int g_tile4, g_tile8, g_elem, g_gen;
+    int stepR, g_step;  // tuned K1 shape
 };
 
 static gm_status launch_resid(Solver& S, int start) {
@@ -707,26 +759,30 @@ static gm_status launch_trueres(Solver& S, int op) {
     return GM_OK;
 }
 
-static gm_status launch_cg_iter(Solver& S, int verify) {
+template <int R, int FMT, int PM>
+static void launch_step_t(Solver& S, const void* A, size_t pitch) {
+    Workspace* w = S.w;
+    hipLaunchKernelGGL((k_cgstep25<R, FMT, PM>), dim3(S.g_step), dim3(kBlock), 0, S.hs, A, pitch, S.plan.W, S.N, w->r,
+                       w->dinv, w->p0, w->p1, w->Ap, w->sc, S.rs.partials, S.rs.counter);
+}
+
+static gm_status launch_step(Solver& S) {
     Workspace* w = S.w;
     if (S.tiled) {
         if (S.pm == kScaled) {
-            if (S.fmt == kF16)
-                hipLaunchKernelGGL((k_cgstep25<8, kF16, kScaled>), dim3(S.g_tile8), dim3(kBlock), 0, S.hs, w->A16,
-                                   w->pitch16, S.plan.W, S.N, w->r, w->dinv, w->p0, w->p1, w->Ap, w->sc, S.rs.partials,
-                                   S.rs.counter);
-            else
-                hipLaunchKernelGGL((k_cgstep25<8, kBF16, kScaled>), dim3(S.g_tile8), dim3(kBlock), 0, S.hs, w->A16,
-                                   w->pitch16, S.plan.W, S.N, w->r, w->dinv, w->p0, w->p1, w->Ap, w->sc, S.rs.partials,
-                                   S.rs.counter);
+            if (S.fmt == kF16) {
+                if (S.stepR == 4) launch_step_t<4, kF16, kScaled>(S, w->A16, w->pitch16);
+                else launch_step_t<8, kF16, kScaled>(S, w->A16, w->pitch16);
+            } else {
+                if (S.stepR == 4) launch_step_t<4, kBF16, kScaled>(S, w->A16, w->pitch16);
+                else launch_step_t<8, kBF16, kScaled>(S, w->A16, w->pitch16);
+            }
         } else if (S.pm == kJacobi) {
-            hipLaunchKernelGGL((k_cgstep25<4, kF32, kJacobi>), dim3(S.g_tile4), dim3(kBlock), 0, S.hs, S.A->a,
-                               S.A->apitch, S.plan.W, S.N, w->r, w->dinv, w->p0, w->p1, w->Ap, w->sc, S.rs.partials,
-                               S.rs.counter);
+            if (S.stepR == 2) launch_step_t<2, kF32, kJacobi>(S, S.A->a, S.A->apitch);
+            else launch_step_t<4, kF32, kJacobi>(S, S.A->a, S.A->apitch);
         } else {
-            hipLaunchKernelGGL((k_cgstep25<4, kF32, kPlain>), dim3(S.g_tile4), dim3(kBlock), 0, S.hs, S.A->a,
-                               S.A->apitch, S.plan.W, S.N, w->r, w->dinv, w->p0, w->p1, w->Ap, w->sc, S.rs.partials,
-                               S.rs.counter);
+            if (S.stepR == 2) launch_step_t<2, kF32, kPlain>(S, S.A->a, S.A->apitch);
+            else launch_step_t<4, kF32, kPlain>(S, S.A->a, S.A->apitch);
         }
     } else {
         if (S.pm == kJacobi)
@@ -739,6 +795,19 @@ static gm_status launch_cg_iter(Solver& S, int verify) {
                                S.rs.counter);
     }
     GM_LAUNCH_CHECK();
+    return GM_OK;
+}
+
+static void set_step_shape(Solver& S, int R, int per_cu) {
+    S.stepR = R;
+    const long rows = (long)kBlock * R;
+    S.g_step = grid_for_tiles(S.dv, (S.N + rows - 1) / rows, per_cu);
+}
+
+static gm_status launch_cg_iter(Solver& S, int verify) {
+    Workspace* w = S.w;
+    gm_status st = launch_step(S);
+    if (st != GM_OK) return st;
     if (S.pm == kJacobi)
         hipLaunchKernelGGL(k_cgupdate<kJacobi>, dim3(S.g_elem), dim3(kBlock), 0, S.hs, S.x, w->r, w->dinv, w->p0,
                            w->p1, w->Ap, S.N, verify, w->sc, S.rs.partials, S.rs.counter);
@@ -820,7 +889,18 @@ gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, c
     S.g_tile8 = grid_for_tiles(S.dv, (S.N + Tile<8>::ROWS - 1) / Tile<8>::ROWS, 3);
     S.g_elem = elementwise_grid(S.dv, S.N, 4);
     S.g_gen = elementwise_grid(S.dv, S.N, 1);
+    {
+        const TuneVal tv = tune_lookup(S.fmt, S.plan.W, S.N);
+        set_step_shape(S, tv.R, tv.per_cu);
+    }
     Workspace* w = S.w;
+    if (!S.tiled) {
+        // generic kernels read r, p and D^-1 at i + offset up to N + pad: clear
+        // whatever a larger earlier solve left beyond N in this workspace
+        for (float* v : {w->r, w->p0, w->p1, w->Ap, w->dinv, w->d})
+            hipLaunchKernelGGL(k_fill, dim3(1), dim3(kBlock), 0, S.hs, v + S.N, w->pad, 0.0f);
+        GM_LAUNCH_CHECK();
+    }
 
     hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(64), 0, S.hs, w->sc, o->max_iter);
     GM_LAUNCH_CHECK();
@@ -1033,6 +1113,93 @@ gm_status gm_true_relres(const BandedMatrix* A, const float* x, const float* b, 
 gm_status gm_release_workspaces(void) {
     release_workspaces();
     release_red_scratch();
+    return GM_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_result* out, gm_stream_t s) {
+    Solver S{};
+    gm_status st = get_plan(L, &S.plan);
+    if (st != GM_OK) return st;
+    if (precision < GM_PREC_FP32 || precision > GM_PREC_BF16) return fail(GM_ERR_INVALID, "autotune: precision");
+    S.A = L;
+    S.N = L->rows;
+    S.hs = (hipStream_t)s;
+    S.tiled = tiled_ok(S.plan, L->a, L->apitch);
+    if (!S.tiled || S.N == 0) return fail(GM_ERR_UNSUPPORTED, "autotune: needs the 5x5 Levin band structure");
+    const bool mixed = precision != GM_PREC_FP32;
+    S.pm = mixed ? kScaled : kJacobi;
+    S.fmt = precision == GM_PREC_FP16 ? kF16 : (precision == GM_PREC_BF16 ? kBF16 : kF32);
+    if ((st = dev_info(&S.dv)) != GM_OK) return st;
+    if ((st = red_scratch(s, &S.rs)) != GM_OK) return st;
+    if ((st = ws_get(s, S.N, S.plan.maxoff, mixed, &S.w)) != GM_OK) return st;
+    Workspace* w = S.w;
+    S.g_elem = elementwise_grid(S.dv, S.N, 4);
+    hipLaunchKernelGGL(k_fill, dim3(S.g_elem), dim3(kBlock), 0, S.hs, w->r, S.N, 1.0f);
+    hipLaunchKernelGGL(k_fill, dim3(S.g_elem), dim3(kBlock), 0, S.hs, w->p0, S.N, 0.5f);
+    hipLaunchKernelGGL(k_fill, dim3(S.g_elem), dim3(kBlock), 0, S.hs, w->p1, S.N, 0.5f);
+    if (mixed) {
+        if (S.fmt == kF16)
+            hipLaunchKernelGGL((k_scale16<8, kF16>), dim3(grid_for_tiles(S.dv, (S.N + 2047) / 2048, 3)), dim3(kBlock),
+                               0, S.hs, L->a, L->apitch, S.plan.W, S.N, w->A16, w->pitch16, w->dinv);
+        else
+            hipLaunchKernelGGL((k_scale16<8, kBF16>), dim3(grid_for_tiles(S.dv, (S.N + 2047) / 2048, 3)),
+                               dim3(kBlock), 0, S.hs, L->a, L->apitch, S.plan.W, S.N, w->A16, w->pitch16, w->dinv);
+    } else {
+        hipLaunchKernelGGL(k_diag_inv, dim3(S.g_elem), dim3(kBlock), 0, S.hs,
+                           L->a + (size_t)S.plan.args.center * L->apitch, w->dinv, S.N);
+    }
+    GM_LAUNCH_CHECK();
+    const int Rs32[] = {2, 4}, Rs16[] = {4, 8};
+    const int pcs[] = {1, 2, 3, 4, 6, 8};
+    hipEvent_t e0, e1;
+    GM_HIP(hipEventCreate(&e0));
+    GM_HIP(hipEventCreate(&e1));
+    TuneVal best{0, 0, 1e30};
+    if (reps < 1) reps = 5;
+    for (int ri = 0; ri < 2; ++ri) {
+        const int R = mixed ? Rs16[ri] : Rs32[ri];
+        for (int pc : pcs) {
+            set_step_shape(S, R, pc);
+            double tot = 0;
+            for (int k = 0; k < reps + 1; ++k) {
+                hipLaunchKernelGGL(k_tune_reset, dim3(1), dim3(64), 0, S.hs, w->sc);
+                GM_HIP(hipEventRecord(e0, S.hs));
+                if ((st = launch_step(S)) != GM_OK) return st;
+                GM_HIP(hipEventRecord(e1, S.hs));
+                GM_HIP(hipEventSynchronize(e1));
+                float ms = 0;
+                GM_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (k > 0) tot += ms;  // first launch warms caches
+            }
+            const double us = tot * 1e3 / reps;
+            if (us < best.us) best = TuneVal{R, pc, us};
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        g_tune[TuneKey{S.fmt, S.plan.W, (long)S.N}] = best;
+    }
+    if (out) {
+        out->rows_per_thread = best.R;
+        out->wg_per_cu = best.per_cu;
+        out->step_us = best.us;
+    }
+    return GM_OK;
+}
+
+gm_status gm_tune_set(int W, int N, int precision, int rows_per_thread, int wg_per_cu) {
+    const int fmt = precision == GM_PREC_FP16 ? kF16 : (precision == GM_PREC_BF16 ? kBF16 : kF32);
+    const bool ok = (fmt == kF32 && (rows_per_thread == 2 || rows_per_thread == 4)) ||
+                    (fmt != kF32 && (rows_per_thread == 4 || rows_per_thread == 8));
+    if (!ok || wg_per_cu < 1 || wg_per_cu > 16) return fail(GM_ERR_INVALID, "gm_tune_set: unsupported shape");
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune[TuneKey{fmt, W, (long)N}] = TuneVal{rows_per_thread, wg_per_cu, 0};
     return GM_OK;
 }
 

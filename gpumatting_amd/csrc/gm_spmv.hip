@@ -35,13 +35,15 @@ __global__ __launch_bounds__(kBlock) void k_spmv25(float* __restrict__ out, cons
     const TileRange tr = xcd_tiles(ntiles);
     for (int t = tr.begin; t < tr.end; t += tr.step) {
         const int i0 = t * Tile<R>::ROWS;
+        const int ib = i0 + R * threadIdx.x;
+        Bands<R, FMT> av;
+        if (ib < N) load_bands<R, FMT, kDiagStored>(A, pitch, ib, av);
         __syncthreads();
         stage5<R>(lds, i0, W, N, [&](int g) { return x[g]; });
         __syncthreads();
-        const int ib = i0 + R * threadIdx.x;
         if (ib < N) {
             float y[R];
-            tile_rows<R, FMT, false>(A, pitch, ib, lds + R * threadIdx.x, y);
+            apply_bands<R, FMT, kDiagStored>(av, lds + R * threadIdx.x, y);
             if (MODE != 0) {
                 float zz[R];
                 load_rows<R>(z, ib, N, zz);

@@ -202,3 +202,10 @@ def levin_backward(image, mu: torch.Tensor, alpha: torch.Tensor, lam=None, tgt=N
     call("gm_levin_backward", _p(gim), _p(glam), _p(gtg), _p(im), nch, W, _p(mu.contiguous()),
          _p(alpha.contiguous()), _p(la), _p(tg), eps, W, H, _stream())
     return gim.view(H, W, 3), glam, gtg
+
+
+def autotune(L: DeviceMatrix, precision: int = _lib.PREC_FP32, reps: int = 5):
+    """gm_autotune: returns (rows_per_thread, wg_per_cu, step_us) and records it."""
+    r = _lib.TuneResult()
+    call("gm_autotune", L.ref, precision, reps, C.byref(r), _stream())
+    return r.rows_per_thread, r.wg_per_cu, r.step_us

@@ -141,3 +141,18 @@ def config_1k(seed: int = 2000) -> MattingProblem:
 def config_4k(seed: int = 3000) -> MattingProblem:
     """3840x2160, three ellipses, 4 % constraints."""
     return make_problem(3840, 2160, seed, 3, 0.04)
+
+
+def config_batched(n_images: int = 64, nrhs: int = 4, seed: int = 4000, wmin=240, wmax=1080, hmin=320, hmax=1920):
+    """Batched workload: n_images images with W ~ U[wmin,wmax], H ~ U[hmin,hmax]
+    (one ellipse each), and nrhs independent constraint sets per image with
+    densities ~ U[2 %, 10 %].  Returns a list of (problem, [(lam, tgt), ...])."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_images):
+        W = int(rng.integers(wmin, wmax + 1))
+        H = int(rng.integers(hmin, hmax + 1))
+        p = make_problem(W, H, seed + 1 + i, 1, 0.0)
+        rhs = [extra_rhs(p, seed * 100 + i * nrhs + j, float(rng.uniform(0.02, 0.10))) for j in range(nrhs)]
+        out.append((p, rhs))
+    return out

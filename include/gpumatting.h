@@ -138,7 +138,21 @@ typedef struct gm_solve_info_s {
    int iterations;      /* iterations performed */
    int status;          /* gm_status of the solve */
    double relres;       /* last residual norm / ||b|| the iteration saw */
-   double true_relres;  /* FP64-accumulated ||b - A x|| / ||b|| at exit (-1 if not computed) */
+   double true_relres;  /* Project autotuner (no reference counterpart): times the fused CG step on
+ * this matrix for every (rows per thread, workgroups per CU) candidate with
+ * HIP events and records the fastest for solves of the same shape and
+ * precision.  Clobbers the calling stream's solver workspace (call before a
+ * solve, not during). */
+typedef struct gm_tune_result_s {
+   int rows_per_thread;
+   int wg_per_cu;
+   double step_us;      /* mean duration of one fused CG step with that shape */
+} gm_tune_result;
+gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_result* out, gm_stream_t stream);
+/* Pin a shape without timing (rows_per_thread 2|4 for FP32, 4|8 for 16-bit). */
+gm_status gm_tune_set(int W, int N, int precision, int rows_per_thread, int wg_per_cu);
+
+/* FP64-accumulated ||b - A x|| / ||b|| at exit (-1 if not computed) */
    int replacements;    /* FP64 residual replacements performed */
 } gm_solve_info;
 
@@ -149,6 +163,20 @@ void gm_solve_opts_default(gm_solve_opts* o);
  * `pad` (>= max |band offset|), initial guess in, solution out. */
 gm_status gm_solve(float* alpha, const BandedMatrix* L, const float* b, int pad, const gm_solve_opts* opts,
                    gm_solve_info* info, gm_stream_t stream);
+
+/* Project autotuner (no reference counterpart): times the fused CG step on
+ * this matrix for every (rows per thread, workgroups per CU) candidate with
+ * HIP events and records the fastest for solves of the same shape and
+ * precision.  Clobbers the calling stream's solver workspace (call before a
+ * solve, not during). */
+typedef struct gm_tune_result_s {
+   int rows_per_thread;
+   int wg_per_cu;
+   double step_us;      /* mean duration of one fused CG step with that shape */
+} gm_tune_result;
+gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_result* out, gm_stream_t stream);
+/* Pin a shape without timing (rows_per_thread 2|4 for FP32, 4|8 for 16-bit). */
+gm_status gm_tune_set(int W, int N, int precision, int rows_per_thread, int wg_per_cu);
 
 /* FP64-accumulated ||b - A x|| / ||b|| (synchronises; writes *relres). */
 gm_status gm_true_relres(const BandedMatrix* A, const float* x, const float* b, double* relres,
@@ -201,6 +229,37 @@ gm_status gm_levin_backward(float* grad_im, float* grad_lam, float* grad_tgt, co
  * ((2W+2)/32)*32+32 (main.cu:228-229). */
 void gm_levin_bands(int W, int* bands25);
 int gm_levin_pad(int W);
+
+/* ---------------------------------------------------------------------- */
+/* Batched solve: many images of uneven size, several right-hand sides each  */
+/* (no reference counterpart: the north star's batched form).                */
+
+typedef struct gm_batch_s gm_batch;
+
+/* Allocate a batch: nimages images (widths[i] x heights[i]) with nrhs <= 8
+ * constraint sets each; precision GM_PREC_FP32 or GM_PREC_FP16 (FP16
+ * off-diagonal bands, FP32 diagonals and vectors, FP64 residual replacement). */
+gm_status gm_batch_create(gm_batch** batch, int nimages, const int* widths, const int* heights, int nrhs,
+                          int precision);
+gm_status gm_batch_destroy(gm_batch* batch);
+/* Assemble image i's Laplacian (accurate rounding, no data term) from a device
+ * image (nch 3|4 floats per pixel, pitch in pixels). */
+gm_status gm_batch_assemble(gm_batch* batch, int image, const float* im, int nch, size_t im_pitch, float eps,
+                            gm_stream_t stream);
+/* Set system (image, rhs): weights lam and targets tgt (device, W*H, NULL =
+ * none), initial guess x0 (device or NULL = 0). */
+gm_status gm_batch_set_rhs(gm_batch* batch, int image, int rhs, const float* lam, const float* tgt, const float* x0,
+                           gm_stream_t stream);
+/* Jacobi-PCG on every system at once (one launch per phase for all images,
+ * hipGraph-captured iteration chunks); per-system convergence. */
+gm_status gm_batch_solve(gm_batch* batch, const gm_solve_opts* opts, gm_stream_t stream);
+/* Device pointer of system (image, rhs)'s solution (valid until destroy). */
+gm_status gm_batch_alpha(gm_batch* batch, int image, int rhs, float** alpha);
+/* Per-system result of the last gm_batch_solve. */
+gm_status gm_batch_info(gm_batch* batch, int image, int rhs, gm_solve_info* info);
+/* Persistent grid size in workgroups per CU (autotuned value or manual). */
+gm_status gm_batch_set_grid(gm_batch* batch, int per_cu);
+int gm_batch_tiles(gm_batch* batch);
 
 /* Release cached solver workspaces (device memory held between solves). */
 gm_status gm_release_workspaces(void);

@@ -499,6 +499,16 @@ void orc_levin_dense_f64(double* L, const float* im, int nch, size_t pitch, doub
     free(A);
 }
 
+/* The diagonal band: index nbands/2 (Solve.cpp:10) when its offset is 0, else
+ * the first band with offset 0. */
+static int orc_center_band(const int* bands, int nbands)
+{
+    if (nbands > 0 && bands[nbands / 2] == 0) return nbands / 2;
+    for (int k = 0; k < nbands; ++k)
+        if (bands[k] == 0) return k;
+    return 0;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Banded SpMV, BandedMatrix.cu:148-182: b_i = sum_k a[i+k*apitch]*x[i+bands[k]],
  * float accumulation in band order.  x must be readable at i+bands[k]
@@ -624,9 +634,7 @@ int orc_pcg_f32(float* x, const float* a, size_t apitch, const int* bands, int n
                 double* relres)
 {
     size_t n = (size_t)rows, len = n + 2 * (size_t)pad;
-    int cb = 0;
-    for (int k = 0; k < nbands; ++k)
-        if (bands[k] == 0) cb = k;
+    int cb = orc_center_band(bands, nbands);
     float* rb = (float*)calloc(len, sizeof(float));
     float* pb = (float*)calloc(len, sizeof(float));
     float* Ap = (float*)calloc(n, sizeof(float));
@@ -685,9 +693,7 @@ int orc_pcg_f64(double* x, const double* a, size_t apitch, const int* bands, int
                 const double* b, int pad, double tol, int maxit, double* relres)
 {
     size_t n = (size_t)rows, len = n + 2 * (size_t)pad;
-    int cb = 0;
-    for (int k = 0; k < nbands; ++k)
-        if (bands[k] == 0) cb = k;
+    int cb = orc_center_band(bands, nbands);
     double* rb = (double*)calloc(len, sizeof(double));
     double* pb = (double*)calloc(len, sizeof(double));
     double* Ap = (double*)calloc(n, sizeof(double));
@@ -822,9 +828,7 @@ double orc_baseline_pcg_f32(float* x, const float* a, size_t apitch, const int* 
                             const float* b, int pad, int iters)
 {
     size_t n = (size_t)rows, len = n + 2 * (size_t)pad;
-    int cb = 0;
-    for (int k = 0; k < nbands; ++k)
-        if (bands[k] == 0) cb = k;
+    int cb = orc_center_band(bands, nbands);
     float* rb = (float*)calloc(len, sizeof(float));
     float* pb = (float*)calloc(len, sizeof(float));
     float* Ap = (float*)calloc(n, sizeof(float));

@@ -1,0 +1,49 @@
+"""Batched multi-image, multi-RHS solver vs the FP64 oracle per system."""
+import numpy as np
+import pytest
+
+from gpumatting_amd import PREC_FP16, PREC_FP32, synth
+from gpumatting_amd.batch import Batch
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("prec", [PREC_FP32, PREC_FP16])
+def test_batched_matches_fp64_per_system(gpu, prec):
+    work = synth.config_batched(n_images=5, nrhs=3, seed=77, wmin=20, wmax=130, hmin=15, hmax=90)
+    b = Batch([(p.W, p.H) for p, _ in work], 3, prec)
+    for i, (p, rhs) in enumerate(work):
+        b.assemble(i, p.image, 1e-7)
+        for j, (lam, tgt) in enumerate(rhs):
+            b.set_rhs(i, j, lam, tgt)
+    b.solve(tol=1e-8, max_iter=20000, check_every=5)
+    for i, (p, rhs) in enumerate(work):
+        for j, (lam, tgt) in enumerate(rhs):
+            inf = b.info(i, j)
+            assert inf.status == 0 and inf.true_relres <= 1.5e-7, (i, j, inf.true_relres)
+            a, bb, _ = O.levin_gather_acc_f32(p.image, lam, tgt, 1e-7)
+            xe, _, _ = O.pcg_f64(a.astype(np.float64), O.bands25(p.W), bb.astype(np.float64), np.zeros(p.N),
+                                 O.pad_for(p.W), 1e-13)
+            x = b.alpha(i, j).cpu().numpy()
+            assert np.abs(x - xe).max() <= 2e-4, (i, j)
+    b.close()
+
+
+def test_batched_tolerance_and_iterations_match_single_solver(gpu):
+    from gpumatting_amd import matting, METHOD_PCG
+    work = synth.config_batched(n_images=3, nrhs=2, seed=5, wmin=60, wmax=200, hmin=50, hmax=160)
+    b = Batch([(p.W, p.H) for p, _ in work], 2, PREC_FP32)
+    for i, (p, rhs) in enumerate(work):
+        b.assemble(i, p.image, 1e-7)
+        for j, (lam, tgt) in enumerate(rhs):
+            b.set_rhs(i, j, lam, tgt)
+    b.solve(tol=1e-6, check_every=4)
+    for i, (p, rhs) in enumerate(work):
+        for j, (lam, tgt) in enumerate(rhs):
+            s = matting.assemble(p.image, lam, tgt, 1e-7)
+            x1, info1 = matting.solve(s, method=METHOD_PCG, tol=1e-6)
+            inf = b.info(i, j)
+            assert inf.true_relres <= 1e-6
+            assert abs(inf.iterations - info1.iterations) <= max(6, 0.05 * info1.iterations)
+    b.close()

@@ -78,7 +78,8 @@ def test_mixed_precision_vs_fp64(gpu, small, prec):
     x, info = matting.solve(s, method=METHOD_PCG, precision=prec, tol=1e-6, max_iter=5000)
     assert info.status == 0 and info.true_relres <= 1e-6
     assert info.replacements >= 1
-    assert info.iterations <= 1.5 * info32.iterations + 10
+    # FP16 keeps FP32's iteration count; BF16's 8-bit significand costs iterations
+    assert info.iterations <= (1.2 if prec == PREC_FP16 else 2.5) * info32.iterations + 10
     assert np.abs(x.numpy() - x_exact).max() <= 5e-3
     if prec == PREC_FP16:
         x2, info2 = matting.solve(s, method=METHOD_PCG, precision=prec, tol=5e-8, max_iter=5000)
@@ -229,3 +230,20 @@ def test_4k_solve_converges(gpu, prec):
     print(f"4K prec={prec}: iterations={info.iterations} true_relres={info.true_relres:.3e}")
     assert info.status == 0 and info.true_relres <= 1e-6
     assert np.isfinite(x.numpy()).all()
+
+
+def test_autotune_then_solve(gpu, small):
+    p, a, b, bands, pad, x_exact = small
+    s = _gpu_system(p)
+    for prec in (PREC_FP32, PREC_FP16):
+        R, pc, us = matting.autotune(s.L, prec, reps=2)
+        assert R in ((2, 4) if prec == PREC_FP32 else (4, 8)) and 1 <= pc <= 8 and us > 0
+        x, info = matting.solve(s, method=METHOD_PCG, precision=prec, tol=1e-6)
+        assert info.status == 0 and info.true_relres <= 1e-6
+        assert np.abs(x.numpy() - x_exact).max() <= 5e-3
+    # every shape gives the same answer to rounding
+    outs = []
+    for R in (2, 4):
+        lib().gm_tune_set(p.W, p.N, PREC_FP32, R, 2)
+        outs.append(matting.solve(s, method=METHOD_PCG, tol=1e-6)[0].numpy())
+    assert np.abs(outs[0] - outs[1]).max() <= 1e-3
