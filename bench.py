@@ -165,9 +165,9 @@ def main():
         for _ in range(args.warmup):
             step(prec)
         if not args.no_tune:
-            R, pc, us = matting.autotune(L, prec, reps=3)
-            tuned[name] = {"rows_per_thread": R, "wg_per_cu": pc, "step_us": round(us, 2)}
-            log(f"autotune {name}: R={R} wg/CU={pc} step {us:.1f} us")
+            tr = matting.autotune(L, prec, reps=3, full=True)
+            tuned[name] = {k: (round(v, 2) if isinstance(v, float) else v) for k, v in tr.items()}
+            log(f"autotune {name}: {tuned[name]}")
             step(prec)
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)

@@ -36,11 +36,14 @@ class BandedMatrix(C.Structure):
 
 class SolveOpts(C.Structure):
     _fields_ = [("method", C.c_int), ("precision", C.c_int), ("max_iter", C.c_int), ("tol", C.c_double),
-                ("restart", C.c_int), ("refine_every", C.c_int), ("check_every", C.c_int), ("verify", C.c_int)]
+                ("restart", C.c_int), ("refine_every", C.c_int), ("check_every", C.c_int), ("verify", C.c_int),
+                ("refine_mode", C.c_int)]
 
 
 class TuneResult(C.Structure):
-    _fields_ = [("rows_per_thread", C.c_int), ("wg_per_cu", C.c_int), ("step_us", C.c_double)]
+    _fields_ = [("rows_per_thread", C.c_int), ("tile_rows", C.c_int), ("wg_per_cu", C.c_int),
+                ("step_us", C.c_double), ("spmv_rows_per_thread", C.c_int), ("spmv_tile_rows", C.c_int),
+                ("spmv_wg_per_cu", C.c_int), ("spmv_us", C.c_double)]
 
 
 class SolveInfo(C.Structure):
@@ -88,7 +91,7 @@ _SIGS = {
     "gm_levin_pad": ([ci], ci),
     "gm_release_workspaces": ([], ci),
     "gm_autotune": ([BMp, ci, ci, C.POINTER(TuneResult), vp], ci),
-    "gm_tune_set": ([ci, ci, ci, ci, ci], ci),
+    "gm_tune_set": ([ci, ci, ci, ci, ci, ci], ci),
     "gm_batch_create": ([C.POINTER(vp), ci, vp, vp, ci, ci], ci),
     "gm_batch_destroy": ([vp], ci),
     "gm_batch_assemble": ([vp, ci, vp, ci, sz, cf, vp], ci),

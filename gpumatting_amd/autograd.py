@@ -3,8 +3,8 @@
     alpha = LevinMatte.apply(image, lam, tgt, eps, tol)
 
 Forward: GPU assembly (accurate rounding) of (L(image) + diag(lam)), then
-Jacobi-PCG in FP32 with FP64 residual replacement to `tol` (FP32 vectors
-reach about 1e-7 relative residual; smaller tolerances stagnate).
+Jacobi-PCG in FP32 inside FP64 iterative refinement (FP64 master solution
+and residual) to `tol`.
 Backward: one adjoint solve mu = (L + diag(lam))^-1 grad_alpha (same solver;
 the system is symmetric) and the assembly adjoint kernel, which returns the
 gradients with respect to the RGB pixels, the per-pixel weights lam and the
@@ -20,7 +20,7 @@ from . import _lib, matting
 def _solve(s: matting.System, rhs: torch.Tensor, tol: float, max_iter: int):
     x = matting.Padded(s.N, s.pad)
     _, info = matting.solve(s.L, rhs.reshape(-1).contiguous(), x=x, method=_lib.METHOD_PCG,
-                            precision=_lib.PREC_FP32, tol=tol, max_iter=max_iter, refine_every=50)
+                            precision=_lib.PREC_FP32, tol=tol, max_iter=max_iter, refine_mode=1)
     if info.status != 0:
         raise _lib.GMError(info.status, f"matte solve did not converge (relres {info.relres:.3e})")
     return x.t.clone(), info
@@ -28,7 +28,7 @@ def _solve(s: matting.System, rhs: torch.Tensor, tol: float, max_iter: int):
 
 class LevinMatte(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, image, lam, tgt, eps=1e-7, tol=1e-7, max_iter=20000):
+    def forward(ctx, image, lam, tgt, eps=1e-7, tol=1e-8, max_iter=20000):
         H, W = image.shape[:2]
         s = matting.assemble(image.detach(), lam.detach(), tgt.detach(), eps, _lib.ASSEMBLY_ACCURATE)
         alpha, info = _solve(s, s.b, tol, max_iter)
@@ -49,5 +49,5 @@ class LevinMatte(torch.autograd.Function):
         return gi, glam.view_as(lam).to(lam.dtype), gtg.view_as(tgt).to(tgt.dtype), None, None, None
 
 
-def matte(image, lam, tgt, eps=1e-7, tol=1e-7, max_iter=20000):
+def matte(image, lam, tgt, eps=1e-7, tol=1e-8, max_iter=20000):
     return LevinMatte.apply(image, lam, tgt, eps, tol, max_iter)

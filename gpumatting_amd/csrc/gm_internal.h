@@ -372,9 +372,10 @@ __device__ __forceinline__ void load_bands(const void* __restrict__ A, size_t pi
 
 // y[j] = sum_k A_k[i+j] * X(i+j+off_k), k = 0..24 in band order (FMA), with X
 // read from the staged LDS window (band 12 per DIAG).
-template <int R, int FMT, int DIAG>
+// STRIDE: floats between the window rows dy = -2..2 in LDS.
+template <int R, int FMT, int DIAG, int STRIDE = Tile<R>::SEG>
 __device__ __forceinline__ void apply_bands(const Bands<R, FMT>& av, const float* __restrict__ xs, float (&y)[R]) {
-    constexpr int SEG = Tile<R>::SEG;
+    constexpr int SEG = STRIDE;
 #pragma unroll
     for (int j = 0; j < R; ++j) y[j] = 0.f;
 #pragma unroll
@@ -415,5 +416,112 @@ __device__ __forceinline__ void tile_rows(const void* __restrict__ A, size_t pit
     load_bands<R, FMT, DIAG>(A, pitch, ib, av);
     apply_bands<R, FMT, DIAG>(av, xs, y);
 }
+
+// ---------------------------------------------------------------------------
+// Tile geometries of the Levin kernels.  Both give every thread R consecutive
+// rows (pixels of one image row) and an LDS window whose rows dy = -2..2 are
+// STRIDE floats apart, so the same apply_bands serves both.
+//
+//   Geom1D<R>     : ROWS = kBlock*R consecutive matrix rows; the window is 5
+//                   segments of ROWS+4 (each x element is staged by 5 tiles).
+//   Geom2D<R, TY> : TX x TY pixels (TX = kBlock/TY*R); the window is the
+//                   (TY+4) x (TX+4) pixel halo, ~3x fewer staged elements.
+//                   Needs W % R == 0 (aligned rows); window cells are
+//                   addressed by matrix-row index, so the product equals the
+//                   1-D one for any band values.
+template <int R_>
+struct Geom1D {
+    static constexpr int R = R_;
+    static constexpr int ROWS = kBlock * R;
+    static constexpr int STRIDE = ROWS + 4;
+    static constexpr int LDS_FLOATS = 5 * STRIDE;
+    static constexpr bool kIs2D = false;
+    int ib;      // first row of this thread
+    bool valid;  // thread has at least one row < N
+    int i0;
+    __device__ static int ntiles(int W, int H, int N) { return (N + ROWS - 1) / ROWS; }
+    __device__ __forceinline__ void at(int t, int W, int H, int N) {
+        i0 = t * ROWS;
+        ib = i0 + R * (int)threadIdx.x;
+        valid = ib < N;
+    }
+    template <class F>
+    __device__ __forceinline__ void stage(float* __restrict__ lds, int W, int H, int N, F f) const {
+        stage5<R>(lds, i0, W, N, f);
+    }
+    __device__ __forceinline__ const float* xs(const float* lds) const { return lds + R * threadIdx.x; }
+    __device__ __forceinline__ float center(const float* lds, int j) const {
+        return lds[2 * STRIDE + R * threadIdx.x + 2 + j];
+    }
+};
+
+template <int R_, int TY_>
+struct Geom2D {
+    static constexpr int R = R_;
+    static constexpr int TY = TY_;
+    static constexpr int TPR = kBlock / TY;  // threads per tile row
+    static constexpr int TX = TPR * R;
+    static constexpr int STRIDE = TX + 4;
+    static constexpr int LDS_FLOATS = (TY + 4) * STRIDE;
+    static constexpr bool kIs2D = true;
+    static_assert(kBlock % TY == 0, "tile rows must divide the workgroup");
+    int ib;
+    bool valid;
+    int x0, y0;
+    __device__ static int ntiles(int W, int H, int N) { return ((W + TX - 1) / TX) * ((H + TY - 1) / TY); }
+    __device__ __forceinline__ void at(int t, int W, int H, int N) {
+        const int txn = (W + TX - 1) / TX;
+        x0 = (t % txn) * TX;
+        y0 = (t / txn) * TY;
+        const int x = x0 + R * ((int)threadIdx.x % TPR);
+        const int y = y0 + (int)threadIdx.x / TPR;
+        valid = x < W && y < H;
+        ib = y * W + x;
+    }
+    template <class F>
+    __device__ __forceinline__ void stage(float* __restrict__ lds, int W, int H, int N, F f) const {
+        constexpr int NE = (TY + 4) * STRIDE;
+        for (int e = threadIdx.x; e < NE; e += kBlock) {
+            const int ly = e / STRIDE, lx = e - ly * STRIDE;
+            // the matrix-row index of this window cell; cells left/right of the
+            // image alias the neighbouring image row exactly as the 1-D
+            // product does, so both geometries compute the same A x
+            const long long g = (long long)(y0 - 2 + ly) * W + (x0 - 2 + lx);
+            lds[e] = (g >= 0 && g < N) ? f((int)g) : 0.f;
+        }
+    }
+    __device__ __forceinline__ const float* xs(const float* lds) const {
+        return lds + ((int)threadIdx.x / TPR) * STRIDE + R * ((int)threadIdx.x % TPR);
+    }
+    __device__ __forceinline__ float center(const float* lds, int j) const {
+        return lds[((int)threadIdx.x / TPR + 2) * STRIDE + R * ((int)threadIdx.x % TPR) + 2 + j];
+    }
+};
+
+// Tile shapes selectable at run time (autotuner / defaults): geometry, rows
+// per thread R, tile rows TY (0 = 1-D).
+enum Shape { kS1R2 = 0, kS1R4, kS1R8, kS2R4Y4, kS2R4Y8, kS2R4Y16, kS2R8Y8, kS2R8Y16, kNumShapes };
+constexpr int kShapeR[kNumShapes] = {2, 4, 8, 4, 4, 4, 8, 8};
+constexpr int kShapeTY[kNumShapes] = {0, 0, 0, 4, 8, 16, 8, 16};
+inline bool shape_ok(int sh, int fmt, int W, long N) {
+    const int R = kShapeR[sh];
+    if (fmt == kF32 ? (R == 8) : (R == 2)) return false;
+    if (kShapeTY[sh] == 0) return true;
+    return W > 0 && N % W == 0 && W % R == 0;
+}
+inline long shape_tiles(int sh, int W, long N) {
+    if (kShapeTY[sh] == 0) {
+        const long rows = (long)kBlock * kShapeR[sh];
+        return (N + rows - 1) / rows;
+    }
+    const int TX = kBlock / kShapeTY[sh] * kShapeR[sh];
+    const long H = N / W;
+    return (long)((W + TX - 1) / TX) * ((H + kShapeTY[sh] - 1) / kShapeTY[sh]);
+}
+
+// SpMV shape table (gm_spmv.hip) and its tuner.
+void spmv_tune_get(int fmt, int W, long N, int* shape, int* per_cu);
+gm_status spmv_autotune(const void* A, size_t pitch, int fmt, int W, int N, float* out, const float* x, int reps,
+                        gm_stream_t s, int* best_shape, int* best_pc, double* best_us);
 
 }  // namespace gm

@@ -36,6 +36,7 @@ __global__ __launch_bounds__(kBlock) void k_levin(const float* __restrict__ im, 
     //              ws[7..9] = float mean (held exactly as double)
     //   ACCURATE : ws[0..5] = inverse S11 S12 S13 S22 S23 S33, ws[7..9] = mean
     __shared__ double ws[10][kNW];
+    __shared__ float wm[6][kNW];  // ACCURATE: float hi/lo split of the mean
     __shared__ unsigned char wvalid[kNW];
 
     const int x0 = blockIdx.x * kTX, y0 = blockIdx.y * kTY;
@@ -125,6 +126,11 @@ __global__ __launch_bounds__(kBlock) void k_levin(const float* __restrict__ im, 
             ws[0][e] = d11 / det; ws[1][e] = d12 / det; ws[2][e] = d13 / det;
             ws[3][e] = d22 / det; ws[4][e] = d23 / det; ws[5][e] = d33 / det;
             ws[7][e] = m0; ws[8][e] = m1; ws[9][e] = m2;
+            const float h0 = (float)m0, h1 = (float)m1, h2 = (float)m2;
+            wm[0][e] = h0; wm[1][e] = h1; wm[2][e] = h2;
+            wm[3][e] = (float)(m0 - (double)h0);
+            wm[4][e] = (float)(m1 - (double)h1);
+            wm[5][e] = (float)(m2 - (double)h2);
         }
     }
     __syncthreads();
@@ -159,6 +165,13 @@ __global__ __launch_bounds__(kBlock) void k_levin(const float* __restrict__ im, 
                 wy = (float)(ws[1][e] * rx + ws[3][e] * ry + ws[4][e] * rz);
                 wz = (float)(ws[2][e] * rx + ws[4][e] * ry + ws[5][e] * rz);
             }
+            float mh0, mh1, mh2, ml0 = 0.f, ml1 = 0.f, ml2 = 0.f;
+            if constexpr (MODE == GM_ASSEMBLY_REFERENCE) {
+                mh0 = (float)ws[7][e]; mh1 = (float)ws[8][e]; mh2 = (float)ws[9][e];
+            } else {
+                mh0 = wm[0][e]; mh1 = wm[1][e]; mh2 = wm[2][e];
+                ml0 = wm[3][e]; ml1 = wm[4][e]; ml2 = wm[5][e];
+            }
 #pragma unroll
             for (int vy = 0; vy < 3; ++vy)
 #pragma unroll
@@ -166,19 +179,22 @@ __global__ __launch_bounds__(kBlock) void k_levin(const float* __restrict__ im, 
                     const int lx = tx + wx2 + vx, ly = ty + wy2 + vy;  // pixel j in the halo
                     float rx, ry, rz;
                     if constexpr (MODE == GM_ASSEMBLY_REFERENCE) {
-                        rx = px[0][ly][lx] - (float)ws[7][e];
-                        ry = px[1][ly][lx] - (float)ws[8][e];
-                        rz = px[2][ly][lx] - (float)ws[9][e];
+                        rx = px[0][ly][lx] - mh0;
+                        ry = px[1][ly][lx] - mh1;
+                        rz = px[2][ly][lx] - mh2;
                     } else {
-                        rx = (float)((double)px[0][ly][lx] - ws[7][e]);
-                        ry = (float)((double)px[1][ly][lx] - ws[8][e]);
-                        rz = (float)((double)px[2][ly][lx] - ws[9][e]);
+                        rx = (px[0][ly][lx] - mh0) - ml0;
+                        ry = (px[1][ly][lx] - mh1) - ml1;
+                        rz = (px[2][ly][lx] - mh2) - ml2;
                     }
                     // Matting.cu:177
                     float s = 1.0f + wx * rx;
                     s = s + wy * ry;
                     s = s + wz * rz;
-                    acc[12 + (wx2 - 1 + vx - 1) + 5 * (wy2 - 1 + vy - 1)] += -s / 9.0f;
+                    if constexpr (MODE == GM_ASSEMBLY_REFERENCE)
+                        acc[12 + (wx2 - 1 + vx - 1) + 5 * (wy2 - 1 + vy - 1)] += -s / 9.0f;
+                    else  // window sums of (1 + w.r); scaled by -1/9 once below
+                        acc[12 + (wx2 - 1 + vx - 1) + 5 * (wy2 - 1 + vy - 1)] += s;
                 }
         }
 
@@ -202,7 +218,10 @@ __global__ __launch_bounds__(kBlock) void k_levin(const float* __restrict__ im, 
         double off = 0;
 #pragma unroll
         for (int k = 0; k < 25; ++k)
-            if (k != 12) off += (double)acc[k];
+            if (k != 12) {
+                acc[k] = -acc[k] / 9.0f;
+                off += (double)acc[k];
+            }
         acc[12] = (float)(-off);
     }
     float bi = 0.f;

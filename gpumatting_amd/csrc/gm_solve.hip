@@ -141,42 +141,44 @@ __global__ __launch_bounds__(kBlock) void k_resid25(const float* __restrict__ A,
 }
 
 // K1: p_new = -z + beta p_old (staged), Ap = A p_new, <p_new, Ap>.
-template <int R, int FMT, int PM>
-__global__ __launch_bounds__(kBlock) void k_cgstep25(const void* __restrict__ A, size_t pitch, int W, int N,
-                                                     const float* __restrict__ r, const float* __restrict__ dinv,
-                                                     float* __restrict__ p0, float* __restrict__ p1,
-                                                     float* __restrict__ Ap, Scalars* sc, double* partials,
-                                                     unsigned int* counter) {
+// G: tile geometry (Geom1D / Geom2D).
+template <class G, int FMT, int PM>
+__global__ __launch_bounds__(kBlock) void k_cgstep(const void* __restrict__ A, size_t pitch, int W, int H, int N,
+                                                   const float* __restrict__ r, const float* __restrict__ dinv,
+                                                   float* __restrict__ p0, float* __restrict__ p1,
+                                                   float* __restrict__ Ap, Scalars* sc, double* partials,
+                                                   unsigned int* counter) {
     if (stopped(sc)) return;
+    constexpr int R = G::R;
+    constexpr int DIAG = PM == kScaled ? kDiagUnit : kDiagStored;
     const float beta = sc->beta;
     const int par = sc->iters & 1;
     const float* __restrict__ pold = par ? p1 : p0;
     float* __restrict__ pnew = par ? p0 : p1;
-    __shared__ __attribute__((aligned(16))) float lds[Tile<R>::LDS_FLOATS];
-    const int ntiles = (N + Tile<R>::ROWS - 1) / Tile<R>::ROWS;
-    const TileRange tr = xcd_tiles(ntiles);
+    __shared__ __attribute__((aligned(16))) float lds[G::LDS_FLOATS];
+    const TileRange tr = xcd_tiles(G::ntiles(W, H, N));
     double v[1] = {0.0};
     for (int t = tr.begin; t < tr.end; t += tr.step) {
-        const int i0 = t * Tile<R>::ROWS;
-        const int ib = i0 + R * threadIdx.x;
+        G g;
+        g.at(t, W, H, N);
         Bands<R, FMT> av;
-        if (ib < N) load_bands<R, FMT, PM == kScaled ? kDiagUnit : kDiagStored>(A, pitch, ib, av);
+        if (g.valid) load_bands<R, FMT, DIAG>(A, pitch, g.ib, av);
         __syncthreads();
         if (beta != 0.f)
-            stage5<R>(lds, i0, W, N, [&](int g) { return fmaf(beta, pold[g], -zval<PM>(r, dinv, g)); });
+            g.stage(lds, W, H, N, [&](int i) { return fmaf(beta, pold[i], -zval<PM>(r, dinv, i)); });
         else
-            stage5<R>(lds, i0, W, N, [&](int g) { return -zval<PM>(r, dinv, g); });
+            g.stage(lds, W, H, N, [&](int i) { return -zval<PM>(r, dinv, i); });
         __syncthreads();
-        if (ib < N) {
+        if (g.valid) {
             float y[R], pc[R];
-            apply_bands<R, FMT, PM == kScaled ? kDiagUnit : kDiagStored>(av, lds + R * threadIdx.x, y);
+            apply_bands<R, FMT, DIAG, G::STRIDE>(av, g.xs(lds), y);
 #pragma unroll
-            for (int j = 0; j < R; ++j) pc[j] = lds[2 * Tile<R>::SEG + R * threadIdx.x + 2 + j];
-            store_rows<R>(pnew, ib, N, pc);
-            store_rows<R>(Ap, ib, N, y);
+            for (int j = 0; j < R; ++j) pc[j] = g.center(lds, j);
+            store_rows<R>(pnew, g.ib, N, pc);
+            store_rows<R>(Ap, g.ib, N, y);
 #pragma unroll
             for (int j = 0; j < R; ++j)
-                if (ib + j < N) v[0] += (double)(pc[j] * y[j]);
+                if (g.ib + j < N) v[0] += (double)(pc[j] * y[j]);
         }
     }
     grid_reduce<1>(v, partials, counter, [&](double* tt) {
@@ -189,8 +191,29 @@ __global__ __launch_bounds__(kBlock) void k_cgstep25(const void* __restrict__ A,
 }
 
 // K2: x += alpha p (scaled: alpha s p), r += alpha Ap, <r,z>, ||r||^2 (scaled:
-// the unscaled residual (r/s)^2).
+// the unscaled residual (r/s)^2).  VEC: every pointer is 16-byte aligned and
+// the body moves float4s (the kernel is a pure stream: wide loads keep enough
+// bytes in flight to reach HBM bandwidth).
 template <int PM>
+__device__ __forceinline__ void upd_elem(float& x, float& r, float d, float p, float ap, float alpha, double& v0,
+                                         double& v1) {
+    if (PM == kScaled) x = fmaf(alpha * d, p, x);
+    else x = fmaf(alpha, p, x);
+    const float rn = fmaf(alpha, ap, r);
+    r = rn;
+    if (PM == kJacobi) {
+        v0 += (double)(rn * (d * rn));
+        v1 += (double)(rn * rn);
+    } else if (PM == kScaled) {
+        const float u = rn / d;  // unscaled residual
+        v0 += (double)(rn * rn);
+        v1 += (double)(u * u);
+    } else {
+        v0 += (double)(rn * rn);
+    }
+}
+
+template <int PM, bool VEC>
 __global__ __launch_bounds__(kBlock) void k_cgupdate(float* __restrict__ x, float* __restrict__ r,
                                                      const float* __restrict__ dinv, const float* __restrict__ p0,
                                                      const float* __restrict__ p1, const float* __restrict__ Ap,
@@ -200,24 +223,41 @@ __global__ __launch_bounds__(kBlock) void k_cgupdate(float* __restrict__ x, floa
     const float alpha = sc->alpha;
     const float* __restrict__ p = (sc->iters & 1) ? p0 : p1;  // the p_new K1 wrote
     double v[2] = {0.0, 0.0};
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < N; i += gridDim.x * kBlock) {
-        const float pi = p[i];
-        if (PM == kScaled) x[i] = fmaf(alpha * dinv[i], pi, x[i]);
-        else x[i] = fmaf(alpha, pi, x[i]);
-        const float rn = fmaf(alpha, Ap[i], r[i]);
-        r[i] = rn;
-        if (PM == kJacobi) {
-            v[0] += (double)(rn * (dinv[i] * rn));
-            v[1] += (double)(rn * rn);
-        } else if (PM == kScaled) {
-            const float u = rn / dinv[i];  // unscaled residual
-            v[0] += (double)(rn * rn);
-            v[1] += (double)(u * u);
-        } else {
-            v[0] += (double)(rn * rn);
-            v[1] = v[0];
+    const int tid = blockIdx.x * kBlock + threadIdx.x, nth = gridDim.x * kBlock;
+    if (VEC) {
+        const int n4 = N >> 2;
+        for (int i = tid; i < n4; i += nth) {
+            f32x4 xv = reinterpret_cast<const f32x4*>(x)[i];
+            f32x4 rv = reinterpret_cast<const f32x4*>(r)[i];
+            const f32x4 pv = reinterpret_cast<const f32x4*>(p)[i];
+            const f32x4 av = reinterpret_cast<const f32x4*>(Ap)[i];
+            f32x4 dv = {1.f, 1.f, 1.f, 1.f};
+            if (PM != kPlain) dv = reinterpret_cast<const f32x4*>(dinv)[i];
+            float xs[4] = {xv.x, xv.y, xv.z, xv.w}, rs[4] = {rv.x, rv.y, rv.z, rv.w};
+            const float ds[4] = {dv.x, dv.y, dv.z, dv.w}, ps[4] = {pv.x, pv.y, pv.z, pv.w};
+            const float as[4] = {av.x, av.y, av.z, av.w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) upd_elem<PM>(xs[q], rs[q], ds[q], ps[q], as[q], alpha, v[0], v[1]);
+            xv = f32x4{xs[0], xs[1], xs[2], xs[3]};
+            rv = f32x4{rs[0], rs[1], rs[2], rs[3]};
+            reinterpret_cast<f32x4*>(x)[i] = xv;
+            reinterpret_cast<f32x4*>(r)[i] = rv;
+        }
+        for (int i = (n4 << 2) + tid; i < N; i += nth) {
+            float xi = x[i], ri = r[i];
+            upd_elem<PM>(xi, ri, PM != kPlain ? dinv[i] : 1.f, p[i], Ap[i], alpha, v[0], v[1]);
+            x[i] = xi;
+            r[i] = ri;
+        }
+    } else {
+        for (int i = tid; i < N; i += nth) {
+            float xi = x[i], ri = r[i];
+            upd_elem<PM>(xi, ri, PM != kPlain ? dinv[i] : 1.f, p[i], Ap[i], alpha, v[0], v[1]);
+            x[i] = xi;
+            r[i] = ri;
         }
     }
+    if (PM == kPlain) v[1] = v[0];
     grid_reduce<2>(v, partials, counter, [&](double* tt) {
         const double rz = tt[0], rr = tt[1];
         sc->iters += 1;
@@ -559,6 +599,38 @@ __global__ __launch_bounds__(kBlock) void k_gd2_g(const float* __restrict__ a, s
     });
 }
 
+// ---------------------------------------------------------------------------
+// FP64 iterative refinement (refine_mode 1)
+
+// t = b - A x64 (FP64 accumulation, x64 padded), rhs = (float)t, g0 = ||t||^2
+__global__ __launch_bounds__(kBlock) void k_refres64(const float* __restrict__ a, size_t pitch, BandArgs ba, int N,
+                                                     const double* __restrict__ x64, const float* __restrict__ b,
+                                                     float* __restrict__ rhs, Scalars* sc, double* partials,
+                                                     unsigned int* counter) {
+    double v[1] = {0.0};
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < N; i += gridDim.x * kBlock) {
+        double s = 0.0;
+        for (int k = 0; k < ba.nb; ++k) s = fma((double)a[(size_t)i + (size_t)k * pitch], x64[(long long)i + ba.off[k]], s);
+        const double t = (double)b[i] - s;
+        rhs[i] = (float)t;
+        v[0] += t * t;
+    }
+    grid_reduce<1>(v, partials, counter, [&](double* tt) { sc->g0 = tt[0]; });
+}
+
+__global__ __launch_bounds__(kBlock) void k_zero64(double* v, int n) {
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) v[i] = 0.0;
+}
+
+// x64 = x (init) or x64 += x (accumulate a correction); x = (float)x64 (finish)
+__global__ __launch_bounds__(kBlock) void k_x64(double* __restrict__ x64, float* __restrict__ x, int N, int op) {
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < N; i += gridDim.x * kBlock) {
+        if (op == 0) x64[i] = (double)x[i];
+        else if (op == 1) x64[i] += (double)x[i];
+        else x[i] = (float)x64[i];
+    }
+}
+
 __global__ void k_tune_reset(Scalars* sc) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         sc->done = 0;
@@ -593,6 +665,9 @@ struct Workspace {
     unsigned short* A16 = nullptr;
     size_t pitch16 = 0;
     long cap16 = 0;
+    double* x64 = nullptr;  // FP64 master x of the refinement mode (padded)
+    long cap64 = 0;
+    int pad64 = 0;
     Scalars* sc = nullptr;
     Scalars* hsc = nullptr;  // pinned, 2 slots
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -656,6 +731,7 @@ void release_workspaces() {
         Workspace* w = kv.second;
         ws_free_vectors(w);
         if (w->A16) (void)hipFree(w->A16);
+        if (w->x64) (void)hipFree(w->x64 - w->pad64);
         if (w->sc) (void)hipFree(w->sc);
         if (w->hsc) (void)hipHostFree(w->hsc);
         for (int i = 0; i < 2; ++i)
@@ -679,7 +755,7 @@ struct TuneKey {
     }
 };
 struct TuneVal {
-    int R, per_cu;
+    int shape, per_cu;
     double us;
 };
 static std::mutex g_tune_mu;
@@ -689,7 +765,8 @@ static TuneVal tune_lookup(int fmt, int W, long N) {
     std::lock_guard<std::mutex> g(g_tune_mu);
     auto it = g_tune.find(TuneKey{fmt, W, N});
     if (it != g_tune.end()) return it->second;
-    return fmt == kF32 ? TuneVal{4, 4, 0} : TuneVal{8, 3, 0};
+    if (fmt == kF32) return shape_ok(kS2R4Y8, fmt, W, N) ? TuneVal{kS2R4Y8, 4, 0} : TuneVal{kS1R4, 4, 0};
+    return shape_ok(kS2R8Y8, fmt, W, N) ? TuneVal{kS2R8Y8, 3, 0} : TuneVal{kS1R8, 3, 0};
 }
 
 struct Solver {
@@ -706,7 +783,9 @@ struct Solver {
     int pm;        // PMode
     int fmt;       // kF32 / kF16 / kBF16
     int g_tile4, g_tile8, g_elem, g_gen;
-    int stepR, g_step;  // tuned K1 shape
+    int shape, g_step;  // tuned K1 shape
+    bool vec;           // x (and the workspace) 16-byte aligned: float4 update kernel
+    int g_upd;
 };
 
 static gm_status launch_resid(Solver& S, int start) {
@@ -759,30 +838,37 @@ static gm_status launch_trueres(Solver& S, int op) {
     return GM_OK;
 }
 
-template <int R, int FMT, int PM>
+template <class G, int FMT, int PM>
 static void launch_step_t(Solver& S, const void* A, size_t pitch) {
     Workspace* w = S.w;
-    hipLaunchKernelGGL((k_cgstep25<R, FMT, PM>), dim3(S.g_step), dim3(kBlock), 0, S.hs, A, pitch, S.plan.W, S.N, w->r,
-                       w->dinv, w->p0, w->p1, w->Ap, w->sc, S.rs.partials, S.rs.counter);
+    hipLaunchKernelGGL((k_cgstep<G, FMT, PM>), dim3(S.g_step), dim3(kBlock), 0, S.hs, A, pitch, S.plan.W,
+                       S.N / S.plan.W, S.N, w->r, w->dinv, w->p0, w->p1, w->Ap, w->sc, S.rs.partials, S.rs.counter);
+}
+
+template <int FMT, int PM>
+static void launch_step_shape(Solver& S, const void* A, size_t pitch) {
+    switch (S.shape) {
+        case kS1R2: launch_step_t<Geom1D<2>, FMT, PM>(S, A, pitch); break;
+        case kS1R4: launch_step_t<Geom1D<4>, FMT, PM>(S, A, pitch); break;
+        case kS1R8: launch_step_t<Geom1D<8>, FMT, PM>(S, A, pitch); break;
+        case kS2R4Y4: launch_step_t<Geom2D<4, 4>, FMT, PM>(S, A, pitch); break;
+        case kS2R4Y8: launch_step_t<Geom2D<4, 8>, FMT, PM>(S, A, pitch); break;
+        case kS2R4Y16: launch_step_t<Geom2D<4, 16>, FMT, PM>(S, A, pitch); break;
+        case kS2R8Y8: launch_step_t<Geom2D<8, 8>, FMT, PM>(S, A, pitch); break;
+        default: launch_step_t<Geom2D<8, 16>, FMT, PM>(S, A, pitch); break;
+    }
 }
 
 static gm_status launch_step(Solver& S) {
     Workspace* w = S.w;
     if (S.tiled) {
         if (S.pm == kScaled) {
-            if (S.fmt == kF16) {
-                if (S.stepR == 4) launch_step_t<4, kF16, kScaled>(S, w->A16, w->pitch16);
-                else launch_step_t<8, kF16, kScaled>(S, w->A16, w->pitch16);
-            } else {
-                if (S.stepR == 4) launch_step_t<4, kBF16, kScaled>(S, w->A16, w->pitch16);
-                else launch_step_t<8, kBF16, kScaled>(S, w->A16, w->pitch16);
-            }
+            if (S.fmt == kF16) launch_step_shape<kF16, kScaled>(S, w->A16, w->pitch16);
+            else launch_step_shape<kBF16, kScaled>(S, w->A16, w->pitch16);
         } else if (S.pm == kJacobi) {
-            if (S.stepR == 2) launch_step_t<2, kF32, kJacobi>(S, S.A->a, S.A->apitch);
-            else launch_step_t<4, kF32, kJacobi>(S, S.A->a, S.A->apitch);
+            launch_step_shape<kF32, kJacobi>(S, S.A->a, S.A->apitch);
         } else {
-            if (S.stepR == 2) launch_step_t<2, kF32, kPlain>(S, S.A->a, S.A->apitch);
-            else launch_step_t<4, kF32, kPlain>(S, S.A->a, S.A->apitch);
+            launch_step_shape<kF32, kPlain>(S, S.A->a, S.A->apitch);
         }
     } else {
         if (S.pm == kJacobi)
@@ -798,25 +884,36 @@ static gm_status launch_step(Solver& S) {
     return GM_OK;
 }
 
-static void set_step_shape(Solver& S, int R, int per_cu) {
-    S.stepR = R;
-    const long rows = (long)kBlock * R;
-    S.g_step = grid_for_tiles(S.dv, (S.N + rows - 1) / rows, per_cu);
+static void set_step_shape(Solver& S, int shape, int per_cu) {
+    S.shape = shape;
+    S.g_step = grid_for_tiles(S.dv, shape_tiles(shape, S.plan.W, S.N), per_cu);
 }
 
 static gm_status launch_cg_iter(Solver& S, int verify) {
     Workspace* w = S.w;
     gm_status st = launch_step(S);
     if (st != GM_OK) return st;
-    if (S.pm == kJacobi)
-        hipLaunchKernelGGL(k_cgupdate<kJacobi>, dim3(S.g_elem), dim3(kBlock), 0, S.hs, S.x, w->r, w->dinv, w->p0,
-                           w->p1, w->Ap, S.N, verify, w->sc, S.rs.partials, S.rs.counter);
-    else if (S.pm == kScaled)
-        hipLaunchKernelGGL(k_cgupdate<kScaled>, dim3(S.g_elem), dim3(kBlock), 0, S.hs, S.x, w->r, w->dinv, w->p0,
-                           w->p1, w->Ap, S.N, verify, w->sc, S.rs.partials, S.rs.counter);
-    else
-        hipLaunchKernelGGL(k_cgupdate<kPlain>, dim3(S.g_elem), dim3(kBlock), 0, S.hs, S.x, w->r, w->dinv, w->p0,
-                           w->p1, w->Ap, S.N, verify, w->sc, S.rs.partials, S.rs.counter);
+    if (S.vec) {
+        if (S.pm == kJacobi)
+            hipLaunchKernelGGL((k_cgupdate<kJacobi, true>), dim3(S.g_upd), dim3(kBlock), 0, S.hs, S.x, w->r, w->dinv,
+                               w->p0, w->p1, w->Ap, S.N, verify, w->sc, S.rs.partials, S.rs.counter);
+        else if (S.pm == kScaled)
+            hipLaunchKernelGGL((k_cgupdate<kScaled, true>), dim3(S.g_upd), dim3(kBlock), 0, S.hs, S.x, w->r, w->dinv,
+                               w->p0, w->p1, w->Ap, S.N, verify, w->sc, S.rs.partials, S.rs.counter);
+        else
+            hipLaunchKernelGGL((k_cgupdate<kPlain, true>), dim3(S.g_upd), dim3(kBlock), 0, S.hs, S.x, w->r, w->dinv,
+                               w->p0, w->p1, w->Ap, S.N, verify, w->sc, S.rs.partials, S.rs.counter);
+    } else {
+        if (S.pm == kJacobi)
+            hipLaunchKernelGGL((k_cgupdate<kJacobi, false>), dim3(S.g_elem), dim3(kBlock), 0, S.hs, S.x, w->r,
+                               w->dinv, w->p0, w->p1, w->Ap, S.N, verify, w->sc, S.rs.partials, S.rs.counter);
+        else if (S.pm == kScaled)
+            hipLaunchKernelGGL((k_cgupdate<kScaled, false>), dim3(S.g_elem), dim3(kBlock), 0, S.hs, S.x, w->r,
+                               w->dinv, w->p0, w->p1, w->Ap, S.N, verify, w->sc, S.rs.partials, S.rs.counter);
+        else
+            hipLaunchKernelGGL((k_cgupdate<kPlain, false>), dim3(S.g_elem), dim3(kBlock), 0, S.hs, S.x, w->r,
+                               w->dinv, w->p0, w->p1, w->Ap, S.N, verify, w->sc, S.rs.partials, S.rs.counter);
+    }
     GM_LAUNCH_CHECK();
     return GM_OK;
 }
@@ -854,7 +951,108 @@ static gm_status status_of_done(int done) {
 }
 
 gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, const gm_solve_opts* o,
+                     gm_solve_info* info, gm_stream_t s);
+
+// Iterative refinement with an FP64 master x: r = b - A x64 (FP64), inner
+// solve A d = r with the requested method/precision to a relative 1e-4
+// (recursive residual), x64 += d; until ||b - A x64|| <= tol ||b||.
+static gm_status refine_impl(float* x, const BandedMatrix* A, const float* b, int pad, const gm_solve_opts* o,
+                             gm_solve_info* info, gm_stream_t s) {
+    BandPlan plan;
+    gm_status st = get_plan(A, &plan);
+    if (st != GM_OK) return st;
+    if (!x || !b) return fail(GM_ERR_INVALID, "solve: null vector");
+    const int N = A->rows;
+    if (info) {
+        memset(info, 0, sizeof(*info));
+        info->true_relres = -1;
+    }
+    if (N == 0) return GM_OK;
+    DevInfo dv;
+    RedScratch rs;
+    Workspace* w;
+    if ((st = dev_info(&dv)) != GM_OK) return st;
+    if ((st = red_scratch(s, &rs)) != GM_OK) return st;
+    if ((st = ws_get(s, N, plan.maxoff, o->precision != GM_PREC_FP32, &w)) != GM_OK) return st;
+    hipStream_t hs = (hipStream_t)s;
+    const int pad64 = (plan.maxoff + 63) / 64 * 64;
+    if (w->cap64 < N || w->pad64 < pad64) {
+        if (w->x64) (void)hipFree(w->x64 - w->pad64);
+        double* p = nullptr;
+        GM_HIP(hipMalloc(&p, sizeof(double) * ((size_t)N + 2 * pad64)));
+        GM_HIP(hipMemset(p, 0, sizeof(double) * ((size_t)N + 2 * pad64)));
+        w->x64 = p + pad64;
+        w->cap64 = N;
+        w->pad64 = pad64;
+    }
+    const int ge = elementwise_grid(dv, N, 4);
+    // clear x64 beyond N (a larger earlier solve may have left values there)
+    hipLaunchKernelGGL(k_x64, dim3(ge), dim3(kBlock), 0, hs, w->x64, x, N, 0);
+    GM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_zero64, dim3(1), dim3(kBlock), 0, hs, w->x64 + N, pad64);
+    hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(64), 0, hs, w->sc, 1);
+    hipLaunchKernelGGL(k_bnorm, dim3(ge), dim3(kBlock), 0, hs, b, N, 0.0, w->sc, rs.partials, rs.counter);
+    GM_LAUNCH_CHECK();
+    Scalars h;
+    GM_HIP(hipMemcpyAsync(&h, w->sc, sizeof(Scalars), hipMemcpyDeviceToHost, hs));
+    GM_HIP(hipStreamSynchronize(hs));
+    const double bb = h.bb > 0 ? h.bb : 1.0;
+    gm_solve_opts in = *o;
+    in.refine_mode = 0;
+    in.verify = 0;
+    in.tol = 1e-4;
+    int iters = 0, outer = 0;
+    double relres = 0;
+    gm_status result = GM_ERR_NOT_CONVERGED;
+    float* rhs = w->d;
+    double prev = 1e300;
+    for (outer = 0; outer < 30; ++outer) {
+        hipLaunchKernelGGL(k_refres64, dim3(elementwise_grid(dv, N, 1)), dim3(kBlock), 0, hs, A->a, A->apitch,
+                           plan.args, N, w->x64, b, rhs, w->sc, rs.partials, rs.counter);
+        GM_LAUNCH_CHECK();
+        GM_HIP(hipMemcpyAsync(&h, w->sc, sizeof(Scalars), hipMemcpyDeviceToHost, hs));
+        GM_HIP(hipStreamSynchronize(hs));
+        relres = sqrt(h.g0 / bb);
+        if (!std::isfinite(relres)) {
+            result = GM_ERR_NONFINITE;
+            break;
+        }
+        if (relres <= o->tol) {
+            result = GM_OK;
+            break;
+        }
+        if (iters >= o->max_iter || relres > 0.9 * prev) break;  // budget spent / no progress
+        prev = relres;
+        in.max_iter = o->max_iter - iters;
+        hipLaunchKernelGGL(k_fill, dim3(ge), dim3(kBlock), 0, hs, x, N, 0.0f);
+        gm_solve_info ii;
+        st = solve_impl(x, A, rhs, pad, &in, &ii, s);
+        iters += ii.iterations;
+        if (st != GM_OK && st != GM_ERR_NOT_CONVERGED) {
+            result = st;
+            break;
+        }
+        hipLaunchKernelGGL(k_x64, dim3(ge), dim3(kBlock), 0, hs, w->x64, x, N, 1);
+        GM_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_x64, dim3(ge), dim3(kBlock), 0, hs, w->x64, x, N, 2);
+    GM_LAUNCH_CHECK();
+    GM_HIP(hipStreamSynchronize(hs));
+    if (info) {
+        info->iterations = iters;
+        info->status = result;
+        info->relres = relres;
+        info->true_relres = relres;
+        info->replacements = outer;
+    }
+    if (result == GM_ERR_NOT_CONVERGED)
+        set_error("refinement: relres %.3e after %d outer steps / %d iterations", relres, outer, iters);
+    return result;
+}
+
+gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, const gm_solve_opts* o,
                      gm_solve_info* info, gm_stream_t s) {
+    if (o->refine_mode == 1) return refine_impl(x, A, b, pad, o, info, s);
     Solver S{};
     gm_status st = get_plan(A, &S.plan);
     if (st != GM_OK) return st;
@@ -889,9 +1087,11 @@ gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, c
     S.g_tile8 = grid_for_tiles(S.dv, (S.N + Tile<8>::ROWS - 1) / Tile<8>::ROWS, 3);
     S.g_elem = elementwise_grid(S.dv, S.N, 4);
     S.g_gen = elementwise_grid(S.dv, S.N, 1);
+    S.vec = (reinterpret_cast<uintptr_t>(x) & 15u) == 0;
+    S.g_upd = elementwise_grid(S.dv, (S.N + 3) / 4, 2);
     {
         const TuneVal tv = tune_lookup(S.fmt, S.plan.W, S.N);
-        set_step_shape(S, tv.R, tv.per_cu);
+        if (S.tiled) set_step_shape(S, tv.shape, tv.per_cu);
     }
     Workspace* w = S.w;
     if (!S.tiled) {
@@ -1034,6 +1234,7 @@ void gm_solve_opts_default(gm_solve_opts* o) {
     o->refine_every = 0;
     o->check_every = 16;
     o->verify = 1;
+    o->refine_mode = 0;
 }
 
 gm_status gm_solve(float* alpha, const BandedMatrix* L, const float* b, int pad, const gm_solve_opts* opts,
@@ -1153,17 +1354,16 @@ gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_re
                            L->a + (size_t)S.plan.args.center * L->apitch, w->dinv, S.N);
     }
     GM_LAUNCH_CHECK();
-    const int Rs32[] = {2, 4}, Rs16[] = {4, 8};
     const int pcs[] = {1, 2, 3, 4, 6, 8};
     hipEvent_t e0, e1;
     GM_HIP(hipEventCreate(&e0));
     GM_HIP(hipEventCreate(&e1));
     TuneVal best{0, 0, 1e30};
     if (reps < 1) reps = 5;
-    for (int ri = 0; ri < 2; ++ri) {
-        const int R = mixed ? Rs16[ri] : Rs32[ri];
+    for (int sh = 0; sh < kNumShapes; ++sh) {
+        if (!shape_ok(sh, S.fmt, S.plan.W, S.N)) continue;
         for (int pc : pcs) {
-            set_step_shape(S, R, pc);
+            set_step_shape(S, sh, pc);
             double tot = 0;
             for (int k = 0; k < reps + 1; ++k) {
                 hipLaunchKernelGGL(k_tune_reset, dim3(1), dim3(64), 0, S.hs, w->sc);
@@ -1176,7 +1376,7 @@ gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_re
                 if (k > 0) tot += ms;  // first launch warms caches
             }
             const double us = tot * 1e3 / reps;
-            if (us < best.us) best = TuneVal{R, pc, us};
+            if (us < best.us) best = TuneVal{sh, pc, us};
         }
     }
     (void)hipEventDestroy(e0);
@@ -1186,20 +1386,40 @@ gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_re
         g_tune[TuneKey{S.fmt, S.plan.W, (long)S.N}] = best;
     }
     if (out) {
-        out->rows_per_thread = best.R;
+        out->rows_per_thread = kShapeR[best.shape];
+        out->tile_rows = kShapeTY[best.shape];
         out->wg_per_cu = best.per_cu;
         out->step_us = best.us;
+        out->spmv_rows_per_thread = 0;
+        out->spmv_tile_rows = 0;
+        out->spmv_wg_per_cu = 0;
+        out->spmv_us = 0;
+    }
+    if (!mixed) {
+        // the plain FP32 product y = A x (gm_bm_ax) on the same matrix
+        int sh = 0, pc = 0;
+        double us = 0;
+        if ((st = spmv_autotune(L->a, L->apitch, kF32, S.plan.W, S.N, w->Ap, w->r, reps, s, &sh, &pc, &us)) != GM_OK)
+            return st;
+        if (out) {
+            out->spmv_rows_per_thread = kShapeR[sh];
+            out->spmv_tile_rows = kShapeTY[sh];
+            out->spmv_wg_per_cu = pc;
+            out->spmv_us = us;
+        }
     }
     return GM_OK;
 }
 
-gm_status gm_tune_set(int W, int N, int precision, int rows_per_thread, int wg_per_cu) {
+gm_status gm_tune_set(int W, int N, int precision, int rows_per_thread, int tile_rows, int wg_per_cu) {
     const int fmt = precision == GM_PREC_FP16 ? kF16 : (precision == GM_PREC_BF16 ? kBF16 : kF32);
-    const bool ok = (fmt == kF32 && (rows_per_thread == 2 || rows_per_thread == 4)) ||
-                    (fmt != kF32 && (rows_per_thread == 4 || rows_per_thread == 8));
-    if (!ok || wg_per_cu < 1 || wg_per_cu > 16) return fail(GM_ERR_INVALID, "gm_tune_set: unsupported shape");
+    int sh = -1;
+    for (int k = 0; k < kNumShapes; ++k)
+        if (kShapeR[k] == rows_per_thread && kShapeTY[k] == tile_rows) sh = k;
+    if (sh < 0 || !shape_ok(sh, fmt, W, N) || wg_per_cu < 1 || wg_per_cu > 16)
+        return fail(GM_ERR_INVALID, "gm_tune_set: unsupported shape");
     std::lock_guard<std::mutex> g(g_tune_mu);
-    g_tune[TuneKey{fmt, W, (long)N}] = TuneVal{rows_per_thread, wg_per_cu, 0};
+    g_tune[TuneKey{fmt, W, (long)N}] = TuneVal{sh, wg_per_cu, 0};
     return GM_OK;
 }
 

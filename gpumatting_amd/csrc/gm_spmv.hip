@@ -1,13 +1,18 @@
 // gm_spmv.hip -- banded matrix-vector products (BandedMatrix.cu:61-207 roles).
 //
 // Two kernels compute y = A x (or A x +- y):
-//   * k_spmv25: the Levin 5x5-band structure (offsets dy*W+dx).  A tile of
-//     kBlock*R rows stages its x window (5 image-row segments of ROWS+4 floats)
-//     in LDS with coalesced loads, then each thread streams the 25 diagonals
-//     of its R consecutive rows with 16-byte non-temporal loads.  Persistent,
-//     XCD-aware grid.  HBM-bound: N*(25*s_A + 8) algorithmic bytes.
+//   * k_spmv<Geometry>: the Levin 5x5-band structure (offsets dy*W+dx).  Each
+//     thread first issues the 16-byte non-temporal loads of its R rows' 25
+//     diagonals, the workgroup then stages its tile's x window in LDS with
+//     coalesced loads (a 2-D pixel halo, or 5 row segments for the 1-D
+//     geometry), and the products read x from LDS.  Persistent, XCD-aware
+//     grid; tile shape and grid size from the autotuner's table.
+//     HBM-bound: N*(25*s_A + 8) algorithmic bytes.
 //   * k_bm_generic: any band set (<= 64 bands), one row per thread, x read
 //     directly at i + off_k (caller's zero padding, as in the reference).
+#include <map>
+#include <utility>
+
 #include "gm_internal.h"
 
 namespace gm {
@@ -26,33 +31,145 @@ __global__ __launch_bounds__(kBlock) void k_bm_generic(float* __restrict__ out, 
     }
 }
 
-template <int R, int FMT, int MODE>
-__global__ __launch_bounds__(kBlock) void k_spmv25(float* __restrict__ out, const void* __restrict__ A, size_t pitch,
-                                                   int W, int N, const float* __restrict__ x,
-                                                   const float* __restrict__ z) {
-    __shared__ __attribute__((aligned(16))) float lds[Tile<R>::LDS_FLOATS];
-    const int ntiles = (N + Tile<R>::ROWS - 1) / Tile<R>::ROWS;
-    const TileRange tr = xcd_tiles(ntiles);
+template <class G, int FMT, int MODE>
+__global__ __launch_bounds__(kBlock) void k_spmv(float* __restrict__ out, const void* __restrict__ A, size_t pitch,
+                                                 int W, int H, int N, const float* __restrict__ x,
+                                                 const float* __restrict__ z) {
+    constexpr int R = G::R;
+    __shared__ __attribute__((aligned(16))) float lds[G::LDS_FLOATS];
+    const TileRange tr = xcd_tiles(G::ntiles(W, H, N));
     for (int t = tr.begin; t < tr.end; t += tr.step) {
-        const int i0 = t * Tile<R>::ROWS;
-        const int ib = i0 + R * threadIdx.x;
+        G g;
+        g.at(t, W, H, N);
         Bands<R, FMT> av;
-        if (ib < N) load_bands<R, FMT, kDiagStored>(A, pitch, ib, av);
+        if (g.valid) load_bands<R, FMT, kDiagStored>(A, pitch, g.ib, av);
         __syncthreads();
-        stage5<R>(lds, i0, W, N, [&](int g) { return x[g]; });
+        g.stage(lds, W, H, N, [&](int i) { return x[i]; });
         __syncthreads();
-        if (ib < N) {
+        if (g.valid) {
             float y[R];
-            apply_bands<R, FMT, kDiagStored>(av, lds + R * threadIdx.x, y);
+            apply_bands<R, FMT, kDiagStored, G::STRIDE>(av, g.xs(lds), y);
             if (MODE != 0) {
                 float zz[R];
-                load_rows<R>(z, ib, N, zz);
+                load_rows<R>(z, g.ib, N, zz);
 #pragma unroll
                 for (int j = 0; j < R; ++j) y[j] = MODE == 1 ? y[j] + zz[j] : y[j] - zz[j];
             }
-            store_rows<R>(out, ib, N, y);
+            store_rows<R>(out, g.ib, N, y);
         }
     }
+}
+
+template <class G, int FMT>
+static void spmv_t(float* out, const void* A, size_t pitch, int W, int N, const float* x, const float* z, int mode,
+                   int grid, hipStream_t hs) {
+    const int H = G::kIs2D ? N / W : 0;
+    if (mode == 0)
+        hipLaunchKernelGGL((k_spmv<G, FMT, 0>), dim3(grid), dim3(kBlock), 0, hs, out, A, pitch, W, H, N, x, z);
+    else if (mode == 1)
+        hipLaunchKernelGGL((k_spmv<G, FMT, 1>), dim3(grid), dim3(kBlock), 0, hs, out, A, pitch, W, H, N, x, z);
+    else
+        hipLaunchKernelGGL((k_spmv<G, FMT, 2>), dim3(grid), dim3(kBlock), 0, hs, out, A, pitch, W, H, N, x, z);
+}
+
+template <int FMT>
+static void spmv_shape_t(float* out, const void* A, size_t pitch, int W, int N, const float* x, const float* z,
+                         int mode, int shape, int grid, hipStream_t hs) {
+    switch (shape) {
+        case kS1R2: spmv_t<Geom1D<2>, FMT>(out, A, pitch, W, N, x, z, mode, grid, hs); break;
+        case kS1R4: spmv_t<Geom1D<4>, FMT>(out, A, pitch, W, N, x, z, mode, grid, hs); break;
+        case kS1R8: spmv_t<Geom1D<8>, FMT>(out, A, pitch, W, N, x, z, mode, grid, hs); break;
+        case kS2R4Y4: spmv_t<Geom2D<4, 4>, FMT>(out, A, pitch, W, N, x, z, mode, grid, hs); break;
+        case kS2R4Y8: spmv_t<Geom2D<4, 8>, FMT>(out, A, pitch, W, N, x, z, mode, grid, hs); break;
+        case kS2R4Y16: spmv_t<Geom2D<4, 16>, FMT>(out, A, pitch, W, N, x, z, mode, grid, hs); break;
+        case kS2R8Y8: spmv_t<Geom2D<8, 8>, FMT>(out, A, pitch, W, N, x, z, mode, grid, hs); break;
+        default: spmv_t<Geom2D<8, 16>, FMT>(out, A, pitch, W, N, x, z, mode, grid, hs); break;
+    }
+}
+
+// y = A x (+- z) for the Levin structure with a given tile shape.
+static gm_status spmv_shape_launch(float* out, const void* A, size_t pitch, int fmt, int W, int N, const float* x,
+                                   const float* z, int mode, int shape, int per_cu, gm_stream_t s) {
+    DevInfo d;
+    gm_status st = dev_info(&d);
+    if (st != GM_OK) return st;
+    if (N == 0) return GM_OK;
+    if (!shape_ok(shape, fmt, W, N)) return fail(GM_ERR_INVALID, "spmv: tile shape %d unusable here", shape);
+    const int grid = grid_for_tiles(d, shape_tiles(shape, W, N), per_cu);
+    hipStream_t hs = (hipStream_t)s;
+    if (fmt == kF32) spmv_shape_t<kF32>(out, A, pitch, W, N, x, z, mode, shape, grid, hs);
+    else if (fmt == kF16) spmv_shape_t<kF16>(out, A, pitch, W, N, x, z, mode, shape, grid, hs);
+    else spmv_shape_t<kBF16>(out, A, pitch, W, N, x, z, mode, shape, grid, hs);
+    GM_LAUNCH_CHECK();
+    return GM_OK;
+}
+
+struct SpTuneKey {
+    int fmt, W;
+    long N;
+    bool operator<(const SpTuneKey& o) const {
+        return fmt != o.fmt ? fmt < o.fmt : (W != o.W ? W < o.W : N < o.N);
+    }
+};
+static std::mutex g_sp_mu;
+static std::map<SpTuneKey, std::pair<int, int>> g_sp_tune;
+
+void spmv_tune_get(int fmt, int W, long N, int* shape, int* per_cu) {
+    {
+        std::lock_guard<std::mutex> g(g_sp_mu);
+        auto it = g_sp_tune.find(SpTuneKey{fmt, W, N});
+        if (it != g_sp_tune.end()) {
+            *shape = it->second.first;
+            *per_cu = it->second.second;
+            return;
+        }
+    }
+    const int pref = fmt == kF32 ? kS2R4Y8 : kS2R8Y8;
+    *shape = shape_ok(pref, fmt, W, N) ? pref : (fmt == kF32 ? kS1R4 : kS1R8);
+    *per_cu = 4;
+}
+
+gm_status spmv_autotune(const void* A, size_t pitch, int fmt, int W, int N, float* out, const float* x, int reps,
+                        gm_stream_t s, int* best_shape, int* best_pc, double* best_us) {
+    const int pcs[] = {1, 2, 3, 4, 6, 8};
+    hipEvent_t e0, e1;
+    GM_HIP(hipEventCreate(&e0));
+    GM_HIP(hipEventCreate(&e1));
+    double best = 1e30;
+    int bs = kS1R4, bp = 4;
+    gm_status st = GM_OK;
+    for (int sh = 0; sh < kNumShapes && st == GM_OK; ++sh) {
+        if (!shape_ok(sh, fmt, W, N)) continue;
+        for (int pc : pcs) {
+            double tot = 0;
+            for (int k = 0; k < reps + 1 && st == GM_OK; ++k) {
+                GM_HIP(hipEventRecord(e0, (hipStream_t)s));
+                st = spmv_shape_launch(out, A, pitch, fmt, W, N, x, nullptr, 0, sh, pc, s);
+                GM_HIP(hipEventRecord(e1, (hipStream_t)s));
+                GM_HIP(hipEventSynchronize(e1));
+                float ms = 0;
+                GM_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (k > 0) tot += ms;
+            }
+            const double us = tot * 1e3 / reps;
+            if (us < best) {
+                best = us;
+                bs = sh;
+                bp = pc;
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st != GM_OK) return st;
+    {
+        std::lock_guard<std::mutex> g(g_sp_mu);
+        g_sp_tune[SpTuneKey{fmt, W, (long)N}] = std::make_pair(bs, bp);
+    }
+    *best_shape = bs;
+    *best_pc = bp;
+    *best_us = best;
+    return GM_OK;
 }
 
 __global__ __launch_bounds__(kBlock) void k_jacobi_generic(float* __restrict__ xx, const float* __restrict__ x,
@@ -82,15 +199,9 @@ gm_status spmv_launch(float* out, const BandedMatrix* A, const BandPlan& p, cons
     if (N == 0) return GM_OK;
     hipStream_t hs = (hipStream_t)s;
     if (tiled_ok(p, A->a, A->apitch)) {
-        constexpr int R = 4;
-        const long ntiles = (N + Tile<R>::ROWS - 1) / Tile<R>::ROWS;
-        const int g = grid_for_tiles(d, ntiles, 4);
-        if (mode == 0)
-            hipLaunchKernelGGL((k_spmv25<R, kF32, 0>), dim3(g), dim3(kBlock), 0, hs, out, A->a, A->apitch, p.W, N, x, z);
-        else if (mode == 1)
-            hipLaunchKernelGGL((k_spmv25<R, kF32, 1>), dim3(g), dim3(kBlock), 0, hs, out, A->a, A->apitch, p.W, N, x, z);
-        else
-            hipLaunchKernelGGL((k_spmv25<R, kF32, 2>), dim3(g), dim3(kBlock), 0, hs, out, A->a, A->apitch, p.W, N, x, z);
+        int shape, pc;
+        spmv_tune_get(kF32, p.W, N, &shape, &pc);
+        return spmv_shape_launch(out, A->a, A->apitch, kF32, p.W, N, x, z, mode, shape, pc, s);
     } else {
         const int g = elementwise_grid(d, N, 1);
         if (mode == 0)
@@ -99,33 +210,6 @@ gm_status spmv_launch(float* out, const BandedMatrix* A, const BandPlan& p, cons
             hipLaunchKernelGGL(k_bm_generic<1>, dim3(g), dim3(kBlock), 0, hs, out, A->a, A->apitch, p.args, N, x, z);
         else
             hipLaunchKernelGGL(k_bm_generic<2>, dim3(g), dim3(kBlock), 0, hs, out, A->a, A->apitch, p.args, N, x, z);
-    }
-    GM_LAUNCH_CHECK();
-    return GM_OK;
-}
-
-// 16-bit / FP32 tiled product on a raw band array (benchmarks and the mixed
-// solver's operator checks).  fmt: kF32, kF16, kBF16; R = 4 (FP32) or 8 (16-bit).
-gm_status spmv25_raw(float* out, const void* A, size_t pitch, int fmt, int W, int N, const float* x, int per_cu,
-                     gm_stream_t s) {
-    DevInfo d;
-    gm_status st = dev_info(&d);
-    if (st != GM_OK) return st;
-    if (N == 0) return GM_OK;
-    hipStream_t hs = (hipStream_t)s;
-    if (fmt == kF32) {
-        const long ntiles = (N + Tile<4>::ROWS - 1) / Tile<4>::ROWS;
-        hipLaunchKernelGGL((k_spmv25<4, kF32, 0>), dim3(grid_for_tiles(d, ntiles, per_cu)), dim3(kBlock), 0, hs, out,
-                           A, pitch, W, N, x, (const float*)nullptr);
-    } else {
-        const long ntiles = (N + Tile<8>::ROWS - 1) / Tile<8>::ROWS;
-        const int g = grid_for_tiles(d, ntiles, per_cu);
-        if (fmt == kF16)
-            hipLaunchKernelGGL((k_spmv25<8, kF16, 0>), dim3(g), dim3(kBlock), 0, hs, out, A, pitch, W, N, x,
-                               (const float*)nullptr);
-        else
-            hipLaunchKernelGGL((k_spmv25<8, kBF16, 0>), dim3(g), dim3(kBlock), 0, hs, out, A, pitch, W, N, x,
-                               (const float*)nullptr);
     }
     GM_LAUNCH_CHECK();
     return GM_OK;

@@ -149,11 +149,12 @@ def assemble_u8(image, scribs, eps: float = 1e-2, gamma: float = 1.0, mode: int 
 
 
 def solve_opts(method=_lib.METHOD_PCG, precision=_lib.PREC_FP32, tol=1e-6, max_iter=10000, restart=0,
-               refine_every=0, check_every=16, verify=True) -> SolveOpts:
+               refine_every=0, check_every=16, verify=True, refine_mode=0) -> SolveOpts:
     o = SolveOpts()
     lib().gm_solve_opts_default(C.byref(o))
     o.method, o.precision, o.tol, o.max_iter = method, precision, tol, max_iter
     o.restart, o.refine_every, o.check_every, o.verify = restart, refine_every, check_every, int(verify)
+    o.refine_mode = refine_mode
     return o
 
 
@@ -204,8 +205,11 @@ def levin_backward(image, mu: torch.Tensor, alpha: torch.Tensor, lam=None, tgt=N
     return gim.view(H, W, 3), glam, gtg
 
 
-def autotune(L: DeviceMatrix, precision: int = _lib.PREC_FP32, reps: int = 5):
-    """gm_autotune: returns (rows_per_thread, wg_per_cu, step_us) and records it."""
+def autotune(L: DeviceMatrix, precision: int = _lib.PREC_FP32, reps: int = 5, full: bool = False):
+    """gm_autotune: returns (rows_per_thread, tile_rows, wg_per_cu, step_us) and
+    records it (full=True: the whole gm_tune_result)."""
     r = _lib.TuneResult()
     call("gm_autotune", L.ref, precision, reps, C.byref(r), _stream())
-    return r.rows_per_thread, r.wg_per_cu, r.step_us
+    if full:
+        return {f: getattr(r, f) for f, _ in r._fields_}
+    return r.rows_per_thread, r.tile_rows, r.wg_per_cu, r.step_us

@@ -132,6 +132,9 @@ typedef struct gm_solve_opts_s {
                            (50 for the 16-bit forms, none for FP32); < 0 = none */
    int check_every;     /* iterations enqueued between host status polls (0 -> 16) */
    int verify;          /* 1: confirm convergence with the FP64 true residual */
+   int refine_mode;     /* 0: FP32 x, residual replacement; 1: iterative refinement with an FP64
+                           master x (outer r = b - A x in FP64, inner solves of A d = r), which
+                           removes the FP32-x floor (~u*|A||x|/|b|) of the true residual */
 } gm_solve_opts;
 
 typedef struct gm_solve_info_s {
@@ -139,18 +142,25 @@ typedef struct gm_solve_info_s {
    int status;          /* gm_status of the solve */
    double relres;       /* last residual norm / ||b|| the iteration saw */
    double true_relres;  /* Project autotuner (no reference counterpart): times the fused CG step on
- * this matrix for every (rows per thread, workgroups per CU) candidate with
- * HIP events and records the fastest for solves of the same shape and
- * precision.  Clobbers the calling stream's solver workspace (call before a
+ * this matrix for every (tile geometry, rows per thread, workgroups per CU)
+ * candidate with HIP events and records the fastest for solves of the same
+ * shape and precision (FP32: also the plain product gm_bm_ax).  Clobbers the calling stream's solver workspace (call before a
  * solve, not during). */
 typedef struct gm_tune_result_s {
    int rows_per_thread;
+   int tile_rows;       /* 0: 1-D row tiles; > 0: 2-D pixel tiles of that many image rows */
    int wg_per_cu;
    double step_us;      /* mean duration of one fused CG step with that shape */
+   int spmv_rows_per_thread;  /* FP32 only: the plain product y = A x, tuned the same way */
+   int spmv_tile_rows;
+   int spmv_wg_per_cu;
+   double spmv_us;
 } gm_tune_result;
 gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_result* out, gm_stream_t stream);
-/* Pin a shape without timing (rows_per_thread 2|4 for FP32, 4|8 for 16-bit). */
-gm_status gm_tune_set(int W, int N, int precision, int rows_per_thread, int wg_per_cu);
+/* Pin a shape without timing: (rows_per_thread, tile_rows) one of (2,0) (4,0)
+ * (4,4) (4,8) (4,16) for FP32 and (4,0) (8,0) (4,4) (4,8) (4,16) (8,8) (8,16)
+ * for 16-bit; 2-D shapes need W % rows_per_thread == 0. */
+gm_status gm_tune_set(int W, int N, int precision, int rows_per_thread, int tile_rows, int wg_per_cu);
 
 /* FP64-accumulated ||b - A x|| / ||b|| at exit (-1 if not computed) */
    int replacements;    /* FP64 residual replacements performed */
@@ -165,18 +175,25 @@ gm_status gm_solve(float* alpha, const BandedMatrix* L, const float* b, int pad,
                    gm_solve_info* info, gm_stream_t stream);
 
 /* Project autotuner (no reference counterpart): times the fused CG step on
- * this matrix for every (rows per thread, workgroups per CU) candidate with
- * HIP events and records the fastest for solves of the same shape and
- * precision.  Clobbers the calling stream's solver workspace (call before a
+ * this matrix for every (tile geometry, rows per thread, workgroups per CU)
+ * candidate with HIP events and records the fastest for solves of the same
+ * shape and precision (FP32: also the plain product gm_bm_ax).  Clobbers the calling stream's solver workspace (call before a
  * solve, not during). */
 typedef struct gm_tune_result_s {
    int rows_per_thread;
+   int tile_rows;       /* 0: 1-D row tiles; > 0: 2-D pixel tiles of that many image rows */
    int wg_per_cu;
    double step_us;      /* mean duration of one fused CG step with that shape */
+   int spmv_rows_per_thread;  /* FP32 only: the plain product y = A x, tuned the same way */
+   int spmv_tile_rows;
+   int spmv_wg_per_cu;
+   double spmv_us;
 } gm_tune_result;
 gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_result* out, gm_stream_t stream);
-/* Pin a shape without timing (rows_per_thread 2|4 for FP32, 4|8 for 16-bit). */
-gm_status gm_tune_set(int W, int N, int precision, int rows_per_thread, int wg_per_cu);
+/* Pin a shape without timing: (rows_per_thread, tile_rows) one of (2,0) (4,0)
+ * (4,4) (4,8) (4,16) for FP32 and (4,0) (8,0) (4,4) (4,8) (4,16) (8,8) (8,16)
+ * for 16-bit; 2-D shapes need W % rows_per_thread == 0. */
+gm_status gm_tune_set(int W, int N, int precision, int rows_per_thread, int tile_rows, int wg_per_cu);
 
 /* FP64-accumulated ||b - A x|| / ||b|| (synchronises; writes *relres). */
 gm_status gm_true_relres(const BandedMatrix* A, const float* x, const float* b, double* relres,

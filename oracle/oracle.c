@@ -254,14 +254,16 @@ void orc_levin_gather_f32(float* a, size_t apitch, float* b, float* dinv, const 
  * The reference forms the window covariance from float means and float
  * products (Matting.cu:96-120), which loses ~4 digits when the colour variance
  * is ~1e-4 and eps is 1e-7.  Here the window mean and covariance sums use
- * exact double products, the inverse is adjugate/det in double, the whitened
- * pixel S(x_i - mu) is rounded to float once, (x_j - mu) is formed in double
- * and rounded to float, and the pair term -(1 + w.r)/9 and the accumulation
- * over windows (window-raster order) are float, as in the reference.  The
- * diagonal is -sum of the off-diagonal entries in double, then + lam_i. */
+ * exact double products and the inverse is adjugate/det in double; the
+ * whitened pixel S(x_i - mu) is rounded to float once; the window mean is
+ * split into float hi + lo parts and (x_j - mu) = (x_j - hi) - lo in float;
+ * per entry the window terms (1 + w.r) are summed in float in window-raster
+ * order and the sum is scaled by -1/9 once (a division by 9).  The diagonal
+ * is -sum of the off-diagonal entries in double, then + lam_i. */
 typedef struct {
     double m[3];
     double S[6]; /* S11 S12 S13 S22 S23 S33 */
+    float mh[3], ml[3];
 } orc_wacc_t;
 
 static void orc_window_stats_acc(orc_wacc_t* w, const float* im, int nch, size_t pitch,
@@ -294,6 +296,10 @@ static void orc_window_stats_acc(orc_wacc_t* w, const float* im, int nch, size_t
     w->S[0] = d11 / det; w->S[1] = d12 / det; w->S[2] = d13 / det;
     w->S[3] = d22 / det; w->S[4] = d23 / det; w->S[5] = d33 / det;
     w->m[0] = m0; w->m[1] = m1; w->m[2] = m2;
+    for (int c = 0; c < 3; ++c) {
+        w->mh[c] = (float)w->m[c];
+        w->ml[c] = (float)(w->m[c] - (double)w->mh[c]);
+    }
 }
 
 void orc_levin_gather_acc_f32(float* a, size_t apitch, float* b, float* dinv, const float* im,
@@ -325,19 +331,22 @@ void orc_levin_gather_acc_f32(float* a, size_t apitch, float* b, float* dinv, co
                     for (int vy = cy - 1; vy <= cy + 1; ++vy)
                         for (int vx = cx - 1; vx <= cx + 1; ++vx) {
                             const float* pj = im + ((size_t)vy * pitch + vx) * nch;
-                            float rx = (float)((double)pj[0] - w->m[0]);
-                            float ry = (float)((double)pj[1] - w->m[1]);
-                            float rz = (float)((double)pj[2] - w->m[2]);
+                            float rx = (pj[0] - w->mh[0]) - w->ml[0];
+                            float ry = (pj[1] - w->mh[1]) - w->ml[1];
+                            float rz = (pj[2] - w->mh[2]) - w->ml[2];
                             float s = 1.0f + wx * rx;
                             s = s + wy * ry;
                             s = s + wz * rz;
-                            acc[CB + (vx - px) + 5 * (vy - py)] += -s / 9.0f;
+                            acc[CB + (vx - px) + 5 * (vy - py)] += s;
                         }
                 }
             size_t i = (size_t)px + (size_t)py * W;
             double off = 0;
             for (int k = 0; k < NB; ++k)
-                if (k != CB) off += (double)acc[k];
+                if (k != CB) {
+                    acc[k] = -acc[k] / 9.0f;
+                    off += (double)acc[k];
+                }
             acc[CB] = (float)(-off);
             float l = lam ? lam[i] : 0.f;
             if (l != 0.f) {

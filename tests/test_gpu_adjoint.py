@@ -34,7 +34,7 @@ def test_autograd_vs_fp64_reference_gradient(gpu, W, H, seed):
     lam = torch.from_numpy(p.lam).cuda().requires_grad_(True)
     tgt = torch.from_numpy(p.target).cuda().requires_grad_(True)
     gt = torch.from_numpy(p.alpha_gt).cuda()
-    alpha = autograd.matte(im, lam, tgt, 1e-7, 1e-7)
+    alpha = autograd.matte(im, lam, tgt, 1e-7, 1e-9)
     loss = ((alpha - gt) ** 2).sum()
     loss.backward()
     L64, a64, gim, glam, gtg = O.matte_loss_grad_f64(p.image, p.lam, p.target, p.alpha_gt, 1e-7)
@@ -42,4 +42,4 @@ def test_autograd_vs_fp64_reference_gradient(gpu, W, H, seed):
     for got, ref in ((im.grad, gim), (lam.grad, glam.reshape(H, W)), (tgt.grad, gtg.reshape(H, W))):
         g = got.detach().cpu().numpy().astype(np.float64)
         rel = np.linalg.norm(g - ref) / np.linalg.norm(ref)
-        assert rel <= 2e-3, rel
+        assert rel <= 1e-4, rel

@@ -17,16 +17,16 @@ def test_batched_matches_fp64_per_system(gpu, prec):
         b.assemble(i, p.image, 1e-7)
         for j, (lam, tgt) in enumerate(rhs):
             b.set_rhs(i, j, lam, tgt)
-    b.solve(tol=1e-8, max_iter=20000, check_every=5)
+    b.solve(tol=1e-7, max_iter=20000, check_every=5)
     for i, (p, rhs) in enumerate(work):
         for j, (lam, tgt) in enumerate(rhs):
             inf = b.info(i, j)
-            assert inf.status == 0 and inf.true_relres <= 1.5e-7, (i, j, inf.true_relres)
+            assert inf.status == 0 and inf.true_relres <= 1e-7, (i, j, inf.true_relres)
             a, bb, _ = O.levin_gather_acc_f32(p.image, lam, tgt, 1e-7)
             xe, _, _ = O.pcg_f64(a.astype(np.float64), O.bands25(p.W), bb.astype(np.float64), np.zeros(p.N),
                                  O.pad_for(p.W), 1e-13)
             x = b.alpha(i, j).cpu().numpy()
-            assert np.abs(x - xe).max() <= 2e-4, (i, j)
+            assert np.abs(x - xe).max() <= 1e-3, (i, j)
     b.close()
 
 
