@@ -168,6 +168,7 @@ def main():
             tr = matting.autotune(L, prec, reps=3, full=True)
             tuned[name] = {k: (round(v, 2) if isinstance(v, float) else v) for k, v in tr.items()}
             log(f"autotune {name}: {tuned[name]}")
+            log("autotune candidates:\n" + lib().gm_autotune_report().decode())
             step(prec)
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -203,6 +204,10 @@ def main():
     e1.record()
     torch.cuda.synchronize()
     spmv_ms = e0.elapsed_time(e1) / args.spmv_reps
+    ft = tuned.get("fp32")
+    sp_r, sp_ty, sp_wg = (ft["spmv_rows_per_thread"], ft["spmv_tile_rows"], ft["spmv_wg_per_cu"]) if ft else (4, 4, 8)
+    spmv_kernel = (f"gm::k_spmv<gm::Geom2D<{sp_r}, {sp_ty}>, 0, 0>" if sp_ty > 0 else f"gm::k_spmv<gm::Geom1D<{sp_r}>, 0, 0>") \
+        + f" ({sp_wg} workgroups/CU; banded SpMV, 25 FP32 diagonals)"
     spmv_bytes = N * (25 * 4 + 8)
     spmv_gbs = spmv_bytes / (spmv_ms * 1e-3) / 1e9
     log(f"spmv: {spmv_ms * 1e3:.1f} us/launch, {spmv_gbs:.0f} GB/s algorithmic")
@@ -246,7 +251,7 @@ def main():
         "autotuned_step": tuned,
         "spmv_gbs": round(spmv_gbs, 1),
         "spmv_us": round(spmv_ms * 1e3, 2),
-        "roofline": {"bound": "hbm", "kernel": "k_spmv25<4,F32,0> (banded SpMV, 25 FP32 diagonals)",
+        "roofline": {"bound": "hbm", "kernel": spmv_kernel,
                      "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": None,
                      "algorithmic_bytes": spmv_bytes},

@@ -21,6 +21,7 @@
 #include <string.h>
 
 #include <map>
+#include <string>
 #include <utility>
 
 #include "gm_internal.h"
@@ -757,7 +758,9 @@ struct TuneKey {
 struct TuneVal {
     int shape, per_cu;
     double us;
+    int upd_pc;  // workgroups per CU of the update kernel (0 = default)
 };
+static thread_local std::string g_tune_report;
 static std::mutex g_tune_mu;
 static std::map<TuneKey, TuneVal> g_tune;
 
@@ -765,8 +768,9 @@ static TuneVal tune_lookup(int fmt, int W, long N) {
     std::lock_guard<std::mutex> g(g_tune_mu);
     auto it = g_tune.find(TuneKey{fmt, W, N});
     if (it != g_tune.end()) return it->second;
-    if (fmt == kF32) return shape_ok(kS2R4Y8, fmt, W, N) ? TuneVal{kS2R4Y8, 4, 0} : TuneVal{kS1R4, 4, 0};
-    return shape_ok(kS2R8Y8, fmt, W, N) ? TuneVal{kS2R8Y8, 3, 0} : TuneVal{kS1R8, 3, 0};
+    // defaults: the shapes the tuner picks for the 3840x2160 Levin matrix
+    if (fmt == kF32) return shape_ok(kS2R4Y4, fmt, W, N) ? TuneVal{kS2R4Y4, 2, 0, 0} : TuneVal{kS1R4, 4, 0, 0};
+    return shape_ok(kS2R8Y16, fmt, W, N) ? TuneVal{kS2R8Y16, 2, 0, 0} : TuneVal{kS1R8, 3, 0, 0};
 }
 
 struct Solver {
@@ -1092,6 +1096,7 @@ gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, c
     {
         const TuneVal tv = tune_lookup(S.fmt, S.plan.W, S.N);
         if (S.tiled) set_step_shape(S, tv.shape, tv.per_cu);
+        if (tv.upd_pc > 0) S.g_upd = grid_for_tiles(S.dv, (S.N + 4L * kBlock - 1) / (4L * kBlock), tv.upd_pc);
     }
     Workspace* w = S.w;
     if (!S.tiled) {
@@ -1358,8 +1363,10 @@ gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_re
     hipEvent_t e0, e1;
     GM_HIP(hipEventCreate(&e0));
     GM_HIP(hipEventCreate(&e1));
-    TuneVal best{0, 0, 1e30};
+    TuneVal best{0, 0, 1e30, 0};
     if (reps < 1) reps = 5;
+    char line[160];
+    g_tune_report = "kernel shape R TY wg_per_cu us\n";
     for (int sh = 0; sh < kNumShapes; ++sh) {
         if (!shape_ok(sh, S.fmt, S.plan.W, S.N)) continue;
         for (int pc : pcs) {
@@ -1376,7 +1383,42 @@ gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_re
                 if (k > 0) tot += ms;  // first launch warms caches
             }
             const double us = tot * 1e3 / reps;
-            if (us < best.us) best = TuneVal{sh, pc, us};
+            snprintf(line, sizeof(line), "cgstep %d %d %d %d %.2f\n", sh, kShapeR[sh], kShapeTY[sh], pc, us);
+            g_tune_report += line;
+            if (us < best.us) best = TuneVal{sh, pc, us, 0};
+        }
+    }
+    // the update kernel's grid: fewer workgroups = fewer device-scope
+    // fence + atomic arrivals in its reduction
+    set_step_shape(S, best.shape, best.per_cu);
+    S.x = w->d;
+    S.vec = true;
+    double best_upd = 1e30;
+    for (int pc : {1, 2, 4, 8, 16}) {
+        S.g_upd = grid_for_tiles(S.dv, (S.N + 4L * kBlock - 1) / (4L * kBlock), pc);
+        double tot = 0;
+        for (int k = 0; k < reps + 1; ++k) {
+            hipLaunchKernelGGL(k_tune_reset, dim3(1), dim3(64), 0, S.hs, w->sc);
+            GM_HIP(hipEventRecord(e0, S.hs));
+            if (S.pm == kJacobi)
+                hipLaunchKernelGGL((k_cgupdate<kJacobi, true>), dim3(S.g_upd), dim3(kBlock), 0, S.hs, S.x, w->r, w->dinv,
+                                   w->p0, w->p1, w->Ap, S.N, 0, w->sc, S.rs.partials, S.rs.counter);
+            else
+                hipLaunchKernelGGL((k_cgupdate<kScaled, true>), dim3(S.g_upd), dim3(kBlock), 0, S.hs, S.x, w->r, w->dinv,
+                                   w->p0, w->p1, w->Ap, S.N, 0, w->sc, S.rs.partials, S.rs.counter);
+            GM_LAUNCH_CHECK();
+            GM_HIP(hipEventRecord(e1, S.hs));
+            GM_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            GM_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (k > 0) tot += ms;
+        }
+        const double us = tot * 1e3 / reps;
+        snprintf(line, sizeof(line), "cgupdate - - - %d %.2f (grid %d)\n", pc, us, S.g_upd);
+        g_tune_report += line;
+        if (us < best_upd) {
+            best_upd = us;
+            best.upd_pc = pc;
         }
     }
     (void)hipEventDestroy(e0);
@@ -1386,6 +1428,8 @@ gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_re
         g_tune[TuneKey{S.fmt, S.plan.W, (long)S.N}] = best;
     }
     if (out) {
+        out->update_wg_per_cu = best.upd_pc;
+        out->update_us = best_upd;
         out->rows_per_thread = kShapeR[best.shape];
         out->tile_rows = kShapeTY[best.shape];
         out->wg_per_cu = best.per_cu;
@@ -1411,6 +1455,8 @@ gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_re
     return GM_OK;
 }
 
+const char* gm_autotune_report(void) { return g_tune_report.c_str(); }
+
 gm_status gm_tune_set(int W, int N, int precision, int rows_per_thread, int tile_rows, int wg_per_cu) {
     const int fmt = precision == GM_PREC_FP16 ? kF16 : (precision == GM_PREC_BF16 ? kBF16 : kF32);
     int sh = -1;
@@ -1419,7 +1465,7 @@ gm_status gm_tune_set(int W, int N, int precision, int rows_per_thread, int tile
     if (sh < 0 || !shape_ok(sh, fmt, W, N) || wg_per_cu < 1 || wg_per_cu > 16)
         return fail(GM_ERR_INVALID, "gm_tune_set: unsupported shape");
     std::lock_guard<std::mutex> g(g_tune_mu);
-    g_tune[TuneKey{fmt, W, (long)N}] = TuneVal{sh, wg_per_cu, 0};
+    g_tune[TuneKey{fmt, W, (long)N}] = TuneVal{sh, wg_per_cu, 0, 0};
     return GM_OK;
 }
 

@@ -124,9 +124,10 @@ void spmv_tune_get(int fmt, int W, long N, int* shape, int* per_cu) {
             return;
         }
     }
-    const int pref = fmt == kF32 ? kS2R4Y8 : kS2R8Y8;
+    // defaults: the shapes the tuner picks for the 3840x2160 Levin matrix
+    const int pref = fmt == kF32 ? kS2R4Y4 : kS2R8Y8;
     *shape = shape_ok(pref, fmt, W, N) ? pref : (fmt == kF32 ? kS1R4 : kS1R8);
-    *per_cu = 4;
+    *per_cu = fmt == kF32 ? 8 : 4;
 }
 
 gm_status spmv_autotune(const void* A, size_t pitch, int fmt, int W, int N, float* out, const float* x, int reps,

@@ -155,8 +155,12 @@ typedef struct gm_tune_result_s {
    int spmv_tile_rows;
    int spmv_wg_per_cu;
    double spmv_us;
+   int update_wg_per_cu;   /* grid of the CG update kernel */
+   double update_us;
 } gm_tune_result;
 gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_result* out, gm_stream_t stream);
+/* Text table of every candidate timed by the last gm_autotune on this thread. */
+const char* gm_autotune_report(void);
 /* Pin a shape without timing: (rows_per_thread, tile_rows) one of (2,0) (4,0)
  * (4,4) (4,8) (4,16) for FP32 and (4,0) (8,0) (4,4) (4,8) (4,16) (8,8) (8,16)
  * for 16-bit; 2-D shapes need W % rows_per_thread == 0. */
@@ -188,8 +192,12 @@ typedef struct gm_tune_result_s {
    int spmv_tile_rows;
    int spmv_wg_per_cu;
    double spmv_us;
+   int update_wg_per_cu;   /* grid of the CG update kernel */
+   double update_us;
 } gm_tune_result;
 gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_result* out, gm_stream_t stream);
+/* Text table of every candidate timed by the last gm_autotune on this thread. */
+const char* gm_autotune_report(void);
 /* Pin a shape without timing: (rows_per_thread, tile_rows) one of (2,0) (4,0)
  * (4,4) (4,8) (4,16) for FP32 and (4,0) (8,0) (4,4) (4,8) (4,16) (8,8) (8,16)
  * for 16-bit; 2-D shapes need W % rows_per_thread == 0. */
