@@ -236,7 +236,7 @@ __global__ __launch_bounds__(kBlock) void kb_trueres(const BImg* __restrict__ im
     constexpr int R = kBR;
     constexpr int SEG = Tile<R>::SEG;
     __shared__ __attribute__((aligned(16))) float lds[Tile<R>::LDS_FLOATS];
-    __shared__ double part[kMaxRhs][2];
+    __shared__ double part[kMaxRhs][3];
     const TileRange tr = xcd_tiles(ntot);
     for (int t = tr.begin; t < tr.end; t += tr.step) {
         const int im = tile_img[t];
@@ -255,14 +255,16 @@ __global__ __launch_bounds__(kBlock) void kb_trueres(const BImg* __restrict__ im
         if (ib < I.N) load_bands<R, kF32, kDiagCaller>(I.L, I.pitch, ib, av);
         for (int j = 0; j < I.nsys; ++j) {
             if (!((mask >> j) & 1u)) {
-                if (threadIdx.x == 0) part[j][0] = part[j][1] = 0;
+                if (threadIdx.x == 0) part[j][0] = part[j][1] = part[j][2] = 0;
                 continue;
             }
             const BSys S = sys[I.sys0 + j];
+            // periodic replacement of a running system: measure the drift
+            const bool drift = op == 1 && sc[I.sys0 + j].verify == 0;
             __syncthreads();
             stage5<R>(lds, i0, I.W, I.N, [&](int g) { return S.x[g]; });
             __syncthreads();
-            double v0 = 0, v1 = 0;
+            double v0 = 0, v1 = 0, v2 = 0;
             if (ib < I.N) {
                 const float* xs = lds + R * threadIdx.x;
                 double acc[R];
@@ -276,10 +278,11 @@ __global__ __launch_bounds__(kBlock) void kb_trueres(const BImg* __restrict__ im
                     for (int q = 0; q < R; ++q)
                         acc[q] = fma((double)av.v[k].get(q), (double)xs[s5 * SEG + q + dx], acc[q]);
                 }
-                float dd[R], bb[R], di[R], out[R];
+                float dd[R], bb[R], di[R], out[R], ro[R];
                 load_rows<R>(S.d, ib, I.N, dd);
                 load_rows<R>(S.b, ib, I.N, bb);
                 load_rows<R>(S.dinv, ib, I.N, di);
+                if (drift) load_rows<R>(S.r, ib, I.N, ro);
 #pragma unroll
                 for (int q = 0; q < R; ++q) {
                     const double tq = fma((double)dd[q], (double)xs[2 * SEG + q + 2], acc[q]) - (double)bb[q];
@@ -287,27 +290,32 @@ __global__ __launch_bounds__(kBlock) void kb_trueres(const BImg* __restrict__ im
                     if (ib + q < I.N) {
                         v1 += tq * tq;
                         v0 += (double)(out[q] * (di[q] * out[q]));
+                        if (drift) v2 += (double)(out[q] - ro[q]) * (double)(out[q] - ro[q]);
                     }
                 }
                 if (op != 0) store_rows<R>(S.r, ib, I.N, out);
             }
             const double b0 = block_sum(v0);
             const double b1 = block_sum(v1);
+            const double b2 = block_sum(v2);
             if (threadIdx.x == 0) {
                 part[j][0] = b0;
                 part[j][1] = b1;
+                part[j][2] = b2;
             }
         }
         if (threadIdx.x == 0)
             for (int j = 0; j < I.nsys; ++j) {
-                partials[((size_t)t * kMaxRhs + j) * 2 + 0] = part[j][0];
-                partials[((size_t)t * kMaxRhs + j) * 2 + 1] = part[j][1];
+                partials[((size_t)t * kMaxRhs + j) * 3 + 0] = part[j][0];
+                partials[((size_t)t * kMaxRhs + j) * 3 + 1] = part[j][1];
+                partials[((size_t)t * kMaxRhs + j) * 3 + 2] = part[j][2];
             }
         if (image_arrive(&counters[im], I.ntiles)) {
             for (int j = 0; j < I.nsys; ++j) {
                 if (!((mask >> j) & 1u)) continue;
-                const double rz = image_sum(partials, I.tile0, I.ntiles, 2 * kMaxRhs, 2 * j);
-                const double rr = image_sum(partials, I.tile0, I.ntiles, 2 * kMaxRhs, 2 * j + 1);
+                const double rz = image_sum(partials, I.tile0, I.ntiles, 3 * kMaxRhs, 3 * j);
+                const double rr = image_sum(partials, I.tile0, I.ntiles, 3 * kMaxRhs, 3 * j + 1);
+                const double d2 = image_sum(partials, I.tile0, I.ntiles, 3 * kMaxRhs, 3 * j + 2);
                 if (threadIdx.x == 0) {
                     Scalars* s = &sc[I.sys0 + j];
                     s->g0 = rr;
@@ -325,7 +333,9 @@ __global__ __launch_bounds__(kBlock) void kb_trueres(const BImg* __restrict__ im
                                 else s->stall = 0;
                                 s->g1 = rr;
                             } else {
-                                s->beta = s->rz_prev > 0 ? (float)(rz / s->rz_prev) : 0.f;
+                                // periodic: keep the direction unless the recursion drifted
+                                s->beta = d2 > kDrift2 * rr ? 0.f
+                                        : (s->rz_prev > 0 ? (float)(rz / s->rz_prev) : 0.f);
                                 s->replacements += 1;
                             }
                             s->rz = rz;
@@ -524,7 +534,7 @@ gm_status gm_batch_create(gm_batch** out, int nimages, const int* widths, const 
     BTRY(hipMalloc(&b->dsc, sizeof(Scalars) * b->nsys));
     BTRY(hipMemset(b->dsc, 0, sizeof(Scalars) * b->nsys));
     BTRY(hipMalloc(&b->dtile, sizeof(int) * tiles));
-    const size_t nparts = (size_t)tiles * kMaxRhs * 2;
+    const size_t nparts = (size_t)tiles * kMaxRhs * 3;
     BTRY(hipMalloc(&b->dpart, sizeof(double) * nparts + sizeof(unsigned int) * nimages + 256));
     BTRY(hipMemset(b->dpart, 0, sizeof(double) * nparts + sizeof(unsigned int) * nimages + 256));
     b->dcnt = reinterpret_cast<unsigned int*>(b->dpart + nparts);

@@ -85,6 +85,12 @@ int elementwise_grid(const DevInfo& d, long n, int per_thread);
 
 // ---------------------------------------------------------------------------
 // device scalar block driving the iterative solvers
+// Periodic residual replacement keeps the search direction only while the
+// recursive residual agrees with the true one: ||r_true - r_rec||^2 <=
+// kDrift2 * ||r_true||^2 (10 %); past that the recursion (run with an operator
+// rounded to 16 bits, or just drifted) restarts with beta = 0.
+constexpr double kDrift2 = 0.01;
+
 struct Scalars {
     double rz;       // <r, z> of the current residual (z = M r)
     double rz_prev;  // previous <r, z> (kept for residual replacement)

@@ -41,7 +41,10 @@ __device__ __forceinline__ bool stopped(const Scalars* sc) { return sc->done != 
 
 // Finaliser shared by the residual-producing kernels: record <r,z>, ||r||^2
 // and test convergence.  start: beta = 0 (next step is p = -z).
-__device__ __forceinline__ void fin_residual(Scalars* sc, double rz, double rr, bool start, bool replace) {
+// drift2 = ||r_true - r_rec||^2 and rsq = ||r_true||^2 in the space r is kept in
+// (periodic replacements only).
+__device__ __forceinline__ void fin_residual(Scalars* sc, double rz, double rr, bool start, bool replace,
+                                             double drift2 = 0.0, double rsq = 0.0) {
     if (!isfinite(rz) || !isfinite(rr)) {
         sc->done = kNonFinite;
         return;
@@ -56,6 +59,8 @@ __device__ __forceinline__ void fin_residual(Scalars* sc, double rz, double rr, 
             if (sc->g1 > 0 && rr > 0.25 * sc->g1) sc->stall += 1;
             else sc->stall = 0;
             sc->g1 = rr;
+        } else if (drift2 > kDrift2 * rsq) {
+            sc->beta = 0.f;  // periodic replacement, recursion drifted: restart
         } else {
             // periodic replacement: keep the search direction
             sc->beta = sc->rz_prev > 0 ? (float)(rz / sc->rz_prev) : 0.f;
@@ -292,7 +297,7 @@ __global__ __launch_bounds__(kBlock) void k_trueres25(const float* __restrict__ 
     const int ntiles = (N + Tile<R>::ROWS - 1) / Tile<R>::ROWS;
     const TileRange tr = xcd_tiles(ntiles);
     constexpr int SEG = Tile<R>::SEG;
-    double v[2] = {0.0, 0.0};
+    double v[3] = {0.0, 0.0, 0.0};
     for (int t = tr.begin; t < tr.end; t += tr.step) {
         const int i0 = t * Tile<R>::ROWS;
         __syncthreads();
@@ -312,9 +317,10 @@ __global__ __launch_bounds__(kBlock) void k_trueres25(const float* __restrict__ 
 #pragma unroll
                 for (int j = 0; j < R; ++j) acc[j] = fma((double)a.get(j), (double)xs[s * SEG + j + dx], acc[j]);
             }
-            float bb[R], dd[R], out[R];
+            float bb[R], dd[R], out[R], ro[R];
             load_rows<R>(b, ib, N, bb);
             if (PM != kPlain) load_rows<R>(dinv, ib, N, dd);
+            if (op == 1) load_rows<R>(r, ib, N, ro);
 #pragma unroll
             for (int j = 0; j < R; ++j) {
                 const double tj = acc[j] - (double)bb[j];
@@ -325,15 +331,16 @@ __global__ __launch_bounds__(kBlock) void k_trueres25(const float* __restrict__ 
                     v[1] += tj * tj;
                     if (PM == kJacobi) v[0] += (double)(rj * (dd[j] * rj));
                     else v[0] += (double)(rj * rj);
+                    if (op == 1) v[2] += (double)(rj - ro[j]) * (double)(rj - ro[j]);
                 }
             }
             if (op != 0) store_rows<R>(r, ib, N, out);
         }
     }
-    grid_reduce<2>(v, partials, counter, [&](double* tt) {
+    grid_reduce<3>(v, partials, counter, [&](double* tt) {
         sc->g0 = tt[1];
         if (op == 0) return;
-        fin_residual(sc, tt[0], tt[1], op == 2, true);
+        fin_residual(sc, tt[0], tt[1], op == 2, true, tt[2], PM == kScaled ? tt[0] : tt[1]);
     });
 }
 
@@ -537,21 +544,22 @@ __global__ __launch_bounds__(kBlock) void k_trueres_g(const float* __restrict__ 
                                                       const float* __restrict__ dinv, float* __restrict__ r, int op,
                                                       Scalars* sc, double* partials, unsigned int* counter) {
     if (op == 1 && sc->done != kRun) return;
-    double v[2] = {0.0, 0.0};
+    double v[3] = {0.0, 0.0, 0.0};
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < N; i += gridDim.x * kBlock) {
         double s = 0.0;
         for (int k = 0; k < ba.nb; ++k)
             s = fma((double)a[(size_t)i + (size_t)k * pitch], (double)x[(long long)i + ba.off[k]], s);
         const double t = s - (double)b[i];
         const float rj = (float)t;
+        if (op == 1) v[2] += (double)(rj - r[i]) * (double)(rj - r[i]);
         if (op != 0) r[i] = rj;
         v[1] += t * t;
         v[0] += PM == kJacobi ? (double)(rj * (dinv[i] * rj)) : (double)(rj * rj);
     }
-    grid_reduce<2>(v, partials, counter, [&](double* tt) {
+    grid_reduce<3>(v, partials, counter, [&](double* tt) {
         sc->g0 = tt[1];
         if (op == 0) return;
-        fin_residual(sc, tt[0], tt[1], op == 2, true);
+        fin_residual(sc, tt[0], tt[1], op == 2, true, tt[2], tt[1]);
     });
 }
 
