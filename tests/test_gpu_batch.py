@@ -9,8 +9,23 @@ from oracle import pyoracle as O
 pytestmark = pytest.mark.gpu
 
 
+def _sparse(a64, W, N):
+    import scipy.sparse as sp
+    rows, cols, vals = [], [], []
+    for k, o in enumerate(O.bands25(W)):
+        i = np.arange(N)
+        m = (i + o >= 0) & (i + o < N)
+        rows.append(i[m]), cols.append(i[m] + o), vals.append(a64[k][:N][m])
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(N, N))
+
+
 @pytest.mark.parametrize("prec", [PREC_FP32, PREC_FP16])
 def test_batched_matches_fp64_per_system(gpu, prec):
+    """Every system reaches the tolerance, checked twice: by the solver's own
+    FP64 true residual and by an independent FP64 residual of the returned
+    alpha on the host; the alpha error against the FP64 solution is within
+    what that residual allows (||x - x*|| <= ||A x - b|| / lambda_min(A))."""
+    import scipy.sparse.linalg as sla
     work = synth.config_batched(n_images=5, nrhs=3, seed=77, wmin=20, wmax=130, hmin=15, hmax=90)
     b = Batch([(p.W, p.H) for p, _ in work], 3, prec)
     for i, (p, rhs) in enumerate(work):
@@ -23,10 +38,17 @@ def test_batched_matches_fp64_per_system(gpu, prec):
             inf = b.info(i, j)
             assert inf.status == 0 and inf.true_relres <= 1e-7, (i, j, inf.true_relres)
             a, bb, _ = O.levin_gather_acc_f32(p.image, lam, tgt, 1e-7)
-            xe, _, _ = O.pcg_f64(a.astype(np.float64), O.bands25(p.W), bb.astype(np.float64), np.zeros(p.N),
-                                 O.pad_for(p.W), 1e-13)
-            x = b.alpha(i, j).cpu().numpy()
-            assert np.abs(x - xe).max() <= 1e-3, (i, j)
+            a64, b64 = a.astype(np.float64), bb.astype(np.float64)
+            xe, _, _ = O.pcg_f64(a64, O.bands25(p.W), b64, np.zeros(p.N), O.pad_for(p.W), 1e-13)
+            x = b.alpha(i, j).cpu().numpy().astype(np.float64)
+            A = _sparse(a64, p.W, p.N)
+            res = np.linalg.norm(A @ x - b64)
+            assert res <= 1.01e-7 * np.linalg.norm(b64), (i, j, res)
+            lmin = sla.eigsh(A, k=1, sigma=0, which="LM", return_eigenvectors=False)[0]
+            assert lmin > 0
+            err = np.linalg.norm(x - xe)
+            assert err <= res / lmin + 1e-6 * np.linalg.norm(xe), (i, j, err, res / lmin)
+            assert np.abs(x - xe).max() <= 1e-2, (i, j)
     b.close()
 
 
