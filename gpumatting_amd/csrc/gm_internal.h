@@ -91,6 +91,26 @@ int elementwise_grid(const DevInfo& d, long n, int per_thread);
 // rounded to 16 bits, or just drifted) restarts with beta = 0.
 constexpr double kDrift2 = 0.01;
 
+// Hand-off from the CG step kernel to the update kernel (see Scalars).
+struct Hand {
+    double rz, rz_prev, rr;
+    float beta;
+    int iters;
+    int mode;    // kHandRun: update; kHandStop: publish this state and stop; kHandIdle: nothing to do
+    int done, verify;
+    int npart;   // partial sums the step kernel left (one per workgroup)
+};
+enum { kHandRun = 0, kHandStop = 1, kHandIdle = 2 };
+
+// The two CG kernels have no grid-wide reduction of their own: each
+// workgroup stores its partial sums and the NEXT kernel folds them (every
+// workgroup, in the same fixed order, so all agree bit for bit).  The kernel
+// boundary orders the stores before the loads, so the iteration needs no
+// device-scope fence or atomic (each costs an L2 write-back on gfx950, and they
+// serialise per XCD).  To keep every field read-only while a kernel runs, the
+// step kernel reads the Scalars and writes only `hand`; the update kernel
+// reads only `hand` and writes the rest; `pend` marks update-kernel partials
+// not yet folded into rz / rr.
 struct Scalars {
     double rz;       // <r, z> of the current residual (z = M r)
     double rz_prev;  // previous <r, z> (kept for residual replacement)
@@ -108,7 +128,10 @@ struct Scalars {
     int replacements;
     int stall;       // verifications in a row that failed to reduce the true residual 4x
     unsigned int counter;  // arrival counter of the last-block reduction
+    int pend;        // 1: partial sums of <r,z>, ||r||^2 of the last update await folding
+    int npart;       // number of those partial sums
     unsigned int pad2_;
+    Hand hand;
 };
 
 // ---------------------------------------------------------------------------
@@ -192,6 +215,52 @@ __device__ __forceinline__ double block_sum(double v) {
     if (threadIdx.x == 0)
         for (int w = 0; w < kWaves; ++w) s += red[w];
     return s;
+}
+
+// Deferred partial sums: part[q * kMaxGrid + i], i < n.  Every thread of the
+// workgroup gets the totals; the summation order is fixed, so every
+// workgroup of a grid computes bit-identical totals.
+template <int NQ>
+__device__ __forceinline__ void fold_all(const double* __restrict__ part, int n, double (&out)[NQ]) {
+    __shared__ double fred[NQ][kWaves];
+    __shared__ double ftot[NQ];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        double s = 0;
+        for (int i = threadIdx.x; i < n; i += kBlock) s += part[(size_t)q * kMaxGrid + i];
+        s = wave_sum(s);
+        if (lane == 0) fred[q][wid] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < NQ) {
+        double s = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) s += fred[threadIdx.x][w];
+        ftot[threadIdx.x] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) out[q] = ftot[q];
+}
+
+// Store one workgroup's partial sums for the next kernel to fold.
+template <int NQ>
+__device__ __forceinline__ void store_partials(double (&v)[NQ], double* part) {
+    __shared__ double sred[NQ][kWaves];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const double s = wave_sum(v[q]);
+        if (lane == 0) sred[q][wid] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < NQ) {
+        double s = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) s += sred[threadIdx.x][w];
+        part[(size_t)threadIdx.x * kMaxGrid + blockIdx.x] = s;
+    }
 }
 
 // XCD-aware tile schedule for a persistent grid (gridDim.x a multiple of

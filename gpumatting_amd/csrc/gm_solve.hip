@@ -62,15 +62,20 @@ __device__ __forceinline__ void fin_residual(Scalars* sc, double rz, double rr, 
         } else if (drift2 > kDrift2 * rsq) {
             sc->beta = 0.f;  // periodic replacement, recursion drifted: restart
         } else {
-            // periodic replacement: keep the search direction
-            sc->beta = sc->rz_prev > 0 ? (float)(rz / sc->rz_prev) : 0.f;
+            // periodic replacement: keep the search direction.  The <r,z> the
+            // last step used is rz while its update's partials are pending
+            // (they are superseded by this residual), else rz_prev.
+            const double rzp = sc->pend ? sc->rz : sc->rz_prev;
+            sc->beta = rzp > 0 ? (float)(rz / rzp) : 0.f;
         }
     } else {
         sc->beta = 0.f;
     }
+    sc->rz_prev = sc->pend ? sc->rz : sc->rz_prev;
     sc->rz = rz;
     sc->rr = rr;
     sc->verify = 0;
+    sc->pend = 0;
     if (sc->tol2 >= 0 && rr <= sc->tol2) sc->done = kConverged;
     else if (sc->iters >= sc->maxit || sc->stall >= 3) sc->done = kMaxIt;  // max_iter or stagnation
 }
@@ -84,6 +89,7 @@ __global__ void k_init_scalars(Scalars* sc, int maxit) {
         z.maxit = maxit;
         z.tol2 = -1.0;
         z.counter = sc->counter;  // the reduction counter lives in the scratch block; keep it
+        z.hand.mode = kHandIdle;
         *sc = z;
     }
 }
@@ -146,19 +152,72 @@ __global__ __launch_bounds__(kBlock) void k_resid25(const float* __restrict__ A,
     grid_reduce<2>(v, partials, counter, [&](double* tt) { fin_residual(sc, tt[0], tt[1], true, false); });
 }
 
+// Start of the CG step kernel: fold the update kernel's pending partial sums
+// (<r,z>, ||r||^2 of the new residual) and run the convergence test the
+// update would have run.  Returns false when the iteration stops here (block 0
+// has recorded why in sc->hand); fills rz, rz_prev, rr, beta, iters otherwise.
+struct StepIn {
+    double rz, rz_prev, rr;
+    float beta;
+    int iters;
+};
+__device__ __forceinline__ bool step_begin(Scalars* sc, const double* __restrict__ dpb, int verify, StepIn& in) {
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (sc->done != kRun || sc->verify) {
+        if (lead) sc->hand.mode = kHandIdle;
+        return false;
+    }
+    in.rz = sc->rz;
+    in.rz_prev = sc->rz_prev;
+    in.rr = sc->rr;
+    in.beta = sc->beta;
+    in.iters = sc->iters;
+    if (!sc->pend) return true;
+    double t[2];
+    fold_all<2>(dpb, sc->npart, t);
+    int stop = kRun, ver = 0;
+    if (!isfinite(t[0]) || !isfinite(t[1])) {
+        stop = kNonFinite;
+    } else {
+        in.rz_prev = in.rz;
+        in.beta = (float)(t[0] / in.rz);
+        in.rz = t[0];
+        in.rr = t[1];
+        if (sc->tol2 >= 0 && in.rr <= sc->tol2) {
+            if (verify) ver = 1;
+            else stop = kConverged;
+        } else if (in.iters >= sc->maxit) {
+            stop = kMaxIt;
+        }
+    }
+    if (stop == kRun && !ver) return true;
+    if (lead) sc->hand = Hand{in.rz, in.rz_prev, in.rr, in.beta, in.iters, kHandStop, stop, ver, 0};
+    return false;
+}
+
+// End of the CG step kernel: this workgroup's <p, Ap> partial; block 0 hands
+// the iteration state to the update kernel.
+__device__ __forceinline__ void step_end(Scalars* sc, double v, double* __restrict__ dpa, const StepIn& in) {
+    double vv[1] = {v};
+    store_partials<1>(vv, dpa);
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        sc->hand = Hand{in.rz, in.rz_prev, in.rr, in.beta, in.iters, kHandRun, kRun, 0, (int)gridDim.x};
+}
+
 // K1: p_new = -z + beta p_old (staged), Ap = A p_new, <p_new, Ap>.
 // G: tile geometry (Geom1D / Geom2D).
 template <class G, int FMT, int PM>
 __global__ __launch_bounds__(kBlock) void k_cgstep(const void* __restrict__ A, size_t pitch, int W, int H, int N,
                                                    const float* __restrict__ r, const float* __restrict__ dinv,
                                                    float* __restrict__ p0, float* __restrict__ p1,
-                                                   float* __restrict__ Ap, Scalars* sc, double* partials,
-                                                   unsigned int* counter) {
-    if (stopped(sc)) return;
+                                                   float* __restrict__ Ap, Scalars* sc, double* __restrict__ dpa,
+                                                   const double* __restrict__ dpb, int verify) {
+    StepIn in;
+    if (!step_begin(sc, dpb, verify, in)) return;
     constexpr int R = G::R;
     constexpr int DIAG = PM == kScaled ? kDiagUnit : kDiagStored;
-    const float beta = sc->beta;
-    const int par = sc->iters & 1;
+    const float beta = in.beta;
+    const int par = in.iters & 1;
     const float* __restrict__ pold = par ? p1 : p0;
     float* __restrict__ pnew = par ? p0 : p1;
     __shared__ __attribute__((aligned(16))) float lds[G::LDS_FLOATS];
@@ -187,13 +246,7 @@ __global__ __launch_bounds__(kBlock) void k_cgstep(const void* __restrict__ A, s
                 if (g.ib + j < N) v[0] += (double)(pc[j] * y[j]);
         }
     }
-    grid_reduce<1>(v, partials, counter, [&](double* tt) {
-        const double pAp = tt[0];
-        sc->pAp = pAp;
-        if (!isfinite(pAp)) sc->done = kNonFinite;
-        else if (!(pAp > 0)) sc->done = kBreakdown;
-        else sc->alpha = (float)(sc->rz / pAp);
-    });
+    step_end(sc, v[0], dpa, in);
 }
 
 // K2: x += alpha p (scaled: alpha s p), r += alpha Ap, <r,z>, ||r||^2 (scaled:
@@ -219,15 +272,44 @@ __device__ __forceinline__ void upd_elem(float& x, float& r, float d, float p, f
     }
 }
 
+// Publish a hand-off state into the Scalars (block 0 of the update kernel).
+__device__ __forceinline__ void publish(Scalars* sc, const Hand& h, int done, int verify, int iters, int pend,
+                                        int npart) {
+    sc->rz = h.rz;
+    sc->rz_prev = h.rz_prev;
+    sc->rr = h.rr;
+    sc->beta = h.beta;
+    sc->iters = iters;
+    sc->done = done;
+    sc->verify = verify;
+    sc->pend = pend;
+    sc->npart = npart;
+}
+
 template <int PM, bool VEC>
 __global__ __launch_bounds__(kBlock) void k_cgupdate(float* __restrict__ x, float* __restrict__ r,
                                                      const float* __restrict__ dinv, const float* __restrict__ p0,
                                                      const float* __restrict__ p1, const float* __restrict__ Ap,
-                                                     int N, int verify, Scalars* sc, double* partials,
-                                                     unsigned int* counter) {
-    if (stopped(sc)) return;
-    const float alpha = sc->alpha;
-    const float* __restrict__ p = (sc->iters & 1) ? p0 : p1;  // the p_new K1 wrote
+                                                     int N, Scalars* sc, const double* __restrict__ dpa,
+                                                     double* __restrict__ dpb) {
+    const Hand h = sc->hand;
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (h.mode != kHandRun) {
+        if (h.mode == kHandStop && lead) publish(sc, h, h.done, h.verify, h.iters, 0, 0);
+        return;
+    }
+    double t[1];
+    fold_all<1>(dpa, h.npart, t);
+    const double pAp = t[0];
+    if (!isfinite(pAp) || !(pAp > 0)) {
+        if (lead) {
+            publish(sc, h, isfinite(pAp) ? kBreakdown : kNonFinite, 0, h.iters, 0, 0);
+            sc->pAp = pAp;
+        }
+        return;
+    }
+    const float alpha = (float)(h.rz / pAp);
+    const float* __restrict__ p = (h.iters & 1) ? p0 : p1;  // the p_new K1 wrote
     double v[2] = {0.0, 0.0};
     const int tid = blockIdx.x * kBlock + threadIdx.x, nth = gridDim.x * kBlock;
     if (VEC) {
@@ -264,24 +346,36 @@ __global__ __launch_bounds__(kBlock) void k_cgupdate(float* __restrict__ x, floa
         }
     }
     if (PM == kPlain) v[1] = v[0];
-    grid_reduce<2>(v, partials, counter, [&](double* tt) {
-        const double rz = tt[0], rr = tt[1];
-        sc->iters += 1;
-        if (!isfinite(rz) || !isfinite(rr)) {
-            sc->done = kNonFinite;
-            return;
-        }
-        sc->rz_prev = sc->rz;
-        sc->beta = (float)(rz / sc->rz);
-        sc->rz = rz;
-        sc->rr = rr;
-        if (sc->tol2 >= 0 && rr <= sc->tol2) {
-            if (verify) sc->verify = 1;
-            else sc->done = kConverged;
-        } else if (sc->iters >= sc->maxit) {
-            sc->done = kMaxIt;
-        }
-    });
+    store_partials<2>(v, dpb);
+    if (lead) {
+        publish(sc, h, kRun, 0, h.iters + 1, 1, (int)gridDim.x);
+        sc->pAp = pAp;
+        sc->alpha = alpha;
+    }
+}
+
+// Fold pending update partials into the Scalars (one workgroup; the host
+// launches it before reading the final state).
+__global__ __launch_bounds__(kBlock) void k_fold(Scalars* sc, const double* __restrict__ dpb, int verify) {
+    if (sc->done != kRun || sc->verify || !sc->pend) return;
+    double t[2];
+    fold_all<2>(dpb, sc->npart, t);
+    if (threadIdx.x != 0) return;
+    sc->pend = 0;
+    if (!isfinite(t[0]) || !isfinite(t[1])) {
+        sc->done = kNonFinite;
+        return;
+    }
+    sc->rz_prev = sc->rz;
+    sc->beta = (float)(t[0] / sc->rz);
+    sc->rz = t[0];
+    sc->rr = t[1];
+    if (sc->tol2 >= 0 && sc->rr <= sc->tol2) {
+        if (verify) sc->verify = 1;
+        else sc->done = kConverged;
+    } else if (sc->iters >= sc->maxit) {
+        sc->done = kMaxIt;
+    }
 }
 
 // FP64-accumulated true residual t = A x - b.  op 0: measure only (g0 = ||t||^2);
@@ -507,11 +601,12 @@ template <int PM>
 __global__ __launch_bounds__(kBlock) void k_cgstep_g(const float* __restrict__ a, size_t pitch, BandArgs ba, int N,
                                                      const float* __restrict__ r, const float* __restrict__ dinv,
                                                      float* __restrict__ p0, float* __restrict__ p1,
-                                                     float* __restrict__ Ap, Scalars* sc, double* partials,
-                                                     unsigned int* counter) {
-    if (stopped(sc)) return;
-    const float beta = sc->beta;
-    const int par = sc->iters & 1;
+                                                     float* __restrict__ Ap, Scalars* sc, double* __restrict__ dpa,
+                                                     const double* __restrict__ dpb, int verify) {
+    StepIn in;
+    if (!step_begin(sc, dpb, verify, in)) return;
+    const float beta = in.beta;
+    const int par = in.iters & 1;
     const float* __restrict__ pold = par ? p1 : p0;
     float* __restrict__ pnew = par ? p0 : p1;
     double v[1] = {0.0};
@@ -529,13 +624,7 @@ __global__ __launch_bounds__(kBlock) void k_cgstep_g(const float* __restrict__ a
         Ap[i] = s;
         v[0] += (double)(pi * s);
     }
-    grid_reduce<1>(v, partials, counter, [&](double* tt) {
-        const double pAp = tt[0];
-        sc->pAp = pAp;
-        if (!isfinite(pAp)) sc->done = kNonFinite;
-        else if (!(pAp > 0)) sc->done = kBreakdown;
-        else sc->alpha = (float)(sc->rz / pAp);
-    });
+    step_end(sc, v[0], dpa, in);
 }
 
 template <int PM>
@@ -645,6 +734,7 @@ __global__ void k_tune_reset(Scalars* sc) {
         sc->done = 0;
         sc->verify = 0;
         sc->iters = 0;
+        sc->pend = 0;
         sc->beta = 0.5f;
         sc->rz = 1.0;
     }
@@ -678,6 +768,7 @@ struct Workspace {
     long cap64 = 0;
     int pad64 = 0;
     Scalars* sc = nullptr;
+    double* dpart = nullptr;  // deferred partial sums of the CG kernels: 3 x kMaxGrid
     Scalars* hsc = nullptr;  // pinned, 2 slots
     hipEvent_t ev[2] = {nullptr, nullptr};
 };
@@ -701,6 +792,8 @@ static gm_status ws_get(gm_stream_t s, long n, int pad, bool need16, Workspace**
         w->dev = dev;
         GM_HIP(hipMalloc(&w->sc, sizeof(Scalars)));
         GM_HIP(hipMemset(w->sc, 0, sizeof(Scalars)));
+        GM_HIP(hipMalloc(&w->dpart, sizeof(double) * 3 * kMaxGrid));
+        GM_HIP(hipMemset(w->dpart, 0, sizeof(double) * 3 * kMaxGrid));
         GM_HIP(hipHostMalloc(&w->hsc, 2 * sizeof(Scalars), hipHostMallocDefault));
         GM_HIP(hipEventCreateWithFlags(&w->ev[0], hipEventDisableTiming));
         GM_HIP(hipEventCreateWithFlags(&w->ev[1], hipEventDisableTiming));
@@ -742,6 +835,7 @@ void release_workspaces() {
         if (w->A16) (void)hipFree(w->A16);
         if (w->x64) (void)hipFree(w->x64 - w->pad64);
         if (w->sc) (void)hipFree(w->sc);
+        if (w->dpart) (void)hipFree(w->dpart);
         if (w->hsc) (void)hipHostFree(w->hsc);
         for (int i = 0; i < 2; ++i)
             if (w->ev[i]) (void)hipEventDestroy(w->ev[i]);
@@ -798,6 +892,7 @@ struct Solver {
     int shape, g_step;  // tuned K1 shape
     bool vec;           // x (and the workspace) 16-byte aligned: float4 update kernel
     int g_upd;
+    int verify;         // recursive convergence asks for an FP64 check first
 };
 
 static gm_status launch_resid(Solver& S, int start) {
@@ -854,7 +949,8 @@ template <class G, int FMT, int PM>
 static void launch_step_t(Solver& S, const void* A, size_t pitch) {
     Workspace* w = S.w;
     hipLaunchKernelGGL((k_cgstep<G, FMT, PM>), dim3(S.g_step), dim3(kBlock), 0, S.hs, A, pitch, S.plan.W,
-                       S.N / S.plan.W, S.N, w->r, w->dinv, w->p0, w->p1, w->Ap, w->sc, S.rs.partials, S.rs.counter);
+                       S.N / S.plan.W, S.N, w->r, w->dinv, w->p0, w->p1, w->Ap, w->sc, w->dpart,
+                       w->dpart + kMaxGrid, S.verify);
 }
 
 template <int FMT, int PM>
@@ -885,12 +981,12 @@ static gm_status launch_step(Solver& S) {
     } else {
         if (S.pm == kJacobi)
             hipLaunchKernelGGL(k_cgstep_g<kJacobi>, dim3(S.g_gen), dim3(kBlock), 0, S.hs, S.A->a, S.A->apitch,
-                               S.plan.args, S.N, w->r, w->dinv, w->p0, w->p1, w->Ap, w->sc, S.rs.partials,
-                               S.rs.counter);
+                               S.plan.args, S.N, w->r, w->dinv, w->p0, w->p1, w->Ap, w->sc, w->dpart,
+                               w->dpart + kMaxGrid, S.verify);
         else
             hipLaunchKernelGGL(k_cgstep_g<kPlain>, dim3(S.g_gen), dim3(kBlock), 0, S.hs, S.A->a, S.A->apitch,
-                               S.plan.args, S.N, w->r, w->dinv, w->p0, w->p1, w->Ap, w->sc, S.rs.partials,
-                               S.rs.counter);
+                               S.plan.args, S.N, w->r, w->dinv, w->p0, w->p1, w->Ap, w->sc, w->dpart,
+                               w->dpart + kMaxGrid, S.verify);
     }
     GM_LAUNCH_CHECK();
     return GM_OK;
@@ -901,31 +997,36 @@ static void set_step_shape(Solver& S, int shape, int per_cu) {
     S.g_step = grid_for_tiles(S.dv, shape_tiles(shape, S.plan.W, S.N), per_cu);
 }
 
-static gm_status launch_cg_iter(Solver& S, int verify) {
+template <int PM, bool VEC>
+static void launch_update_t(Solver& S, int grid) {
     Workspace* w = S.w;
+    hipLaunchKernelGGL((k_cgupdate<PM, VEC>), dim3(grid), dim3(kBlock), 0, S.hs, S.x, w->r, w->dinv, w->p0, w->p1,
+                       w->Ap, S.N, w->sc, w->dpart, w->dpart + kMaxGrid);
+}
+
+static gm_status launch_update(Solver& S) {
+    if (S.vec) {
+        if (S.pm == kJacobi) launch_update_t<kJacobi, true>(S, S.g_upd);
+        else if (S.pm == kScaled) launch_update_t<kScaled, true>(S, S.g_upd);
+        else launch_update_t<kPlain, true>(S, S.g_upd);
+    } else {
+        if (S.pm == kJacobi) launch_update_t<kJacobi, false>(S, S.g_elem);
+        else if (S.pm == kScaled) launch_update_t<kScaled, false>(S, S.g_elem);
+        else launch_update_t<kPlain, false>(S, S.g_elem);
+    }
+    GM_LAUNCH_CHECK();
+    return GM_OK;
+}
+
+static gm_status launch_cg_iter(Solver& S) {
     gm_status st = launch_step(S);
     if (st != GM_OK) return st;
-    if (S.vec) {
-        if (S.pm == kJacobi)
-            hipLaunchKernelGGL((k_cgupdate<kJacobi, true>), dim3(S.g_upd), dim3(kBlock), 0, S.hs, S.x, w->r, w->dinv,
-                               w->p0, w->p1, w->Ap, S.N, verify, w->sc, S.rs.partials, S.rs.counter);
-        else if (S.pm == kScaled)
-            hipLaunchKernelGGL((k_cgupdate<kScaled, true>), dim3(S.g_upd), dim3(kBlock), 0, S.hs, S.x, w->r, w->dinv,
-                               w->p0, w->p1, w->Ap, S.N, verify, w->sc, S.rs.partials, S.rs.counter);
-        else
-            hipLaunchKernelGGL((k_cgupdate<kPlain, true>), dim3(S.g_upd), dim3(kBlock), 0, S.hs, S.x, w->r, w->dinv,
-                               w->p0, w->p1, w->Ap, S.N, verify, w->sc, S.rs.partials, S.rs.counter);
-    } else {
-        if (S.pm == kJacobi)
-            hipLaunchKernelGGL((k_cgupdate<kJacobi, false>), dim3(S.g_elem), dim3(kBlock), 0, S.hs, S.x, w->r,
-                               w->dinv, w->p0, w->p1, w->Ap, S.N, verify, w->sc, S.rs.partials, S.rs.counter);
-        else if (S.pm == kScaled)
-            hipLaunchKernelGGL((k_cgupdate<kScaled, false>), dim3(S.g_elem), dim3(kBlock), 0, S.hs, S.x, w->r,
-                               w->dinv, w->p0, w->p1, w->Ap, S.N, verify, w->sc, S.rs.partials, S.rs.counter);
-        else
-            hipLaunchKernelGGL((k_cgupdate<kPlain, false>), dim3(S.g_elem), dim3(kBlock), 0, S.hs, S.x, w->r,
-                               w->dinv, w->p0, w->p1, w->Ap, S.N, verify, w->sc, S.rs.partials, S.rs.counter);
-    }
+    return launch_update(S);
+}
+
+// fold the last update's pending partial sums into the Scalars
+static gm_status launch_fold(Solver& S) {
+    hipLaunchKernelGGL(k_fold, dim3(1), dim3(kBlock), 0, S.hs, S.w->sc, S.w->dpart + kMaxGrid, S.verify);
     GM_LAUNCH_CHECK();
     return GM_OK;
 }
@@ -1121,7 +1222,7 @@ gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, c
                        S.rs.counter);
     GM_LAUNCH_CHECK();
 
-    const int verify = (o->tol > 0 && o->verify) ? 1 : 0;
+    S.verify = (o->tol > 0 && o->verify) ? 1 : 0;
     if (S.pm == kJacobi) {
         hipLaunchKernelGGL(k_diag_inv, dim3(S.g_elem), dim3(kBlock), 0, S.hs, A->a + (size_t)S.plan.args.center * A->apitch,
                            w->dinv, S.N);
@@ -1165,7 +1266,7 @@ gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, c
         long since_restart = 0;
         while (!finished) {
             for (int j = 0; j < chunk && enq < o->max_iter; ++j) {
-                if ((st = launch_cg_iter(S, verify)) != GM_OK) return st;
+                if ((st = launch_cg_iter(S)) != GM_OK) return st;
                 ++enq;
                 ++since_restart;
                 if (o->restart > 0 && since_restart >= o->restart && enq < o->max_iter) {
@@ -1190,7 +1291,9 @@ gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, c
             }
             ++polls;
             if (enq >= o->max_iter) {
-                // drain: a pending verification may still need the final check
+                // drain: fold the last update, then a pending verification may
+                // still need the final check
+                if ((st = launch_fold(S)) != GM_OK) return st;
                 GM_HIP(hipMemcpyAsync(&w->hsc[polls & 1], w->sc, sizeof(Scalars), hipMemcpyDeviceToHost, S.hs));
                 GM_HIP(hipStreamSynchronize(S.hs));
                 const Scalars& h = w->hsc[polls & 1];
@@ -1201,6 +1304,7 @@ gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, c
             }
         }
     }
+    if (o->method != GM_METHOD_GD && (st = launch_fold(S)) != GM_OK) return st;
     GM_HIP(hipStreamSynchronize(S.hs));
     Scalars fin;
     GM_HIP(hipMemcpy(&fin, w->sc, sizeof(Scalars), hipMemcpyDeviceToHost));
@@ -1398,7 +1502,10 @@ gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_re
     }
     // the update kernel's grid: fewer workgroups = fewer device-scope
     // fence + atomic arrivals in its reduction
+    // one step so the update kernels below find a hand-off to run
     set_step_shape(S, best.shape, best.per_cu);
+    hipLaunchKernelGGL(k_tune_reset, dim3(1), dim3(64), 0, S.hs, w->sc);
+    if ((st = launch_step(S)) != GM_OK) return st;
     S.x = w->d;
     S.vec = true;
     double best_upd = 1e30;
@@ -1406,15 +1513,8 @@ gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_re
         S.g_upd = grid_for_tiles(S.dv, (S.N + 4L * kBlock - 1) / (4L * kBlock), pc);
         double tot = 0;
         for (int k = 0; k < reps + 1; ++k) {
-            hipLaunchKernelGGL(k_tune_reset, dim3(1), dim3(64), 0, S.hs, w->sc);
             GM_HIP(hipEventRecord(e0, S.hs));
-            if (S.pm == kJacobi)
-                hipLaunchKernelGGL((k_cgupdate<kJacobi, true>), dim3(S.g_upd), dim3(kBlock), 0, S.hs, S.x, w->r, w->dinv,
-                                   w->p0, w->p1, w->Ap, S.N, 0, w->sc, S.rs.partials, S.rs.counter);
-            else
-                hipLaunchKernelGGL((k_cgupdate<kScaled, true>), dim3(S.g_upd), dim3(kBlock), 0, S.hs, S.x, w->r, w->dinv,
-                                   w->p0, w->p1, w->Ap, S.N, 0, w->sc, S.rs.partials, S.rs.counter);
-            GM_LAUNCH_CHECK();
+            if ((st = launch_update(S)) != GM_OK) return st;
             GM_HIP(hipEventRecord(e1, S.hs));
             GM_HIP(hipEventSynchronize(e1));
             float ms = 0;
