@@ -52,42 +52,96 @@ enum { kRun = 0, kConverged = 1, kBreakdown = 2, kNonFinite = 3, kMaxIt = 4 };
 
 __device__ __forceinline__ bool b_stopped(const Scalars* s) { return s->done != kRun || s->verify != 0; }
 
-// Arrive at the image's counter; returns true (on every thread) for the last tile.
-__device__ __forceinline__ bool image_arrive(unsigned int* counter, int ntiles) {
-    __shared__ int last;
+// Sum part[(tile0 + k) * stride + off] over k < ntiles in index order; every
+// thread gets the total, and every workgroup computes the same bits.
+__device__ __forceinline__ double image_fold(const double* __restrict__ part, int tile0, int ntiles, int stride,
+                                             int off) {
+    __shared__ double fr[kWaves];
+    __shared__ double ft;
+    double s = 0;
+    for (int k = threadIdx.x; k < ntiles; k += kBlock) s += part[(size_t)(tile0 + k) * stride + off];
+    s = wave_sum(s);
+    __syncthreads();  // an earlier call's readers are done with fr / ft
+    if ((threadIdx.x & 63) == 0) fr[threadIdx.x >> 6] = s;
     __syncthreads();
     if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned int prev = atomicAdd(counter, 1u);
-        last = (prev == (unsigned int)ntiles - 1u);
+        double t = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) t += fr[w];
+        ft = t;
     }
     __syncthreads();
-    if (last) __threadfence();
-    return last != 0;
+    return ft;
 }
 
-// Sum partials[(tile0 + k) * stride + off] over k < ntiles in a fixed order.
-__device__ __forceinline__ double image_sum(const double* partials, int tile0, int ntiles, int stride, int off) {
-    double s = 0;
-    for (int k = threadIdx.x; k < ntiles; k += kBlock) s += load_agent(partials + (size_t)(tile0 + k) * stride + off);
-    return block_sum(s);
+// The batched iteration uses the same fence-free hand-off as the single
+// system's CG kernels (see Scalars), per system: the step kernel folds the
+// update kernel's per-tile partials of the image (once per image a workgroup
+// visits) and only the workgroup holding the image's first tile writes the
+// system's `hand`; the update kernel reads only `hand` and only that same
+// workgroup publishes the system's Scalars.
+
+// Step-kernel view of one system on entering an image: fold pending update
+// partials and run the convergence test (every thread; results to chand).
+__device__ __forceinline__ void b_step_enter(const BImg& I, const Scalars* sc, const double* __restrict__ part2,
+                                             int verify, Hand* chand) {
+    __syncthreads();  // chand of the previous image is no longer read
+    for (int j = 0; j < I.nsys; ++j) {
+        const Scalars* s = &sc[I.sys0 + j];
+        Hand h{s->rz, s->rz_prev, s->rr, s->beta, s->iters, kHandRun, kRun, 0, I.ntiles};
+        if (s->done != kRun || s->verify) {
+            h.mode = kHandIdle;
+        } else if (s->pend) {
+            const double rz = image_fold(part2, I.tile0, I.ntiles, 2 * kMaxRhs, 2 * j);
+            const double rr = image_fold(part2, I.tile0, I.ntiles, 2 * kMaxRhs, 2 * j + 1);
+            if (!isfinite(rz) || !isfinite(rr)) {
+                h.mode = kHandStop;
+                h.done = kNonFinite;
+            } else {
+                h.rz_prev = h.rz;
+                h.beta = (float)(rz / h.rz);
+                h.rz = rz;
+                h.rr = rr;
+                if (s->tol2 >= 0 && rr <= s->tol2) {
+                    h.mode = kHandStop;
+                    if (verify) h.verify = 1;
+                    else h.done = kConverged;
+                } else if (h.iters >= s->maxit) {
+                    h.mode = kHandStop;
+                    h.done = kMaxIt;
+                }
+            }
+        }
+        if (threadIdx.x == 0) chand[j] = h;
+    }
+    __syncthreads();
 }
 
-// K1: p = -D^-1 r + beta p_old staged per system, Ap = (L_off + diag d) p, <p, Ap>.
+// K1: p = -D^-1 r + beta p_old staged per system, Ap = (L_off + diag d) p,
+// per-tile <p, Ap> partials.
 template <int FMT>
 __global__ __launch_bounds__(kBlock) void kb_cgstep(const BImg* __restrict__ imgs, const BSys* __restrict__ sys,
                                                     Scalars* sc, const int* __restrict__ tile_img, int ntot,
-                                                    double* partials, unsigned int* counters) {
+                                                    double* __restrict__ part1, const double* __restrict__ part2,
+                                                    int verify) {
     constexpr int R = kBR;
     __shared__ __attribute__((aligned(16))) float lds[Tile<R>::LDS_FLOATS];
+    __shared__ Hand chand[kMaxRhs];
     __shared__ double part[kMaxRhs];
     const TileRange tr = xcd_tiles(ntot);
+    int cimg = -1;
     for (int t = tr.begin; t < tr.end; t += tr.step) {
         const int im = tile_img[t];
         const BImg I = imgs[im];
+        if (im != cimg) {
+            b_step_enter(I, sc, part2, verify, chand);
+            cimg = im;
+        }
+        if (t == I.tile0 && threadIdx.x == 0)
+            for (int j = 0; j < I.nsys; ++j) sc[I.sys0 + j].hand = chand[j];
         unsigned mask = 0;
         for (int j = 0; j < I.nsys; ++j)
-            if (!b_stopped(&sc[I.sys0 + j])) mask |= 1u << j;
+            if (chand[j].mode == kHandRun) mask |= 1u << j;
         if (mask == 0) continue;
         const int i0 = (t - I.tile0) * Tile<R>::ROWS;
         const int ib = i0 + R * threadIdx.x;
@@ -102,9 +156,8 @@ __global__ __launch_bounds__(kBlock) void kb_cgstep(const BImg* __restrict__ img
                 continue;
             }
             const BSys S = sys[I.sys0 + j];
-            const Scalars* s = &sc[I.sys0 + j];
-            const float beta = s->beta;
-            const int par = s->iters & 1;
+            const float beta = chand[j].beta;
+            const int par = chand[j].iters & 1;
             const float* __restrict__ pold = par ? S.p1 : S.p0;
             float* __restrict__ pnew = par ? S.p0 : S.p1;
             __syncthreads();
@@ -132,38 +185,74 @@ __global__ __launch_bounds__(kBlock) void kb_cgstep(const BImg* __restrict__ img
             const double bs = block_sum(v);
             if (threadIdx.x == 0) part[j] = bs;
         }
-        if (threadIdx.x == 0)
-            for (int j = 0; j < I.nsys; ++j) partials[(size_t)t * kMaxRhs + j] = part[j];
-        if (image_arrive(&counters[im], I.ntiles)) {
-            for (int j = 0; j < I.nsys; ++j) {
-                if (!((mask >> j) & 1u)) continue;
-                const double pAp = image_sum(partials, I.tile0, I.ntiles, kMaxRhs, j);
-                if (threadIdx.x == 0) {
-                    Scalars* s = &sc[I.sys0 + j];
-                    s->pAp = pAp;
-                    if (!isfinite(pAp)) s->done = kNonFinite;
-                    else if (!(pAp > 0)) s->done = kBreakdown;
-                    else s->alpha = (float)(s->rz / pAp);
-                }
-            }
-            if (threadIdx.x == 0) counters[im] = 0u;
-        }
+        __syncthreads();
+        if (threadIdx.x < I.nsys) part1[(size_t)t * kMaxRhs + threadIdx.x] = part[threadIdx.x];
+        __syncthreads();  // part is rewritten by the next tile
     }
 }
 
-// K2: x += alpha p, r += alpha Ap, <r, D^-1 r>, ||r||^2 per system.
+enum { kBRun = 0, kBIdle = 10, kBStop = 11 };  // update-kernel outcome per system (else kBreakdown / kNonFinite)
+
+// K2: x += alpha p, r += alpha Ap, per-tile <r, D^-1 r>, ||r||^2 partials.
 __global__ __launch_bounds__(kBlock) void kb_cgupdate(const BImg* __restrict__ imgs, const BSys* __restrict__ sys,
                                                       Scalars* sc, const int* __restrict__ tile_img, int ntot,
-                                                      int verify, double* partials, unsigned int* counters) {
+                                                      const double* __restrict__ part1, double* __restrict__ part2) {
     constexpr int R = kBR;
+    __shared__ Hand chand[kMaxRhs];
+    __shared__ int cst[kMaxRhs];
+    __shared__ float calpha[kMaxRhs];
+    __shared__ double cpap[kMaxRhs];
     __shared__ double part[kMaxRhs][2];
     const TileRange tr = xcd_tiles(ntot);
+    int cimg = -1;
     for (int t = tr.begin; t < tr.end; t += tr.step) {
         const int im = tile_img[t];
         const BImg I = imgs[im];
+        if (im != cimg) {
+            __syncthreads();
+            for (int j = 0; j < I.nsys; ++j) {
+                const Hand h = sc[I.sys0 + j].hand;
+                int st = kBRun;
+                float a = 0.f;
+                double pAp = 0;
+                if (h.mode == kHandIdle) {
+                    st = kBIdle;
+                } else if (h.mode == kHandStop) {
+                    st = kBStop;
+                } else {
+                    pAp = image_fold(part1, I.tile0, I.ntiles, kMaxRhs, j);
+                    if (!isfinite(pAp)) st = kNonFinite;
+                    else if (!(pAp > 0)) st = kBreakdown;
+                    else a = (float)(h.rz / pAp);
+                }
+                if (threadIdx.x == 0) {
+                    chand[j] = h;
+                    cst[j] = st;
+                    calpha[j] = a;
+                    cpap[j] = pAp;
+                }
+            }
+            __syncthreads();
+            cimg = im;
+        }
+        if (t == I.tile0 && threadIdx.x == 0)
+            for (int j = 0; j < I.nsys; ++j) {
+                Scalars* s = &sc[I.sys0 + j];
+                const Hand& h = chand[j];
+                if (cst[j] == kBStop) {
+                    publish(s, h, h.done, h.verify, h.iters, 0, 0);
+                } else if (cst[j] == kBRun) {
+                    publish(s, h, kRun, 0, h.iters + 1, 1, I.ntiles);
+                    s->pAp = cpap[j];
+                    s->alpha = calpha[j];
+                } else if (cst[j] != kBIdle) {
+                    publish(s, h, cst[j], 0, h.iters, 0, 0);
+                    s->pAp = cpap[j];
+                }
+            }
         unsigned mask = 0;
         for (int j = 0; j < I.nsys; ++j)
-            if (!b_stopped(&sc[I.sys0 + j])) mask |= 1u << j;
+            if (cst[j] == kBRun) mask |= 1u << j;
         if (mask == 0) continue;
         const int i0 = (t - I.tile0) * Tile<R>::ROWS;
         for (int j = 0; j < I.nsys; ++j) {
@@ -172,9 +261,8 @@ __global__ __launch_bounds__(kBlock) void kb_cgupdate(const BImg* __restrict__ i
                 continue;
             }
             const BSys S = sys[I.sys0 + j];
-            const Scalars* s = &sc[I.sys0 + j];
-            const float alpha = s->alpha;
-            const float* __restrict__ p = (s->iters & 1) ? S.p0 : S.p1;
+            const float alpha = calpha[j];
+            const float* __restrict__ p = (chand[j].iters & 1) ? S.p0 : S.p1;
             double v0 = 0, v1 = 0;
             for (int q = 0; q < R; ++q) {
                 const int i = i0 + q * kBlock + threadIdx.x;  // coalesced across the workgroup
@@ -192,47 +280,55 @@ __global__ __launch_bounds__(kBlock) void kb_cgupdate(const BImg* __restrict__ i
                 part[j][1] = b1;
             }
         }
-        if (threadIdx.x == 0)
-            for (int j = 0; j < I.nsys; ++j) {
-                partials[((size_t)t * kMaxRhs + j) * 2 + 0] = part[j][0];
-                partials[((size_t)t * kMaxRhs + j) * 2 + 1] = part[j][1];
-            }
-        if (image_arrive(&counters[im], I.ntiles)) {
-            for (int j = 0; j < I.nsys; ++j) {
-                if (!((mask >> j) & 1u)) continue;
-                const double rz = image_sum(partials, I.tile0, I.ntiles, 2 * kMaxRhs, 2 * j);
-                const double rr = image_sum(partials, I.tile0, I.ntiles, 2 * kMaxRhs, 2 * j + 1);
-                if (threadIdx.x == 0) {
-                    Scalars* s = &sc[I.sys0 + j];
-                    s->iters += 1;
-                    if (!isfinite(rz) || !isfinite(rr)) {
-                        s->done = kNonFinite;
-                    } else {
-                        s->rz_prev = s->rz;
-                        s->beta = (float)(rz / s->rz);
-                        s->rz = rz;
-                        s->rr = rr;
-                        if (s->tol2 >= 0 && rr <= s->tol2) {
-                            if (verify) s->verify = 1;
-                            else s->done = kConverged;
-                        } else if (s->iters >= s->maxit) {
-                            s->done = kMaxIt;
-                        }
-                    }
-                }
-            }
-            if (threadIdx.x == 0) counters[im] = 0u;
+        __syncthreads();
+        if (threadIdx.x < 2 * I.nsys)
+            part2[(size_t)t * 2 * kMaxRhs + threadIdx.x] = part[threadIdx.x >> 1][threadIdx.x & 1];
+        __syncthreads();  // part is rewritten by the next tile
+    }
+}
+
+// Fold pending update partials into the Scalars (one workgroup per image;
+// launched before the host reads the final state).
+__global__ __launch_bounds__(kBlock) void kb_fold(const BImg* __restrict__ imgs, Scalars* sc,
+                                                  const double* __restrict__ part2, int verify) {
+    const BImg I = imgs[blockIdx.x];
+    for (int j = 0; j < I.nsys; ++j) {
+        Scalars* s = &sc[I.sys0 + j];
+        if (s->done != kRun || s->verify || !s->pend) continue;
+        const double rz = image_fold(part2, I.tile0, I.ntiles, 2 * kMaxRhs, 2 * j);
+        const double rr = image_fold(part2, I.tile0, I.ntiles, 2 * kMaxRhs, 2 * j + 1);
+        if (threadIdx.x != 0) continue;
+        s->pend = 0;
+        if (!isfinite(rz) || !isfinite(rr)) {
+            s->done = kNonFinite;
+            continue;
+        }
+        s->rz_prev = s->rz;
+        s->beta = (float)(rz / s->rz);
+        s->rz = rz;
+        s->rr = rr;
+        if (s->tol2 >= 0 && rr <= s->tol2) {
+            if (verify) s->verify = 1;
+            else s->done = kConverged;
+        } else if (s->iters >= s->maxit) {
+            s->done = kMaxIt;
         }
     }
 }
 
-// FP64-accumulated true residual t = (L_off + diag d) x - b per system.
-// op 0: measure (g0 = ||t||^2) for every system; 1: replace r for running
-// systems; 2: start (all systems, beta = 0); 3: replace r for systems whose
-// recursive residual asked for verification.
+// systems a true-residual pass selects (op: see kb_trueres)
+__device__ __forceinline__ bool tr_selected(const Scalars* s, int op) {
+    return op == 0 || op == 2 || (op == 1 && s->done == kRun) || (op == 3 && s->done == kRun && s->verify);
+}
+
+// FP64-accumulated true residual t = (L_off + diag d) x - b per system, per-tile
+// partials (<t, D^-1 t>, ||t||^2, drift ||t - r||^2); kb_trueres_fin folds
+// them.  op 0: measure only (g0); 1: replace r for running systems; 2: start
+// (all systems, beta = 0); 3: replace r for systems whose recursive residual
+// asked for verification.
 __global__ __launch_bounds__(kBlock) void kb_trueres(const BImg* __restrict__ imgs, const BSys* __restrict__ sys,
-                                                     Scalars* sc, const int* __restrict__ tile_img, int ntot, int op,
-                                                     double* partials, unsigned int* counters) {
+                                                     const Scalars* sc, const int* __restrict__ tile_img, int ntot,
+                                                     int op, double* __restrict__ part3) {
     constexpr int R = kBR;
     constexpr int SEG = Tile<R>::SEG;
     __shared__ __attribute__((aligned(16))) float lds[Tile<R>::LDS_FLOATS];
@@ -242,12 +338,8 @@ __global__ __launch_bounds__(kBlock) void kb_trueres(const BImg* __restrict__ im
         const int im = tile_img[t];
         const BImg I = imgs[im];
         unsigned mask = 0;
-        for (int j = 0; j < I.nsys; ++j) {
-            const Scalars* s = &sc[I.sys0 + j];
-            const bool sel = op == 0 || op == 2 || (op == 1 && s->done == kRun) ||
-                             (op == 3 && s->done == kRun && s->verify);
-            if (sel) mask |= 1u << j;
-        }
+        for (int j = 0; j < I.nsys; ++j)
+            if (tr_selected(&sc[I.sys0 + j], op)) mask |= 1u << j;
         if (mask == 0) continue;
         const int i0 = (t - I.tile0) * Tile<R>::ROWS;
         const int ib = i0 + R * threadIdx.x;
@@ -304,51 +396,55 @@ __global__ __launch_bounds__(kBlock) void kb_trueres(const BImg* __restrict__ im
                 part[j][2] = b2;
             }
         }
-        if (threadIdx.x == 0)
-            for (int j = 0; j < I.nsys; ++j) {
-                partials[((size_t)t * kMaxRhs + j) * 3 + 0] = part[j][0];
-                partials[((size_t)t * kMaxRhs + j) * 3 + 1] = part[j][1];
-                partials[((size_t)t * kMaxRhs + j) * 3 + 2] = part[j][2];
-            }
-        if (image_arrive(&counters[im], I.ntiles)) {
-            for (int j = 0; j < I.nsys; ++j) {
-                if (!((mask >> j) & 1u)) continue;
-                const double rz = image_sum(partials, I.tile0, I.ntiles, 3 * kMaxRhs, 3 * j);
-                const double rr = image_sum(partials, I.tile0, I.ntiles, 3 * kMaxRhs, 3 * j + 1);
-                const double d2 = image_sum(partials, I.tile0, I.ntiles, 3 * kMaxRhs, 3 * j + 2);
-                if (threadIdx.x == 0) {
-                    Scalars* s = &sc[I.sys0 + j];
-                    s->g0 = rr;
-                    if (op != 0) {
-                        if (!isfinite(rz) || !isfinite(rr)) {
-                            s->done = kNonFinite;
-                        } else {
-                            if (op == 2) {
-                                s->beta = 0.f;
-                            } else if (s->verify) {
-                                // drifted recursion: restart from the true residual
-                                s->beta = 0.f;
-                                s->replacements += 1;
-                                if (s->g1 > 0 && rr > 0.25 * s->g1) s->stall += 1;
-                                else s->stall = 0;
-                                s->g1 = rr;
-                            } else {
-                                // periodic: keep the direction unless the recursion drifted
-                                s->beta = d2 > kDrift2 * rr ? 0.f
-                                        : (s->rz_prev > 0 ? (float)(rz / s->rz_prev) : 0.f);
-                                s->replacements += 1;
-                            }
-                            s->rz = rz;
-                            s->rr = rr;
-                            s->verify = 0;
-                            if (s->tol2 >= 0 && rr <= s->tol2) s->done = kConverged;
-                            else if (s->iters >= s->maxit || s->stall >= 3) s->done = kMaxIt;
-                        }
-                    }
-                }
-            }
-            if (threadIdx.x == 0) counters[im] = 0u;
+        __syncthreads();
+        if (threadIdx.x < 3 * I.nsys)
+            part3[(size_t)t * 3 * kMaxRhs + threadIdx.x] = part[threadIdx.x / 3][threadIdx.x % 3];
+        __syncthreads();  // part is rewritten by the next tile
+    }
+}
+
+// Finish a true-residual pass: one workgroup per image folds its tiles'
+// partials and updates the selected systems' scalars.
+__global__ __launch_bounds__(kBlock) void kb_trueres_fin(const BImg* __restrict__ imgs, Scalars* sc,
+                                                         const double* __restrict__ part3, int op) {
+    const BImg I = imgs[blockIdx.x];
+    for (int j = 0; j < I.nsys; ++j) {
+        Scalars* s = &sc[I.sys0 + j];
+        if (!tr_selected(s, op)) continue;
+        const double rz = image_fold(part3, I.tile0, I.ntiles, 3 * kMaxRhs, 3 * j);
+        const double rr = image_fold(part3, I.tile0, I.ntiles, 3 * kMaxRhs, 3 * j + 1);
+        const double d2 = image_fold(part3, I.tile0, I.ntiles, 3 * kMaxRhs, 3 * j + 2);
+        if (threadIdx.x != 0) continue;
+        s->g0 = rr;
+        if (op == 0) continue;
+        if (!isfinite(rz) || !isfinite(rr)) {
+            s->done = kNonFinite;
+            continue;
         }
+        if (op == 2) {
+            s->beta = 0.f;
+        } else if (s->verify) {
+            // drifted recursion: restart from the true residual
+            s->beta = 0.f;
+            s->replacements += 1;
+            if (s->g1 > 0 && rr > 0.25 * s->g1) s->stall += 1;
+            else s->stall = 0;
+            s->g1 = rr;
+        } else {
+            // periodic: keep the direction unless the recursion drifted; the
+            // <r,z> of the last step is rz while its update's partials are
+            // pending (superseded by this residual), else rz_prev
+            const double rzp = s->pend ? s->rz : s->rz_prev;
+            s->beta = d2 > kDrift2 * rr ? 0.f : (rzp > 0 ? (float)(rz / rzp) : 0.f);
+            s->replacements += 1;
+        }
+        s->rz_prev = s->pend ? s->rz : s->rz_prev;
+        s->rz = rz;
+        s->rr = rr;
+        s->verify = 0;
+        s->pend = 0;
+        if (s->tol2 >= 0 && rr <= s->tol2) s->done = kConverged;
+        else if (s->iters >= s->maxit || s->stall >= 3) s->done = kMaxIt;
     }
 }
 
@@ -387,6 +483,7 @@ __global__ void kb_init(Scalars* sc, int nsys, int maxit, double tol) {
     Scalars z = {};
     z.maxit = maxit;
     z.tol2 = -1.0;
+    z.hand.mode = kHandIdle;
     sc[j] = z;
     (void)tol;
 }
@@ -421,8 +518,7 @@ struct gm_batch_s {
     Scalars* dsc = nullptr;
     Scalars* hsc = nullptr;  // pinned, nsys
     int* dtile = nullptr;
-    double* dpart = nullptr;
-    unsigned int* dcnt = nullptr;
+    double* dpart = nullptr;  // per-tile partial sums: step, update (2), true residual (3)
     float* Lmem = nullptr;
     unsigned short* L16mem = nullptr;
     float* vmem = nullptr;
@@ -534,10 +630,9 @@ gm_status gm_batch_create(gm_batch** out, int nimages, const int* widths, const 
     BTRY(hipMalloc(&b->dsc, sizeof(Scalars) * b->nsys));
     BTRY(hipMemset(b->dsc, 0, sizeof(Scalars) * b->nsys));
     BTRY(hipMalloc(&b->dtile, sizeof(int) * tiles));
-    const size_t nparts = (size_t)tiles * kMaxRhs * 3;
-    BTRY(hipMalloc(&b->dpart, sizeof(double) * nparts + sizeof(unsigned int) * nimages + 256));
-    BTRY(hipMemset(b->dpart, 0, sizeof(double) * nparts + sizeof(unsigned int) * nimages + 256));
-    b->dcnt = reinterpret_cast<unsigned int*>(b->dpart + nparts);
+    const size_t nparts = (size_t)tiles * kMaxRhs * 6;
+    BTRY(hipMalloc(&b->dpart, sizeof(double) * nparts));
+    BTRY(hipMemset(b->dpart, 0, sizeof(double) * nparts));
     BTRY(hipMemcpy(b->dimg, b->himg.data(), sizeof(BImg) * nimages, hipMemcpyHostToDevice));
     BTRY(hipMemcpy(b->dsys, b->hsys.data(), sizeof(BSys) * b->nsys, hipMemcpyHostToDevice));
     BTRY(hipMemcpy(b->dsysN, b->hsysN.data(), sizeof(int) * b->nsys, hipMemcpyHostToDevice));
@@ -610,23 +705,35 @@ gm_status gm_batch_info(gm_batch* b, int image, int rhs, gm_solve_info* info) {
     return GM_OK;
 }
 
+static double* part1(gm_batch* b) { return b->dpart; }
+static double* part2(gm_batch* b) { return b->dpart + (size_t)b->ntiles * kMaxRhs; }
+static double* part3(gm_batch* b) { return b->dpart + (size_t)b->ntiles * kMaxRhs * 3; }
+
 static gm_status batch_launch_iter(gm_batch* b, int grid, int verify, hipStream_t hs) {
     if (b->precision == GM_PREC_FP16)
         hipLaunchKernelGGL(kb_cgstep<kF16>, dim3(grid), dim3(kBlock), 0, hs, b->dimg, b->dsys, b->dsc, b->dtile,
-                           b->ntiles, b->dpart, b->dcnt);
+                           b->ntiles, part1(b), part2(b), verify);
     else
         hipLaunchKernelGGL(kb_cgstep<kF32>, dim3(grid), dim3(kBlock), 0, hs, b->dimg, b->dsys, b->dsc, b->dtile,
-                           b->ntiles, b->dpart, b->dcnt);
+                           b->ntiles, part1(b), part2(b), verify);
     GM_LAUNCH_CHECK();
     hipLaunchKernelGGL(kb_cgupdate, dim3(grid), dim3(kBlock), 0, hs, b->dimg, b->dsys, b->dsc, b->dtile, b->ntiles,
-                       verify, b->dpart, b->dcnt);
+                       part1(b), part2(b));
     GM_LAUNCH_CHECK();
     return GM_OK;
 }
 
 static gm_status batch_trueres(gm_batch* b, int grid, int op, hipStream_t hs) {
     hipLaunchKernelGGL(kb_trueres, dim3(grid), dim3(kBlock), 0, hs, b->dimg, b->dsys, b->dsc, b->dtile, b->ntiles, op,
-                       b->dpart, b->dcnt);
+                       part3(b));
+    GM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(kb_trueres_fin, dim3(b->nimg), dim3(kBlock), 0, hs, b->dimg, b->dsc, part3(b), op);
+    GM_LAUNCH_CHECK();
+    return GM_OK;
+}
+
+static gm_status batch_fold(gm_batch* b, int verify, hipStream_t hs) {
+    hipLaunchKernelGGL(kb_fold, dim3(b->nimg), dim3(kBlock), 0, hs, b->dimg, b->dsc, part2(b), verify);
     GM_LAUNCH_CHECK();
     return GM_OK;
 }
@@ -701,14 +808,14 @@ gm_status gm_batch_solve(gm_batch* b, const gm_solve_opts* opts, gm_stream_t s) 
             }
         }
         if (running == 0 || enq >= o.max_iter) finished = true;
-        if (verifying > 0) {
+        if (verifying > 0 && !finished) {
             if ((st = batch_trueres(b, grid, 3, hs)) != GM_OK) return st;
-            if (finished && enq >= o.max_iter) {
-                // the last replacement may have converged them; nothing more to run
-            }
         }
     }
-    // final FP64 true residual of every system (g0), then a host copy of all scalars
+    // fold the last update's partials, check any recursive convergence it
+    // shows, then the FP64 true residual of every system (g0) and a host copy
+    if ((st = batch_fold(b, verify, hs)) != GM_OK) return st;
+    if ((st = batch_trueres(b, grid, 3, hs)) != GM_OK) return st;
     if ((st = batch_trueres(b, grid, 0, hs)) != GM_OK) return st;
     GM_HIP(hipMemcpyAsync(b->hsc, b->dsc, sizeof(Scalars) * b->nsys, hipMemcpyDeviceToHost, hs));
     GM_HIP(hipStreamSynchronize(hs));

@@ -217,6 +217,20 @@ __device__ __forceinline__ double block_sum(double v) {
     return s;
 }
 
+// Publish a hand-off state into the Scalars (the one workgroup of the update kernel that does so).
+__device__ __forceinline__ void publish(Scalars* sc, const Hand& h, int done, int verify, int iters, int pend,
+                                        int npart) {
+    sc->rz = h.rz;
+    sc->rz_prev = h.rz_prev;
+    sc->rr = h.rr;
+    sc->beta = h.beta;
+    sc->iters = iters;
+    sc->done = done;
+    sc->verify = verify;
+    sc->pend = pend;
+    sc->npart = npart;
+}
+
 // Deferred partial sums: part[q * kMaxGrid + i], i < n.  Every thread of the
 // workgroup gets the totals; the summation order is fixed, so every
 // workgroup of a grid computes bit-identical totals.

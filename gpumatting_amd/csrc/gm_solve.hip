@@ -272,20 +272,6 @@ __device__ __forceinline__ void upd_elem(float& x, float& r, float d, float p, f
     }
 }
 
-// Publish a hand-off state into the Scalars (block 0 of the update kernel).
-__device__ __forceinline__ void publish(Scalars* sc, const Hand& h, int done, int verify, int iters, int pend,
-                                        int npart) {
-    sc->rz = h.rz;
-    sc->rz_prev = h.rz_prev;
-    sc->rr = h.rr;
-    sc->beta = h.beta;
-    sc->iters = iters;
-    sc->done = done;
-    sc->verify = verify;
-    sc->pend = pend;
-    sc->npart = npart;
-}
-
 template <int PM, bool VEC>
 __global__ __launch_bounds__(kBlock) void k_cgupdate(float* __restrict__ x, float* __restrict__ r,
                                                      const float* __restrict__ dinv, const float* __restrict__ p0,
