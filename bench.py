@@ -131,7 +131,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (the solve would shard by image "
-              "strips with a 2-row halo; see DESIGN.md).", file=sys.stderr)
+              "strips with a 2-row halo and a scalar all-reduce per iteration).", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -206,8 +206,19 @@ def main():
     spmv_ms = e0.elapsed_time(e1) / args.spmv_reps
     ft = tuned.get("fp32")
     sp_r, sp_ty, sp_wg = (ft["spmv_rows_per_thread"], ft["spmv_tile_rows"], ft["spmv_wg_per_cu"]) if ft else (4, 4, 8)
-    spmv_kernel = (f"gm::k_spmv<gm::Geom2D<{sp_r}, {sp_ty}>, 0, 0>" if sp_ty > 0 else f"gm::k_spmv<gm::Geom1D<{sp_r}>, 0, 0>") \
-        + f" ({sp_wg} workgroups/CU; banded SpMV, 25 FP32 diagonals)"
+    spmv_name = f"gm::k_spmv<gm::Geom2D<{sp_r}, {sp_ty}>, 0, 0>" if sp_ty > 0 else f"gm::k_spmv<gm::Geom1D<{sp_r}>, 0, 0>"
+    spmv_kernel = spmv_name + f" ({sp_wg} workgroups/CU; banded SpMV, 25 FP32 diagonals)"
+    # HBM bytes per launch of this kernel on this workload, from the committed
+    # rocprofv3 FETCH_SIZE / WRITE_SIZE runs (copy-calibrated); null if the
+    # tuner picked a shape that was not profiled
+    traffic = None
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            t = json.load(f).get(spmv_name)
+        if t and t.get("workload", "").startswith(f"{W}x{H}"):
+            traffic = t["bytes_per_launch"]
+    except (OSError, ValueError):
+        pass
     spmv_bytes = N * (25 * 4 + 8)
     spmv_gbs = spmv_bytes / (spmv_ms * 1e-3) / 1e9
     log(f"spmv: {spmv_ms * 1e3:.1f} us/launch, {spmv_gbs:.0f} GB/s algorithmic")
@@ -253,7 +264,7 @@ def main():
         "spmv_us": round(spmv_ms * 1e3, 2),
         "roofline": {"bound": "hbm", "kernel": spmv_kernel,
                      "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes": spmv_bytes},
         "cpu_baseline": cpu,
     }

@@ -1,5 +1,5 @@
 // gm_batch.hip -- many images of uneven size, several right-hand sides each,
-// solved together (no reference counterpart: the north star's batched form).
+// solved together (no reference counterpart).
 //
 // Each image i has one Levin Laplacian L_i (25 FP32 bands, no data term) and
 // nrhs independent constraint sets j, i.e. nrhs systems

@@ -141,38 +141,13 @@ typedef struct gm_solve_info_s {
    int iterations;      /* iterations performed */
    int status;          /* gm_status of the solve */
    double relres;       /* last residual norm / ||b|| the iteration saw */
-   double true_relres;  /* Project autotuner (no reference counterpart): times the fused CG step on
- * this matrix for every (tile geometry, rows per thread, workgroups per CU)
- * candidate with HIP events and records the fastest for solves of the same
- * shape and precision (FP32: also the plain product gm_bm_ax).  Clobbers the calling stream's solver workspace (call before a
- * solve, not during). */
-typedef struct gm_tune_result_s {
-   int rows_per_thread;
-   int tile_rows;       /* 0: 1-D row tiles; > 0: 2-D pixel tiles of that many image rows */
-   int wg_per_cu;
-   double step_us;      /* mean duration of one fused CG step with that shape */
-   int spmv_rows_per_thread;  /* FP32 only: the plain product y = A x, tuned the same way */
-   int spmv_tile_rows;
-   int spmv_wg_per_cu;
-   double spmv_us;
-   int update_wg_per_cu;   /* grid of the CG update kernel */
-   double update_us;
-} gm_tune_result;
-gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_result* out, gm_stream_t stream);
-/* Text table of every candidate timed by the last gm_autotune on this thread. */
-const char* gm_autotune_report(void);
-/* Pin a shape without timing: (rows_per_thread, tile_rows) one of (2,0) (4,0)
- * (4,4) (4,8) (4,16) for FP32 and (4,0) (8,0) (4,4) (4,8) (4,16) (8,8) (8,16)
- * for 16-bit; 2-D shapes need W % rows_per_thread == 0. */
-gm_status gm_tune_set(int W, int N, int precision, int rows_per_thread, int tile_rows, int wg_per_cu);
-
-/* FP64-accumulated ||b - A x|| / ||b|| at exit (-1 if not computed) */
+   double true_relres;  /* FP64-accumulated ||b - A x|| / ||b|| at exit (-1 if not computed) */
    int replacements;    /* FP64 residual replacements performed */
 } gm_solve_info;
 
 void gm_solve_opts_default(gm_solve_opts* o);
 
-/* North-star solver entry: (L + lambda D) alpha = lambda D b as assembled by
+/* Solver entry: (L + lambda D) alpha = lambda D b as assembled by
  * gm_levin_laplacian, or any SPD banded system.  alpha: device, padded by
  * `pad` (>= max |band offset|), initial guess in, solution out. */
 gm_status gm_solve(float* alpha, const BandedMatrix* L, const float* b, int pad, const gm_solve_opts* opts,
@@ -257,7 +232,7 @@ int gm_levin_pad(int W);
 
 /* ---------------------------------------------------------------------- */
 /* Batched solve: many images of uneven size, several right-hand sides each  */
-/* (no reference counterpart: the north star's batched form).                */
+/* (no reference counterpart).                                              */
 
 typedef struct gm_batch_s gm_batch;
 

@@ -889,7 +889,7 @@ double orc_baseline_pcg_f32(float* x, const float* a, size_t apitch, const int* 
 }
 
 /* ------------------------------------------------------------------------ */
-/* Adjoint of the assembly (FP64; no reference counterpart -- the north star's
+/* Adjoint of the assembly (FP64; no reference counterpart: the
  * differentiation path).  For a loss whose gradient with respect to the
  * stored off-diagonal entries is G_ij (the diagonal being -sum of the
  * off-diagonals + lam), and G_ij = -mu_i alpha_j for a linear solve

@@ -66,3 +66,28 @@ def test_cpp_dropin_headers_compile():
     r = subprocess.run(["g++", "-std=c++14", "-fsyntax-only", "-I" + os.path.join(ROOT, "include"),
                         "-I/opt/rocm/include", "-D__HIP_PLATFORM_AMD__", src], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_c_abi_header_is_plain_c():
+    """include/gpumatting.h and BandedMatrix.h compile as C99 with every
+    warning an error, so C, cgo-style and ctypes callers can include them."""
+    inc = os.path.join(ROOT, "include")
+    for h in ("gpumatting.h", "BandedMatrix.h"):
+        r = subprocess.run(["gcc", "-x", "c", "-std=c99", "-Wall", "-Wextra", "-Werror", "-pedantic", "-fsyntax-only",
+                            "-I" + inc, os.path.join(inc, h)], capture_output=True, text=True)
+        assert r.returncode == 0, (h, r.stderr)
+
+
+def test_ctypes_structs_match_the_header():
+    """The ctypes mirrors of the option / info / tune structs have the C
+    layout the header declares (field names, order, sizes)."""
+    text = open(os.path.join(ROOT, "include", "gpumatting.h")).read()
+
+    def fields(struct):
+        body = re.search(r"typedef struct " + struct + r"\s*\{(.*?)\}", text, re.S).group(1)
+        body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+        return [m.group(2) for m in re.finditer(r"\b(int|double|float)\s+(\w+)\s*;", body)]
+
+    for struct, cls in (("gm_solve_opts_s", _lib.SolveOpts), ("gm_solve_info_s", _lib.SolveInfo),
+                        ("gm_tune_result_s", _lib.TuneResult)):
+        assert fields(struct) == [f for f, _ in cls._fields_], struct
