@@ -598,6 +598,15 @@ inline bool shape_ok(int sh, int fmt, int W, long N) {
     if (kShapeTY[sh] == 0) return true;
     return W > 0 && N % W == 0 && W % R == 0;
 }
+// The 16-bit CG forms use the symmetric 12-band layout when the image width
+// is a multiple of 4 (2-D tiles, 4 pixels per thread), else the full 24-band
+// layout with 1-D tiles; these are the step-kernel shapes each may use.
+inline bool sym16_ok(int W, long N) { return W > 0 && N % W == 0 && W % 4 == 0; }
+inline bool step_shape_ok(int sh, int fmt, int W, long N) {
+    if (fmt == kF32) return shape_ok(sh, fmt, W, N);
+    if (sym16_ok(W, N)) return sh == kS2R4Y4 || sh == kS2R4Y8 || sh == kS2R4Y16;
+    return sh == kS1R4 || sh == kS1R8;
+}
 inline long shape_tiles(int sh, int W, long N) {
     if (kShapeTY[sh] == 0) {
         const long rows = (long)kBlock * kShapeR[sh];

@@ -249,6 +249,119 @@ __global__ __launch_bounds__(kBlock) void k_cgstep(const void* __restrict__ A, s
     step_end(sc, v[0], dpa, in);
 }
 
+// One 16-bit band word (low half of h) as float.
+template <int FMT>
+__device__ __forceinline__ float h2f(unsigned int h) {
+    if constexpr (FMT == kF16) return (float)__builtin_bit_cast(_Float16, (unsigned short)(h & 0xffffu));
+    else return __uint_as_float(h << 16);
+}
+
+// K1 on the symmetric scaled operator.  Only the 12 upper bands k = 13..24
+// (offset o_k > 0) of As = S A S are stored, the unit diagonal is implicit,
+// and row i's entry in the lower band 24 - k is the upper entry of row
+// i - o_k, U_k[i - o_k].  Each thread loads its own rows' upper entries into
+// registers and also into an LDS window of U covering the tile plus 2 image
+// rows above and 2 columns on each side (window cells are addressed by
+// matrix-row index, like the x window); the lower entries are read from it.
+// U holds 0 for pairs outside the image or wrapping a row, so the product
+// equals the full 24-band one wherever As is symmetric.  16-bit bands move
+// 24 B per row instead of 48.
+template <int TY, int FMT>
+__global__ __launch_bounds__(kBlock) void k_cgstep_sym(const unsigned short* __restrict__ U, size_t pitch, int W,
+                                                       int H, int N, const float* __restrict__ r,
+                                                       float* __restrict__ p0, float* __restrict__ p1,
+                                                       float* __restrict__ Ap, Scalars* sc, double* __restrict__ dpa,
+                                                       const double* __restrict__ dpb, int verify) {
+    using G = Geom2D<4, TY>;
+    constexpr int R = 4, TPR = G::TPR, TX = G::TX, ST = G::STRIDE;
+    constexpr int UR = TY + 2;       // staged U rows: 2 above the tile, then the tile's
+    constexpr int SW = TX / 2 + 2;   // 32-bit words (half pairs) per staged U row: TX + 4 halves
+    StepIn in;
+    if (!step_begin(sc, dpb, verify, in)) return;
+    const float beta = in.beta;
+    const int par = in.iters & 1;
+    const float* __restrict__ pold = par ? p1 : p0;
+    float* __restrict__ pnew = par ? p0 : p1;
+    __shared__ __attribute__((aligned(16))) float lds[G::LDS_FLOATS];
+    __shared__ __attribute__((aligned(16))) unsigned int ub[12][UR][SW];
+    const TileRange tr = xcd_tiles(G::ntiles(W, H, N));
+    const int tx = (int)threadIdx.x % TPR, ty = (int)threadIdx.x / TPR;
+    double v[1] = {0.0};
+    for (int t = tr.begin; t < tr.end; t += tr.step) {
+        G g;
+        g.at(t, W, H, N);
+        Raw<R, FMT> u[12];
+        if (g.valid) {
+#pragma unroll
+            for (int m = 0; m < 12; ++m) load_band<R, FMT>(U, (size_t)(13 + m) * pitch + (size_t)g.ib, u[m]);
+        }
+        __syncthreads();  // the previous tile is done with lds and ub
+        // own cells: window row ty + 2, halves 2 + 4 tx .. 5 + 4 tx (words 1 + 2 tx, 2 + 2 tx)
+#pragma unroll
+        for (int m = 0; m < 12; ++m) {
+            ub[m][ty + 2][1 + 2 * tx] = g.valid ? u[m].d[0] : 0u;
+            ub[m][ty + 2][2 + 2 * tx] = g.valid ? u[m].d[1] : 0u;
+        }
+        // halo: every word of window rows 0 and 1, and words 0 and SW - 1 of the others
+        constexpr int NH = 2 * SW + 2 * TY;
+        for (int e = threadIdx.x; e < NH; e += kBlock) {
+            int wy, ww;
+            if (e < 2 * SW) {
+                wy = e / SW;
+                ww = e - wy * SW;
+            } else {
+                const int f = e - 2 * SW;
+                wy = 2 + (f >> 1);
+                ww = (f & 1) ? SW - 1 : 0;
+            }
+            const long long gi = (long long)(g.y0 - 2 + wy) * W + (g.x0 - 2 + 2 * ww);  // even: W % 4 == 0
+            const bool in = gi >= 0 && gi < N;
+#pragma unroll
+            for (int m = 0; m < 12; ++m)
+                ub[m][wy][ww] =
+                    in ? *reinterpret_cast<const unsigned int*>(U + (size_t)(13 + m) * pitch + (size_t)gi) : 0u;
+        }
+        if (beta != 0.f)
+            g.stage(lds, W, H, N, [&](int i) { return fmaf(beta, pold[i], -r[i]); });
+        else
+            g.stage(lds, W, H, N, [&](int i) { return -r[i]; });
+        __syncthreads();
+        if (g.valid) {
+            const float* __restrict__ xs = g.xs(lds);
+            float y[R], pc[R];
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                pc[j] = xs[2 * ST + 2 + j];
+                y[j] = pc[j];  // unit diagonal
+            }
+#pragma unroll
+            for (int m = 0; m < 12; ++m) {
+                const int k = 13 + m, dy = k / 5 - 2, dx = k % 5 - 2;
+                // upper entry of this row: x at (+dy, +dx)
+#pragma unroll
+                for (int j = 0; j < R; ++j) y[j] = fmaf(u[m].get(j), xs[(2 + dy) * ST + 2 + dx + j], y[j]);
+                // lower entry: U_k of the row at (-dy, -dx), x there; halves 4 tx .. 4 tx + 7 of window row ty + 2 - dy
+                const unsigned int* wr = &ub[m][ty + 2 - dy][2 * tx];
+                const uint2 w01 = *reinterpret_cast<const uint2*>(wr);
+                const uint2 w23 = *reinterpret_cast<const uint2*>(wr + 2);
+                const unsigned int w[4] = {w01.x, w01.y, w23.x, w23.y};
+#pragma unroll
+                for (int j = 0; j < R; ++j) {
+                    const int q = 2 + j - dx;  // half index 0..7
+                    const float c = h2f<FMT>(w[q >> 1] >> (16 * (q & 1)));
+                    y[j] = fmaf(c, xs[(2 - dy) * ST + 2 - dx + j], y[j]);
+                }
+            }
+            store_rows<R>(pnew, g.ib, N, pc);
+            store_rows<R>(Ap, g.ib, N, y);
+#pragma unroll
+            for (int j = 0; j < R; ++j)
+                if (g.ib + j < N) v[0] += (double)(pc[j] * y[j]);
+        }
+    }
+    step_end(sc, v[0], dpa, in);
+}
+
 // K2: x += alpha p (scaled: alpha s p), r += alpha Ap, <r,z>, ||r||^2 (scaled:
 // the unscaled residual (r/s)^2).  VEC: every pointer is 16-byte aligned and
 // the body moves float4s (the kernel is a pure stream: wide loads keep enough
@@ -426,10 +539,12 @@ __global__ __launch_bounds__(kBlock) void k_trueres25(const float* __restrict__ 
 
 // Mixed-precision operator: As = S A S with S = diag(1/sqrt(a_ii)) (unit
 // diagonal, band 12 not stored), rounded to FP16 or BF16; s written to svec.
-template <int R, int FMT>
+// SYM: only the upper bands 13..24 (the symmetric step kernel's form).
+template <int R, int FMT, bool SYM>
 __global__ __launch_bounds__(kBlock) void k_scale16(const float* __restrict__ A, size_t pitch, int W, int N,
                                                     unsigned short* __restrict__ A16, size_t pitch16,
                                                     float* __restrict__ svec) {
+    static_assert(R % 2 == 0, "half pairs");
     __shared__ __attribute__((aligned(16))) float lds[Tile<R>::LDS_FLOATS];
     constexpr int SEG = Tile<R>::SEG;
     const int ntiles = (N + Tile<R>::ROWS - 1) / Tile<R>::ROWS;
@@ -446,20 +561,24 @@ __global__ __launch_bounds__(kBlock) void k_scale16(const float* __restrict__ A,
         if (ib >= N) continue;
         const float* xs = lds + R * threadIdx.x;
 #pragma unroll
-        for (int k = 0; k < 25; ++k) {
+        for (int k = SYM ? 13 : 0; k < 25; ++k) {
             if (k == 12) continue;
             const int s = k / 5, dx = k % 5;
             Raw<R, kF32> a;
             load_band<R, kF32>(A, (size_t)k * pitch + (size_t)ib, a);
-            unsigned short h[R];
+            unsigned int h[R / 2];
 #pragma unroll
-            for (int j = 0; j < R; ++j) {
-                const float v = (a.get(j) * xs[2 * SEG + j + 2]) * xs[s * SEG + j + dx];
-                h[j] = FMT == kF16 ? f32_to_f16_rne(v) : f32_to_bf16_rne(v);
+            for (int j = 0; j < R; j += 2) {
+                const float v0 = (a.get(j) * xs[2 * SEG + j + 2]) * xs[s * SEG + j + dx];
+                const float v1 = (a.get(j + 1) * xs[2 * SEG + j + 3]) * xs[s * SEG + j + 1 + dx];
+                const unsigned int h0 = FMT == kF16 ? f32_to_f16_rne(v0) : f32_to_bf16_rne(v0);
+                const unsigned int h1 = FMT == kF16 ? f32_to_f16_rne(v1) : f32_to_bf16_rne(v1);
+                h[j / 2] = h0 | (h1 << 16);
             }
-            unsigned short* dst = A16 + (size_t)k * pitch16 + ib;
+            // rows ib .. ib + R - 1 lie inside the padded pitch; 4R-byte aligned
+            unsigned int* dst = reinterpret_cast<unsigned int*>(A16 + (size_t)k * pitch16 + ib);
 #pragma unroll
-            for (int j = 0; j < R; ++j) dst[j] = h[j];
+            for (int j = 0; j < R / 2; ++j) dst[j] = h[j];
         }
         float sc[R];
 #pragma unroll
@@ -858,7 +977,7 @@ static TuneVal tune_lookup(int fmt, int W, long N) {
     if (it != g_tune.end()) return it->second;
     // defaults: the shapes the tuner picks for the 3840x2160 Levin matrix
     if (fmt == kF32) return shape_ok(kS2R4Y4, fmt, W, N) ? TuneVal{kS2R4Y4, 2, 0, 0} : TuneVal{kS1R4, 4, 0, 0};
-    return shape_ok(kS2R8Y16, fmt, W, N) ? TuneVal{kS2R8Y16, 2, 0, 0} : TuneVal{kS1R8, 3, 0, 0};
+    return sym16_ok(W, N) ? TuneVal{kS2R4Y8, 4, 0, 0} : TuneVal{kS1R8, 3, 0, 0};
 }
 
 struct Solver {
@@ -879,6 +998,7 @@ struct Solver {
     bool vec;           // x (and the workspace) 16-byte aligned: float4 update kernel
     int g_upd;
     int verify;         // recursive convergence asks for an FP64 check first
+    bool sym;           // 16-bit form stored as its 12 upper bands (k_cgstep_sym)
 };
 
 static gm_status launch_resid(Solver& S, int start) {
@@ -931,12 +1051,56 @@ static gm_status launch_trueres(Solver& S, int op) {
     return GM_OK;
 }
 
+static void launch_scale16(Solver& S, int grid) {
+    Workspace* w = S.w;
+    const BandedMatrix* A = S.A;
+    if (S.fmt == kF16) {
+        if (S.sym)
+            hipLaunchKernelGGL((k_scale16<8, kF16, true>), dim3(grid), dim3(kBlock), 0, S.hs, A->a, A->apitch,
+                               S.plan.W, S.N, w->A16, w->pitch16, w->dinv);
+        else
+            hipLaunchKernelGGL((k_scale16<8, kF16, false>), dim3(grid), dim3(kBlock), 0, S.hs, A->a, A->apitch,
+                               S.plan.W, S.N, w->A16, w->pitch16, w->dinv);
+    } else {
+        if (S.sym)
+            hipLaunchKernelGGL((k_scale16<8, kBF16, true>), dim3(grid), dim3(kBlock), 0, S.hs, A->a, A->apitch,
+                               S.plan.W, S.N, w->A16, w->pitch16, w->dinv);
+        else
+            hipLaunchKernelGGL((k_scale16<8, kBF16, false>), dim3(grid), dim3(kBlock), 0, S.hs, A->a, A->apitch,
+                               S.plan.W, S.N, w->A16, w->pitch16, w->dinv);
+    }
+}
+
+template <int TY, int FMT>
+static void launch_sym_t(Solver& S) {
+    Workspace* w = S.w;
+    hipLaunchKernelGGL((k_cgstep_sym<TY, FMT>), dim3(S.g_step), dim3(kBlock), 0, S.hs, w->A16, w->pitch16, S.plan.W,
+                       S.N / S.plan.W, S.N, w->r, w->p0, w->p1, w->Ap, w->sc, w->dpart, w->dpart + kMaxGrid,
+                       S.verify);
+}
+
+template <int FMT>
+static void launch_sym(Solver& S) {
+    switch (S.shape) {
+        case kS2R4Y4: launch_sym_t<4, FMT>(S); break;
+        case kS2R4Y16: launch_sym_t<16, FMT>(S); break;
+        default: launch_sym_t<8, FMT>(S); break;
+    }
+}
+
 template <class G, int FMT, int PM>
 static void launch_step_t(Solver& S, const void* A, size_t pitch) {
     Workspace* w = S.w;
     hipLaunchKernelGGL((k_cgstep<G, FMT, PM>), dim3(S.g_step), dim3(kBlock), 0, S.hs, A, pitch, S.plan.W,
                        S.N / S.plan.W, S.N, w->r, w->dinv, w->p0, w->p1, w->Ap, w->sc, w->dpart,
                        w->dpart + kMaxGrid, S.verify);
+}
+
+// 16-bit forms without the symmetric layout (image width not a multiple of 4): 1-D tiles
+template <int FMT>
+static void launch_step_16(Solver& S, const void* A, size_t pitch) {
+    if (S.shape == kS1R4) launch_step_t<Geom1D<4>, FMT, kScaled>(S, A, pitch);
+    else launch_step_t<Geom1D<8>, FMT, kScaled>(S, A, pitch);
 }
 
 template <int FMT, int PM>
@@ -956,9 +1120,12 @@ static void launch_step_shape(Solver& S, const void* A, size_t pitch) {
 static gm_status launch_step(Solver& S) {
     Workspace* w = S.w;
     if (S.tiled) {
-        if (S.pm == kScaled) {
-            if (S.fmt == kF16) launch_step_shape<kF16, kScaled>(S, w->A16, w->pitch16);
-            else launch_step_shape<kBF16, kScaled>(S, w->A16, w->pitch16);
+        if (S.pm == kScaled && S.sym) {
+            if (S.fmt == kF16) launch_sym<kF16>(S);
+            else launch_sym<kBF16>(S);
+        } else if (S.pm == kScaled) {
+            if (S.fmt == kF16) launch_step_16<kF16>(S, w->A16, w->pitch16);
+            else launch_step_16<kBF16>(S, w->A16, w->pitch16);
         } else if (S.pm == kJacobi) {
             launch_step_shape<kF32, kJacobi>(S, S.A->a, S.A->apitch);
         } else {
@@ -1174,6 +1341,7 @@ gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, c
         return fail(GM_ERR_INVALID, "solve: Jacobi scaling needs a band with offset 0");
     S.pm = mixed ? kScaled : (o->method == GM_METHOD_PCG ? kJacobi : kPlain);
     S.fmt = o->precision == GM_PREC_FP16 ? kF16 : (o->precision == GM_PREC_BF16 ? kBF16 : kF32);
+    S.sym = mixed && S.tiled && sym16_ok(S.plan.W, S.N);
     if ((st = dev_info(&S.dv)) != GM_OK) return st;
     if ((st = red_scratch(s, &S.rs)) != GM_OK) return st;
     if ((st = ws_get(s, S.N > 0 ? S.N : 1, S.plan.maxoff, mixed, &S.w)) != GM_OK) return st;
@@ -1214,12 +1382,7 @@ gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, c
                            w->dinv, S.N);
         GM_LAUNCH_CHECK();
     } else if (S.pm == kScaled) {
-        if (S.fmt == kF16)
-            hipLaunchKernelGGL((k_scale16<8, kF16>), dim3(S.g_tile8), dim3(kBlock), 0, S.hs, A->a, A->apitch, S.plan.W,
-                               S.N, w->A16, w->pitch16, w->dinv);
-        else
-            hipLaunchKernelGGL((k_scale16<8, kBF16>), dim3(S.g_tile8), dim3(kBlock), 0, S.hs, A->a, A->apitch,
-                               S.plan.W, S.N, w->A16, w->pitch16, w->dinv);
+        launch_scale16(S, S.g_tile8);
         GM_LAUNCH_CHECK();
     }
 
@@ -1446,12 +1609,8 @@ gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_re
     hipLaunchKernelGGL(k_fill, dim3(S.g_elem), dim3(kBlock), 0, S.hs, w->p0, S.N, 0.5f);
     hipLaunchKernelGGL(k_fill, dim3(S.g_elem), dim3(kBlock), 0, S.hs, w->p1, S.N, 0.5f);
     if (mixed) {
-        if (S.fmt == kF16)
-            hipLaunchKernelGGL((k_scale16<8, kF16>), dim3(grid_for_tiles(S.dv, (S.N + 2047) / 2048, 3)), dim3(kBlock),
-                               0, S.hs, L->a, L->apitch, S.plan.W, S.N, w->A16, w->pitch16, w->dinv);
-        else
-            hipLaunchKernelGGL((k_scale16<8, kBF16>), dim3(grid_for_tiles(S.dv, (S.N + 2047) / 2048, 3)),
-                               dim3(kBlock), 0, S.hs, L->a, L->apitch, S.plan.W, S.N, w->A16, w->pitch16, w->dinv);
+        S.sym = sym16_ok(S.plan.W, S.N);
+        launch_scale16(S, grid_for_tiles(S.dv, (S.N + 2047) / 2048, 3));
     } else {
         hipLaunchKernelGGL(k_diag_inv, dim3(S.g_elem), dim3(kBlock), 0, S.hs,
                            L->a + (size_t)S.plan.args.center * L->apitch, w->dinv, S.N);
@@ -1466,7 +1625,7 @@ gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_re
     char line[160];
     g_tune_report = "kernel shape R TY wg_per_cu us\n";
     for (int sh = 0; sh < kNumShapes; ++sh) {
-        if (!shape_ok(sh, S.fmt, S.plan.W, S.N)) continue;
+        if (!step_shape_ok(sh, S.fmt, S.plan.W, S.N)) continue;
         for (int pc : pcs) {
             set_step_shape(S, sh, pc);
             double tot = 0;
@@ -1556,7 +1715,7 @@ gm_status gm_tune_set(int W, int N, int precision, int rows_per_thread, int tile
     int sh = -1;
     for (int k = 0; k < kNumShapes; ++k)
         if (kShapeR[k] == rows_per_thread && kShapeTY[k] == tile_rows) sh = k;
-    if (sh < 0 || !shape_ok(sh, fmt, W, N) || wg_per_cu < 1 || wg_per_cu > 16)
+    if (sh < 0 || !step_shape_ok(sh, fmt, W, N) || wg_per_cu < 1 || wg_per_cu > 16)
         return fail(GM_ERR_INVALID, "gm_tune_set: unsupported shape");
     std::lock_guard<std::mutex> g(g_tune_mu);
     g_tune[TuneKey{fmt, W, (long)N}] = TuneVal{sh, wg_per_cu, 0, 0};

@@ -248,12 +248,14 @@ def test_autotune_then_solve(gpu, small):
         assert np.abs(x.numpy() - x_exact).max() <= 5e-3
 
 
-@pytest.mark.parametrize("prec,shapes", [(PREC_FP32, [(2, 0), (4, 0), (4, 4), (4, 8), (4, 16)]),
-                                         (PREC_FP16, [(4, 0), (8, 0), (4, 8), (8, 8), (8, 16)])])
-def test_every_tile_shape_same_iteration(gpu, small, prec, shapes):
+@pytest.mark.parametrize("prec,shapes,width", [
+    (PREC_FP32, [(2, 0), (4, 0), (4, 4), (4, 8), (4, 16)], 256),
+    (PREC_FP16, [(4, 4), (4, 8), (4, 16)], 256),   # symmetric 12-band layout (W % 4 == 0)
+    (PREC_FP16, [(4, 0), (8, 0)], 254)])           # full 24-band layout, 1-D tiles
+def test_every_tile_shape_same_iteration(gpu, small, prec, shapes, width):
     """Every tile geometry computes the same products: the first iterations of
     a fixed-count CG agree to rounding across shapes."""
-    p = small[0]
+    p = small[0] if width == 256 else synth.make_problem(width, 190, 1001, 1, 0.08)
     s = _gpu_system(p)
     outs = []
     for (R, ty) in shapes:
@@ -262,4 +264,58 @@ def test_every_tile_shape_same_iteration(gpu, small, prec, shapes):
         outs.append(x.numpy())
     for o in outs[1:]:
         assert np.abs(o - outs[0]).max() <= 1e-4
-    call("gm_tune_set", p.W, p.N, prec, *((4, 8) if prec == PREC_FP32 else (8, 8)), 4)
+    call("gm_tune_set", p.W, p.N, prec, *shapes[0], 4)
+
+
+@pytest.mark.parametrize("width", [256, 250])
+def test_fp16_operator_matches_emulation(gpu, width):
+    """The 16-bit scaled operator the solver applies (the symmetric 12-band
+    layout for W % 4 == 0, the full 24-band one otherwise) is the one a host
+    emulation builds: As = S A S with S = 1/sqrt(a_ii), entries rounded to
+    FP16 (RNE) -- for the symmetric layout the upper entry of each pair serves
+    both.  Checked through k fixed-count CG iterations (no residual
+    replacement): a wrong lower index, sign or missing band moves x by far
+    more than the FP32-vs-FP64 arithmetic difference."""
+    import scipy.sparse as sp
+    p = synth.make_problem(width, 96, 77, 1, 0.08)
+    W, N = p.W, p.N
+    a, b, _ = O.levin_gather_acc_f32(p.image, p.lam, p.target, 1e-7)
+    s32 = (np.float32(1) / np.sqrt(a[12][:N].astype(np.float32))).astype(np.float32)
+    offs = O.bands25(W)
+    rows, cols, vals = [], [], []
+    for k, o in enumerate(offs):
+        if k == 12:
+            continue
+        i = np.arange(N)
+        m = (i + o >= 0) & (i + o < N)
+        if W % 4 == 0 and k < 12:
+            # lower band: the upper entry of the partner row i + o_k (band 24 - k)
+            src = i + o
+            v = np.zeros(N, np.float32)
+            v[m] = (a[24 - k][src[m]] * s32[src[m]]) * s32[i[m]]
+        else:
+            j = np.clip(i + o, 0, N - 1)
+            v = (a[k][:N] * s32) * s32[j]
+        v16 = v.astype(np.float16).astype(np.float64)
+        rows.append(i[m]), cols.append(i[m] + o), vals.append(v16[m])
+    As = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(N, N))
+    As = As + sp.identity(N)
+    sv = s32.astype(np.float64)
+    s = _gpu_system(p)
+    for k in (2, 5):
+        x, _ = matting.solve(s, method=METHOD_PCG, precision=PREC_FP16, tol=0, max_iter=k, verify=False,
+                             refine_every=-1)
+        y = np.zeros(N)
+        r = -sv * b.astype(np.float64)
+        pv = np.zeros(N)
+        rz = r @ r
+        beta = 0.0
+        for _ in range(k):
+            pv = beta * pv - r
+            Ap = As @ pv
+            al = rz / (pv @ Ap)
+            y += al * pv
+            r += al * Ap
+            rz, beta = r @ r, (r @ r) / rz
+        xe = sv * y
+        assert np.abs(x.numpy() - xe).max() <= 1e-5 * np.abs(xe).max(), k
