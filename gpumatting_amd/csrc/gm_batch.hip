@@ -4,25 +4,30 @@
 // Each image i has one Levin Laplacian L_i (25 FP32 bands, no data term) and
 // nrhs independent constraint sets j, i.e. nrhs systems
 //     (L_i + diag(lam_ij)) x_ij = lam_ij * t_ij .
-// The systems of one image share L_i's off-diagonal bands, so one pass over a
+// The systems of one image share L_i's off-diagonal bands: one pass over a
 // tile loads those 24 bands into registers once and applies them to every
 // right-hand side; each system brings its own FP32 diagonal d_ij = L_ii +
-// lam_ij.  Jacobi-PCG per system, with the single-system kernels' sign and
-// reduction conventions.
+// lam_ij.  Jacobi-PCG per system, with the single-system kernels' sign,
+// residual-replacement and hand-off conventions (see Scalars).
 //
-// One launch per phase covers all images: a global tile list (tile -> image)
-// is walked by a persistent, XCD-aware grid.  Reductions are per image: every
-// tile writes its partial sums, and the last tile of an image to arrive sums
-// that image's tiles in index order and finalises its systems' scalars
-// (deterministic).  Systems that have converged are skipped; an image whose
-// systems are all done costs nothing.  Iterations are captured in a hipGraph
-// (kernels only, no memset nodes) and replayed.
+// Work lists.  The step and true-residual kernels walk 64x4-pixel tiles (one
+// pixel per thread, any image width); the update kernel walks 1024-pixel
+// chunks.  Each list concatenates all images.  A launch splits its list into
+// contiguous ranges, one per workgroup, so a workgroup touches few images: it
+// accumulates its partial sums per (image, system) in registers and stores
+// one slot per image it touches.  The consumer kernel folds an image's slots
+// -- those of the workgroups whose ranges cover the image, in index order --
+// with no fence or atomic (the kernel boundary orders them).  Systems that
+// have converged are skipped.  Iterations are captured in a hipGraph (kernel
+// nodes only) and replayed.
 //
 // Mixed precision: the off-diagonal bands are also kept in FP16 (unscaled;
-// |L_ij| <= 9), the per-system diagonals stay FP32, and the residual is
-// replaced by the FP64-accumulated true residual (FP32 L) every 50 iterations.
+// |L_ij| < 20), the per-system diagonals stay FP32, and the residual is
+// replaced by the FP64-accumulated true residual (FP32 L) every 50
+// iterations (see kDrift2 for when the search direction is kept).
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "gm_internal.h"
@@ -34,10 +39,29 @@ gm_status levin_assemble_raw(float* A, size_t apitch, float* b, float* dinv, con
                              int mode, int W, int H, gm_stream_t s);
 
 constexpr int kMaxRhs = 8;
-constexpr int kBR = 4;  // rows per thread of the batched kernels
+constexpr int kTX = 64, kTY = kBlock / kTX;  // step / true-residual tile: 64 x 4 pixels
+constexpr int kWX = kTX + 4, kWY = kTY + 4;   // staged window (2-pixel halo)
+constexpr int kWin = kWX * kWY;               // 544 cells
+constexpr int kWinPer = (kWin + kBlock - 1) / kBlock;
+constexpr int kChunk = 4 * kBlock;            // update-kernel chunk: 1024 pixels, 4 per thread
+static_assert(kTY * kTX == kBlock, "one pixel per thread");
+
+// Static split of a work list over the workgroups of a launch.
+struct BPart {
+    int T;            // items in the list
+    int G;            // workgroups
+    int K;            // slots per workgroup: the most images one range touches
+    const int* img;   // item -> image
+    const int* wimg;  // workgroup -> image of its first item
+};
+__device__ __forceinline__ int part_begin(const BPart& P, int b) { return (int)(((long long)b * P.T) / P.G); }
 
 struct BImg {
-    int W, N, ntiles, tile0, sys0, nsys;
+    int W, H, N, sys0, nsys;
+    int t0, nt, ntx;  // step tiles: first index in the tile list, count, tiles per image row
+    int c0, nc;       // update chunks
+    int wf1, wl1;     // first / last workgroup of the tile partition covering the image
+    int wf2, wl2;     // same for the chunk partition
     const float* L;
     size_t pitch;
     const unsigned short* L16;
@@ -48,18 +72,15 @@ struct BSys {
     float *x, *r, *p0, *p1, *Ap, *d, *dinv, *b;
 };
 
-enum { kRun = 0, kConverged = 1, kBreakdown = 2, kNonFinite = 3, kMaxIt = 4 };
-
-__device__ __forceinline__ bool b_stopped(const Scalars* s) { return s->done != kRun || s->verify != 0; }
-
-// Sum part[(tile0 + k) * stride + off] over k < ntiles in index order; every
-// thread gets the total, and every workgroup computes the same bits.
-__device__ __forceinline__ double image_fold(const double* __restrict__ part, int tile0, int ntiles, int stride,
-                                             int off) {
+// Sum an image's slots: slot[(b * K + im - wimg[b]) * stride + off] over the
+// workgroups b in [wf, wl], in a fixed order; every thread gets the total.
+__device__ __forceinline__ double img_fold(const double* __restrict__ slot, const BPart& P, int im, int wf, int wl,
+                                           int stride, int off) {
     __shared__ double fr[kWaves];
     __shared__ double ft;
     double s = 0;
-    for (int k = threadIdx.x; k < ntiles; k += kBlock) s += part[(size_t)(tile0 + k) * stride + off];
+    for (int b = wf + (int)threadIdx.x; b <= wl; b += kBlock)
+        s += slot[((size_t)b * P.K + (im - P.wimg[b])) * stride + off];
     s = wave_sum(s);
     __syncthreads();  // an earlier call's readers are done with fr / ft
     if ((threadIdx.x & 63) == 0) fr[threadIdx.x >> 6] = s;
@@ -74,26 +95,44 @@ __device__ __forceinline__ double image_fold(const double* __restrict__ part, in
     return ft;
 }
 
-// The batched iteration uses the same fence-free hand-off as the single
-// system's CG kernels (see Scalars), per system: the step kernel folds the
-// update kernel's per-tile partials of the image (once per image a workgroup
-// visits) and only the workgroup holding the image's first tile writes the
-// system's `hand`; the update kernel reads only `hand` and only that same
-// workgroup publishes the system's Scalars.
+// Store this workgroup's per-system partial sums for one image (block
+// reduction of acc[j][q], j < kMaxRhs, q < NQ) and clear them.
+template <int NQ>
+__device__ __forceinline__ void flush_slot(double (&acc)[kMaxRhs][NQ], double* __restrict__ dst) {
+    __shared__ double red[kMaxRhs * NQ][kWaves];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < kMaxRhs; ++j)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const double s = wave_sum(acc[j][q]);
+            if (lane == 0) red[j * NQ + q][wid] = s;
+            acc[j][q] = 0.0;
+        }
+    __syncthreads();
+    if (threadIdx.x < kMaxRhs * NQ) {
+        double t = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) t += red[threadIdx.x][w];
+        dst[threadIdx.x] = t;
+    }
+    __syncthreads();  // red is reused by the next flush
+}
 
-// Step-kernel view of one system on entering an image: fold pending update
-// partials and run the convergence test (every thread; results to chand).
-__device__ __forceinline__ void b_step_enter(const BImg& I, const Scalars* sc, const double* __restrict__ part2,
-                                             int verify, Hand* chand) {
+// Step-kernel view of one image's systems: fold pending update partials and
+// run the convergence test (every thread; results to chand).
+__device__ __forceinline__ void b_step_enter(const BImg& I, int im, const Scalars* sc,
+                                             const double* __restrict__ slot2, const BPart& P2, int verify,
+                                             Hand* chand) {
     __syncthreads();  // chand of the previous image is no longer read
     for (int j = 0; j < I.nsys; ++j) {
         const Scalars* s = &sc[I.sys0 + j];
-        Hand h{s->rz, s->rz_prev, s->rr, s->beta, s->iters, kHandRun, kRun, 0, I.ntiles};
+        Hand h{s->rz, s->rz_prev, s->rr, s->beta, s->iters, kHandRun, kRun, 0, 0};
         if (s->done != kRun || s->verify) {
             h.mode = kHandIdle;
         } else if (s->pend) {
-            const double rz = image_fold(part2, I.tile0, I.ntiles, 2 * kMaxRhs, 2 * j);
-            const double rr = image_fold(part2, I.tile0, I.ntiles, 2 * kMaxRhs, 2 * j + 1);
+            const double rz = img_fold(slot2, P2, im, I.wf2, I.wl2, 2 * kMaxRhs, 2 * j);
+            const double rr = img_fold(slot2, P2, im, I.wf2, I.wl2, 2 * kMaxRhs, 2 * j + 1);
             if (!isfinite(rz) || !isfinite(rr)) {
                 h.mode = kHandStop;
                 h.done = kNonFinite;
@@ -117,186 +156,233 @@ __device__ __forceinline__ void b_step_enter(const BImg& I, const Scalars* sc, c
     __syncthreads();
 }
 
-// K1: p = -D^-1 r + beta p_old staged per system, Ap = (L_off + diag d) p,
-// per-tile <p, Ap> partials.
+// The 24 off-diagonal band values of pixel i (band order, 12 skipped).
 template <int FMT>
-__global__ __launch_bounds__(kBlock) void kb_cgstep(const BImg* __restrict__ imgs, const BSys* __restrict__ sys,
-                                                    Scalars* sc, const int* __restrict__ tile_img, int ntot,
-                                                    double* __restrict__ part1, const double* __restrict__ part2,
-                                                    int verify) {
-    constexpr int R = kBR;
-    __shared__ __attribute__((aligned(16))) float lds[Tile<R>::LDS_FLOATS];
+__device__ __forceinline__ void load_pixel_bands(const BImg& I, int i, float (&a)[24]) {
+#pragma unroll
+    for (int k = 0, kk = 0; k < 25; ++k) {
+        if (k == 12) continue;
+        if constexpr (FMT == kF32) {
+            a[kk] = __builtin_nontemporal_load(I.L + (size_t)k * I.pitch + i);
+        } else {
+            const unsigned short h = __builtin_nontemporal_load(I.L16 + (size_t)k * I.pitch16 + i);
+            a[kk] = (float)__builtin_bit_cast(_Float16, h);
+        }
+        ++kk;
+    }
+}
+
+// K1: per running system p = -D^-1 r + beta p_old (staged), Ap = (L_off +
+// diag d) p, <p, Ap> partials.
+template <int FMT>
+__global__ __launch_bounds__(kBlock) void kb_step(const BImg* __restrict__ imgs, const BSys* __restrict__ sys,
+                                                  Scalars* sc, BPart P1, BPart P2, double* __restrict__ slot1,
+                                                  const double* __restrict__ slot2, int verify) {
+    __shared__ float win[kMaxRhs][kWin];
     __shared__ Hand chand[kMaxRhs];
-    __shared__ double part[kMaxRhs];
-    const TileRange tr = xcd_tiles(ntot);
-    int cimg = -1;
-    for (int t = tr.begin; t < tr.end; t += tr.step) {
-        const int im = tile_img[t];
-        const BImg I = imgs[im];
-        if (im != cimg) {
-            b_step_enter(I, sc, part2, verify, chand);
-            cimg = im;
+    const int b = blockIdx.x;
+    const int tb = part_begin(P1, b), te = part_begin(P1, b + 1);
+    const int lane = (int)threadIdx.x % kTX, row = (int)threadIdx.x / kTX;
+    double acc[kMaxRhs][1] = {};
+    int cur = -1;
+    unsigned mask = 0;
+    BImg I{};
+    for (int t = tb; t < te; ++t) {
+        const int im = P1.img[t];
+        if (im != cur) {
+            if (cur >= 0) flush_slot<1>(acc, slot1 + ((size_t)b * P1.K + (cur - P1.wimg[b])) * kMaxRhs);
+            I = imgs[im];
+            b_step_enter(I, im, sc, slot2, P2, verify, chand);
+            if (b == I.wf1 && threadIdx.x == 0)
+                for (int j = 0; j < I.nsys; ++j) sc[I.sys0 + j].hand = chand[j];
+            mask = 0;
+            for (int j = 0; j < I.nsys; ++j)
+                if (chand[j].mode == kHandRun) mask |= 1u << j;
+            cur = im;
         }
-        if (t == I.tile0 && threadIdx.x == 0)
-            for (int j = 0; j < I.nsys; ++j) sc[I.sys0 + j].hand = chand[j];
-        unsigned mask = 0;
-        for (int j = 0; j < I.nsys; ++j)
-            if (chand[j].mode == kHandRun) mask |= 1u << j;
         if (mask == 0) continue;
-        const int i0 = (t - I.tile0) * Tile<R>::ROWS;
-        const int ib = i0 + R * threadIdx.x;
-        Bands<R, FMT> av;
-        if (ib < I.N) {
-            if (FMT == kF32) load_bands<R, FMT, kDiagCaller>(I.L, I.pitch, ib, av);
-            else load_bands<R, FMT, kDiagCaller>(I.L16, I.pitch16, ib, av);
-        }
-        for (int j = 0; j < I.nsys; ++j) {
-            if (!((mask >> j) & 1u)) {
-                if (threadIdx.x == 0) part[j] = 0;
-                continue;
-            }
+        const int lt = t - I.t0;
+        const int x0 = (lt % I.ntx) * kTX, y0 = (lt / I.ntx) * kTY;
+        const int x = x0 + lane, y = y0 + row;
+        const bool valid = x < I.W && y < I.H;
+        const int i = y * I.W + x;
+        // every global load of the tile first, then LDS
+        float a[24];
+        if (valid) load_pixel_bands<FMT>(I, i, a);
+        float sv[kMaxRhs][kWinPer];
+#pragma unroll
+        for (int j = 0; j < kMaxRhs; ++j) {
+            if (!((mask >> j) & 1u)) continue;
             const BSys S = sys[I.sys0 + j];
             const float beta = chand[j].beta;
-            const int par = chand[j].iters & 1;
-            const float* __restrict__ pold = par ? S.p1 : S.p0;
-            float* __restrict__ pnew = par ? S.p0 : S.p1;
-            __syncthreads();
-            if (beta != 0.f)
-                stage5<R>(lds, i0, I.W, I.N, [&](int g) { return fmaf(beta, pold[g], -(S.dinv[g] * S.r[g])); });
-            else
-                stage5<R>(lds, i0, I.W, I.N, [&](int g) { return -(S.dinv[g] * S.r[g]); });
-            __syncthreads();
-            double v = 0;
-            if (ib < I.N) {
-                float y[R], pc[R], dd[R];
-                apply_bands<R, FMT, kDiagCaller>(av, lds + R * threadIdx.x, y);
-                load_rows<R>(S.d, ib, I.N, dd);
+            const float* __restrict__ pold = (chand[j].iters & 1) ? S.p1 : S.p0;
 #pragma unroll
-                for (int q = 0; q < R; ++q) {
-                    pc[q] = lds[2 * Tile<R>::SEG + R * threadIdx.x + 2 + q];
-                    y[q] = fmaf(dd[q], pc[q], y[q]);
+            for (int q = 0; q < kWinPer; ++q) {
+                const int e = (int)threadIdx.x + q * kBlock;
+                const int wy = e / kWX, wx = e - wy * kWX;
+                // matrix-row index of the window cell (cells beside the image alias the neighbouring
+                // row, exactly like the row-wise product; their band entries are 0)
+                const long long g = (long long)(y0 - 2 + wy) * I.W + (x0 - 2 + wx);
+                float v = 0.f;
+                if (e < kWin && g >= 0 && g < I.N) {
+                    const float z = S.dinv[g] * S.r[g];
+                    v = beta != 0.f ? fmaf(beta, pold[g], -z) : -z;
                 }
-                store_rows<R>(pnew, ib, I.N, pc);
-                store_rows<R>(S.Ap, ib, I.N, y);
-#pragma unroll
-                for (int q = 0; q < R; ++q)
-                    if (ib + q < I.N) v += (double)(pc[q] * y[q]);
+                sv[j][q] = v;
             }
-            const double bs = block_sum(v);
-            if (threadIdx.x == 0) part[j] = bs;
+        }
+        __syncthreads();  // the previous tile is done with win
+#pragma unroll
+        for (int j = 0; j < kMaxRhs; ++j) {
+            if (!((mask >> j) & 1u)) continue;
+#pragma unroll
+            for (int q = 0; q < kWinPer; ++q) {
+                const int e = (int)threadIdx.x + q * kBlock;
+                if (e < kWin) win[j][e] = sv[j][q];
+            }
         }
         __syncthreads();
-        if (threadIdx.x < I.nsys) part1[(size_t)t * kMaxRhs + threadIdx.x] = part[threadIdx.x];
-        __syncthreads();  // part is rewritten by the next tile
+        if (!valid) continue;
+#pragma unroll
+        for (int j = 0; j < kMaxRhs; ++j) {
+            if (!((mask >> j) & 1u)) continue;
+            const BSys S = sys[I.sys0 + j];
+            const float* __restrict__ w = &win[j][row * kWX + lane];  // cell (row, lane) is pixel (y - 2, x - 2)
+            float yv = 0.f;
+#pragma unroll
+            for (int k = 0, kk = 0; k < 25; ++k) {
+                if (k == 12) continue;
+                yv = fmaf(a[kk], w[(k / 5) * kWX + (k % 5)], yv);
+                ++kk;
+            }
+            const float pc = w[2 * kWX + 2];
+            yv = fmaf(S.d[i], pc, yv);
+            float* __restrict__ pnew = (chand[j].iters & 1) ? S.p0 : S.p1;
+            pnew[i] = pc;
+            S.Ap[i] = yv;
+            acc[j][0] += (double)(pc * yv);
+        }
     }
+    if (cur >= 0) flush_slot<1>(acc, slot1 + ((size_t)b * P1.K + (cur - P1.wimg[b])) * kMaxRhs);
 }
 
 enum { kBRun = 0, kBIdle = 10, kBStop = 11 };  // update-kernel outcome per system (else kBreakdown / kNonFinite)
 
-// K2: x += alpha p, r += alpha Ap, per-tile <r, D^-1 r>, ||r||^2 partials.
-__global__ __launch_bounds__(kBlock) void kb_cgupdate(const BImg* __restrict__ imgs, const BSys* __restrict__ sys,
-                                                      Scalars* sc, const int* __restrict__ tile_img, int ntot,
-                                                      const double* __restrict__ part1, double* __restrict__ part2) {
-    constexpr int R = kBR;
+// K2: per running system x += alpha p, r += alpha Ap; <r, D^-1 r>, ||r||^2 partials.
+__global__ __launch_bounds__(kBlock) void kb_update(const BImg* __restrict__ imgs, const BSys* __restrict__ sys,
+                                                    Scalars* sc, BPart P1, BPart P2, const double* __restrict__ slot1,
+                                                    double* __restrict__ slot2) {
     __shared__ Hand chand[kMaxRhs];
     __shared__ int cst[kMaxRhs];
     __shared__ float calpha[kMaxRhs];
     __shared__ double cpap[kMaxRhs];
-    __shared__ double part[kMaxRhs][2];
-    const TileRange tr = xcd_tiles(ntot);
-    int cimg = -1;
-    for (int t = tr.begin; t < tr.end; t += tr.step) {
-        const int im = tile_img[t];
-        const BImg I = imgs[im];
-        if (im != cimg) {
-            __syncthreads();
+    const int b = blockIdx.x;
+    const int cb = part_begin(P2, b), ce = part_begin(P2, b + 1);
+    double acc[kMaxRhs][2] = {};
+    int cur = -1;
+    unsigned mask = 0;
+    BImg I{};
+    for (int c = cb; c < ce; ++c) {
+        const int im = P2.img[c];
+        if (im != cur) {
+            if (cur >= 0) flush_slot<2>(acc, slot2 + ((size_t)b * P2.K + (cur - P2.wimg[b])) * 2 * kMaxRhs);
+            I = imgs[im];
+            __syncthreads();  // chand / cst of the previous image are no longer read
             for (int j = 0; j < I.nsys; ++j) {
                 const Hand h = sc[I.sys0 + j].hand;
                 int st = kBRun;
-                float a = 0.f;
+                float al = 0.f;
                 double pAp = 0;
                 if (h.mode == kHandIdle) {
                     st = kBIdle;
                 } else if (h.mode == kHandStop) {
                     st = kBStop;
                 } else {
-                    pAp = image_fold(part1, I.tile0, I.ntiles, kMaxRhs, j);
+                    pAp = img_fold(slot1, P1, im, I.wf1, I.wl1, kMaxRhs, j);
                     if (!isfinite(pAp)) st = kNonFinite;
                     else if (!(pAp > 0)) st = kBreakdown;
-                    else a = (float)(h.rz / pAp);
+                    else al = (float)(h.rz / pAp);
                 }
                 if (threadIdx.x == 0) {
                     chand[j] = h;
                     cst[j] = st;
-                    calpha[j] = a;
+                    calpha[j] = al;
                     cpap[j] = pAp;
                 }
             }
             __syncthreads();
-            cimg = im;
+            if (b == I.wf2 && threadIdx.x == 0)
+                for (int j = 0; j < I.nsys; ++j) {
+                    Scalars* s = &sc[I.sys0 + j];
+                    const Hand& h = chand[j];
+                    if (cst[j] == kBStop) {
+                        publish(s, h, h.done, h.verify, h.iters, 0, 0);
+                    } else if (cst[j] == kBRun) {
+                        publish(s, h, kRun, 0, h.iters + 1, 1, 0);
+                        s->pAp = cpap[j];
+                        s->alpha = calpha[j];
+                    } else if (cst[j] != kBIdle) {
+                        publish(s, h, cst[j], 0, h.iters, 0, 0);
+                        s->pAp = cpap[j];
+                    }
+                }
+            mask = 0;
+            for (int j = 0; j < I.nsys; ++j)
+                if (cst[j] == kBRun) mask |= 1u << j;
+            cur = im;
         }
-        if (t == I.tile0 && threadIdx.x == 0)
-            for (int j = 0; j < I.nsys; ++j) {
-                Scalars* s = &sc[I.sys0 + j];
-                const Hand& h = chand[j];
-                if (cst[j] == kBStop) {
-                    publish(s, h, h.done, h.verify, h.iters, 0, 0);
-                } else if (cst[j] == kBRun) {
-                    publish(s, h, kRun, 0, h.iters + 1, 1, I.ntiles);
-                    s->pAp = cpap[j];
-                    s->alpha = calpha[j];
-                } else if (cst[j] != kBIdle) {
-                    publish(s, h, cst[j], 0, h.iters, 0, 0);
-                    s->pAp = cpap[j];
+        if (mask == 0) continue;
+        const int i0 = (c - I.c0) * kChunk + 4 * (int)threadIdx.x;
+        if (i0 >= I.N) continue;
+#pragma unroll
+        for (int j = 0; j < kMaxRhs; ++j) {
+            if (!((mask >> j) & 1u)) continue;
+            const BSys S = sys[I.sys0 + j];
+            const float al = calpha[j];
+            const float* __restrict__ p = (chand[j].iters & 1) ? S.p0 : S.p1;  // the p_new K1 wrote
+            if (i0 + 4 <= I.N) {  // per-system vectors are 256-byte aligned and i0 % 4 == 0
+                f32x4 xv = *reinterpret_cast<const f32x4*>(S.x + i0);
+                f32x4 rv = *reinterpret_cast<const f32x4*>(S.r + i0);
+                const f32x4 pv = *reinterpret_cast<const f32x4*>(p + i0);
+                const f32x4 av = *reinterpret_cast<const f32x4*>(S.Ap + i0);
+                const f32x4 dv = *reinterpret_cast<const f32x4*>(S.dinv + i0);
+                float xs[4] = {xv.x, xv.y, xv.z, xv.w}, rs[4] = {rv.x, rv.y, rv.z, rv.w};
+                const float ps[4] = {pv.x, pv.y, pv.z, pv.w}, as[4] = {av.x, av.y, av.z, av.w};
+                const float ds[4] = {dv.x, dv.y, dv.z, dv.w};
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    xs[q] = fmaf(al, ps[q], xs[q]);
+                    rs[q] = fmaf(al, as[q], rs[q]);
+                    acc[j][0] += (double)(rs[q] * (ds[q] * rs[q]));
+                    acc[j][1] += (double)(rs[q] * rs[q]);
+                }
+                *reinterpret_cast<f32x4*>(S.x + i0) = f32x4{xs[0], xs[1], xs[2], xs[3]};
+                *reinterpret_cast<f32x4*>(S.r + i0) = f32x4{rs[0], rs[1], rs[2], rs[3]};
+            } else {
+                for (int i = i0; i < I.N; ++i) {
+                    S.x[i] = fmaf(al, p[i], S.x[i]);
+                    const float rn = fmaf(al, S.Ap[i], S.r[i]);
+                    S.r[i] = rn;
+                    acc[j][0] += (double)(rn * (S.dinv[i] * rn));
+                    acc[j][1] += (double)(rn * rn);
                 }
             }
-        unsigned mask = 0;
-        for (int j = 0; j < I.nsys; ++j)
-            if (cst[j] == kBRun) mask |= 1u << j;
-        if (mask == 0) continue;
-        const int i0 = (t - I.tile0) * Tile<R>::ROWS;
-        for (int j = 0; j < I.nsys; ++j) {
-            if (!((mask >> j) & 1u)) {
-                if (threadIdx.x == 0) part[j][0] = part[j][1] = 0;
-                continue;
-            }
-            const BSys S = sys[I.sys0 + j];
-            const float alpha = calpha[j];
-            const float* __restrict__ p = (chand[j].iters & 1) ? S.p0 : S.p1;
-            double v0 = 0, v1 = 0;
-            for (int q = 0; q < R; ++q) {
-                const int i = i0 + q * kBlock + threadIdx.x;  // coalesced across the workgroup
-                if (i >= I.N) break;
-                S.x[i] = fmaf(alpha, p[i], S.x[i]);
-                const float rn = fmaf(alpha, S.Ap[i], S.r[i]);
-                S.r[i] = rn;
-                v0 += (double)(rn * (S.dinv[i] * rn));
-                v1 += (double)(rn * rn);
-            }
-            const double b0 = block_sum(v0);
-            const double b1 = block_sum(v1);
-            if (threadIdx.x == 0) {
-                part[j][0] = b0;
-                part[j][1] = b1;
-            }
         }
-        __syncthreads();
-        if (threadIdx.x < 2 * I.nsys)
-            part2[(size_t)t * 2 * kMaxRhs + threadIdx.x] = part[threadIdx.x >> 1][threadIdx.x & 1];
-        __syncthreads();  // part is rewritten by the next tile
     }
+    if (cur >= 0) flush_slot<2>(acc, slot2 + ((size_t)b * P2.K + (cur - P2.wimg[b])) * 2 * kMaxRhs);
 }
 
 // Fold pending update partials into the Scalars (one workgroup per image;
 // launched before the host reads the final state).
-__global__ __launch_bounds__(kBlock) void kb_fold(const BImg* __restrict__ imgs, Scalars* sc,
-                                                  const double* __restrict__ part2, int verify) {
-    const BImg I = imgs[blockIdx.x];
+__global__ __launch_bounds__(kBlock) void kb_fold(const BImg* __restrict__ imgs, Scalars* sc, BPart P2,
+                                                  const double* __restrict__ slot2, int verify) {
+    const int im = blockIdx.x;
+    const BImg I = imgs[im];
     for (int j = 0; j < I.nsys; ++j) {
         Scalars* s = &sc[I.sys0 + j];
         if (s->done != kRun || s->verify || !s->pend) continue;
-        const double rz = image_fold(part2, I.tile0, I.ntiles, 2 * kMaxRhs, 2 * j);
-        const double rr = image_fold(part2, I.tile0, I.ntiles, 2 * kMaxRhs, 2 * j + 1);
+        const double rz = img_fold(slot2, P2, im, I.wf2, I.wl2, 2 * kMaxRhs, 2 * j);
+        const double rr = img_fold(slot2, P2, im, I.wf2, I.wl2, 2 * kMaxRhs, 2 * j + 1);
         if (threadIdx.x != 0) continue;
         s->pend = 0;
         if (!isfinite(rz) || !isfinite(rr)) {
@@ -321,130 +407,116 @@ __device__ __forceinline__ bool tr_selected(const Scalars* s, int op) {
     return op == 0 || op == 2 || (op == 1 && s->done == kRun) || (op == 3 && s->done == kRun && s->verify);
 }
 
-// FP64-accumulated true residual t = (L_off + diag d) x - b per system, per-tile
-// partials (<t, D^-1 t>, ||t||^2, drift ||t - r||^2); kb_trueres_fin folds
-// them.  op 0: measure only (g0); 1: replace r for running systems; 2: start
-// (all systems, beta = 0); 3: replace r for systems whose recursive residual
-// asked for verification.
+// FP64-accumulated true residual t = (L_off + diag d) x - b of the selected
+// systems, partials (<t, D^-1 t>, ||t||^2, drift ||t - r||^2, ||b||^2) per
+// image slot; kb_trueres_fin folds them.  op 0: measure only (g0); 1: replace
+// r for running systems; 2: start (all systems, beta = 0, also ||b||^2); 3:
+// replace r for systems whose recursive residual asked for verification.
 __global__ __launch_bounds__(kBlock) void kb_trueres(const BImg* __restrict__ imgs, const BSys* __restrict__ sys,
-                                                     const Scalars* sc, const int* __restrict__ tile_img, int ntot,
-                                                     int op, double* __restrict__ part3) {
-    constexpr int R = kBR;
-    constexpr int SEG = Tile<R>::SEG;
-    __shared__ __attribute__((aligned(16))) float lds[Tile<R>::LDS_FLOATS];
-    __shared__ double part[kMaxRhs][3];
-    const TileRange tr = xcd_tiles(ntot);
-    for (int t = tr.begin; t < tr.end; t += tr.step) {
-        const int im = tile_img[t];
-        const BImg I = imgs[im];
-        unsigned mask = 0;
-        for (int j = 0; j < I.nsys; ++j)
-            if (tr_selected(&sc[I.sys0 + j], op)) mask |= 1u << j;
+                                                     const Scalars* sc, BPart P1, int op, double* __restrict__ slot3) {
+    __shared__ float win[kMaxRhs][kWin];
+    const int b = blockIdx.x;
+    const int tb = part_begin(P1, b), te = part_begin(P1, b + 1);
+    const int lane = (int)threadIdx.x % kTX, row = (int)threadIdx.x / kTX;
+    double acc[kMaxRhs][4] = {};
+    int cur = -1;
+    unsigned mask = 0, dmask = 0;
+    BImg I{};
+    for (int t = tb; t < te; ++t) {
+        const int im = P1.img[t];
+        if (im != cur) {
+            if (cur >= 0) flush_slot<4>(acc, slot3 + ((size_t)b * P1.K + (cur - P1.wimg[b])) * 4 * kMaxRhs);
+            I = imgs[im];
+            mask = dmask = 0;
+            for (int j = 0; j < I.nsys; ++j) {
+                const Scalars* s = &sc[I.sys0 + j];
+                if (tr_selected(s, op)) mask |= 1u << j;
+                if (op == 1 && s->verify == 0) dmask |= 1u << j;  // periodic replacement: measure the drift
+            }
+            cur = im;
+        }
         if (mask == 0) continue;
-        const int i0 = (t - I.tile0) * Tile<R>::ROWS;
-        const int ib = i0 + R * threadIdx.x;
-        Bands<R, kF32> av;
-        if (ib < I.N) load_bands<R, kF32, kDiagCaller>(I.L, I.pitch, ib, av);
-        for (int j = 0; j < I.nsys; ++j) {
-            if (!((mask >> j) & 1u)) {
-                if (threadIdx.x == 0) part[j][0] = part[j][1] = part[j][2] = 0;
-                continue;
-            }
+        const int lt = t - I.t0;
+        const int x0 = (lt % I.ntx) * kTX, y0 = (lt / I.ntx) * kTY;
+        const int x = x0 + lane, y = y0 + row;
+        const bool valid = x < I.W && y < I.H;
+        const int i = y * I.W + x;
+        float a[24];
+        if (valid) load_pixel_bands<kF32>(I, i, a);
+        float sv[kMaxRhs][kWinPer];
+#pragma unroll
+        for (int j = 0; j < kMaxRhs; ++j) {
+            if (!((mask >> j) & 1u)) continue;
             const BSys S = sys[I.sys0 + j];
-            // periodic replacement of a running system: measure the drift
-            const bool drift = op == 1 && sc[I.sys0 + j].verify == 0;
-            __syncthreads();
-            stage5<R>(lds, i0, I.W, I.N, [&](int g) { return S.x[g]; });
-            __syncthreads();
-            double v0 = 0, v1 = 0, v2 = 0;
-            if (ib < I.N) {
-                const float* xs = lds + R * threadIdx.x;
-                double acc[R];
 #pragma unroll
-                for (int q = 0; q < R; ++q) acc[q] = 0.0;
-#pragma unroll
-                for (int k = 0; k < 25; ++k) {
-                    if (k == 12) continue;
-                    const int s5 = k / 5, dx = k % 5;
-#pragma unroll
-                    for (int q = 0; q < R; ++q)
-                        acc[q] = fma((double)av.v[k].get(q), (double)xs[s5 * SEG + q + dx], acc[q]);
-                }
-                float dd[R], bb[R], di[R], out[R], ro[R];
-                load_rows<R>(S.d, ib, I.N, dd);
-                load_rows<R>(S.b, ib, I.N, bb);
-                load_rows<R>(S.dinv, ib, I.N, di);
-                if (drift) load_rows<R>(S.r, ib, I.N, ro);
-#pragma unroll
-                for (int q = 0; q < R; ++q) {
-                    const double tq = fma((double)dd[q], (double)xs[2 * SEG + q + 2], acc[q]) - (double)bb[q];
-                    out[q] = (float)tq;
-                    if (ib + q < I.N) {
-                        v1 += tq * tq;
-                        v0 += (double)(out[q] * (di[q] * out[q]));
-                        if (drift) v2 += (double)(out[q] - ro[q]) * (double)(out[q] - ro[q]);
-                    }
-                }
-                if (op != 0) store_rows<R>(S.r, ib, I.N, out);
+            for (int q = 0; q < kWinPer; ++q) {
+                const int e = (int)threadIdx.x + q * kBlock;
+                const int wy = e / kWX, wx = e - wy * kWX;
+                const long long g = (long long)(y0 - 2 + wy) * I.W + (x0 - 2 + wx);
+                sv[j][q] = (e < kWin && g >= 0 && g < I.N) ? S.x[g] : 0.f;
             }
-            const double b0 = block_sum(v0);
-            const double b1 = block_sum(v1);
-            const double b2 = block_sum(v2);
-            if (threadIdx.x == 0) {
-                part[j][0] = b0;
-                part[j][1] = b1;
-                part[j][2] = b2;
+        }
+        __syncthreads();  // the previous tile is done with win
+#pragma unroll
+        for (int j = 0; j < kMaxRhs; ++j) {
+            if (!((mask >> j) & 1u)) continue;
+#pragma unroll
+            for (int q = 0; q < kWinPer; ++q) {
+                const int e = (int)threadIdx.x + q * kBlock;
+                if (e < kWin) win[j][e] = sv[j][q];
             }
         }
         __syncthreads();
-        if (threadIdx.x < 3 * I.nsys)
-            part3[(size_t)t * 3 * kMaxRhs + threadIdx.x] = part[threadIdx.x / 3][threadIdx.x % 3];
-        __syncthreads();  // part is rewritten by the next tile
+        if (!valid) continue;
+#pragma unroll
+        for (int j = 0; j < kMaxRhs; ++j) {
+            if (!((mask >> j) & 1u)) continue;
+            const BSys S = sys[I.sys0 + j];
+            const float* __restrict__ w = &win[j][row * kWX + lane];
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0, kk = 0; k < 25; ++k) {
+                if (k == 12) continue;
+                s = fma((double)a[kk], (double)w[(k / 5) * kWX + (k % 5)], s);
+                ++kk;
+            }
+            const float bi = S.b[i];
+            const double tq = fma((double)S.d[i], (double)w[2 * kWX + 2], s) - (double)bi;
+            const float out = (float)tq;
+            acc[j][0] += (double)(out * (S.dinv[i] * out));
+            acc[j][1] += tq * tq;
+            if ((dmask >> j) & 1u) {
+                const double dd = (double)out - (double)S.r[i];
+                acc[j][2] += dd * dd;
+            }
+            acc[j][3] += (double)bi * (double)bi;
+            if (op != 0) S.r[i] = out;
+        }
     }
+    if (cur >= 0) flush_slot<4>(acc, slot3 + ((size_t)b * P1.K + (cur - P1.wimg[b])) * 4 * kMaxRhs);
 }
 
-// Finish a true-residual pass: one workgroup per image folds its tiles'
-// partials and updates the selected systems' scalars.
-__global__ __launch_bounds__(kBlock) void kb_trueres_fin(const BImg* __restrict__ imgs, Scalars* sc,
-                                                         const double* __restrict__ part3, int op) {
-    const BImg I = imgs[blockIdx.x];
+// Finish a true-residual pass: one workgroup per image folds its slots and
+// updates the selected systems' scalars.
+__global__ __launch_bounds__(kBlock) void kb_trueres_fin(const BImg* __restrict__ imgs, Scalars* sc, BPart P1,
+                                                         const double* __restrict__ slot3, int op, double tol) {
+    const int im = blockIdx.x;
+    const BImg I = imgs[im];
     for (int j = 0; j < I.nsys; ++j) {
         Scalars* s = &sc[I.sys0 + j];
         if (!tr_selected(s, op)) continue;
-        const double rz = image_fold(part3, I.tile0, I.ntiles, 3 * kMaxRhs, 3 * j);
-        const double rr = image_fold(part3, I.tile0, I.ntiles, 3 * kMaxRhs, 3 * j + 1);
-        const double d2 = image_fold(part3, I.tile0, I.ntiles, 3 * kMaxRhs, 3 * j + 2);
+        const double rz = img_fold(slot3, P1, im, I.wf1, I.wl1, 4 * kMaxRhs, 4 * j);
+        const double rr = img_fold(slot3, P1, im, I.wf1, I.wl1, 4 * kMaxRhs, 4 * j + 1);
+        const double d2 = img_fold(slot3, P1, im, I.wf1, I.wl1, 4 * kMaxRhs, 4 * j + 2);
+        const double bb = img_fold(slot3, P1, im, I.wf1, I.wl1, 4 * kMaxRhs, 4 * j + 3);
         if (threadIdx.x != 0) continue;
         s->g0 = rr;
         if (op == 0) continue;
-        if (!isfinite(rz) || !isfinite(rr)) {
-            s->done = kNonFinite;
-            continue;
-        }
         if (op == 2) {
-            s->beta = 0.f;
-        } else if (s->verify) {
-            // drifted recursion: restart from the true residual
-            s->beta = 0.f;
-            s->replacements += 1;
-            if (s->g1 > 0 && rr > 0.25 * s->g1) s->stall += 1;
-            else s->stall = 0;
-            s->g1 = rr;
-        } else {
-            // periodic: keep the direction unless the recursion drifted; the
-            // <r,z> of the last step is rz while its update's partials are
-            // pending (superseded by this residual), else rz_prev
-            const double rzp = s->pend ? s->rz : s->rz_prev;
-            s->beta = d2 > kDrift2 * rr ? 0.f : (rzp > 0 ? (float)(rz / rzp) : 0.f);
-            s->replacements += 1;
+            s->bb = bb;
+            s->tol2 = tol > 0 ? tol * tol * (bb > 0 ? bb : 1.0) : -1.0;
         }
-        s->rz_prev = s->pend ? s->rz : s->rz_prev;
-        s->rz = rz;
-        s->rr = rr;
-        s->verify = 0;
-        s->pend = 0;
-        if (s->tol2 >= 0 && rr <= s->tol2) s->done = kConverged;
-        else if (s->iters >= s->maxit || s->stall >= 3) s->done = kMaxIt;
+        replace_residual(s, rz, rr, op == 2, d2, rr);
     }
 }
 
@@ -477,7 +549,7 @@ __global__ __launch_bounds__(kBlock) void kb_to16(const float* __restrict__ L, s
             if (k != 12) L16[(size_t)k * pitch16 + i] = f32_to_f16_rne(L[(size_t)k * pitch + i]);
 }
 
-__global__ void kb_init(Scalars* sc, int nsys, int maxit, double tol) {
+__global__ void kb_init(Scalars* sc, int nsys, int maxit) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nsys) return;
     Scalars z = {};
@@ -485,50 +557,49 @@ __global__ void kb_init(Scalars* sc, int nsys, int maxit, double tol) {
     z.tol2 = -1.0;
     z.hand.mode = kHandIdle;
     sc[j] = z;
-    (void)tol;
-}
-
-// per system <b,b> -> tol2 (one workgroup per system)
-__global__ __launch_bounds__(kBlock) void kb_bnorm(const BSys* __restrict__ sys, const int* __restrict__ sysN,
-                                                   Scalars* sc, double tol) {
-    const int j = blockIdx.x;
-    const BSys S = sys[j];
-    const int N = sysN[j];
-    double v = 0;
-    for (int i = threadIdx.x; i < N; i += kBlock) v += (double)S.b[i] * (double)S.b[i];
-    const double bb = block_sum(v);
-    if (threadIdx.x == 0) {
-        sc[j].bb = bb;
-        sc[j].tol2 = tol > 0 ? tol * tol * (bb > 0 ? bb : 1.0) : -1.0;
-    }
 }
 
 }  // namespace gm
 
 using namespace gm;
 
+// Host side of a work-list partition.
+struct HostPart {
+    int T = 0, G = 0, K = 1;
+    std::vector<int> img;   // item -> image
+    std::vector<int> wimg;  // workgroup -> image of its first item
+    int* dimg = nullptr;
+    int* dwimg = nullptr;
+    BPart dev() const { return BPart{T, G, K, dimg, dwimg}; }
+};
+
 struct gm_batch_s {
-    int nimg = 0, nrhs = 0, nsys = 0, precision = 0, ntiles = 0;
+    int nimg = 0, nrhs = 0, nsys = 0, precision = 0;
     std::vector<BImg> himg;
     std::vector<BSys> hsys;
-    std::vector<int> hsysN;
     BImg* dimg = nullptr;
     BSys* dsys = nullptr;
-    int* dsysN = nullptr;
     Scalars* dsc = nullptr;
     Scalars* hsc = nullptr;  // pinned, nsys
-    int* dtile = nullptr;
-    double* dpart = nullptr;  // per-tile partial sums: step, update (2), true residual (3)
+    HostPart p1, p2;          // step tiles, update chunks
+    double* dslot = nullptr;  // slots: step (1 per system), update (2), true residual (4)
+    size_t slot1_n = 0, slot2_n = 0, slot3_n = 0;
     float* Lmem = nullptr;
     unsigned short* L16mem = nullptr;
     float* vmem = nullptr;
     hipStream_t cap = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
-    int graph_iters = 0, graph_grid = 0, graph_verify = -1;
+    int graph_iters = 0, graph_verify = -1;
     hipEvent_t ev = nullptr;
-    int per_cu = 2;
+    int per_cu = 4;
 };
+
+static void free_part(HostPart& p) {
+    if (p.dimg) (void)hipFree(p.dimg);
+    if (p.dwimg) (void)hipFree(p.dwimg);
+    p.dimg = p.dwimg = nullptr;
+}
 
 static void batch_free(gm_batch* b) {
     if (!b) return;
@@ -536,11 +607,76 @@ static void batch_free(gm_batch* b) {
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->cap) (void)hipStreamDestroy(b->cap);
     if (b->ev) (void)hipEventDestroy(b->ev);
-    for (void* p : {(void*)b->dimg, (void*)b->dsys, (void*)b->dsysN, (void*)b->dsc, (void*)b->dtile, (void*)b->dpart,
-                    (void*)b->Lmem, (void*)b->L16mem, (void*)b->vmem})
+    free_part(b->p1);
+    free_part(b->p2);
+    for (void* p : {(void*)b->dimg, (void*)b->dsys, (void*)b->dsc, (void*)b->dslot, (void*)b->Lmem,
+                    (void*)b->L16mem, (void*)b->vmem})
         if (p) (void)hipFree(p);
     if (b->hsc) (void)hipHostFree(b->hsc);
     delete b;
+}
+
+// Split list p (p.img filled, image-sorted) over G workgroups in contiguous
+// ranges; record each image's first and last covering workgroup.
+static void split(HostPart& p, int G, std::vector<BImg>& imgs, bool chunks) {
+    p.T = (int)p.img.size();
+    p.G = G;
+    p.wimg.assign(G, 0);
+    p.K = 1;
+    for (auto& I : imgs) {
+        (chunks ? I.wf2 : I.wf1) = G;
+        (chunks ? I.wl2 : I.wl1) = -1;
+    }
+    for (int b = 0; b < G; ++b) {
+        const long tb = (long)b * p.T / G, te = (long)(b + 1) * p.T / G;
+        if (tb >= te) {
+            p.wimg[b] = p.img[std::min<long>(tb, p.T - 1)];
+            continue;
+        }
+        const int i0 = p.img[tb], i1 = p.img[te - 1];
+        p.wimg[b] = i0;
+        p.K = std::max(p.K, i1 - i0 + 1);
+        for (int im = i0; im <= i1; ++im) {
+            BImg& I = imgs[im];
+            int& wf = chunks ? I.wf2 : I.wf1;
+            int& wl = chunks ? I.wl2 : I.wl1;
+            wf = std::min(wf, b);
+            wl = std::max(wl, b);
+        }
+    }
+}
+
+// (Re)build both partitions for the batch's grid and upload them.
+static gm_status batch_partition(gm_batch* b) {
+    DevInfo d;
+    gm_status st = dev_info(&d);
+    if (st != GM_OK) return st;
+    const int G1 = grid_for_tiles(d, (long)b->p1.img.size(), b->per_cu);
+    const int G2 = grid_for_tiles(d, (long)b->p2.img.size(), 2 * b->per_cu);
+    split(b->p1, G1, b->himg, false);
+    split(b->p2, G2, b->himg, true);
+    for (HostPart* p : {&b->p1, &b->p2}) {
+        free_part(*p);
+        GM_HIP(hipMalloc(&p->dimg, sizeof(int) * p->T));
+        GM_HIP(hipMalloc(&p->dwimg, sizeof(int) * p->G));
+        GM_HIP(hipMemcpy(p->dimg, p->img.data(), sizeof(int) * p->T, hipMemcpyHostToDevice));
+        GM_HIP(hipMemcpy(p->dwimg, p->wimg.data(), sizeof(int) * p->G, hipMemcpyHostToDevice));
+    }
+    b->slot1_n = (size_t)b->p1.G * b->p1.K * kMaxRhs;
+    b->slot2_n = (size_t)b->p2.G * b->p2.K * kMaxRhs * 2;
+    b->slot3_n = (size_t)b->p1.G * b->p1.K * kMaxRhs * 4;
+    if (b->dslot) (void)hipFree(b->dslot);
+    b->dslot = nullptr;
+    const size_t n = b->slot1_n + b->slot2_n + b->slot3_n;
+    GM_HIP(hipMalloc(&b->dslot, sizeof(double) * n));
+    GM_HIP(hipMemset(b->dslot, 0, sizeof(double) * n));
+    GM_HIP(hipMemcpy(b->dimg, b->himg.data(), sizeof(BImg) * b->nimg, hipMemcpyHostToDevice));
+    // a captured iteration graph bakes in the partition
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+    return GM_OK;
 }
 
 extern "C" {
@@ -556,7 +692,6 @@ gm_status gm_batch_create(gm_batch** out, int nimages, const int* widths, const 
     b->nsys = nimages * nrhs;
     b->precision = precision;
     size_t Lfloats = 0, L16n = 0, vfloats = 0;
-    int tiles = 0;
     std::vector<size_t> loff(nimages), l16off(nimages), voff(nimages);
     for (int i = 0; i < nimages; ++i) {
         const int W = widths[i], H = heights[i];
@@ -568,14 +703,19 @@ gm_status gm_batch_create(gm_batch** out, int nimages, const int* widths, const 
         const size_t pitch = (size_t)(N + 63) / 64 * 64;
         BImg I{};
         I.W = W;
+        I.H = H;
         I.N = N;
-        I.ntiles = (N + Tile<kBR>::ROWS - 1) / Tile<kBR>::ROWS;
-        I.tile0 = tiles;
         I.sys0 = i * nrhs;
         I.nsys = nrhs;
+        I.ntx = (W + kTX - 1) / kTX;
+        I.nt = I.ntx * ((H + kTY - 1) / kTY);
+        I.t0 = (int)b->p1.img.size();
+        I.nc = (N + kChunk - 1) / kChunk;
+        I.c0 = (int)b->p2.img.size();
+        b->p1.img.insert(b->p1.img.end(), I.nt, i);
+        b->p2.img.insert(b->p2.img.end(), I.nc, i);
         I.pitch = pitch;
         I.pitch16 = pitch;
-        tiles += I.ntiles;
         loff[i] = Lfloats;
         Lfloats += 25 * pitch;
         l16off[i] = L16n;
@@ -584,15 +724,14 @@ gm_status gm_batch_create(gm_batch** out, int nimages, const int* widths, const 
         vfloats += (size_t)nrhs * 8 * pitch;
         b->himg.push_back(I);
     }
-    b->ntiles = tiles;
     hipError_t e = hipSuccess;
-#define BTRY(x)                                                        \
-    do {                                                               \
-        e = (x);                                                       \
-        if (e != hipSuccess) {                                         \
-            batch_free(b);                                             \
-            return hip_fail(e, #x, __FILE__, __LINE__);                \
-        }                                                              \
+#define BTRY(x)                                         \
+    do {                                                \
+        e = (x);                                        \
+        if (e != hipSuccess) {                          \
+            batch_free(b);                              \
+            return hip_fail(e, #x, __FILE__, __LINE__); \
+        }                                               \
     } while (0)
     BTRY(hipMalloc(&b->Lmem, sizeof(float) * Lfloats + 256));
     BTRY(hipMemset(b->Lmem, 0, sizeof(float) * Lfloats + 256));
@@ -618,29 +757,22 @@ gm_status gm_batch_create(gm_batch** out, int nimages, const int* widths, const 
             S.dinv = base + 6 * I.pitch;
             S.b = base + 7 * I.pitch;
             b->hsys.push_back(S);
-            b->hsysN.push_back(I.N);
         }
     }
-    std::vector<int> tile_img(tiles);
-    for (int i = 0; i < nimages; ++i)
-        for (int k = 0; k < b->himg[i].ntiles; ++k) tile_img[b->himg[i].tile0 + k] = i;
     BTRY(hipMalloc(&b->dimg, sizeof(BImg) * nimages));
     BTRY(hipMalloc(&b->dsys, sizeof(BSys) * b->nsys));
-    BTRY(hipMalloc(&b->dsysN, sizeof(int) * b->nsys));
     BTRY(hipMalloc(&b->dsc, sizeof(Scalars) * b->nsys));
     BTRY(hipMemset(b->dsc, 0, sizeof(Scalars) * b->nsys));
-    BTRY(hipMalloc(&b->dtile, sizeof(int) * tiles));
-    const size_t nparts = (size_t)tiles * kMaxRhs * 6;
-    BTRY(hipMalloc(&b->dpart, sizeof(double) * nparts));
-    BTRY(hipMemset(b->dpart, 0, sizeof(double) * nparts));
-    BTRY(hipMemcpy(b->dimg, b->himg.data(), sizeof(BImg) * nimages, hipMemcpyHostToDevice));
     BTRY(hipMemcpy(b->dsys, b->hsys.data(), sizeof(BSys) * b->nsys, hipMemcpyHostToDevice));
-    BTRY(hipMemcpy(b->dsysN, b->hsysN.data(), sizeof(int) * b->nsys, hipMemcpyHostToDevice));
-    BTRY(hipMemcpy(b->dtile, tile_img.data(), sizeof(int) * tiles, hipMemcpyHostToDevice));
     BTRY(hipHostMalloc(&b->hsc, sizeof(Scalars) * b->nsys, hipHostMallocDefault));
     BTRY(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
     BTRY(hipEventCreateWithFlags(&b->ev, hipEventDisableTiming));
 #undef BTRY
+    gm_status st = batch_partition(b);
+    if (st != GM_OK) {
+        batch_free(b);
+        return st;
+    }
     *out = b;
     return GM_OK;
 }
@@ -654,9 +786,8 @@ gm_status gm_batch_assemble(gm_batch* b, int image, const float* im, int nch, si
                             gm_stream_t s) {
     if (!b || image < 0 || image >= b->nimg) return fail(GM_ERR_INVALID, "gm_batch_assemble: bad image index");
     const BImg& I = b->himg[image];
-    const int H = I.N / I.W;
     gm_status st = levin_assemble_raw(const_cast<float*>(I.L), I.pitch, nullptr, nullptr, im, nch, im_pitch, nullptr,
-                                      nullptr, nullptr, 0.f, eps, GM_ASSEMBLY_ACCURATE, I.W, H, s);
+                                      nullptr, nullptr, 0.f, eps, GM_ASSEMBLY_ACCURATE, I.W, I.H, s);
     if (st != GM_OK) return st;
     if (b->precision == GM_PREC_FP16) {
         DevInfo d;
@@ -705,50 +836,53 @@ gm_status gm_batch_info(gm_batch* b, int image, int rhs, gm_solve_info* info) {
     return GM_OK;
 }
 
-static double* part1(gm_batch* b) { return b->dpart; }
-static double* part2(gm_batch* b) { return b->dpart + (size_t)b->ntiles * kMaxRhs; }
-static double* part3(gm_batch* b) { return b->dpart + (size_t)b->ntiles * kMaxRhs * 3; }
+}  // extern "C"
 
-static gm_status batch_launch_iter(gm_batch* b, int grid, int verify, hipStream_t hs) {
+static double* slot1(gm_batch* b) { return b->dslot; }
+static double* slot2(gm_batch* b) { return b->dslot + b->slot1_n; }
+static double* slot3(gm_batch* b) { return b->dslot + b->slot1_n + b->slot2_n; }
+
+static gm_status batch_launch_iter(gm_batch* b, int verify, hipStream_t hs) {
+    const BPart P1 = b->p1.dev(), P2 = b->p2.dev();
     if (b->precision == GM_PREC_FP16)
-        hipLaunchKernelGGL(kb_cgstep<kF16>, dim3(grid), dim3(kBlock), 0, hs, b->dimg, b->dsys, b->dsc, b->dtile,
-                           b->ntiles, part1(b), part2(b), verify);
+        hipLaunchKernelGGL(kb_step<kF16>, dim3(P1.G), dim3(kBlock), 0, hs, b->dimg, b->dsys, b->dsc, P1, P2, slot1(b),
+                           slot2(b), verify);
     else
-        hipLaunchKernelGGL(kb_cgstep<kF32>, dim3(grid), dim3(kBlock), 0, hs, b->dimg, b->dsys, b->dsc, b->dtile,
-                           b->ntiles, part1(b), part2(b), verify);
+        hipLaunchKernelGGL(kb_step<kF32>, dim3(P1.G), dim3(kBlock), 0, hs, b->dimg, b->dsys, b->dsc, P1, P2, slot1(b),
+                           slot2(b), verify);
     GM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(kb_cgupdate, dim3(grid), dim3(kBlock), 0, hs, b->dimg, b->dsys, b->dsc, b->dtile, b->ntiles,
-                       part1(b), part2(b));
+    hipLaunchKernelGGL(kb_update, dim3(P2.G), dim3(kBlock), 0, hs, b->dimg, b->dsys, b->dsc, P1, P2, slot1(b),
+                       slot2(b));
     GM_LAUNCH_CHECK();
     return GM_OK;
 }
 
-static gm_status batch_trueres(gm_batch* b, int grid, int op, hipStream_t hs) {
-    hipLaunchKernelGGL(kb_trueres, dim3(grid), dim3(kBlock), 0, hs, b->dimg, b->dsys, b->dsc, b->dtile, b->ntiles, op,
-                       part3(b));
+static gm_status batch_trueres(gm_batch* b, int op, double tol, hipStream_t hs) {
+    const BPart P1 = b->p1.dev();
+    hipLaunchKernelGGL(kb_trueres, dim3(P1.G), dim3(kBlock), 0, hs, b->dimg, b->dsys, b->dsc, P1, op, slot3(b));
     GM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(kb_trueres_fin, dim3(b->nimg), dim3(kBlock), 0, hs, b->dimg, b->dsc, part3(b), op);
+    hipLaunchKernelGGL(kb_trueres_fin, dim3(b->nimg), dim3(kBlock), 0, hs, b->dimg, b->dsc, P1, slot3(b), op, tol);
     GM_LAUNCH_CHECK();
     return GM_OK;
 }
 
 static gm_status batch_fold(gm_batch* b, int verify, hipStream_t hs) {
-    hipLaunchKernelGGL(kb_fold, dim3(b->nimg), dim3(kBlock), 0, hs, b->dimg, b->dsc, part2(b), verify);
+    hipLaunchKernelGGL(kb_fold, dim3(b->nimg), dim3(kBlock), 0, hs, b->dimg, b->dsc, b->p2.dev(), slot2(b), verify);
     GM_LAUNCH_CHECK();
     return GM_OK;
 }
 
-// Capture `iters` iterations (K1, K2 pairs) into a graph on the batch's own
-// stream (capture needs a non-null stream); replayed on the caller's stream.
-static gm_status batch_graph(gm_batch* b, int iters, int grid, int verify) {
-    if (b->gexec && b->graph_iters == iters && b->graph_grid == grid && b->graph_verify == verify) return GM_OK;
+// Capture `iters` iterations (step, update pairs) into a graph on the batch's
+// own stream (capture needs a non-null stream); replayed on the caller's.
+static gm_status batch_graph(gm_batch* b, int iters, int verify) {
+    if (b->gexec && b->graph_iters == iters && b->graph_verify == verify) return GM_OK;
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     b->gexec = nullptr;
     b->graph = nullptr;
     GM_HIP(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
     gm_status st = GM_OK;
-    for (int k = 0; k < iters && st == GM_OK; ++k) st = batch_launch_iter(b, grid, verify, b->cap);
+    for (int k = 0; k < iters && st == GM_OK; ++k) st = batch_launch_iter(b, verify, b->cap);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(b->cap, &g);
     if (st != GM_OK) {
@@ -759,10 +893,11 @@ static gm_status batch_graph(gm_batch* b, int iters, int grid, int verify) {
     b->graph = g;
     GM_HIP(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
     b->graph_iters = iters;
-    b->graph_grid = grid;
     b->graph_verify = verify;
     return GM_OK;
 }
+
+extern "C" {
 
 gm_status gm_batch_solve(gm_batch* b, const gm_solve_opts* opts, gm_stream_t s) {
     if (!b) return fail(GM_ERR_INVALID, "gm_batch_solve: null batch");
@@ -771,23 +906,18 @@ gm_status gm_batch_solve(gm_batch* b, const gm_solve_opts* opts, gm_stream_t s) 
     else gm_solve_opts_default(&o);
     if (o.method != GM_METHOD_PCG) return fail(GM_ERR_UNSUPPORTED, "gm_batch_solve: batched solver is Jacobi-PCG");
     if (o.max_iter <= 0) return fail(GM_ERR_INVALID, "gm_batch_solve: max_iter");
-    DevInfo d;
-    gm_status st = dev_info(&d);
-    if (st != GM_OK) return st;
+    gm_status st = GM_OK;
     hipStream_t hs = (hipStream_t)s;
-    const int grid = grid_for_tiles(d, b->ntiles, b->per_cu);
     const int verify = (o.tol > 0 && o.verify) ? 1 : 0;
     const int chunk = o.check_every > 0 ? o.check_every : 10;
     const bool mixed = b->precision == GM_PREC_FP16;
     int refine = o.refine_every > 0 ? o.refine_every : (o.refine_every == 0 && mixed ? 50 : 0);
     if (refine > 0 && refine % chunk) refine = (refine / chunk + 1) * chunk;  // replacement at chunk boundaries
 
-    hipLaunchKernelGGL(kb_init, dim3((b->nsys + 63) / 64), dim3(64), 0, hs, b->dsc, b->nsys, o.max_iter, o.tol);
+    hipLaunchKernelGGL(kb_init, dim3((b->nsys + 63) / 64), dim3(64), 0, hs, b->dsc, b->nsys, o.max_iter);
     GM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(kb_bnorm, dim3(b->nsys), dim3(kBlock), 0, hs, b->dsys, b->dsysN, b->dsc, o.tol);
-    GM_LAUNCH_CHECK();
-    if ((st = batch_trueres(b, grid, 2, hs)) != GM_OK) return st;
-    if ((st = batch_graph(b, chunk, grid, verify)) != GM_OK) return st;
+    if ((st = batch_trueres(b, 2, o.tol, hs)) != GM_OK) return st;  // r = A x - b; ||b||^2 -> tol2
+    if ((st = batch_graph(b, chunk, verify)) != GM_OK) return st;
 
     long enq = 0;
     bool finished = false;
@@ -795,7 +925,7 @@ gm_status gm_batch_solve(gm_batch* b, const gm_solve_opts* opts, gm_stream_t s) 
         GM_HIP(hipGraphLaunch(b->gexec, hs));
         enq += chunk;
         if (refine > 0 && enq % refine == 0) {
-            if ((st = batch_trueres(b, grid, 1, hs)) != GM_OK) return st;
+            if ((st = batch_trueres(b, 1, o.tol, hs)) != GM_OK) return st;
         }
         GM_HIP(hipMemcpyAsync(b->hsc, b->dsc, sizeof(Scalars) * b->nsys, hipMemcpyDeviceToHost, hs));
         GM_HIP(hipEventRecord(b->ev, hs));
@@ -809,14 +939,14 @@ gm_status gm_batch_solve(gm_batch* b, const gm_solve_opts* opts, gm_stream_t s) 
         }
         if (running == 0 || enq >= o.max_iter) finished = true;
         if (verifying > 0 && !finished) {
-            if ((st = batch_trueres(b, grid, 3, hs)) != GM_OK) return st;
+            if ((st = batch_trueres(b, 3, o.tol, hs)) != GM_OK) return st;
         }
     }
     // fold the last update's partials, check any recursive convergence it
     // shows, then the FP64 true residual of every system (g0) and a host copy
     if ((st = batch_fold(b, verify, hs)) != GM_OK) return st;
-    if ((st = batch_trueres(b, grid, 3, hs)) != GM_OK) return st;
-    if ((st = batch_trueres(b, grid, 0, hs)) != GM_OK) return st;
+    if ((st = batch_trueres(b, 3, o.tol, hs)) != GM_OK) return st;
+    if ((st = batch_trueres(b, 0, o.tol, hs)) != GM_OK) return st;
     GM_HIP(hipMemcpyAsync(b->hsc, b->dsc, sizeof(Scalars) * b->nsys, hipMemcpyDeviceToHost, hs));
     GM_HIP(hipStreamSynchronize(hs));
     int bad = 0;
@@ -830,9 +960,9 @@ gm_status gm_batch_solve(gm_batch* b, const gm_solve_opts* opts, gm_stream_t s) 
 gm_status gm_batch_set_grid(gm_batch* b, int per_cu) {
     if (!b || per_cu < 1 || per_cu > 16) return fail(GM_ERR_INVALID, "gm_batch_set_grid: per_cu 1..16");
     b->per_cu = per_cu;
-    return GM_OK;
+    return batch_partition(b);
 }
 
-int gm_batch_tiles(gm_batch* b) { return b ? b->ntiles : 0; }
+int gm_batch_tiles(gm_batch* b) { return b ? b->p1.T : 0; }
 
 }  // extern "C"

@@ -85,11 +85,19 @@ int elementwise_grid(const DevInfo& d, long n, int per_thread);
 
 // ---------------------------------------------------------------------------
 // device scalar block driving the iterative solvers
-// Periodic residual replacement keeps the search direction only while the
-// recursive residual agrees with the true one: ||r_true - r_rec||^2 <=
-// kDrift2 * ||r_true||^2 (10 %); past that the recursion (run with an operator
-// rounded to 16 bits, or just drifted) restarts with beta = 0.
+// Periodic residual replacement (r := the FP64 true residual) keeps the
+// search direction unless the recursion has drifted: it restarts (beta = 0)
+// when ||r_true - r_rec|| > 50 % of ||r_true||, or > 10 % while the true
+// residual norm has not halved since the previous replacement.  (A recursion
+// run with an operator rounded to 16 bits drifts by 10-30 % per 50 iterations
+// and still converges fastest with its direction kept while it makes
+// progress; keeping it through poor progress has diverged.)
 constexpr double kDrift2 = 0.01;
+constexpr double kDriftMax2 = 0.25;
+constexpr double kProgress2 = 0.25;
+
+// Scalars::done codes
+enum { kRun = 0, kConverged = 1, kBreakdown = 2, kNonFinite = 3, kMaxIt = 4 };
 
 // Hand-off from the CG step kernel to the update kernel (see Scalars).
 struct Hand {
@@ -118,7 +126,8 @@ struct Scalars {
     double pAp;      // <p, A p>
     double bb;       // <b, b>
     double tol2;     // stop when rr <= tol2 (tol^2 * bb)
-    double g0, g1;   // scratch scalars (gradient descent)
+    double g0, g1;   // scratch scalars (gradient descent; g0: last true ||r||^2, g1: at the last verification)
+    double rr_rep;   // true ||r||^2 at the start or the last replacement
     float alpha;     // step length
     float beta;      // direction update factor
     int done;        // 0 running, 1 converged, 2 breakdown, 3 non-finite, 4 max_iter
@@ -229,6 +238,45 @@ __device__ __forceinline__ void publish(Scalars* sc, const Hand& h, int done, in
     sc->verify = verify;
     sc->pend = pend;
     sc->npart = npart;
+}
+
+// Replace the residual by the true one (<r,z> = rz, unscaled ||r||^2 = rr;
+// rsq = ||r||^2 in the space r is kept in, drift2 = ||r_true - r_rec||^2 in
+// that space, periodic replacements only).  start: a fresh start (beta = 0,
+// not counted).  Then the convergence / stagnation test.
+__device__ __forceinline__ void replace_residual(Scalars* s, double rz, double rr, bool start, double drift2,
+                                                 double rsq) {
+    if (!isfinite(rz) || !isfinite(rr)) {
+        s->done = kNonFinite;
+        return;
+    }
+    if (start) {
+        s->beta = 0.f;
+    } else if (s->verify) {
+        // the recursive residual reached the tolerance but the true one did
+        // not: restart from the true residual rather than keep a direction
+        // built on the drifted one
+        s->beta = 0.f;
+        s->replacements += 1;
+        if (s->g1 > 0 && rr > 0.25 * s->g1) s->stall += 1;
+        else s->stall = 0;
+        s->g1 = rr;
+    } else {
+        const bool drifted = drift2 > kDriftMax2 * rsq || (drift2 > kDrift2 * rsq && rr > kProgress2 * s->rr_rep);
+        // the <r,z> the last step used is rz while its update's partials are
+        // pending (superseded by this residual), else rz_prev
+        const double rzp = s->pend ? s->rz : s->rz_prev;
+        s->beta = drifted ? 0.f : (rzp > 0 ? (float)(rz / rzp) : 0.f);
+        s->replacements += 1;
+    }
+    s->rr_rep = rr;
+    s->rz_prev = s->pend ? s->rz : s->rz_prev;
+    s->rz = rz;
+    s->rr = rr;
+    s->verify = 0;
+    s->pend = 0;
+    if (s->tol2 >= 0 && rr <= s->tol2) s->done = kConverged;
+    else if (s->iters >= s->maxit || s->stall >= 3) s->done = kMaxIt;  // max_iter or stagnation
 }
 
 // Deferred partial sums: part[q * kMaxGrid + i], i < n.  Every thread of the

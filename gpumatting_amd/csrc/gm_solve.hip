@@ -34,50 +34,14 @@ bool tiled_ok(const BandPlan& p, const void* a, size_t pitch);
 
 enum PMode { kPlain = 0, kJacobi = 1, kScaled = 2 };
 
-// done codes
-enum { kRun = 0, kConverged = 1, kBreakdown = 2, kNonFinite = 3, kMaxIt = 4 };
 
 __device__ __forceinline__ bool stopped(const Scalars* sc) { return sc->done != kRun || sc->verify != 0; }
 
 // Finaliser shared by the residual-producing kernels: record <r,z>, ||r||^2
-// and test convergence.  start: beta = 0 (next step is p = -z).
-// drift2 = ||r_true - r_rec||^2 and rsq = ||r_true||^2 in the space r is kept in
-// (periodic replacements only).
+// and test convergence.  start / !replace: beta = 0 (the next step is p = -z).
 __device__ __forceinline__ void fin_residual(Scalars* sc, double rz, double rr, bool start, bool replace,
                                              double drift2 = 0.0, double rsq = 0.0) {
-    if (!isfinite(rz) || !isfinite(rr)) {
-        sc->done = kNonFinite;
-        return;
-    }
-    if (replace && !start) {
-        sc->replacements += 1;
-        if (sc->verify) {
-            // The recursive residual reached the tolerance but the true one did
-            // not: the recursion has drifted, so restart from the true residual
-            // (beta = 0) rather than keep a direction built on the drifted one.
-            sc->beta = 0.f;
-            if (sc->g1 > 0 && rr > 0.25 * sc->g1) sc->stall += 1;
-            else sc->stall = 0;
-            sc->g1 = rr;
-        } else if (drift2 > kDrift2 * rsq) {
-            sc->beta = 0.f;  // periodic replacement, recursion drifted: restart
-        } else {
-            // periodic replacement: keep the search direction.  The <r,z> the
-            // last step used is rz while its update's partials are pending
-            // (they are superseded by this residual), else rz_prev.
-            const double rzp = sc->pend ? sc->rz : sc->rz_prev;
-            sc->beta = rzp > 0 ? (float)(rz / rzp) : 0.f;
-        }
-    } else {
-        sc->beta = 0.f;
-    }
-    sc->rz_prev = sc->pend ? sc->rz : sc->rz_prev;
-    sc->rz = rz;
-    sc->rr = rr;
-    sc->verify = 0;
-    sc->pend = 0;
-    if (sc->tol2 >= 0 && rr <= sc->tol2) sc->done = kConverged;
-    else if (sc->iters >= sc->maxit || sc->stall >= 3) sc->done = kMaxIt;  // max_iter or stagnation
+    replace_residual(sc, rz, rr, start || !replace, drift2, rsq);
 }
 
 // ---------------------------------------------------------------------------
@@ -267,7 +231,7 @@ __device__ __forceinline__ float h2f(unsigned int h) {
 // equals the full 24-band one wherever As is symmetric.  16-bit bands move
 // 24 B per row instead of 48.
 template <int TY, int FMT>
-__global__ __launch_bounds__(kBlock) void k_cgstep_sym(const unsigned short* __restrict__ U, size_t pitch, int W,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_cgstep_sym(const unsigned short* __restrict__ U, size_t pitch, int W,
                                                        int H, int N, const float* __restrict__ r,
                                                        float* __restrict__ p0, float* __restrict__ p1,
                                                        float* __restrict__ Ap, Scalars* sc, double* __restrict__ dpa,
@@ -286,26 +250,26 @@ __global__ __launch_bounds__(kBlock) void k_cgstep_sym(const unsigned short* __r
     __shared__ __attribute__((aligned(16))) unsigned int ub[12][UR][SW];
     const TileRange tr = xcd_tiles(G::ntiles(W, H, N));
     const int tx = (int)threadIdx.x % TPR, ty = (int)threadIdx.x / TPR;
+    constexpr int NE = (TY + 4) * ST;  // x window cells
+    constexpr int XE = (NE + kBlock - 1) / kBlock;
+    constexpr int NH = 2 * SW + 2 * TY;  // U halo words: all of window rows 0, 1; words 0 and SW - 1 below
+    constexpr int HE = (NH + kBlock - 1) / kBlock;
     double v[1] = {0.0};
     for (int t = tr.begin; t < tr.end; t += tr.step) {
         G g;
         g.at(t, W, H, N);
+        // every global load of the tile first (one memory round trip), then LDS
         Raw<R, FMT> u[12];
         if (g.valid) {
 #pragma unroll
             for (int m = 0; m < 12; ++m) load_band<R, FMT>(U, (size_t)(13 + m) * pitch + (size_t)g.ib, u[m]);
         }
-        __syncthreads();  // the previous tile is done with lds and ub
-        // own cells: window row ty + 2, halves 2 + 4 tx .. 5 + 4 tx (words 1 + 2 tx, 2 + 2 tx)
+        unsigned int hv[HE][12];
+        int hw[HE];
 #pragma unroll
-        for (int m = 0; m < 12; ++m) {
-            ub[m][ty + 2][1 + 2 * tx] = g.valid ? u[m].d[0] : 0u;
-            ub[m][ty + 2][2 + 2 * tx] = g.valid ? u[m].d[1] : 0u;
-        }
-        // halo: every word of window rows 0 and 1, and words 0 and SW - 1 of the others
-        constexpr int NH = 2 * SW + 2 * TY;
-        for (int e = threadIdx.x; e < NH; e += kBlock) {
-            int wy, ww;
+        for (int q = 0; q < HE; ++q) {
+            const int e = (int)threadIdx.x + q * kBlock;
+            int wy = 0, ww = 0;
             if (e < 2 * SW) {
                 wy = e / SW;
                 ww = e - wy * SW;
@@ -314,24 +278,56 @@ __global__ __launch_bounds__(kBlock) void k_cgstep_sym(const unsigned short* __r
                 wy = 2 + (f >> 1);
                 ww = (f & 1) ? SW - 1 : 0;
             }
+            hw[q] = wy * SW + ww;
             const long long gi = (long long)(g.y0 - 2 + wy) * W + (g.x0 - 2 + 2 * ww);  // even: W % 4 == 0
-            const bool in = gi >= 0 && gi < N;
+            const bool in = e < NH && gi >= 0 && gi < N;
 #pragma unroll
             for (int m = 0; m < 12; ++m)
-                ub[m][wy][ww] =
-                    in ? *reinterpret_cast<const unsigned int*>(U + (size_t)(13 + m) * pitch + (size_t)gi) : 0u;
+                hv[q][m] = in ? *reinterpret_cast<const unsigned int*>(U + (size_t)(13 + m) * pitch + (size_t)gi) : 0u;
         }
-        if (beta != 0.f)
-            g.stage(lds, W, H, N, [&](int i) { return fmaf(beta, pold[i], -r[i]); });
-        else
-            g.stage(lds, W, H, N, [&](int i) { return -r[i]; });
+        float xv[XE];
+#pragma unroll
+        for (int q = 0; q < XE; ++q) {
+            const int e = (int)threadIdx.x + q * kBlock;
+            const int ly = e / ST, lx = e - ly * ST;
+            // matrix-row index of the window cell (as Geom2D::stage)
+            const long long gi = (long long)(g.y0 - 2 + ly) * W + (g.x0 - 2 + lx);
+            float val = 0.f;
+            if (e < NE && gi >= 0 && gi < N) val = beta != 0.f ? fmaf(beta, pold[gi], -r[gi]) : -r[gi];
+            xv[q] = val;
+        }
+        __syncthreads();  // the previous tile is done with lds and ub
+        // own cells: window row ty + 2, halves 2 + 4 tx .. 5 + 4 tx (words 1 + 2 tx, 2 + 2 tx)
+#pragma unroll
+        for (int m = 0; m < 12; ++m) {
+            ub[m][ty + 2][1 + 2 * tx] = g.valid ? u[m].d[0] : 0u;
+            ub[m][ty + 2][2 + 2 * tx] = g.valid ? u[m].d[1] : 0u;
+        }
+#pragma unroll
+        for (int q = 0; q < HE; ++q)
+            if ((int)threadIdx.x + q * kBlock < NH) {
+#pragma unroll
+                for (int m = 0; m < 12; ++m) (&ub[m][0][0])[hw[q]] = hv[q][m];
+            }
+#pragma unroll
+        for (int q = 0; q < XE; ++q)
+            if ((int)threadIdx.x + q * kBlock < NE) lds[threadIdx.x + q * kBlock] = xv[q];
         __syncthreads();
         if (g.valid) {
+            // the thread's x window: rows dy = -2..2, columns 4 tx .. 4 tx + 7 (16-byte aligned)
             const float* __restrict__ xs = g.xs(lds);
+            float xw[5][8];
+#pragma unroll
+            for (int q = 0; q < 5; ++q) {
+                const f32x4 a0 = *reinterpret_cast<const f32x4*>(xs + q * ST);
+                const f32x4 a1 = *reinterpret_cast<const f32x4*>(xs + q * ST + 4);
+                xw[q][0] = a0.x; xw[q][1] = a0.y; xw[q][2] = a0.z; xw[q][3] = a0.w;
+                xw[q][4] = a1.x; xw[q][5] = a1.y; xw[q][6] = a1.z; xw[q][7] = a1.w;
+            }
             float y[R], pc[R];
 #pragma unroll
             for (int j = 0; j < R; ++j) {
-                pc[j] = xs[2 * ST + 2 + j];
+                pc[j] = xw[2][2 + j];
                 y[j] = pc[j];  // unit diagonal
             }
 #pragma unroll
@@ -339,8 +335,8 @@ __global__ __launch_bounds__(kBlock) void k_cgstep_sym(const unsigned short* __r
                 const int k = 13 + m, dy = k / 5 - 2, dx = k % 5 - 2;
                 // upper entry of this row: x at (+dy, +dx)
 #pragma unroll
-                for (int j = 0; j < R; ++j) y[j] = fmaf(u[m].get(j), xs[(2 + dy) * ST + 2 + dx + j], y[j]);
-                // lower entry: U_k of the row at (-dy, -dx), x there; halves 4 tx .. 4 tx + 7 of window row ty + 2 - dy
+                for (int j = 0; j < R; ++j) y[j] = fmaf(u[m].get(j), xw[2 + dy][2 + dx + j], y[j]);
+                // lower entry: U_k of the row at (-dy, -dx); halves 4 tx .. 4 tx + 7 of window row ty + 2 - dy
                 const unsigned int* wr = &ub[m][ty + 2 - dy][2 * tx];
                 const uint2 w01 = *reinterpret_cast<const uint2*>(wr);
                 const uint2 w23 = *reinterpret_cast<const uint2*>(wr + 2);
@@ -349,7 +345,7 @@ __global__ __launch_bounds__(kBlock) void k_cgstep_sym(const unsigned short* __r
                 for (int j = 0; j < R; ++j) {
                     const int q = 2 + j - dx;  // half index 0..7
                     const float c = h2f<FMT>(w[q >> 1] >> (16 * (q & 1)));
-                    y[j] = fmaf(c, xs[(2 - dy) * ST + 2 - dx + j], y[j]);
+                    y[j] = fmaf(c, xw[2 - dy][2 - dx + j], y[j]);
                 }
             }
             store_rows<R>(pnew, g.ib, N, pc);
