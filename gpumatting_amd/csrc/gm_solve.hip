@@ -52,6 +52,7 @@ __global__ void k_init_scalars(Scalars* sc, int maxit) {
         Scalars z = {};
         z.maxit = maxit;
         z.tol2 = -1.0;
+        z.g0 = -1.0;  // no true residual measured yet
         z.counter = sc->counter;  // the reduction counter lives in the scratch block; keep it
         z.hand.mode = kHandIdle;
         *sc = z;
@@ -93,12 +94,23 @@ __global__ __launch_bounds__(kBlock) void k_resid25(const float* __restrict__ A,
     for (int t = tr.begin; t < tr.end; t += tr.step) {
         const int i0 = t * Tile<R>::ROWS;
         __syncthreads();
-        stage5<R>(lds, i0, W, N, [&](int g) { return x[g]; });
-        __syncthreads();
+        int nz = 0;
+        stage5<R>(lds, i0, W, N, [&](int g) {
+            const float v = x[g];
+            nz |= v != 0.f;
+            return v;
+        });
+        // a window of zeros (a zero initial guess) needs no band loads: A x = 0
+        nz = __syncthreads_or(nz);
         const int ib = i0 + R * threadIdx.x;
         if (ib < N) {
             float y[R], bb[R], dd[R];
-            tile_rows<R, kF32, kDiagStored>(A, pitch, ib, lds + R * threadIdx.x, y);
+            if (nz) {
+                tile_rows<R, kF32, kDiagStored>(A, pitch, ib, lds + R * threadIdx.x, y);
+            } else {
+#pragma unroll
+                for (int j = 0; j < R; ++j) y[j] = 0.f;
+            }
             load_rows<R>(b, ib, N, bb);
             if (PM == kJacobi) load_rows<R>(dinv, ib, N, dd);
 #pragma unroll
@@ -113,7 +125,10 @@ __global__ __launch_bounds__(kBlock) void k_resid25(const float* __restrict__ A,
             store_rows<R>(r, ib, N, y);
         }
     }
-    grid_reduce<2>(v, partials, counter, [&](double* tt) { fin_residual(sc, tt[0], tt[1], true, false); });
+    grid_reduce<2>(v, partials, counter, [&](double* tt) {
+        sc->g0 = -1.0;  // an FP32 residual: no FP64 true residual of this x yet
+        fin_residual(sc, tt[0], tt[1], true, false);
+    });
 }
 
 // Start of the CG step kernel: fold the update kernel's pending partial sums
@@ -490,21 +505,29 @@ __global__ __launch_bounds__(kBlock) void k_trueres25(const float* __restrict__ 
     for (int t = tr.begin; t < tr.end; t += tr.step) {
         const int i0 = t * Tile<R>::ROWS;
         __syncthreads();
-        stage5<R>(lds, i0, W, N, [&](int g) { return x[g]; });
-        __syncthreads();
+        int nz = 0;
+        stage5<R>(lds, i0, W, N, [&](int g) {
+            const float v = x[g];
+            nz |= v != 0.f;
+            return v;
+        });
+        // a window of zeros (a zero initial guess) needs no band loads: A x = 0
+        nz = __syncthreads_or(nz);
         const int ib = i0 + R * threadIdx.x;
         if (ib < N) {
             const float* xs = lds + R * threadIdx.x;
             double acc[R];
 #pragma unroll
             for (int j = 0; j < R; ++j) acc[j] = 0.0;
+            if (nz) {
 #pragma unroll
-            for (int k = 0; k < 25; ++k) {
-                Raw<R, kF32> a;
-                load_band<R, kF32>(A, (size_t)k * pitch + (size_t)ib, a);
-                const int s = k / 5, dx = k % 5;
+                for (int k = 0; k < 25; ++k) {
+                    Raw<R, kF32> a;
+                    load_band<R, kF32>(A, (size_t)k * pitch + (size_t)ib, a);
+                    const int s = k / 5, dx = k % 5;
 #pragma unroll
-                for (int j = 0; j < R; ++j) acc[j] = fma((double)a.get(j), (double)xs[s * SEG + j + dx], acc[j]);
+                    for (int j = 0; j < R; ++j) acc[j] = fma((double)a.get(j), (double)xs[s * SEG + j + dx], acc[j]);
+                }
             }
             float bb[R], dd[R], out[R], ro[R];
             load_rows<R>(b, ib, N, bb);
@@ -695,7 +718,10 @@ __global__ __launch_bounds__(kBlock) void k_resid_g(const float* __restrict__ a,
         v[0] += (double)(s * z);
         v[1] += (double)(s * s);
     }
-    grid_reduce<2>(v, partials, counter, [&](double* tt) { fin_residual(sc, tt[0], tt[1], true, false); });
+    grid_reduce<2>(v, partials, counter, [&](double* tt) {
+        sc->g0 = -1.0;  // an FP32 residual: no FP64 true residual of this x yet
+        fin_residual(sc, tt[0], tt[1], true, false);
+    });
 }
 
 template <int PM>
@@ -1465,10 +1491,14 @@ gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, c
         info->true_relres = -1;
     }
     if (o->verify && info && o->method != GM_METHOD_GD) {
-        // FP64-accumulated true residual of the returned x
-        if ((st = launch_trueres(S, 0)) != GM_OK) return st;
-        GM_HIP(hipStreamSynchronize(S.hs));
-        GM_HIP(hipMemcpy(&fin, w->sc, sizeof(Scalars), hipMemcpyDeviceToHost));
+        // FP64-accumulated true residual of the returned x.  With verification
+        // on, convergence is only ever declared by a true-residual pass, after
+        // which x is not updated again: its g0 is already that of the final x.
+        if (!(S.verify && fin.done == kConverged && fin.g0 >= 0)) {
+            if ((st = launch_trueres(S, 0)) != GM_OK) return st;
+            GM_HIP(hipStreamSynchronize(S.hs));
+            GM_HIP(hipMemcpy(&fin, w->sc, sizeof(Scalars), hipMemcpyDeviceToHost));
+        }
         info->true_relres = sqrt(fin.g0 / bb);
     }
     if (result != GM_OK && result != GM_ERR_NOT_CONVERGED) {
