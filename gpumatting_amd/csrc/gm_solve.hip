@@ -246,7 +246,7 @@ __device__ __forceinline__ float h2f(unsigned int h) {
 // equals the full 24-band one wherever As is symmetric.  16-bit bands move
 // 24 B per row instead of 48.
 template <int TY, int FMT>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_cgstep_sym(const unsigned short* __restrict__ U, size_t pitch, int W,
+__global__ __launch_bounds__(kBlock) void k_cgstep_sym(const unsigned short* __restrict__ U, size_t pitch, int W,
                                                        int H, int N, const float* __restrict__ r,
                                                        float* __restrict__ p0, float* __restrict__ p1,
                                                        float* __restrict__ Ap, Scalars* sc, double* __restrict__ dpa,
@@ -269,22 +269,26 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     constexpr int XE = (NE + kBlock - 1) / kBlock;
     constexpr int NH = 2 * SW + 2 * TY;  // U halo words: all of window rows 0, 1; words 0 and SW - 1 below
     constexpr int HE = (NH + kBlock - 1) / kBlock;
-    double v[1] = {0.0};
-    for (int t = tr.begin; t < tr.end; t += tr.step) {
+    // Software pipeline: the global loads of tile t + step are issued right
+    // after tile t's are moved to LDS, so they are in flight while tile t is
+    // computed from LDS.  One register set holds a tile's loads.
+    struct Loads {
+        Raw<R, FMT> u[12];  // own rows' upper bands (0 outside the image)
+        unsigned int hv[HE][12];
+        float xp[XE], xr[XE];  // p_old and r of the x window cells (0 outside)
+    };
+    auto issue = [&](int t, Loads& L) {
         G g;
         g.at(t, W, H, N);
-        // every global load of the tile first (one memory round trip), then LDS
-        Raw<R, FMT> u[12];
-        if (g.valid) {
 #pragma unroll
-            for (int m = 0; m < 12; ++m) load_band<R, FMT>(U, (size_t)(13 + m) * pitch + (size_t)g.ib, u[m]);
+        for (int m = 0; m < 12; ++m) {
+            if (g.valid) load_band<R, FMT>(U, (size_t)(13 + m) * pitch + (size_t)g.ib, L.u[m]);
+            else L.u[m].d[0] = L.u[m].d[1] = 0u;
         }
-        unsigned int hv[HE][12];
-        int hw[HE];
 #pragma unroll
         for (int q = 0; q < HE; ++q) {
             const int e = (int)threadIdx.x + q * kBlock;
-            int wy = 0, ww = 0;
+            int wy, ww;
             if (e < 2 * SW) {
                 wy = e / SW;
                 ww = e - wy * SW;
@@ -293,41 +297,60 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 wy = 2 + (f >> 1);
                 ww = (f & 1) ? SW - 1 : 0;
             }
-            hw[q] = wy * SW + ww;
             const long long gi = (long long)(g.y0 - 2 + wy) * W + (g.x0 - 2 + 2 * ww);  // even: W % 4 == 0
             const bool in = e < NH && gi >= 0 && gi < N;
 #pragma unroll
             for (int m = 0; m < 12; ++m)
-                hv[q][m] = in ? *reinterpret_cast<const unsigned int*>(U + (size_t)(13 + m) * pitch + (size_t)gi) : 0u;
+                L.hv[q][m] = in ? *reinterpret_cast<const unsigned int*>(U + (size_t)(13 + m) * pitch + (size_t)gi) : 0u;
         }
-        float xv[XE];
 #pragma unroll
         for (int q = 0; q < XE; ++q) {
             const int e = (int)threadIdx.x + q * kBlock;
             const int ly = e / ST, lx = e - ly * ST;
             // matrix-row index of the window cell (as Geom2D::stage)
             const long long gi = (long long)(g.y0 - 2 + ly) * W + (g.x0 - 2 + lx);
-            float val = 0.f;
-            if (e < NE && gi >= 0 && gi < N) val = beta != 0.f ? fmaf(beta, pold[gi], -r[gi]) : -r[gi];
-            xv[q] = val;
+            const bool in = e < NE && gi >= 0 && gi < N;
+            L.xp[q] = in ? pold[gi] : 0.f;
+            L.xr[q] = in ? r[gi] : 0.f;
         }
+    };
+    Loads L;
+    if (tr.begin < tr.end) issue(tr.begin, L);
+    double v[1] = {0.0};
+    for (int t = tr.begin; t < tr.end; t += tr.step) {
+        G g;
+        g.at(t, W, H, N);
         __syncthreads();  // the previous tile is done with lds and ub
         // own cells: window row ty + 2, halves 2 + 4 tx .. 5 + 4 tx (words 1 + 2 tx, 2 + 2 tx)
 #pragma unroll
         for (int m = 0; m < 12; ++m) {
-            ub[m][ty + 2][1 + 2 * tx] = g.valid ? u[m].d[0] : 0u;
-            ub[m][ty + 2][2 + 2 * tx] = g.valid ? u[m].d[1] : 0u;
+            ub[m][ty + 2][1 + 2 * tx] = L.u[m].d[0];
+            ub[m][ty + 2][2 + 2 * tx] = L.u[m].d[1];
         }
 #pragma unroll
-        for (int q = 0; q < HE; ++q)
-            if ((int)threadIdx.x + q * kBlock < NH) {
+        for (int q = 0; q < HE; ++q) {
+            const int e = (int)threadIdx.x + q * kBlock;
+            if (e < NH) {
+                int wy, ww;
+                if (e < 2 * SW) {
+                    wy = e / SW;
+                    ww = e - wy * SW;
+                } else {
+                    const int f = e - 2 * SW;
+                    wy = 2 + (f >> 1);
+                    ww = (f & 1) ? SW - 1 : 0;
+                }
 #pragma unroll
-                for (int m = 0; m < 12; ++m) (&ub[m][0][0])[hw[q]] = hv[q][m];
+                for (int m = 0; m < 12; ++m) ub[m][wy][ww] = L.hv[q][m];
             }
+        }
 #pragma unroll
-        for (int q = 0; q < XE; ++q)
-            if ((int)threadIdx.x + q * kBlock < NE) lds[threadIdx.x + q * kBlock] = xv[q];
+        for (int q = 0; q < XE; ++q) {
+            const int e = (int)threadIdx.x + q * kBlock;
+            if (e < NE) lds[e] = beta != 0.f ? fmaf(beta, L.xp[q], -L.xr[q]) : -L.xr[q];
+        }
         __syncthreads();
+        if (t + tr.step < tr.end) issue(t + tr.step, L);
         if (g.valid) {
             // the thread's x window: rows dy = -2..2, columns 4 tx .. 4 tx + 7 (16-byte aligned)
             const float* __restrict__ xs = g.xs(lds);
@@ -348,19 +371,26 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 #pragma unroll
             for (int m = 0; m < 12; ++m) {
                 const int k = 13 + m, dy = k / 5 - 2, dx = k % 5 - 2;
-                // upper entry of this row: x at (+dy, +dx)
-#pragma unroll
-                for (int j = 0; j < R; ++j) y[j] = fmaf(u[m].get(j), xw[2 + dy][2 + dx + j], y[j]);
-                // lower entry: U_k of the row at (-dy, -dx); halves 4 tx .. 4 tx + 7 of window row ty + 2 - dy
-                const unsigned int* wr = &ub[m][ty + 2 - dy][2 * tx];
-                const uint2 w01 = *reinterpret_cast<const uint2*>(wr);
-                const uint2 w23 = *reinterpret_cast<const uint2*>(wr + 2);
-                const unsigned int w[4] = {w01.x, w01.y, w23.x, w23.y};
+                // halves 4 tx .. 4 tx + 7 of window rows ty + 2 (own) and ty + 2 - dy
+                const unsigned int* wo = &ub[m][ty + 2][2 * tx];
+                const uint2 o01 = *reinterpret_cast<const uint2*>(wo);
+                const uint2 o23 = *reinterpret_cast<const uint2*>(wo + 2);
+                const unsigned int wn[4] = {o01.x, o01.y, o23.x, o23.y};
+                unsigned int wl[4] = {o01.x, o01.y, o23.x, o23.y};
+                if (dy != 0) {
+                    const unsigned int* wr = &ub[m][ty + 2 - dy][2 * tx];
+                    const uint2 w01 = *reinterpret_cast<const uint2*>(wr);
+                    const uint2 w23 = *reinterpret_cast<const uint2*>(wr + 2);
+                    wl[0] = w01.x; wl[1] = w01.y; wl[2] = w23.x; wl[3] = w23.y;
+                }
 #pragma unroll
                 for (int j = 0; j < R; ++j) {
-                    const int q = 2 + j - dx;  // half index 0..7
-                    const float c = h2f<FMT>(w[q >> 1] >> (16 * (q & 1)));
-                    y[j] = fmaf(c, xw[2 - dy][2 - dx + j], y[j]);
+                    // upper entry of this row (half 2 + j): x at (+dy, +dx)
+                    const int qo = 2 + j;
+                    y[j] = fmaf(h2f<FMT>(wn[qo >> 1] >> (16 * (qo & 1))), xw[2 + dy][2 + dx + j], y[j]);
+                    // lower entry: U_k of the row at (-dy, -dx) (half 2 + j - dx): x there
+                    const int ql = 2 + j - dx;
+                    y[j] = fmaf(h2f<FMT>(wl[ql >> 1] >> (16 * (ql & 1))), xw[2 - dy][2 - dx + j], y[j]);
                 }
             }
             store_rows<R>(pnew, g.ib, N, pc);
