@@ -38,24 +38,71 @@ __global__ __launch_bounds__(kBlock) void k_spmv(float* __restrict__ out, const 
     constexpr int R = G::R;
     __shared__ __attribute__((aligned(16))) float lds[G::LDS_FLOATS];
     const TileRange tr = xcd_tiles(G::ntiles(W, H, N));
-    for (int t = tr.begin; t < tr.end; t += tr.step) {
-        G g;
-        g.at(t, W, H, N);
-        Bands<R, FMT> av;
-        if (g.valid) load_bands<R, FMT, kDiagStored>(A, pitch, g.ib, av);
-        __syncthreads();
-        g.stage(lds, W, H, N, [&](int i) { return x[i]; });
-        __syncthreads();
-        if (g.valid) {
-            float y[R];
-            apply_bands<R, FMT, kDiagStored, G::STRIDE>(av, g.xs(lds), y);
-            if (MODE != 0) {
-                float zz[R];
-                load_rows<R>(z, g.ib, N, zz);
+    if constexpr (G::kIs2D) {
+        // Software pipeline: tile t + step's bands and x window are loaded
+        // into registers while tile t is computed.
+        constexpr int NE = G::LDS_FLOATS;  // x window cells
+        constexpr int XE = (NE + kBlock - 1) / kBlock;
+        Bands<R, FMT> nb;
+        float nx[XE];
+        auto issue = [&](int t) {
+            G g;
+            g.at(t, W, H, N);
+            if (g.valid) load_bands<R, FMT, kDiagStored>(A, pitch, g.ib, nb);
 #pragma unroll
-                for (int j = 0; j < R; ++j) y[j] = MODE == 1 ? y[j] + zz[j] : y[j] - zz[j];
+            for (int q = 0; q < XE; ++q) {
+                const int e = (int)threadIdx.x + q * kBlock;
+                const int ly = e / G::STRIDE, lx = e - ly * G::STRIDE;
+                // matrix-row index of the window cell (as Geom2D::stage)
+                const long long gi = (long long)(g.y0 - 2 + ly) * W + (g.x0 - 2 + lx);
+                nx[q] = (e < NE && gi >= 0 && gi < N) ? x[gi] : 0.f;
             }
-            store_rows<R>(out, g.ib, N, y);
+        };
+        if (tr.begin < tr.end) issue(tr.begin);
+        for (int t = tr.begin; t < tr.end; t += tr.step) {
+            G g;
+            g.at(t, W, H, N);
+            __syncthreads();  // the previous tile is done with lds
+#pragma unroll
+            for (int q = 0; q < XE; ++q) {
+                const int e = (int)threadIdx.x + q * kBlock;
+                if (e < NE) lds[e] = nx[q];
+            }
+            const Bands<R, FMT> av = nb;
+            __syncthreads();
+            if (t + tr.step < tr.end) issue(t + tr.step);
+            if (g.valid) {
+                float y[R];
+                apply_bands<R, FMT, kDiagStored, G::STRIDE>(av, g.xs(lds), y);
+                if (MODE != 0) {
+                    float zz[R];
+                    load_rows<R>(z, g.ib, N, zz);
+#pragma unroll
+                    for (int j = 0; j < R; ++j) y[j] = MODE == 1 ? y[j] + zz[j] : y[j] - zz[j];
+                }
+                store_rows<R>(out, g.ib, N, y);
+            }
+        }
+    } else {
+        for (int t = tr.begin; t < tr.end; t += tr.step) {
+            G g;
+            g.at(t, W, H, N);
+            Bands<R, FMT> av;
+            if (g.valid) load_bands<R, FMT, kDiagStored>(A, pitch, g.ib, av);
+            __syncthreads();
+            g.stage(lds, W, H, N, [&](int i) { return x[i]; });
+            __syncthreads();
+            if (g.valid) {
+                float y[R];
+                apply_bands<R, FMT, kDiagStored, G::STRIDE>(av, g.xs(lds), y);
+                if (MODE != 0) {
+                    float zz[R];
+                    load_rows<R>(z, g.ib, N, zz);
+#pragma unroll
+                    for (int j = 0; j < R; ++j) y[j] = MODE == 1 ? y[j] + zz[j] : y[j] - zz[j];
+                }
+                store_rows<R>(out, g.ib, N, y);
+            }
         }
     }
 }
