@@ -205,8 +205,13 @@ def main():
     torch.cuda.synchronize()
     spmv_ms = e0.elapsed_time(e1) / args.spmv_reps
     ft = tuned.get("fp32")
-    sp_r, sp_ty, sp_wg = (ft["spmv_rows_per_thread"], ft["spmv_tile_rows"], ft["spmv_wg_per_cu"]) if ft else (4, 4, 8)
-    spmv_name = f"gm::k_spmv<gm::Geom2D<{sp_r}, {sp_ty}>, 0, 0>" if sp_ty > 0 else f"gm::k_spmv<gm::Geom1D<{sp_r}>, 0, 0>"
+    sp_r, sp_ty, sp_wg, sp_d = ((ft["spmv_rows_per_thread"], ft["spmv_tile_rows"], ft["spmv_wg_per_cu"],
+                                 ft["spmv_direct"]) if ft else (4, 4, 8, 0))
+    if sp_d:
+        spmv_name = f"gm::k_spmv_direct<{sp_r}, {sp_ty}, 0>"
+    else:
+        spmv_name = (f"gm::k_spmv<gm::Geom2D<{sp_r}, {sp_ty}>, 0, 0>" if sp_ty > 0
+                     else f"gm::k_spmv<gm::Geom1D<{sp_r}>, 0, 0>")
     spmv_kernel = spmv_name + f" ({sp_wg} workgroups/CU; banded SpMV, 25 FP32 diagonals)"
     # HBM bytes per launch of this kernel on this workload, from the committed
     # rocprofv3 FETCH_SIZE / WRITE_SIZE runs (copy-calibrated); null if the

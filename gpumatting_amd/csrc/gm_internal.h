@@ -667,6 +667,9 @@ inline long shape_tiles(int sh, int W, long N) {
 
 // SpMV shape table (gm_spmv.hip) and its tuner.
 void spmv_tune_get(int fmt, int W, long N, int* shape, int* per_cu);
+void spmv_shape_info(int sh, int* R, int* TY, int* direct);  // rows per thread, tile rows, LDS-free form
+gm_status spmv_tune_set(int W, int N, int R, int TY, int direct, int per_cu);
+const char* spmv_report();  // candidates timed by the last spmv_autotune on this thread
 gm_status spmv_autotune(const void* A, size_t pitch, int fmt, int W, int N, float* out, const float* x, int reps,
                         gm_stream_t s, int* best_shape, int* best_pc, double* best_us);
 

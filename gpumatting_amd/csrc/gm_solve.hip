@@ -1020,6 +1020,7 @@ struct TuneVal {
     int upd_pc;  // workgroups per CU of the update kernel (0 = default)
 };
 static thread_local std::string g_tune_report;
+static thread_local bool g_tune_spmv = false;  // the last gm_autotune on this thread also tuned the SpMV
 static std::mutex g_tune_mu;
 static std::map<TuneKey, TuneVal> g_tune;
 
@@ -1680,6 +1681,7 @@ gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_re
     if (reps < 1) reps = 5;
     char line[160];
     g_tune_report = "kernel shape R TY wg_per_cu us\n";
+    g_tune_spmv = !mixed;
     for (int sh = 0; sh < kNumShapes; ++sh) {
         if (!step_shape_ok(sh, S.fmt, S.plan.W, S.N)) continue;
         for (int pc : pcs) {
@@ -1745,6 +1747,7 @@ gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_re
         out->step_us = best.us;
         out->spmv_rows_per_thread = 0;
         out->spmv_tile_rows = 0;
+        out->spmv_direct = 0;
         out->spmv_wg_per_cu = 0;
         out->spmv_us = 0;
     }
@@ -1755,8 +1758,7 @@ gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_re
         if ((st = spmv_autotune(L->a, L->apitch, kF32, S.plan.W, S.N, w->Ap, w->r, reps, s, &sh, &pc, &us)) != GM_OK)
             return st;
         if (out) {
-            out->spmv_rows_per_thread = kShapeR[sh];
-            out->spmv_tile_rows = kShapeTY[sh];
+            spmv_shape_info(sh, &out->spmv_rows_per_thread, &out->spmv_tile_rows, &out->spmv_direct);
             out->spmv_wg_per_cu = pc;
             out->spmv_us = us;
         }
@@ -1764,7 +1766,15 @@ gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_re
     return GM_OK;
 }
 
-const char* gm_autotune_report(void) { return g_tune_report.c_str(); }
+const char* gm_autotune_report(void) {
+    static thread_local std::string full;
+    full = g_tune_report + (g_tune_spmv ? spmv_report() : "");
+    return full.c_str();
+}
+
+gm_status gm_spmv_tune_set(int W, int N, int rows_per_thread, int tile_rows, int direct, int wg_per_cu) {
+    return spmv_tune_set(W, N, rows_per_thread, tile_rows, direct, wg_per_cu);
+}
 
 gm_status gm_tune_set(int W, int N, int precision, int rows_per_thread, int tile_rows, int wg_per_cu) {
     const int fmt = precision == GM_PREC_FP16 ? kF16 : (precision == GM_PREC_BF16 ? kBF16 : kF32);

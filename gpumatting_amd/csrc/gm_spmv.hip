@@ -13,6 +13,8 @@
 #include <map>
 #include <utility>
 
+#include <string>
+
 #include "gm_internal.h"
 
 namespace gm {
@@ -107,6 +109,81 @@ __global__ __launch_bounds__(kBlock) void k_spmv(float* __restrict__ out, const 
     }
 }
 
+// FP32 banded SpMV without LDS: each thread reads its own x window straight
+// from global memory (neighbouring threads share it in L1 / L2), so there is
+// no staging and no barrier, and fewer registers allow more waves per CU.
+// 2-D tiles of TX x TY pixels, R consecutive pixels per thread (W % R == 0,
+// x 4R-byte aligned).  Tiles whose window reaches the first 3 or last 3
+// image rows guard every x read; the others read x at matrix-row indices
+// that stay inside [0, N) (cells beside the image alias the neighbouring row
+// exactly like the row-wise product; their band entries are 0).
+template <int R, int TY, int MODE>
+__global__ __launch_bounds__(kBlock) void k_spmv_direct(float* __restrict__ out, const float* __restrict__ A,
+                                                        size_t pitch, int W, int H, int N,
+                                                        const float* __restrict__ x, const float* __restrict__ z) {
+    static_assert(R == 2 || R == 4, "rows per thread");
+    using G = Geom2D<R, TY>;
+    constexpr int XL = R == 4 ? 12 : 6;  // x floats read per window row, from column x - (R == 4 ? 4 : 2)
+    constexpr int CO = R == 4 ? 2 : 0;   // index of column x - 2 among them
+    typedef float vecR __attribute__((ext_vector_type(R)));
+    const TileRange tr = xcd_tiles(G::ntiles(W, H, N));
+    for (int t = tr.begin; t < tr.end; t += tr.step) {
+        G g;
+        g.at(t, W, H, N);
+        if (!g.valid) continue;
+        Bands<R, kF32> av;
+        load_bands<R, kF32, kDiagStored>(A, pitch, g.ib, av);
+        const bool interior = g.y0 >= 3 && g.y0 + TY + 3 <= H;
+        float xw[5][XL];
+#pragma unroll
+        for (int sr = 0; sr < 5; ++sr) {
+            const long long base = (long long)g.ib + (long long)(sr - 2) * W - (R == 4 ? 4 : 2);
+            if (interior) {
+#pragma unroll
+                for (int q = 0; q < XL / R; ++q) {
+                    const vecR v = *reinterpret_cast<const vecR*>(x + base + q * R);
+#pragma unroll
+                    for (int c = 0; c < R; ++c) xw[sr][q * R + c] = v[c];
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < XL; ++c) {
+                    const long long gi = base + c;
+                    xw[sr][c] = (gi >= 0 && gi < N) ? x[gi] : 0.f;
+                }
+            }
+        }
+        float y[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) y[j] = 0.f;
+#pragma unroll
+        for (int k = 0; k < 25; ++k) {
+            const int sr = k / 5, dx = k % 5;
+#pragma unroll
+            for (int j = 0; j < R; ++j) y[j] = fmaf(av.v[k].get(j), xw[sr][CO + dx + j], y[j]);
+        }
+        if (MODE != 0) {
+            float zz[R];
+            load_rows<R>(z, g.ib, N, zz);
+#pragma unroll
+            for (int j = 0; j < R; ++j) y[j] = MODE == 1 ? y[j] + zz[j] : y[j] - zz[j];
+        }
+        store_rows<R>(out, g.ib, N, y);
+    }
+}
+
+template <int R, int TY>
+static void spmv_direct_t(float* out, const float* A, size_t pitch, int W, int N, const float* x, const float* z,
+                          int mode, int grid, hipStream_t hs) {
+    const int H = N / W;
+    if (mode == 0)
+        hipLaunchKernelGGL((k_spmv_direct<R, TY, 0>), dim3(grid), dim3(kBlock), 0, hs, out, A, pitch, W, H, N, x, z);
+    else if (mode == 1)
+        hipLaunchKernelGGL((k_spmv_direct<R, TY, 1>), dim3(grid), dim3(kBlock), 0, hs, out, A, pitch, W, H, N, x, z);
+    else
+        hipLaunchKernelGGL((k_spmv_direct<R, TY, 2>), dim3(grid), dim3(kBlock), 0, hs, out, A, pitch, W, H, N, x, z);
+}
+
 template <class G, int FMT>
 static void spmv_t(float* out, const void* A, size_t pitch, int W, int N, const float* x, const float* z, int mode,
                    int grid, hipStream_t hs) {
@@ -134,6 +211,31 @@ static void spmv_shape_t(float* out, const void* A, size_t pitch, int W, int N, 
     }
 }
 
+// SpMV-only shapes beyond the shared table: the LDS-free FP32 kernel.
+enum { kSpD4Y4 = kNumShapes, kSpD4Y8, kSpD2Y8, kSpD2Y16, kNumSpmvShapes };
+static int spmv_R(int sh) { return sh < kNumShapes ? kShapeR[sh] : (sh <= kSpD4Y8 ? 4 : 2); }
+static int spmv_TY(int sh) {
+    if (sh < kNumShapes) return kShapeTY[sh];
+    return sh == kSpD4Y4 ? 4 : (sh == kSpD2Y16 ? 16 : 8);
+}
+static bool spmv_shape_ok(int sh, int fmt, int W, long N, const float* x) {
+    if (sh < kNumShapes) return shape_ok(sh, fmt, W, N);
+    const int R = spmv_R(sh);
+    return fmt == kF32 && W >= 8 && N % W == 0 && W % R == 0 &&
+           (reinterpret_cast<uintptr_t>(x) & (uintptr_t)(4 * R - 1)) == 0;
+}
+static long spmv_tiles(int sh, int W, long N) {
+    if (sh < kNumShapes) return shape_tiles(sh, W, N);
+    const int TX = kBlock / spmv_TY(sh) * spmv_R(sh);
+    return (long)((W + TX - 1) / TX) * ((N / W + spmv_TY(sh) - 1) / spmv_TY(sh));
+}
+
+void spmv_shape_info(int sh, int* R, int* TY, int* direct) {
+    *R = spmv_R(sh);
+    *TY = spmv_TY(sh);
+    *direct = sh >= kNumShapes;
+}
+
 // y = A x (+- z) for the Levin structure with a given tile shape.
 static gm_status spmv_shape_launch(float* out, const void* A, size_t pitch, int fmt, int W, int N, const float* x,
                                    const float* z, int mode, int shape, int per_cu, gm_stream_t s) {
@@ -141,10 +243,15 @@ static gm_status spmv_shape_launch(float* out, const void* A, size_t pitch, int 
     gm_status st = dev_info(&d);
     if (st != GM_OK) return st;
     if (N == 0) return GM_OK;
-    if (!shape_ok(shape, fmt, W, N)) return fail(GM_ERR_INVALID, "spmv: tile shape %d unusable here", shape);
-    const int grid = grid_for_tiles(d, shape_tiles(shape, W, N), per_cu);
+    if (!spmv_shape_ok(shape, fmt, W, N, x)) return fail(GM_ERR_INVALID, "spmv: tile shape %d unusable here", shape);
+    const int grid = grid_for_tiles(d, spmv_tiles(shape, W, N), per_cu);
     hipStream_t hs = (hipStream_t)s;
-    if (fmt == kF32) spmv_shape_t<kF32>(out, A, pitch, W, N, x, z, mode, shape, grid, hs);
+    const float* Af = reinterpret_cast<const float*>(A);
+    if (shape == kSpD4Y4) spmv_direct_t<4, 4>(out, Af, pitch, W, N, x, z, mode, grid, hs);
+    else if (shape == kSpD4Y8) spmv_direct_t<4, 8>(out, Af, pitch, W, N, x, z, mode, grid, hs);
+    else if (shape == kSpD2Y8) spmv_direct_t<2, 8>(out, Af, pitch, W, N, x, z, mode, grid, hs);
+    else if (shape == kSpD2Y16) spmv_direct_t<2, 16>(out, Af, pitch, W, N, x, z, mode, grid, hs);
+    else if (fmt == kF32) spmv_shape_t<kF32>(out, A, pitch, W, N, x, z, mode, shape, grid, hs);
     else if (fmt == kF16) spmv_shape_t<kF16>(out, A, pitch, W, N, x, z, mode, shape, grid, hs);
     else spmv_shape_t<kBF16>(out, A, pitch, W, N, x, z, mode, shape, grid, hs);
     GM_LAUNCH_CHECK();
@@ -177,8 +284,12 @@ void spmv_tune_get(int fmt, int W, long N, int* shape, int* per_cu) {
     *per_cu = fmt == kF32 ? 8 : 4;
 }
 
+static thread_local std::string g_spmv_report;
+const char* spmv_report() { return g_spmv_report.c_str(); }
+
 gm_status spmv_autotune(const void* A, size_t pitch, int fmt, int W, int N, float* out, const float* x, int reps,
                         gm_stream_t s, int* best_shape, int* best_pc, double* best_us) {
+    g_spmv_report.clear();
     const int pcs[] = {1, 2, 3, 4, 6, 8};
     hipEvent_t e0, e1;
     GM_HIP(hipEventCreate(&e0));
@@ -186,8 +297,8 @@ gm_status spmv_autotune(const void* A, size_t pitch, int fmt, int W, int N, floa
     double best = 1e30;
     int bs = kS1R4, bp = 4;
     gm_status st = GM_OK;
-    for (int sh = 0; sh < kNumShapes && st == GM_OK; ++sh) {
-        if (!shape_ok(sh, fmt, W, N)) continue;
+    for (int sh = 0; sh < kNumSpmvShapes && st == GM_OK; ++sh) {
+        if (!spmv_shape_ok(sh, fmt, W, N, x)) continue;
         for (int pc : pcs) {
             double tot = 0;
             for (int k = 0; k < reps + 1 && st == GM_OK; ++k) {
@@ -200,6 +311,10 @@ gm_status spmv_autotune(const void* A, size_t pitch, int fmt, int W, int N, floa
                 if (k > 0) tot += ms;
             }
             const double us = tot * 1e3 / reps;
+            char line[128];
+            snprintf(line, sizeof(line), "spmv %d %d %d%s %d %.2f\n", sh, spmv_R(sh), spmv_TY(sh),
+                     sh >= kNumShapes ? " direct" : "", pc, us);
+            g_spmv_report += line;
             if (us < best) {
                 best = us;
                 bs = sh;
@@ -217,6 +332,19 @@ gm_status spmv_autotune(const void* A, size_t pitch, int fmt, int W, int N, floa
     *best_shape = bs;
     *best_pc = bp;
     *best_us = best;
+    return GM_OK;
+}
+
+// Pin the plain product's shape for one matrix shape (tuner-table override).
+gm_status spmv_tune_set(int W, int N, int R, int TY, int direct, int per_cu) {
+    if (per_cu < 1 || per_cu > 16) return fail(GM_ERR_INVALID, "gm_spmv_tune_set: wg_per_cu 1..16");
+    int sh = -1;
+    for (int k = 0; k < kNumSpmvShapes; ++k)
+        if (spmv_R(k) == R && spmv_TY(k) == TY && (k >= kNumShapes) == (direct != 0)) sh = k;
+    if (sh < 0 || (sh < kNumShapes ? !shape_ok(sh, kF32, W, N) : !(W >= 8 && N % W == 0 && W % R == 0)))
+        return fail(GM_ERR_INVALID, "gm_spmv_tune_set: unsupported shape");
+    std::lock_guard<std::mutex> g(g_sp_mu);
+    g_sp_tune[SpTuneKey{kF32, W, (long)N}] = std::make_pair(sh, per_cu);
     return GM_OK;
 }
 
@@ -249,6 +377,10 @@ gm_status spmv_launch(float* out, const BandedMatrix* A, const BandPlan& p, cons
     if (tiled_ok(p, A->a, A->apitch)) {
         int shape, pc;
         spmv_tune_get(kF32, p.W, N, &shape, &pc);
+        if (!spmv_shape_ok(shape, kF32, p.W, N, x)) {  // e.g. a direct shape and an unaligned x
+            shape = shape_ok(kS2R4Y4, kF32, p.W, N) ? kS2R4Y4 : kS1R4;
+            pc = 8;
+        }
         return spmv_shape_launch(out, A->a, A->apitch, kF32, p.W, N, x, z, mode, shape, pc, s);
     } else {
         const int g = elementwise_grid(d, N, 1);

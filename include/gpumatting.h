@@ -169,6 +169,7 @@ typedef struct gm_tune_result_s {
    double spmv_us;
    int update_wg_per_cu;   /* grid of the CG update kernel */
    double update_us;
+   int spmv_direct;        /* 1: the plain product's LDS-free form (x read through the caches) */
 } gm_tune_result;
 gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_result* out, gm_stream_t stream);
 /* Text table of every candidate timed by the last gm_autotune on this thread. */
@@ -177,6 +178,13 @@ const char* gm_autotune_report(void);
  * (4,4) (4,8) (4,16) for FP32 and (4,0) (8,0) (4,4) (4,8) (4,16) (8,8) (8,16)
  * for 16-bit; 2-D shapes need W % rows_per_thread == 0. */
 gm_status gm_tune_set(int W, int N, int precision, int rows_per_thread, int tile_rows, int wg_per_cu);
+
+/* Pin the plain product's (gm_bm_ax) form for FP32 Levin matrices of this
+ * shape without timing: the staged forms (rows_per_thread, tile_rows) as for
+ * gm_tune_set's FP32 shapes, or direct = 1 with (4,4) (4,8) (2,8) (2,16) for
+ * the LDS-free form (x 4*rows_per_thread-byte aligned, else a staged form
+ * runs).  No reference counterpart. */
+gm_status gm_spmv_tune_set(int W, int N, int rows_per_thread, int tile_rows, int direct, int wg_per_cu);
 
 /* FP64-accumulated ||b - A x|| / ||b|| (synchronises; writes *relres). */
 gm_status gm_true_relres(const BandedMatrix* A, const float* x, const float* b, double* relres,

@@ -158,6 +158,35 @@ def test_tiled_spmv_vs_oracle(gpu, W, H):
     assert (np.abs(got - (ref - y)) / (scale + np.abs(y))).max() <= 1e-6
 
 
+SPMV_FORMS = [(2, 0, 0), (4, 0, 0), (4, 4, 0), (4, 8, 0), (4, 16, 0), (4, 4, 1), (4, 8, 1), (2, 8, 1), (2, 16, 1)]
+
+
+@pytest.mark.parametrize("W,H", [(64, 20), (256, 192), (96, 7), (1000, 37)])
+def test_every_spmv_form_same_product(gpu, W, H):
+    """Every form of the plain product (1-D and 2-D staged tiles, the LDS-free
+    direct tiles with their guarded edge tiles) sums the 25 products of a row in
+    the same order: bit-identical results, and the FP64 oracle's to 1e-6."""
+    p = synth.make_problem(W, H, 400 + W, 1, 0.08)
+    s = matting.assemble(p.image, p.lam, p.target, 1e-7, ASSEMBLY_ACCURATE)
+    a = s.L.to_numpy()
+    rng = np.random.default_rng(H)
+    x = rng.standard_normal(p.N).astype(np.float32)
+    xp = Padded(p.N, s.pad, init=x)
+    ref = O.bm_ax_f64(a.astype(np.float64), O.bands25(W), O.padded(x.astype(np.float64), s.pad), s.pad)
+    scale = np.abs(a).sum(0) * np.abs(x).max() + 1e-30
+    outs = []
+    for (R, ty, direct) in SPMV_FORMS:
+        if W % R:
+            continue
+        call("gm_spmv_tune_set", W, p.N, R, ty, direct, 4)
+        got = matting.bm_ax(s.L, xp).cpu().numpy()
+        assert (np.abs(got - ref) / scale).max() <= 1e-6, (R, ty, direct)
+        outs.append(got)
+    for o in outs[1:]:
+        assert np.array_equal(o.view(np.uint32), outs[0].view(np.uint32))
+    call("gm_spmv_tune_set", W, p.N, 4, 4, 0, 8)
+
+
 def test_spmv_properties_at_4k(gpu):
     """Size-independent properties at 3840x2160: linearity, symmetry
     <x, Ay> = <Ax, y>, and L 1 = lambda D 1 (zero row sums of L)."""
