@@ -12,6 +12,10 @@ replacement) ms per solve; the FP32 solve is timed the same way beside it.
 The roofline object is the FP32 banded SpMV (y = A x, 25 diagonals), timed
 with HIP events on its launch stream; the CPU baseline is the oracle's OpenMP
 restatement of the same assembly + Jacobi-PCG on a bounded sample.
+`alpha_err_vs_fp64` gives max / mean |alpha - alpha_FP64| of both 1e-6 solves
+against the same system solved to 1e-12 with an FP64 master x;
+`accuracy_matched` times both forms at the tolerance where their max alpha
+error is <= 1e-4 (5e-9, FP64 master x inside the iteration).
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -27,6 +31,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+MATCHED_TOL = 5e-9  # relative residual at which both forms reach max |alpha err| <= 1e-4 at 4K
 METRIC = "4K matte solve ms to 1e-6 resid (mixed vs FP32) + banded SpMV HBM GB/s"
 
 
@@ -153,14 +158,32 @@ def main():
     pad = matting.levin_pad(W)
     x = matting.Padded(N, pad)
 
-    def step(prec):
+    def step(prec, tol=1e-6, refine_mode=0):
         s = matting.assemble(img, lam, tgt, p.eps, ASSEMBLY_ACCURATE, L=L)
         x.buf.zero_()
-        _, info = matting.solve(s, x=x, method=METHOD_PCG, precision=prec, tol=1e-6, max_iter=20000)
+        _, info = matting.solve(s, x=x, method=METHOD_PCG, precision=prec, tol=tol, max_iter=20000,
+                                refine_mode=refine_mode)
         return s, info
+
+    def timed(prec, tol=1e-6, refine_mode=0):
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        infos = []
+        t0 = time.perf_counter()
+        e0.record()
+        for _ in range(args.steps):
+            _, info = step(prec, tol, refine_mode)
+            infos.append((info.iterations, info.true_relres, info.status, info.replacements))
+        e1.record()
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) * 1e3 / args.steps
+        ms = e0.elapsed_time(e1) / args.steps
+        return {"ms": ms, "wall_ms": wall, "iterations": infos[-1][0], "true_relres": infos[-1][1],
+                "status": infos[-1][2], "replacements": infos[-1][3]}
 
     results = {}
     tuned = {}
+    alphas = {}
     for name, prec in (("mixed", PREC_FP16), ("fp32", PREC_FP32)):
         for _ in range(args.warmup):
             step(prec)
@@ -170,22 +193,35 @@ def main():
             log(f"autotune {name}: {tuned[name]}")
             log("autotune candidates:\n" + lib().gm_autotune_report().decode())
             step(prec)
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        infos = []
-        t0 = time.perf_counter()
-        e0.record()
-        for _ in range(args.steps):
-            s, info = step(prec)
-            infos.append((info.iterations, info.true_relres, info.status, info.replacements))
-        e1.record()
-        torch.cuda.synchronize()
-        wall = (time.perf_counter() - t0) * 1e3 / args.steps
-        ms = e0.elapsed_time(e1) / args.steps
-        results[name] = {"ms": ms, "wall_ms": wall, "iterations": infos[-1][0], "true_relres": infos[-1][1],
-                         "status": infos[-1][2], "replacements": infos[-1][3]}
-        log(f"{name}: {ms:.2f} ms/solve (wall {wall:.2f}), iterations {infos[-1][0]}, "
-            f"true relres {infos[-1][1]:.3e}")
+        r = results[name] = timed(prec)
+        alphas[name] = x.t.clone()
+        log(f"{name}: {r['ms']:.2f} ms/solve (wall {r['wall_ms']:.2f}), iterations {r['iterations']}, "
+            f"true relres {r['true_relres']:.3e}")
+
+    # alpha accuracy: against the same system solved to 1e-12 with an FP64
+    # master x (outer FP64 residual, refine_mode 1), outside the timed regions
+    s = matting.assemble(img, lam, tgt, p.eps, ASSEMBLY_ACCURATE, L=L)
+    xref, iref = matting.solve(s, method=METHOD_PCG, tol=1e-12, max_iter=20000, refine_mode=1)
+    aref = xref.t.clone()
+    log(f"alpha reference: {iref.iterations} iterations, true relres {iref.true_relres:.2e}")
+
+    def alpha_err(a):
+        d = (a - aref).abs()
+        return {"max": float(d.max()), "mean": float(d.mean())}
+
+    alpha_errs = {name: alpha_err(a) for name, a in alphas.items()}
+    # accuracy-matched comparison: both forms to the tolerance at which their
+    # max alpha error is <= 1e-4, with the FP64 master x inside the iteration
+    # (refine_mode 2; FP32 x alone stalls near 2e-8 here)
+    matched = {"tol": MATCHED_TOL, "refine_mode": 2}
+    for name, prec in (("mixed", PREC_FP16), ("fp32", PREC_FP32)):
+        step(prec, MATCHED_TOL, 2)
+        r = timed(prec, MATCHED_TOL, 2)
+        r["alpha_err"] = alpha_err(x.t)
+        matched[name] = {k: (round(v, 3) if k.endswith("ms") else v) for k, v in r.items()}
+        log(f"{name} to {MATCHED_TOL:g} (FP64 master x): {r['ms']:.2f} ms, {r['iterations']} iterations, "
+            f"max |alpha err| {r['alpha_err']['max']:.2e}")
+    matched["speedup"] = round(matched["fp32"]["ms"] / matched["mixed"]["ms"], 3)
 
     # banded SpMV roofline: y = A x on the assembled FP32 matrix
     s = matting.assemble(img, lam, tgt, p.eps, ASSEMBLY_ACCURATE, L=L)
@@ -264,6 +300,9 @@ def main():
         "iterations": {"mixed": mixed["iterations"], "fp32": fp32["iterations"]},
         "true_relres": {"mixed": mixed["true_relres"], "fp32": fp32["true_relres"]},
         "residual_replacements_mixed": mixed["replacements"],
+        "alpha_err_vs_fp64": dict(alpha_errs, reference=f"same system, PCG with an FP64 master x to 1e-12 "
+                                                        f"({iref.iterations} iterations)"),
+        "accuracy_matched": matched,
         "autotuned_step": tuned,
         "spmv_gbs": round(spmv_gbs, 1),
         "spmv_us": round(spmv_ms * 1e3, 2),

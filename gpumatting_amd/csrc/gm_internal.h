@@ -474,15 +474,15 @@ struct Tile {
     static constexpr int LDS_FLOATS = 5 * SEG;
 };
 
-template <int R, class F>
-__device__ __forceinline__ void stage5(float* __restrict__ lds, int i0, int W, int N, F f) {
+template <int R, class T, class F>
+__device__ __forceinline__ void stage5(T* __restrict__ lds, int i0, int W, int N, F f) {
     constexpr int SEG = Tile<R>::SEG;
 #pragma unroll
     for (int s = 0; s < 5; ++s) {
         const long long gb = (long long)i0 + (long long)(s - 2) * W - 2;
         for (int e = threadIdx.x; e < SEG; e += kBlock) {
             const long long g = gb + e;
-            lds[s * SEG + e] = (g >= 0 && g < N) ? f((int)g) : 0.f;
+            lds[s * SEG + e] = (g >= 0 && g < N) ? f((int)g) : T(0);
         }
     }
 }

@@ -520,14 +520,15 @@ __global__ __launch_bounds__(kBlock) void k_fold(Scalars* sc, const double* __re
 
 // FP64-accumulated true residual t = A x - b.  op 0: measure only (g0 = ||t||^2);
 // op 1: replace (r = t, or s*t for the scaled system; <r,z>, ||t||^2);
-// op 2: start the scaled system (as 1 with beta = 0).
-template <int R, int PM>
+// op 2: start the scaled system (as 1 with beta = 0).  XT: float for the
+// solver's own x, double for the FP64 master x of refine mode 2.
+template <int R, int PM, class XT>
 __global__ __launch_bounds__(kBlock) void k_trueres25(const float* __restrict__ A, size_t pitch, int W, int N,
-                                                      const float* __restrict__ x, const float* __restrict__ b,
+                                                      const XT* __restrict__ x, const float* __restrict__ b,
                                                       const float* __restrict__ dinv, float* __restrict__ r, int op,
                                                       Scalars* sc, double* partials, unsigned int* counter) {
-    if (op == 1 && sc->done != kRun) return;
-    __shared__ __attribute__((aligned(16))) float lds[Tile<R>::LDS_FLOATS];
+    if (op != 0 && sc->done != kRun) return;
+    __shared__ __attribute__((aligned(16))) XT lds[Tile<R>::LDS_FLOATS];
     const int ntiles = (N + Tile<R>::ROWS - 1) / Tile<R>::ROWS;
     const TileRange tr = xcd_tiles(ntiles);
     constexpr int SEG = Tile<R>::SEG;
@@ -537,15 +538,15 @@ __global__ __launch_bounds__(kBlock) void k_trueres25(const float* __restrict__ 
         __syncthreads();
         int nz = 0;
         stage5<R>(lds, i0, W, N, [&](int g) {
-            const float v = x[g];
-            nz |= v != 0.f;
+            const XT v = x[g];
+            nz |= v != XT(0);
             return v;
         });
         // a window of zeros (a zero initial guess) needs no band loads: A x = 0
         nz = __syncthreads_or(nz);
         const int ib = i0 + R * threadIdx.x;
         if (ib < N) {
-            const float* xs = lds + R * threadIdx.x;
+            const XT* xs = lds + R * threadIdx.x;
             double acc[R];
 #pragma unroll
             for (int j = 0; j < R; ++j) acc[j] = 0.0;
@@ -784,12 +785,12 @@ __global__ __launch_bounds__(kBlock) void k_cgstep_g(const float* __restrict__ a
     step_end(sc, v[0], dpa, in);
 }
 
-template <int PM>
+template <int PM, class XT>
 __global__ __launch_bounds__(kBlock) void k_trueres_g(const float* __restrict__ a, size_t pitch, BandArgs ba, int N,
-                                                      const float* __restrict__ x, const float* __restrict__ b,
+                                                      const XT* __restrict__ x, const float* __restrict__ b,
                                                       const float* __restrict__ dinv, float* __restrict__ r, int op,
                                                       Scalars* sc, double* partials, unsigned int* counter) {
-    if (op == 1 && sc->done != kRun) return;
+    if (op != 0 && sc->done != kRun) return;
     double v[3] = {0.0, 0.0, 0.0};
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < N; i += gridDim.x * kBlock) {
         double s = 0.0;
@@ -877,12 +878,25 @@ __global__ __launch_bounds__(kBlock) void k_zero64(double* v, int n) {
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) v[i] = 0.0;
 }
 
-// x64 = x (init) or x64 += x (accumulate a correction); x = (float)x64 (finish)
+// op 0: x64 = x (init); 1: x64 += x (accumulate a correction); 2: x = (float)x64
+// (finish); 3: x64 += x, x = 0 (move the FP32 partial sum into the master);
+// 4: x64 = x, x = 0 (start from the initial guess); 5: x64 += x, x = (float)x64
 __global__ __launch_bounds__(kBlock) void k_x64(double* __restrict__ x64, float* __restrict__ x, int N, int op) {
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < N; i += gridDim.x * kBlock) {
-        if (op == 0) x64[i] = (double)x[i];
-        else if (op == 1) x64[i] += (double)x[i];
-        else x[i] = (float)x64[i];
+        if (op == 0) {
+            x64[i] = (double)x[i];
+        } else if (op == 1) {
+            x64[i] += (double)x[i];
+        } else if (op == 2) {
+            x[i] = (float)x64[i];
+        } else if (op == 5) {
+            const double v = x64[i] + (double)x[i];
+            x64[i] = v;
+            x[i] = (float)v;
+        } else {
+            x64[i] = (op == 3 ? x64[i] : 0.0) + (double)x[i];
+            x[i] = 0.f;
+        }
     }
 }
 
@@ -984,6 +998,27 @@ static gm_status ws_get(gm_stream_t s, long n, int pad, bool need16, Workspace**
     return GM_OK;
 }
 
+// The FP64 master x of the refinement modes: N doubles padded by the largest
+// band offset (the generic residual kernel reads x64[i + off]).  The pad past
+// N is cleared here, since a larger earlier solve may have left values there.
+static gm_status ws_x64(Workspace* w, long N, int maxoff, hipStream_t hs) {
+    const int pad64 = (maxoff + 63) / 64 * 64;
+    if (w->cap64 < N || w->pad64 < pad64) {
+        if (w->x64) (void)hipFree(w->x64 - w->pad64);
+        w->x64 = nullptr;
+        w->cap64 = 0;
+        double* p = nullptr;
+        GM_HIP(hipMalloc(&p, sizeof(double) * ((size_t)N + 2 * pad64)));
+        GM_HIP(hipMemset(p, 0, sizeof(double) * ((size_t)N + 2 * pad64)));
+        w->x64 = p + pad64;
+        w->cap64 = N;
+        w->pad64 = pad64;
+    }
+    hipLaunchKernelGGL(k_zero64, dim3(1), dim3(kBlock), 0, hs, w->x64 + N, w->pad64);
+    GM_LAUNCH_CHECK();
+    return GM_OK;
+}
+
 void release_workspaces() {
     std::lock_guard<std::mutex> g(g_ws_mu);
     for (auto& kv : g_ws) {
@@ -1052,6 +1087,7 @@ struct Solver {
     int g_upd;
     int verify;         // recursive convergence asks for an FP64 check first
     bool sym;           // 16-bit form stored as its 12 upper bands (k_cgstep_sym)
+    double* x64;        // refine mode 2: FP64 master x (x then holds the FP32 partial sum since the last fold)
 };
 
 static gm_status launch_resid(Solver& S, int start) {
@@ -1076,29 +1112,44 @@ static gm_status launch_resid(Solver& S, int start) {
     return GM_OK;
 }
 
-static gm_status launch_trueres(Solver& S, int op) {
+template <class XT>
+static void launch_trueres_t(Solver& S, const XT* x, int op) {
     if (S.tiled) {
         if (S.pm == kJacobi)
-            hipLaunchKernelGGL((k_trueres25<4, kJacobi>), dim3(S.g_tile4), dim3(kBlock), 0, S.hs, S.A->a,
-                               S.A->apitch, S.plan.W, S.N, S.x, S.b, S.w->dinv, S.w->r, op, S.w->sc, S.rs.partials,
+            hipLaunchKernelGGL((k_trueres25<4, kJacobi, XT>), dim3(S.g_tile4), dim3(kBlock), 0, S.hs, S.A->a,
+                               S.A->apitch, S.plan.W, S.N, x, S.b, S.w->dinv, S.w->r, op, S.w->sc, S.rs.partials,
                                S.rs.counter);
         else if (S.pm == kScaled)
-            hipLaunchKernelGGL((k_trueres25<4, kScaled>), dim3(S.g_tile4), dim3(kBlock), 0, S.hs, S.A->a,
-                               S.A->apitch, S.plan.W, S.N, S.x, S.b, S.w->dinv, S.w->r, op, S.w->sc, S.rs.partials,
+            hipLaunchKernelGGL((k_trueres25<4, kScaled, XT>), dim3(S.g_tile4), dim3(kBlock), 0, S.hs, S.A->a,
+                               S.A->apitch, S.plan.W, S.N, x, S.b, S.w->dinv, S.w->r, op, S.w->sc, S.rs.partials,
                                S.rs.counter);
         else
-            hipLaunchKernelGGL((k_trueres25<4, kPlain>), dim3(S.g_tile4), dim3(kBlock), 0, S.hs, S.A->a,
-                               S.A->apitch, S.plan.W, S.N, S.x, S.b, S.w->dinv, S.w->r, op, S.w->sc, S.rs.partials,
+            hipLaunchKernelGGL((k_trueres25<4, kPlain, XT>), dim3(S.g_tile4), dim3(kBlock), 0, S.hs, S.A->a,
+                               S.A->apitch, S.plan.W, S.N, x, S.b, S.w->dinv, S.w->r, op, S.w->sc, S.rs.partials,
                                S.rs.counter);
     } else {
         if (S.pm == kJacobi)
-            hipLaunchKernelGGL(k_trueres_g<kJacobi>, dim3(S.g_gen), dim3(kBlock), 0, S.hs, S.A->a, S.A->apitch,
-                               S.plan.args, S.N, S.x, S.b, S.w->dinv, S.w->r, op, S.w->sc, S.rs.partials,
+            hipLaunchKernelGGL((k_trueres_g<kJacobi, XT>), dim3(S.g_gen), dim3(kBlock), 0, S.hs, S.A->a,
+                               S.A->apitch, S.plan.args, S.N, x, S.b, S.w->dinv, S.w->r, op, S.w->sc, S.rs.partials,
                                S.rs.counter);
         else
-            hipLaunchKernelGGL(k_trueres_g<kPlain>, dim3(S.g_gen), dim3(kBlock), 0, S.hs, S.A->a, S.A->apitch,
-                               S.plan.args, S.N, S.x, S.b, S.w->dinv, S.w->r, op, S.w->sc, S.rs.partials,
+            hipLaunchKernelGGL((k_trueres_g<kPlain, XT>), dim3(S.g_gen), dim3(kBlock), 0, S.hs, S.A->a,
+                               S.A->apitch, S.plan.args, S.N, x, S.b, S.w->dinv, S.w->r, op, S.w->sc, S.rs.partials,
                                S.rs.counter);
+    }
+}
+
+// The FP64-accumulated true residual of the current iterate (op as k_trueres25).
+// Refine mode 2 first moves the FP32 partial sum into the FP64 master x, so the
+// residual is that of the master and the FP32 x restarts from zero: x never
+// carries more than the correction since the last fold, which removes the
+// FP32 rounding floor of the true residual.
+static gm_status launch_trueres(Solver& S, int op) {
+    if (S.x64) {
+        hipLaunchKernelGGL(k_x64, dim3(S.g_elem), dim3(kBlock), 0, S.hs, S.x64, S.x, S.N, 3);
+        launch_trueres_t<double>(S, S.x64, op);
+    } else {
+        launch_trueres_t<float>(S, S.x, op);
     }
     GM_LAUNCH_CHECK();
     return GM_OK;
@@ -1294,21 +1345,10 @@ static gm_status refine_impl(float* x, const BandedMatrix* A, const float* b, in
     if ((st = red_scratch(s, &rs)) != GM_OK) return st;
     if ((st = ws_get(s, N, plan.maxoff, o->precision != GM_PREC_FP32, &w)) != GM_OK) return st;
     hipStream_t hs = (hipStream_t)s;
-    const int pad64 = (plan.maxoff + 63) / 64 * 64;
-    if (w->cap64 < N || w->pad64 < pad64) {
-        if (w->x64) (void)hipFree(w->x64 - w->pad64);
-        double* p = nullptr;
-        GM_HIP(hipMalloc(&p, sizeof(double) * ((size_t)N + 2 * pad64)));
-        GM_HIP(hipMemset(p, 0, sizeof(double) * ((size_t)N + 2 * pad64)));
-        w->x64 = p + pad64;
-        w->cap64 = N;
-        w->pad64 = pad64;
-    }
+    if ((st = ws_x64(w, N, plan.maxoff, hs)) != GM_OK) return st;
     const int ge = elementwise_grid(dv, N, 4);
-    // clear x64 beyond N (a larger earlier solve may have left values there)
     hipLaunchKernelGGL(k_x64, dim3(ge), dim3(kBlock), 0, hs, w->x64, x, N, 0);
     GM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_zero64, dim3(1), dim3(kBlock), 0, hs, w->x64 + N, pad64);
     hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(64), 0, hs, w->sc, 1);
     hipLaunchKernelGGL(k_bnorm, dim3(ge), dim3(kBlock), 0, hs, b, N, 0.0, w->sc, rs.partials, rs.counter);
     GM_LAUNCH_CHECK();
@@ -1372,6 +1412,7 @@ static gm_status refine_impl(float* x, const BandedMatrix* A, const float* b, in
 gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, const gm_solve_opts* o,
                      gm_solve_info* info, gm_stream_t s) {
     if (o->refine_mode == 1) return refine_impl(x, A, b, pad, o, info, s);
+    if (o->refine_mode < 0 || o->refine_mode > 2) return fail(GM_ERR_INVALID, "solve: refine_mode");
     Solver S{};
     gm_status st = get_plan(A, &S.plan);
     if (st != GM_OK) return st;
@@ -1390,6 +1431,8 @@ gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, c
     const bool mixed = o->precision != GM_PREC_FP32;
     if (mixed && (!S.tiled || o->method == GM_METHOD_GD))
         return fail(GM_ERR_UNSUPPORTED, "16-bit forms need the 5x5 Levin band structure and CG/PCG");
+    if (o->refine_mode == 2 && o->method == GM_METHOD_GD)
+        return fail(GM_ERR_UNSUPPORTED, "refine_mode 2 needs CG/PCG");
     if ((o->method == GM_METHOD_PCG || mixed) && S.plan.args.center < 0)
         return fail(GM_ERR_INVALID, "solve: Jacobi scaling needs a band with offset 0");
     S.pm = mixed ? kScaled : (o->method == GM_METHOD_PCG ? kJacobi : kPlain);
@@ -1428,6 +1471,13 @@ gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, c
     hipLaunchKernelGGL(k_bnorm, dim3(S.g_elem), dim3(kBlock), 0, S.hs, b, S.N, o->tol, w->sc, S.rs.partials,
                        S.rs.counter);
     GM_LAUNCH_CHECK();
+    if (o->refine_mode == 2) {
+        // the initial guess moves into the FP64 master; x accumulates from zero
+        if ((st = ws_x64(w, S.N, S.plan.maxoff, S.hs)) != GM_OK) return st;
+        S.x64 = w->x64;
+        hipLaunchKernelGGL(k_x64, dim3(S.g_elem), dim3(kBlock), 0, S.hs, S.x64, x, S.N, 4);
+        GM_LAUNCH_CHECK();
+    }
 
     S.verify = (o->tol > 0 && o->verify) ? 1 : 0;
     if (S.pm == kJacobi) {
@@ -1440,8 +1490,10 @@ gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, c
     }
 
     const int chunk = o->check_every > 0 ? o->check_every : 16;
-    // FP64 residual replacement period: 16-bit forms default to 50, FP32 to none
-    const int refine = o->refine_every > 0 ? o->refine_every : (o->refine_every == 0 && mixed ? 50 : 0);
+    // FP64 residual replacement period: 16-bit forms and refine mode 2 default
+    // to 50, plain FP32 to none
+    const int refine =
+        o->refine_every > 0 ? o->refine_every : (o->refine_every == 0 && (mixed || S.x64) ? 50 : 0);
     long enq = 0;  // iterations enqueued
     int polls = 0;
     bool finished = false;
@@ -1460,7 +1512,7 @@ gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, c
             if (enq >= o->max_iter) finished = true;
         }
     } else {
-        if (S.pm == kScaled) {
+        if (S.pm == kScaled || S.x64) {
             if ((st = launch_trueres(S, 2)) != GM_OK) return st;
         } else {
             if ((st = launch_resid(S, 1)) != GM_OK) return st;
@@ -1473,7 +1525,7 @@ gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, c
                 ++since_restart;
                 if (o->restart > 0 && since_restart >= o->restart && enq < o->max_iter) {
                     // cgSolve's periodic restart (main.cu:483-493): r from scratch, p = -z
-                    if ((st = launch_resid(S, 0)) != GM_OK) return st;
+                    if ((st = S.x64 ? launch_trueres(S, 2) : launch_resid(S, 0)) != GM_OK) return st;
                     since_restart = 0;
                 }
                 if (refine > 0 && enq % refine == 0) {
@@ -1531,6 +1583,12 @@ gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, c
             GM_HIP(hipMemcpy(&fin, w->sc, sizeof(Scalars), hipMemcpyDeviceToHost));
         }
         info->true_relres = sqrt(fin.g0 / bb);
+    }
+    if (S.x64) {
+        // returned x = the FP64 master (with any partial sum not yet folded), rounded
+        hipLaunchKernelGGL(k_x64, dim3(S.g_elem), dim3(kBlock), 0, S.hs, S.x64, x, S.N, 5);
+        GM_LAUNCH_CHECK();
+        GM_HIP(hipStreamSynchronize(S.hs));
     }
     if (result != GM_OK && result != GM_ERR_NOT_CONVERGED) {
         set_error("solve stopped: %s after %d iterations",

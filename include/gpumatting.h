@@ -133,8 +133,14 @@ typedef struct gm_solve_opts_s {
    int check_every;     /* iterations enqueued between host status polls (0 -> 16) */
    int verify;          /* 1: confirm convergence with the FP64 true residual */
    int refine_mode;     /* 0: FP32 x, residual replacement; 1: iterative refinement with an FP64
-                           master x (outer r = b - A x in FP64, inner solves of A d = r), which
-                           removes the FP32-x floor (~u*|A||x|/|b|) of the true residual */
+                           master x (outer r = b - A x in FP64, inner solves of A d = r to 1e-4);
+                           2: FP64 master x inside the iteration: at every residual replacement
+                           (refine_every, default 50) the FP32 x is added into the master and
+                           restarts from zero, and r = b - A x is recomputed from the master in
+                           FP64, keeping the search direction while the recursive residual has
+                           not drifted.  Modes 1 and 2 remove the FP32-x floor (~u*|A||x|/|b|)
+                           of the true residual; true_relres is then that of the FP64 master,
+                           and alpha is returned rounded to FP32.  Mode 2: CG/PCG only */
 } gm_solve_opts;
 
 typedef struct gm_solve_info_s {

@@ -227,6 +227,84 @@ def test_gpu_jacobi_driver(gpu):
         assert np.abs(x.numpy() - ref).max() <= 1e-5
 
 
+@pytest.mark.parametrize("method,prec", [(METHOD_CG, PREC_FP32), (METHOD_PCG, PREC_FP32), (METHOD_PCG, PREC_FP16),
+                                         (METHOD_PCG, PREC_BF16)])
+def test_refine_mode2_fp64_master(gpu, small, method, prec):
+    """refine_mode 2 keeps x's FP64 master inside the iteration: it reaches
+    tolerances below the FP32-x floor (~2e-8 here) and the FP64 answer."""
+    p, a, b, bands, pad, x_exact = small
+    s = _gpu_system(p)
+    x, info = matting.solve(s, method=method, precision=prec, tol=1e-10, max_iter=20000, refine_mode=2)
+    assert info.status == 0 and info.true_relres <= 1e-10
+    assert info.replacements >= 1
+    assert np.abs(x.numpy() - x_exact).max() <= 1e-6
+    # an initial guess is honoured (moved into the master before the first
+    # residual): the FP32-rounded answer (relres ~1e-8) needs few iterations
+    x0 = Padded(p.N, pad, init=x.numpy())
+    _, info2 = matting.solve(s, x=x0, method=method, precision=prec, tol=1e-10, max_iter=20000, refine_mode=2)
+    assert info2.status == 0 and info2.iterations <= info.iterations // 3
+    # with the periodic restart of cgSolve (r recomputed from the master, p = -z)
+    x3, info3 = matting.solve(s, method=method, precision=prec, tol=1e-9, max_iter=20000, refine_mode=2,
+                              restart=40)
+    assert info3.status == 0 and np.abs(x3.numpy() - x_exact).max() <= 1e-5
+
+
+def test_generic_bands_refine_mode2(gpu):
+    """refine_mode 2 through the generic-band kernels (a band set that is not
+    the 5x5 Levin structure), against the oracle's FP64 PCG on the same
+    matrix."""
+    N = 5000
+    bands = np.array([-37, -1, 0, 1, 37], np.int32)
+    a = np.zeros((5, N), np.float32)
+    i = np.arange(N)
+    a[0, i >= 37] = -0.5
+    a[1, i >= 1] = -1
+    a[2] = 3.02
+    a[3, i + 1 < N] = -1
+    a[4, i + 37 < N] = -0.5
+    M = DeviceMatrix(N, bands, a)
+    rhs = np.random.default_rng(5).random(N).astype(np.float32)
+    b = torch.from_numpy(rhs).cuda()
+    xe, _, _ = O.pcg_f64(a.astype(np.float64), bands, rhs.astype(np.float64), np.zeros(N), 37, 1e-14)
+    for method in (METHOD_CG, METHOD_PCG):
+        x, info = matting.solve(M, b, pad=37, method=method, tol=1e-11, max_iter=5000, refine_mode=2)
+        assert info.status == 0 and info.true_relres <= 1e-11
+        assert np.abs(x.numpy() - xe).max() <= 1e-6 * np.abs(xe).max()
+
+
+def test_config_1024_alpha_error_per_mode(gpu, capsys):
+    """1024x1024, one ellipse, 8 % constraints: plain CG, Jacobi-PCG, BF16 and
+    FP16 mixed to 1e-6, then FP16 and FP32 with the FP64 master x to 2e-9;
+    max / mean |alpha - alpha_FP64| against the oracle's FP64 Laplacian + FP64
+    PCG (1e-12)."""
+    p = synth.config_1k()
+    x64, it64, rr64 = O.solve_f64(p.image, p.lam, p.target, p.eps, tol=1e-12)
+    assert rr64 <= 1e-12
+    s = matting.assemble(p.image, p.lam, p.target, p.eps, ASSEMBLY_ACCURATE)
+    rows = []
+    for name, meth, prec, tol, rm in [("fp32-cg", METHOD_CG, PREC_FP32, 1e-6, 0),
+                                      ("fp32-pcg", METHOD_PCG, PREC_FP32, 1e-6, 0),
+                                      ("bf16-pcg", METHOD_PCG, PREC_BF16, 1e-6, 0),
+                                      ("fp16-pcg", METHOD_PCG, PREC_FP16, 1e-6, 0),
+                                      ("fp32-pcg-x64", METHOD_PCG, PREC_FP32, 2e-9, 2),
+                                      ("fp16-pcg-x64", METHOD_PCG, PREC_FP16, 2e-9, 2)]:
+        x, info = matting.solve(s, method=meth, precision=prec, tol=tol, max_iter=20000, refine_mode=rm)
+        d = np.abs(x.numpy().astype(np.float64) - x64)
+        rows.append((name, tol, info.iterations, info.true_relres, d.max(), d.mean()))
+        assert info.status == 0 and info.true_relres <= tol, name
+    with capsys.disabled():
+        for r in rows:
+            print(f"\n[1024^2] {r[0]:13s} tol {r[1]:.0e}: {r[2]:4d} iterations, true relres {r[3]:.2e}, "
+                  f"max |da| {r[4]:.2e}, mean |da| {r[5]:.2e}", end="")
+        print()
+    for name, tol, its, rr, mx, mean in rows:
+        if tol >= 1e-6:
+            # at 1e-6 the error is the tolerance's, not the precision's
+            assert mx <= 2e-2 and mean <= 5e-5, name
+        else:
+            assert mx <= 1e-4 and mean <= 1e-6, name
+
+
 @pytest.mark.parametrize("prec", [PREC_FP32, PREC_FP16])
 def test_4k_solve_converges(gpu, prec):
     p = synth.config_4k()
