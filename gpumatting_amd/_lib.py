@@ -44,7 +44,7 @@ class TuneResult(C.Structure):
     _fields_ = [("rows_per_thread", C.c_int), ("tile_rows", C.c_int), ("wg_per_cu", C.c_int),
                 ("step_us", C.c_double), ("spmv_rows_per_thread", C.c_int), ("spmv_tile_rows", C.c_int),
                 ("spmv_wg_per_cu", C.c_int), ("spmv_us", C.c_double), ("update_wg_per_cu", C.c_int),
-                ("update_us", C.c_double), ("spmv_direct", C.c_int)]
+                ("update_us", C.c_double), ("spmv_direct", C.c_int), ("step_walk", C.c_int)]
 
 
 class SolveInfo(C.Structure):
@@ -93,6 +93,7 @@ _SIGS = {
     "gm_release_workspaces": ([], ci),
     "gm_autotune": ([BMp, ci, ci, C.POINTER(TuneResult), vp], ci),
     "gm_tune_set": ([ci, ci, ci, ci, ci, ci], ci),
+    "gm_tune_set_step": ([ci, ci, ci, ci, ci, ci, ci], ci),
     "gm_autotune_report": ([], C.c_char_p),
     "gm_spmv_tune_set": ([ci, ci, ci, ci, ci, ci], ci),
     "gm_batch_create": ([C.POINTER(vp), ci, vp, vp, ci, ci], ci),

@@ -245,16 +245,75 @@ __device__ __forceinline__ float h2f(unsigned int h) {
 // U holds 0 for pairs outside the image or wrapping a row, so the product
 // equals the full 24-band one wherever As is symmetric.  16-bit bands move
 // 24 B per row instead of 48.
+template <int TY>
+struct SymGeom {
+    using G = Geom2D<4, TY>;
+    static constexpr int R = 4, TPR = G::TPR, TX = G::TX, ST = G::STRIDE;
+    static constexpr int UR = TY + 2;      // staged U rows: 2 above the tile, then the tile's
+    static constexpr int SW = TX / 2 + 2;  // 32-bit words (half pairs) per staged U row: TX + 4 halves
+    static constexpr int XR = TY + 4;      // staged x rows
+};
+
+// One tile's product from the staged windows.  xrow[q]: LDS offset of the
+// thread's 8 x cells (columns 4 tx .. 4 tx + 7) in window row ty + q;
+// urow[d]: LDS row of U window row ty + 2 - d.  Sums in band order, so every
+// staging scheme gives bit-identical results.
+template <int TY, int FMT>
+__device__ __forceinline__ void sym_tile(const float* __restrict__ lds, const int (&xrow)[5],
+                                         const unsigned int (*ub)[SymGeom<TY>::UR][SymGeom<TY>::SW],
+                                         const int (&urow)[3], int tx, float (&y)[4], float (&pc)[4]) {
+    constexpr int R = 4;
+    float xw[5][8];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        const f32x4 a0 = *reinterpret_cast<const f32x4*>(lds + xrow[q]);
+        const f32x4 a1 = *reinterpret_cast<const f32x4*>(lds + xrow[q] + 4);
+        xw[q][0] = a0.x; xw[q][1] = a0.y; xw[q][2] = a0.z; xw[q][3] = a0.w;
+        xw[q][4] = a1.x; xw[q][5] = a1.y; xw[q][6] = a1.z; xw[q][7] = a1.w;
+    }
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        pc[j] = xw[2][2 + j];
+        y[j] = pc[j];  // unit diagonal
+    }
+#pragma unroll
+    for (int m = 0; m < 12; ++m) {
+        const int k = 13 + m, dy = k / 5 - 2, dx = k % 5 - 2;
+        // halves 4 tx .. 4 tx + 7 of window rows ty + 2 (own) and ty + 2 - dy
+        const unsigned int* wo = &ub[m][urow[0]][2 * tx];
+        const uint2 o01 = *reinterpret_cast<const uint2*>(wo);
+        const uint2 o23 = *reinterpret_cast<const uint2*>(wo + 2);
+        const unsigned int wn[4] = {o01.x, o01.y, o23.x, o23.y};
+        unsigned int wl[4] = {o01.x, o01.y, o23.x, o23.y};
+        if (dy != 0) {
+            const unsigned int* wr = &ub[m][urow[dy]][2 * tx];
+            const uint2 w01 = *reinterpret_cast<const uint2*>(wr);
+            const uint2 w23 = *reinterpret_cast<const uint2*>(wr + 2);
+            wl[0] = w01.x; wl[1] = w01.y; wl[2] = w23.x; wl[3] = w23.y;
+        }
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            // upper entry of this row (half 2 + j): x at (+dy, +dx)
+            const int qo = 2 + j;
+            y[j] = fmaf(h2f<FMT>(wn[qo >> 1] >> (16 * (qo & 1))), xw[2 + dy][2 + dx + j], y[j]);
+            // lower entry: U_k of the row at (-dy, -dx) (half 2 + j - dx): x there
+            const int ql = 2 + j - dx;
+            y[j] = fmaf(h2f<FMT>(wl[ql >> 1] >> (16 * (ql & 1))), xw[2 - dy][2 - dx + j], y[j]);
+        }
+    }
+}
+
+// Tile-parallel form: tiles in row-major order, each workgroup takes every
+// step-th tile of its XCD's chunk and stages the whole window.
 template <int TY, int FMT>
 __global__ __launch_bounds__(kBlock) void k_cgstep_sym(const unsigned short* __restrict__ U, size_t pitch, int W,
                                                        int H, int N, const float* __restrict__ r,
                                                        float* __restrict__ p0, float* __restrict__ p1,
                                                        float* __restrict__ Ap, Scalars* sc, double* __restrict__ dpa,
                                                        const double* __restrict__ dpb, int verify) {
-    using G = Geom2D<4, TY>;
-    constexpr int R = 4, TPR = G::TPR, TX = G::TX, ST = G::STRIDE;
-    constexpr int UR = TY + 2;       // staged U rows: 2 above the tile, then the tile's
-    constexpr int SW = TX / 2 + 2;   // 32-bit words (half pairs) per staged U row: TX + 4 halves
+    using SG = SymGeom<TY>;
+    using G = typename SG::G;
+    constexpr int R = 4, TPR = SG::TPR, ST = SG::ST, UR = SG::UR, SW = SG::SW;
     StepIn in;
     if (!step_begin(sc, dpb, verify, in)) return;
     const float beta = in.beta;
@@ -314,6 +373,11 @@ __global__ __launch_bounds__(kBlock) void k_cgstep_sym(const unsigned short* __r
             L.xr[q] = in ? r[gi] : 0.f;
         }
     };
+    int xrow[5], urow[3];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) xrow[q] = (ty + q) * ST + R * tx;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) urow[d] = ty + 2 - d;
     Loads L;
     if (tr.begin < tr.end) issue(tr.begin, L);
     double v[1] = {0.0};
@@ -352,52 +416,199 @@ __global__ __launch_bounds__(kBlock) void k_cgstep_sym(const unsigned short* __r
         __syncthreads();
         if (t + tr.step < tr.end) issue(t + tr.step, L);
         if (g.valid) {
-            // the thread's x window: rows dy = -2..2, columns 4 tx .. 4 tx + 7 (16-byte aligned)
-            const float* __restrict__ xs = g.xs(lds);
-            float xw[5][8];
-#pragma unroll
-            for (int q = 0; q < 5; ++q) {
-                const f32x4 a0 = *reinterpret_cast<const f32x4*>(xs + q * ST);
-                const f32x4 a1 = *reinterpret_cast<const f32x4*>(xs + q * ST + 4);
-                xw[q][0] = a0.x; xw[q][1] = a0.y; xw[q][2] = a0.z; xw[q][3] = a0.w;
-                xw[q][4] = a1.x; xw[q][5] = a1.y; xw[q][6] = a1.z; xw[q][7] = a1.w;
-            }
             float y[R], pc[R];
-#pragma unroll
-            for (int j = 0; j < R; ++j) {
-                pc[j] = xw[2][2 + j];
-                y[j] = pc[j];  // unit diagonal
-            }
-#pragma unroll
-            for (int m = 0; m < 12; ++m) {
-                const int k = 13 + m, dy = k / 5 - 2, dx = k % 5 - 2;
-                // halves 4 tx .. 4 tx + 7 of window rows ty + 2 (own) and ty + 2 - dy
-                const unsigned int* wo = &ub[m][ty + 2][2 * tx];
-                const uint2 o01 = *reinterpret_cast<const uint2*>(wo);
-                const uint2 o23 = *reinterpret_cast<const uint2*>(wo + 2);
-                const unsigned int wn[4] = {o01.x, o01.y, o23.x, o23.y};
-                unsigned int wl[4] = {o01.x, o01.y, o23.x, o23.y};
-                if (dy != 0) {
-                    const unsigned int* wr = &ub[m][ty + 2 - dy][2 * tx];
-                    const uint2 w01 = *reinterpret_cast<const uint2*>(wr);
-                    const uint2 w23 = *reinterpret_cast<const uint2*>(wr + 2);
-                    wl[0] = w01.x; wl[1] = w01.y; wl[2] = w23.x; wl[3] = w23.y;
-                }
-#pragma unroll
-                for (int j = 0; j < R; ++j) {
-                    // upper entry of this row (half 2 + j): x at (+dy, +dx)
-                    const int qo = 2 + j;
-                    y[j] = fmaf(h2f<FMT>(wn[qo >> 1] >> (16 * (qo & 1))), xw[2 + dy][2 + dx + j], y[j]);
-                    // lower entry: U_k of the row at (-dy, -dx) (half 2 + j - dx): x there
-                    const int ql = 2 + j - dx;
-                    y[j] = fmaf(h2f<FMT>(wl[ql >> 1] >> (16 * (ql & 1))), xw[2 - dy][2 - dx + j], y[j]);
-                }
-            }
+            sym_tile<TY, FMT>(lds, xrow, ub, urow, tx, y, pc);
             store_rows<R>(pnew, g.ib, N, pc);
             store_rows<R>(Ap, g.ib, N, y);
 #pragma unroll
             for (int j = 0; j < R; ++j)
                 if (g.ib + j < N) v[0] += (double)(pc[j] * y[j]);
+        }
+    }
+    step_end(sc, v[0], dpa, in);
+}
+
+// Column-walk form: the tiles are numbered column-major (down each TX-wide
+// column of tiles) and every workgroup takes one contiguous run of them.  Going
+// down a column, the window rows a tile shares with the tile above (4 x rows,
+// 2 U rows) are already in LDS: the x and U windows are row rings, and each
+// tile loads only its own U rows (plus the two side words of each) and the TY x
+// rows below the shared ones.  The first tile of a run, and a tile starting a
+// new column, also load the shared rows.  Every x and U element is then read
+// about once per launch.
+template <int TY, int FMT>
+__global__ __launch_bounds__(kBlock) void k_cgstep_symw(const unsigned short* __restrict__ U, size_t pitch, int W,
+                                                        int H, int N, const float* __restrict__ r,
+                                                        float* __restrict__ p0, float* __restrict__ p1,
+                                                        float* __restrict__ Ap, Scalars* sc,
+                                                        double* __restrict__ dpa, const double* __restrict__ dpb,
+                                                        int verify) {
+    using SG = SymGeom<TY>;
+    using G = typename SG::G;
+    constexpr int R = 4, TPR = SG::TPR, TX = SG::TX, ST = SG::ST, UR = SG::UR, SW = SG::SW, XR = SG::XR;
+    StepIn in;
+    if (!step_begin(sc, dpb, verify, in)) return;
+    const float beta = in.beta;
+    const int par = in.iters & 1;
+    const float* __restrict__ pold = par ? p1 : p0;
+    float* __restrict__ pnew = par ? p0 : p1;
+    __shared__ __attribute__((aligned(16))) float lds[XR * ST];
+    __shared__ __attribute__((aligned(16))) unsigned int ub[12][UR][SW];
+    const int ncol = (W + TX - 1) / TX, nrow = (H + TY - 1) / TY;
+    // Work: one run [ubeg, uend) of the column-major tile order per workgroup.
+    // With at least one workgroup per column on each XCD, the runs form a
+    // grid: each XCD takes a band of tile rows, split into s segments, and its
+    // workgroup l walks column l % ncol of segment l / ncol.  Workgroups of one
+    // segment then walk the same rows side by side on the same XCD, so the side
+    // columns a tile reads from its neighbours' tiles come from that XCD's L2.
+    // Otherwise the column-major order is cut into equal runs.
+    int ubeg, uend;
+    {
+        const int g = gridDim.x / kNumXcd, xcd = blockIdx.x % kNumXcd, l = blockIdx.x / kNumXcd;
+        if (gridDim.x % kNumXcd == 0 && g >= ncol) {
+            const int sg = g / ncol, nseg = kNumXcd * sg;
+            if (l < ncol * sg) {
+                const int col = l % ncol, seg = xcd * sg + l / ncol;
+                ubeg = col * nrow + (int)((long long)seg * nrow / nseg);
+                uend = col * nrow + (int)((long long)(seg + 1) * nrow / nseg);
+            } else {
+                ubeg = uend = 0;
+            }
+        } else {
+            const long long T = (long long)ncol * nrow;
+            ubeg = (int)(T * blockIdx.x / gridDim.x);
+            uend = (int)(T * (blockIdx.x + 1) / gridDim.x);
+        }
+    }
+    const int tx = (int)threadIdx.x % TPR, ty = (int)threadIdx.x / TPR;
+    constexpr int NX = TY * ST;  // new x cells per tile: window rows 4 .. TY + 3
+    constexpr int XE = (NX + kBlock - 1) / kBlock;
+    constexpr int NS = 2 * TY;  // side words of the own U rows: words 0 and SW - 1 of window rows 2 .. TY + 1
+    static_assert(NS <= kBlock, "one side word per thread");
+    struct Loads {
+        Raw<R, FMT> u[12];
+        unsigned int sv[12];
+        float xp[XE], xr[XE];
+    };
+    // tile u: column u / nrow, tile row u % nrow
+    auto origin = [&](int u, int& x0, int& y0) {
+        x0 = (u / nrow) * TX;
+        y0 = (u % nrow) * TY;
+    };
+    auto issue = [&](int u, Loads& L) {
+        int x0, y0;
+        origin(u, x0, y0);
+        const int x = x0 + R * tx, y = y0 + ty;
+        const bool valid = x < W && y < H;
+        const size_t ib = (size_t)y * W + x;
+#pragma unroll
+        for (int m = 0; m < 12; ++m) {
+            if (valid) load_band<R, FMT>(U, (size_t)(13 + m) * pitch + ib, L.u[m]);
+            else L.u[m].d[0] = L.u[m].d[1] = 0u;
+        }
+        {
+            const int e = (int)threadIdx.x;
+            const int wy = 2 + (e >> 1), ww = (e & 1) ? SW - 1 : 0;
+            const long long gi = (long long)(y0 - 2 + wy) * W + (x0 - 2 + 2 * ww);  // even: W % 4 == 0
+            const bool in = e < NS && gi >= 0 && gi < N;
+#pragma unroll
+            for (int m = 0; m < 12; ++m)
+                L.sv[m] = in ? *reinterpret_cast<const unsigned int*>(U + (size_t)(13 + m) * pitch + (size_t)gi) : 0u;
+        }
+#pragma unroll
+        for (int q = 0; q < XE; ++q) {
+            const int e = (int)threadIdx.x + q * kBlock;
+            const int ly = 4 + e / ST, lx = e % ST;
+            const long long gi = (long long)(y0 - 2 + ly) * W + (x0 - 2 + lx);
+            const bool in = e < NX && gi >= 0 && gi < N;
+            L.xp[q] = in ? pold[gi] : 0.f;
+            L.xr[q] = in ? r[gi] : 0.f;
+        }
+    };
+    Loads L;
+    if (ubeg < uend) issue(ubeg, L);
+    int xtop = 0, utop = 0;  // LDS rows of window row 0 (x ring of XR rows, U ring of UR rows)
+    double v[1] = {0.0};
+    for (int u = ubeg; u < uend; ++u) {
+        int x0, y0;
+        origin(u, x0, y0);
+        const bool first = u == ubeg || y0 == 0;
+        __syncthreads();  // the previous tile is done with lds and ub
+        if (first) {
+            // the rows shared with the tile above: U window rows 0, 1 and x
+            // window rows 0 .. 3, read straight into LDS
+            xtop = 0;
+            utop = 0;
+            for (int e = (int)threadIdx.x; e < 2 * SW; e += kBlock) {
+                const int wy = e / SW, ww = e - wy * SW;
+                const long long gi = (long long)(y0 - 2 + wy) * W + (x0 - 2 + 2 * ww);
+                const bool in = gi >= 0 && gi < N;
+#pragma unroll
+                for (int m = 0; m < 12; ++m)
+                    ub[m][wy][ww] =
+                        in ? *reinterpret_cast<const unsigned int*>(U + (size_t)(13 + m) * pitch + (size_t)gi) : 0u;
+            }
+            for (int e = (int)threadIdx.x; e < 4 * ST; e += kBlock) {
+                const int ly = e / ST, lx = e - ly * ST;
+                const long long gi = (long long)(y0 - 2 + ly) * W + (x0 - 2 + lx);
+                const bool in = gi >= 0 && gi < N;
+                const float xp = in ? pold[gi] : 0.f, xr = in ? r[gi] : 0.f;
+                lds[e] = beta != 0.f ? fmaf(beta, xp, -xr) : -xr;
+            }
+        } else {
+            // window rows TY.. of the tile above are rows 0.. of this one
+            xtop = xtop + TY < XR ? xtop + TY : xtop + TY - XR;
+            utop = utop + TY < UR ? utop + TY : utop + TY - UR;
+        }
+        // own U rows (window rows 2 .. TY + 1) and their side words
+        {
+            const int wr = utop + 2 + ty < UR ? utop + 2 + ty : utop + 2 + ty - UR;
+#pragma unroll
+            for (int m = 0; m < 12; ++m) {
+                ub[m][wr][1 + 2 * tx] = L.u[m].d[0];
+                ub[m][wr][2 + 2 * tx] = L.u[m].d[1];
+            }
+            const int e = (int)threadIdx.x;
+            if (e < NS) {
+                const int wy = utop + 2 + (e >> 1), ws = wy < UR ? wy : wy - UR;
+                const int ww = (e & 1) ? SW - 1 : 0;
+#pragma unroll
+                for (int m = 0; m < 12; ++m) ub[m][ws][ww] = L.sv[m];
+            }
+        }
+        // x window rows 4 .. TY + 3
+#pragma unroll
+        for (int q = 0; q < XE; ++q) {
+            const int e = (int)threadIdx.x + q * kBlock;
+            if (e < NX) {
+                const int ly = xtop + 4 + e / ST, lx = e % ST;
+                const int ls = ly < XR ? ly : ly - XR;
+                lds[ls * ST + lx] = beta != 0.f ? fmaf(beta, L.xp[q], -L.xr[q]) : -L.xr[q];
+            }
+        }
+        __syncthreads();
+        if (u + 1 < uend) issue(u + 1, L);
+        const int x = x0 + R * tx, y = y0 + ty;
+        if (x < W && y < H) {
+            int xrow[5], urow[3];
+#pragma unroll
+            for (int q = 0; q < 5; ++q) {
+                const int lr = xtop + ty + q;
+                xrow[q] = (lr < XR ? lr : lr - XR) * ST + R * tx;
+            }
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const int lr = utop + ty + 2 - d;
+                urow[d] = lr < UR ? lr : lr - UR;
+            }
+            float yv[R], pc[R];
+            sym_tile<TY, FMT>(lds, xrow, ub, urow, tx, yv, pc);
+            const int ib = y * W + x;
+            store_rows<R>(pnew, ib, N, pc);
+            store_rows<R>(Ap, ib, N, yv);
+#pragma unroll
+            for (int j = 0; j < R; ++j)
+                if (ib + j < N) v[0] += (double)(pc[j] * yv[j]);
         }
     }
     step_end(sc, v[0], dpa, in);
@@ -1053,6 +1264,7 @@ struct TuneVal {
     int shape, per_cu;
     double us;
     int upd_pc;  // workgroups per CU of the update kernel (0 = default)
+    int walk;    // symmetric 16-bit step: column-walk form (k_cgstep_symw)
 };
 static thread_local std::string g_tune_report;
 static thread_local bool g_tune_spmv = false;  // the last gm_autotune on this thread also tuned the SpMV
@@ -1087,6 +1299,7 @@ struct Solver {
     int g_upd;
     int verify;         // recursive convergence asks for an FP64 check first
     bool sym;           // 16-bit form stored as its 12 upper bands (k_cgstep_sym)
+    int walk;           // ... stepped by the column-walk kernel (k_cgstep_symw)
     double* x64;        // refine mode 2: FP64 master x (x then holds the FP32 partial sum since the last fold)
 };
 
@@ -1178,9 +1391,14 @@ static void launch_scale16(Solver& S, int grid) {
 template <int TY, int FMT>
 static void launch_sym_t(Solver& S) {
     Workspace* w = S.w;
-    hipLaunchKernelGGL((k_cgstep_sym<TY, FMT>), dim3(S.g_step), dim3(kBlock), 0, S.hs, w->A16, w->pitch16, S.plan.W,
-                       S.N / S.plan.W, S.N, w->r, w->p0, w->p1, w->Ap, w->sc, w->dpart, w->dpart + kMaxGrid,
-                       S.verify);
+    if (S.walk)
+        hipLaunchKernelGGL((k_cgstep_symw<TY, FMT>), dim3(S.g_step), dim3(kBlock), 0, S.hs, w->A16, w->pitch16,
+                           S.plan.W, S.N / S.plan.W, S.N, w->r, w->p0, w->p1, w->Ap, w->sc, w->dpart,
+                           w->dpart + kMaxGrid, S.verify);
+    else
+        hipLaunchKernelGGL((k_cgstep_sym<TY, FMT>), dim3(S.g_step), dim3(kBlock), 0, S.hs, w->A16, w->pitch16,
+                           S.plan.W, S.N / S.plan.W, S.N, w->r, w->p0, w->p1, w->Ap, w->sc, w->dpart,
+                           w->dpart + kMaxGrid, S.verify);
 }
 
 template <int FMT>
@@ -1455,6 +1673,7 @@ gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, c
     {
         const TuneVal tv = tune_lookup(S.fmt, S.plan.W, S.N);
         if (S.tiled) set_step_shape(S, tv.shape, tv.per_cu);
+        S.walk = tv.walk;
         if (tv.upd_pc > 0) S.g_upd = grid_for_tiles(S.dv, (S.N + 4L * kBlock - 1) / (4L * kBlock), tv.upd_pc);
     }
     Workspace* w = S.w;
@@ -1742,29 +1961,34 @@ gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_re
     g_tune_spmv = !mixed;
     for (int sh = 0; sh < kNumShapes; ++sh) {
         if (!step_shape_ok(sh, S.fmt, S.plan.W, S.N)) continue;
-        for (int pc : pcs) {
-            set_step_shape(S, sh, pc);
-            double tot = 0;
-            for (int k = 0; k < reps + 1; ++k) {
-                hipLaunchKernelGGL(k_tune_reset, dim3(1), dim3(64), 0, S.hs, w->sc);
-                GM_HIP(hipEventRecord(e0, S.hs));
-                if ((st = launch_step(S)) != GM_OK) return st;
-                GM_HIP(hipEventRecord(e1, S.hs));
-                GM_HIP(hipEventSynchronize(e1));
-                float ms = 0;
-                GM_HIP(hipEventElapsedTime(&ms, e0, e1));
-                if (k > 0) tot += ms;  // first launch warms caches
+        for (int walk = 0; walk < (S.sym ? 2 : 1); ++walk) {
+            for (int pc : pcs) {
+                set_step_shape(S, sh, pc);
+                S.walk = walk;
+                double tot = 0;
+                for (int k = 0; k < reps + 1; ++k) {
+                    hipLaunchKernelGGL(k_tune_reset, dim3(1), dim3(64), 0, S.hs, w->sc);
+                    GM_HIP(hipEventRecord(e0, S.hs));
+                    if ((st = launch_step(S)) != GM_OK) return st;
+                    GM_HIP(hipEventRecord(e1, S.hs));
+                    GM_HIP(hipEventSynchronize(e1));
+                    float ms = 0;
+                    GM_HIP(hipEventElapsedTime(&ms, e0, e1));
+                    if (k > 0) tot += ms;  // first launch warms caches
+                }
+                const double us = tot * 1e3 / reps;
+                snprintf(line, sizeof(line), "cgstep %d %d %d %d %.2f%s\n", sh, kShapeR[sh], kShapeTY[sh], pc, us,
+                         walk ? " column-walk" : "");
+                g_tune_report += line;
+                if (us < best.us) best = TuneVal{sh, pc, us, 0, walk};
             }
-            const double us = tot * 1e3 / reps;
-            snprintf(line, sizeof(line), "cgstep %d %d %d %d %.2f\n", sh, kShapeR[sh], kShapeTY[sh], pc, us);
-            g_tune_report += line;
-            if (us < best.us) best = TuneVal{sh, pc, us, 0};
         }
     }
     // the update kernel's grid: fewer workgroups = fewer device-scope
     // fence + atomic arrivals in its reduction
     // one step so the update kernels below find a hand-off to run
     set_step_shape(S, best.shape, best.per_cu);
+    S.walk = best.walk;
     hipLaunchKernelGGL(k_tune_reset, dim3(1), dim3(64), 0, S.hs, w->sc);
     if ((st = launch_step(S)) != GM_OK) return st;
     S.x = w->d;
@@ -1803,6 +2027,7 @@ gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_re
         out->tile_rows = kShapeTY[best.shape];
         out->wg_per_cu = best.per_cu;
         out->step_us = best.us;
+        out->step_walk = best.walk;
         out->spmv_rows_per_thread = 0;
         out->spmv_tile_rows = 0;
         out->spmv_direct = 0;
@@ -1834,16 +2059,23 @@ gm_status gm_spmv_tune_set(int W, int N, int rows_per_thread, int tile_rows, int
     return spmv_tune_set(W, N, rows_per_thread, tile_rows, direct, wg_per_cu);
 }
 
-gm_status gm_tune_set(int W, int N, int precision, int rows_per_thread, int tile_rows, int wg_per_cu) {
+gm_status gm_tune_set_step(int W, int N, int precision, int rows_per_thread, int tile_rows, int wg_per_cu,
+                           int column_walk) {
     const int fmt = precision == GM_PREC_FP16 ? kF16 : (precision == GM_PREC_BF16 ? kBF16 : kF32);
     int sh = -1;
     for (int k = 0; k < kNumShapes; ++k)
         if (kShapeR[k] == rows_per_thread && kShapeTY[k] == tile_rows) sh = k;
     if (sh < 0 || !step_shape_ok(sh, fmt, W, N) || wg_per_cu < 1 || wg_per_cu > 16)
         return fail(GM_ERR_INVALID, "gm_tune_set: unsupported shape");
+    if (column_walk != 0 && (fmt == kF32 || !sym16_ok(W, N)))
+        return fail(GM_ERR_INVALID, "gm_tune_set_step: the column walk is a form of the symmetric 16-bit step");
     std::lock_guard<std::mutex> g(g_tune_mu);
-    g_tune[TuneKey{fmt, W, (long)N}] = TuneVal{sh, wg_per_cu, 0, 0};
+    g_tune[TuneKey{fmt, W, (long)N}] = TuneVal{sh, wg_per_cu, 0, 0, column_walk != 0};
     return GM_OK;
+}
+
+gm_status gm_tune_set(int W, int N, int precision, int rows_per_thread, int tile_rows, int wg_per_cu) {
+    return gm_tune_set_step(W, N, precision, rows_per_thread, tile_rows, wg_per_cu, 0);
 }
 
 }  // extern "C"

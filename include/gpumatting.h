@@ -176,14 +176,21 @@ typedef struct gm_tune_result_s {
    int update_wg_per_cu;   /* grid of the CG update kernel */
    double update_us;
    int spmv_direct;        /* 1: the plain product's LDS-free form (x read through the caches) */
+   int step_walk;          /* 16-bit symmetric step: 1 = column-walk form (workgroups walk down
+                              columns of tiles and keep the rows shared with the tile above in LDS) */
 } gm_tune_result;
 gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_result* out, gm_stream_t stream);
 /* Text table of every candidate timed by the last gm_autotune on this thread. */
 const char* gm_autotune_report(void);
 /* Pin a shape without timing: (rows_per_thread, tile_rows) one of (2,0) (4,0)
- * (4,4) (4,8) (4,16) for FP32 and (4,0) (8,0) (4,4) (4,8) (4,16) (8,8) (8,16)
- * for 16-bit; 2-D shapes need W % rows_per_thread == 0. */
+ * (4,4) (4,8) (4,16) for FP32 (2-D shapes need W % rows_per_thread == 0); for
+ * 16-bit, (4,4) (4,8) (4,16) when W % 4 == 0 (the symmetric 12-band form),
+ * else (4,0) (8,0). */
 gm_status gm_tune_set(int W, int N, int precision, int rows_per_thread, int tile_rows, int wg_per_cu);
+/* As gm_tune_set, also choosing the symmetric 16-bit step's column-walk form
+ * (column_walk = 1; 16-bit with W % 4 == 0 only). */
+gm_status gm_tune_set_step(int W, int N, int precision, int rows_per_thread, int tile_rows, int wg_per_cu,
+                           int column_walk);
 
 /* Pin the plain product's (gm_bm_ax) form for FP32 Levin matrices of this
  * shape without timing: the staged forms (rows_per_thread, tile_rows) as for

@@ -345,6 +345,32 @@ def test_every_tile_shape_same_iteration(gpu, small, prec, shapes, width):
     call("gm_tune_set", p.W, p.N, prec, *shapes[0], 4)
 
 
+@pytest.mark.parametrize("W,H", [(256, 192), (1916, 1023), (2048, 1030), (4096, 300)])
+@pytest.mark.parametrize("ty", [4, 8, 16])
+def test_column_walk_step_matches_tile_parallel(gpu, W, H, ty):
+    """The column-walk form of the symmetric 16-bit step (rows shared with the
+    tile above kept in LDS rings) computes the same products as the
+    tile-parallel form.  One workgroup per CU, so at the larger sizes every
+    workgroup walks several tiles: down one column in a band of rows when each
+    XCD has a workgroup per column, else along a run of the column-major order
+    that crosses into the next column (4096 wide, 16-row tiles); ragged sizes
+    leave partial tiles at the right and bottom edges.  The
+    per-workgroup partial sums differ in order between the forms, so the
+    iterates agree to rounding, far below what a wrong window row would
+    move them."""
+    p = synth.make_problem(W, H, 1234, 1, 0.08)
+    s = _gpu_system(p)
+    outs = []
+    for walk in (0, 1):
+        call("gm_tune_set_step", W, p.N, PREC_FP16, 4, ty, 1, walk)
+        x, info = matting.solve(s, method=METHOD_PCG, precision=PREC_FP16, tol=0, max_iter=8, verify=False,
+                                refine_every=-1)
+        outs.append(x.numpy())
+    scale = np.abs(outs[0]).max()
+    assert scale > 0
+    assert np.abs(outs[1] - outs[0]).max() <= 1e-5 * scale
+
+
 @pytest.mark.parametrize("width", [256, 250])
 def test_fp16_operator_matches_emulation(gpu, width):
     """The 16-bit scaled operator the solver applies (the symmetric 12-band
