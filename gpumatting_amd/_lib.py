@@ -105,6 +105,16 @@ _SIGS = {
     "gm_batch_info": ([vp, ci, ci, C.POINTER(SolveInfo)], ci),
     "gm_batch_set_grid": ([vp, ci], ci),
     "gm_batch_tiles": ([vp], ci),
+    "gm_vec_copy_to_host": ([vp, vp, ci], ci),
+    "gm_free": ([vp], None),
+    "gm_pgm_read": ([C.c_char_p, C.POINTER(vp), C.POINTER(ci), C.POINTER(ci), C.POINTER(ci)], ci),
+    "gm_ppm_read": ([C.c_char_p, C.POINTER(vp), C.POINTER(ci), C.POINTER(ci), C.POINTER(ci)], ci),
+    "gm_pgm_write": ([C.c_char_p, vp, ci, ci, C.c_char_p, ci], ci),
+    "gm_quantize_u8": ([vp, ci, vp], None),
+    "gm_pgm_write_float": ([C.c_char_p, vp, ci, ci, C.c_char_p, ci], ci),
+    "gm_clamped_mse": ([vp, vp, ci, C.POINTER(cd)], ci),
+    "gm_bm_dump": ([BMp, C.c_char_p], ci),
+    "gm_bm_load": ([BMp, C.c_char_p], ci),
 }
 
 

@@ -2,7 +2,8 @@
 
     python -m gpumatting_amd.build
 
-Each csrc/*.hip is compiled to an object in parallel, then linked.  Objects
+Each csrc/*.hip (hipcc) and host-only csrc/*.cpp (g++) is compiled to an
+object in parallel, then linked.  Objects
 go to gpumatting_amd/_build/ (git-ignored); the .so sits next to this file.
 """
 from __future__ import annotations
@@ -29,8 +30,13 @@ CFLAGS = [
 ]
 
 
+CXX = shutil.which("g++") or "g++"
+CXXFLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-I" + os.path.join(ROOT, "include")]
+
+
 def _compile(src: str, obj: str):
-    cmd = [HIPCC, *CFLAGS, "-c", src, "-o", obj]
+    # device code (.hip) through hipcc; host-only sources (.cpp) through g++
+    cmd = [HIPCC, *CFLAGS, "-c", src, "-o", obj] if src.endswith(".hip") else [CXX, *CXXFLAGS, "-c", src, "-o", obj]
     p = subprocess.run(cmd, capture_output=True, text=True)
     err = "\n".join(l for l in (p.stderr or "").splitlines() if "packed-fp32-ops" not in l)
     if p.returncode != 0:
@@ -41,8 +47,8 @@ def _compile(src: str, obj: str):
 def build_library(verbose: bool = False) -> str:
     bdir = os.path.join(HERE, "_build")
     os.makedirs(bdir, exist_ok=True)
-    srcs = sorted(f for f in os.listdir(CSRC) if f.endswith(".hip"))
-    objs = [os.path.join(bdir, s[:-4] + ".o") for s in srcs]
+    srcs = sorted(f for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
+    objs = [os.path.join(bdir, os.path.splitext(s)[0] + ".o") for s in srcs]
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         futs = [ex.submit(_compile, os.path.join(CSRC, s), o) for s, o in zip(srcs, objs)]
         for s, f in zip(srcs, futs):

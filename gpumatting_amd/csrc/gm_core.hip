@@ -238,4 +238,10 @@ gm_status gm_vec_device_free(float* dx, int left) {
     return GM_OK;
 }
 
+gm_status gm_vec_copy_to_host(float* hx, const float* dx, int length) {
+    if (length < 0 || ((!hx || !dx) && length > 0)) return fail(GM_ERR_INVALID, "gm_vec_copy_to_host: bad argument");
+    if (length > 0) GM_HIP(hipMemcpy(hx, dx, sizeof(float) * length, hipMemcpyDeviceToHost));
+    return GM_OK;
+}
+
 }  // extern "C"

@@ -444,7 +444,6 @@ __global__ __launch_bounds__(kBlock) void k_cgstep_symw(const unsigned short* __
                                                         double* __restrict__ dpa, const double* __restrict__ dpb,
                                                         int verify) {
     using SG = SymGeom<TY>;
-    using G = typename SG::G;
     constexpr int R = 4, TPR = SG::TPR, TX = SG::TX, ST = SG::ST, UR = SG::UR, SW = SG::SW, XR = SG::XR;
     StepIn in;
     if (!step_begin(sc, dpb, verify, in)) return;

@@ -61,6 +61,9 @@ gm_status gm_vec_device_malloc(float** dx, int length, int left_pad, int right_p
 gm_status gm_vec_copy_to_device(float** dx, const float* hx, int length, int left_pad, int right_pad);
 /* vecDeviceFree, Vector.cu:43-46 */
 gm_status gm_vec_device_free(float* dx, int left_pad);
+/* The device-to-host copy the reference's driver makes with cudaMemcpy
+ * (main.cu:283, 303): length floats from dx (past any pad) to hx; synchronous. */
+gm_status gm_vec_copy_to_host(float* hx, const float* dx, int length);
 
 /* ---------------------------------------------------------------------- */
 /* Kernels as host launchers (BandedMatrix.cu, Vector.cu)                   */
@@ -284,6 +287,37 @@ int gm_batch_tiles(gm_batch* batch);
 
 /* Release cached solver workspaces (device memory held between solves). */
 gm_status gm_release_workspaces(void);
+
+/* ---- host file I/O of the matting driver (no device work) ---------------
+ * Netpbm: binary or ASCII, header tokens separated by whitespace with '#'
+ * comments, 8-bit samples (maxval 1..255).  Returned buffers are malloc'd:
+ * release them with gm_free. */
+void gm_free(void* p);
+/* pgmread, ppm.h:32-129: "P5" / "P2" grey image, w*h samples row-major.
+ * maxval may be NULL. */
+gm_status gm_pgm_read(const char* path, unsigned char** data, int* w, int* h, int* maxval);
+/* ppmread, ppm.h:178-278: "P6" / "P3" RGB image, 3*w*h samples row-major.
+ * ppmread_float4 (ppm.h:309-337) normalises each sample by maxval. */
+gm_status gm_ppm_read(const char* path, unsigned char** rgb, int* w, int* h, int* maxval);
+/* pgmwrite, ppm.h:345-418: "P5" (binary = 1) or "P2"; comment line
+ * "# <comment> " when comment is not NULL; maxval 255. */
+gm_status gm_pgm_write(const char* path, const unsigned char* data, int w, int h, const char* comment, int binary);
+/* pgmwrite_float's quantisation, ppm.h:430-451: v < 0 -> 0, v > 1 -> 255,
+ * else (unsigned char)(255.f * v); NaN -> 0. */
+void gm_quantize_u8(const float* v, int n, unsigned char* out);
+/* pgmwrite_float, ppm.h:430-451: quantise as gm_quantize_u8, then pgmwrite. */
+gm_status gm_pgm_write_float(const char* path, const float* data, int w, int h, const char* comment, int binary);
+/* computeError, main.cu:558-577: mean over n pixels of (clamp(alpha) - gt)^2
+ * with alpha > 1 scored as 1 and alpha < 0 as 0; each term in FP32, the sum
+ * in FP64.  Host arrays. */
+gm_status gm_clamped_mse(const float* alpha, const float* gt, int n, double* mse);
+/* dump, main.cu:334-355 ("A.bin"): a HOST matrix's bands one after another,
+ * `rows` FP32 values each (pitch padding dropped), no header. */
+gm_status gm_bm_dump(const BandedMatrix* A, const char* path);
+/* Reader of that format (main.cu:334-355; matlab/laplacianFromBmDump.m reads
+ * it in MATLAB): fills a HOST matrix whose nbands, rows, apitch and a are set;
+ * the file must hold exactly nbands * rows floats; padding is zeroed. */
+gm_status gm_bm_load(BandedMatrix* A, const char* path);
 
 #ifdef __cplusplus
 }

@@ -12,6 +12,8 @@
 #define GPUMATTING_HPP
 
 #include <stddef.h>
+#include <stdio.h>
+#include <stdlib.h>
 
 #include "gpumatting.h"
 
@@ -108,6 +110,104 @@ inline void hostLevinLaplacian(BandedMatrix L, float* b, float lambda, Float4* i
     static_assert(sizeof(Float4) == 4 * sizeof(float), "im must be float4 pixels");
     gpumatting_detail::record(gm_host_levin_laplacian(&L, b, lambda, reinterpret_cast<const float*>(im), scribs, imW,
                                                       imH, imPitch));
+}
+
+// ---- driver I/O (ppm.h, main.cu) ------------------------------------------
+// Like the reference, readers return malloc'd buffers (release with free())
+// or NULL on failure (message in gm_last_error()), writers return 0 / -1.
+
+struct gm_float4 {
+    float x, y, z, w;
+};
+
+// ppm.h:32-129
+inline unsigned char* pgmread(const char* filename, int* w, int* h) {
+    unsigned char* d = nullptr;
+    const gm_status s = gm_pgm_read(filename, &d, w, h, nullptr);
+    gpumatting_detail::record(s);
+    if (s != GM_OK) *w = *h = 0;
+    return s == GM_OK ? d : nullptr;
+}
+// ppm.h:134-149: samples / 255
+inline float* pgmread_float(const char* filename, int* w, int* h) {
+    unsigned char* c = pgmread(filename, w, h);
+    if (!c) return nullptr;
+    const size_t n = (size_t)(*w) * (*h);
+    float* f = (float*)malloc(n * sizeof(float));
+    if (f)
+        for (size_t i = 0; i < n; ++i) f[i] = (float)c[i] / 255.0f;
+    free(c);
+    return f;
+}
+// ppm.h:178-278
+inline unsigned char* ppmread(const char* filename, int* w, int* h, int* maxval) {
+    unsigned char* d = nullptr;
+    const gm_status s = gm_ppm_read(filename, &d, w, h, maxval);
+    gpumatting_detail::record(s);
+    if (s != GM_OK) *w = *h = 0;
+    return s == GM_OK ? d : nullptr;
+}
+// ppm.h:285-302: samples / maxval, 3 per pixel
+inline float* ppmread_float(const char* filename, int* w, int* h) {
+    int maxval = 0;
+    unsigned char* c = ppmread(filename, w, h, &maxval);
+    if (!c) return nullptr;
+    const size_t n = (size_t)(*w) * (*h) * 3;
+    float* f = (float*)malloc(n * sizeof(float));
+    if (f)
+        for (size_t i = 0; i < n; ++i) f[i] = (float)c[i] / (float)maxval;
+    free(c);
+    return f;
+}
+// ppm.h:309-337: RGBA pixels, samples / maxval, alpha 1; the 8-bit samples
+// are handed back in *outcdata when it is not NULL
+template <class Float4 = gm_float4>
+inline Float4* ppmread_float4(unsigned char** outcdata, const char* filename, int* w, int* h) {
+    static_assert(sizeof(Float4) == 4 * sizeof(float), "float4 pixels");
+    int maxval = 0;
+    unsigned char* c = ppmread(filename, w, h, &maxval);
+    if (!c) return nullptr;
+    const size_t n = (size_t)(*w) * (*h);
+    float* f = (float*)malloc(n * 4 * sizeof(float));
+    if (f)
+        for (size_t i = 0; i < n; ++i) {
+            f[4 * i + 0] = (float)c[3 * i + 0] / (float)maxval;
+            f[4 * i + 1] = (float)c[3 * i + 1] / (float)maxval;
+            f[4 * i + 2] = (float)c[3 * i + 2] / (float)maxval;
+            f[4 * i + 3] = 1.0f;
+        }
+    if (outcdata) *outcdata = c;
+    else free(c);
+    return reinterpret_cast<Float4*>(f);
+}
+// ppm.h:345-418
+inline int pgmwrite(const char* filename, int w, int h, const unsigned char* data, const char* comment_string,
+                    int binsave) {
+    const gm_status s = gm_pgm_write(filename, data, w, h, comment_string, binsave);
+    gpumatting_detail::record(s);
+    return s == GM_OK ? 0 : -1;
+}
+// ppm.h:430-451
+inline int pgmwrite_float(const char* filename, int w, int h, const float* data, const char* comment_string,
+                          int binsave) {
+    const gm_status s = gm_pgm_write_float(filename, data, w, h, comment_string, binsave);
+    gpumatting_detail::record(s);
+    return s == GM_OK ? 0 : -1;
+}
+// main.cu:558-577: prints "Ground truth MSE: <mse>"
+inline void computeError(const float* alpha, const float* gtAlpha, int imW, int imH) {
+    double mse = 0.0;
+    const gm_status s = gm_clamped_mse(alpha, gtAlpha, imW * imH, &mse);
+    gpumatting_detail::record(s);
+    if (s == GM_OK) printf("Ground truth MSE: %.3e\n", mse);
+}
+// main.cu:334-355: the row count and band offsets to stderr, then the host
+// matrix to A.bin in the working directory
+inline void dump(BandedMatrix const& bm) {
+    fprintf(stderr, "%d\n", bm.rows);
+    for (int i = 0; i < bm.nbands; ++i) fprintf(stderr, "%d,", bm.bands[i]);
+    printf("\n");
+    gpumatting_detail::record(gm_bm_dump(&bm, "A.bin"));
 }
 
 #endif  // GPUMATTING_HPP

@@ -1,0 +1,112 @@
+"""The `matting` command-line driver end to end on the GPU, with the
+reference's driver semantics (main.cu:106-332): Netpbm in, GPU assembly with
+the reference's rounding, the chosen solver, alpha.pgm / A.bin out and the
+clamped ground-truth MSE printed -- checked against the oracle on a crop of
+the reference's own data/05 images."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from gpumatting_amd import netpbm
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "gpumatting_amd", "cli", "matting")
+GOLDEN = os.path.join(ROOT, "tests", "golden", "data05_crop.npz")
+
+
+@pytest.fixture(scope="module")
+def crop(tmp_path_factory):
+    if not os.path.exists(CLI):
+        pytest.fail(f"{CLI} is not built (run __graft_entry__.build())")
+    d = np.load(GOLDEN)
+    rgb, scribs, gt = d["rgb"], d["scribs"], d["gt"]
+    maxval = int(d["maxval"])
+    H, W = scribs.shape
+    t = tmp_path_factory.mktemp("cli")
+    (t / "im.ppm").write_bytes(f"P6\n{W} {H}\n{maxval}\n".encode() + rgb.tobytes())
+    netpbm.write_pgm(t / "scribs.pgm", scribs)
+    netpbm.write_pgm(t / "gt.pgm", gt)
+    image = rgb.astype(np.float32) / np.float32(maxval)
+    a, b = O.levin_scatter_u8(image, scribs, 1e-2, 1.0)
+    return t, W, H, a, b, gt.astype(np.float32) / np.float32(255)
+
+
+def _run(cwd, *args, check=True):
+    print(f"[cli] matting {' '.join(args)}", flush=True)
+    p = subprocess.run([CLI, *args], cwd=cwd, capture_output=True, text=True, timeout=120)
+    if check and p.returncode != 0:
+        raise AssertionError(f"matting failed ({p.returncode}):\n{p.stdout}\n{p.stderr}")
+    return p
+
+
+def _oracle_alpha(kind, a, b, W, iters):
+    N = b.size
+    bands, pad = O.bands25(W), O.pad_for(W)
+    x0 = np.full(N, 0.5, np.float32)
+    if kind == "cg":
+        return O.cg_ref_f32(a, bands, b, x0, pad, iters, 101)
+    if kind == "grad":
+        return O.gd_ref_f32(a, bands, b, x0, pad, iters)
+    if kind == "cpu-jacobi":
+        return O.jacobi_f32(a, bands, b, x0, pad, iters)
+    return O.gs_f32(a, bands, b, x0, pad, iters)
+
+
+@pytest.mark.parametrize("solver,iters", [("cg", 200), ("grad", 150), ("cpu-jacobi", 60), ("cpu-gauss-seidel", 30)])
+def test_cli_matches_oracle(gpu, crop, solver, iters):
+    t, W, H, a, b, gt = crop
+    for f in ("alpha.pgm", "A.bin"):
+        if (t / f).exists():
+            (t / f).unlink()
+    p = _run(t, solver, str(iters), "im.ppm", "scribs.pgm", "gt.pgm")
+    assert f"Image Size: {W} x {H}" in p.stdout
+    assert "Laplacian generation:" in p.stderr
+    # A.bin: the reference-rounding Laplacian, bit for bit
+    A = np.fromfile(t / "A.bin", np.float32).reshape(25, W * H)
+    assert np.array_equal(A.view(np.uint32), a.view(np.uint32))
+    # the matte, quantised as pgmwrite_float does
+    alpha = _oracle_alpha(solver, a, b, W, iters)
+    got, _ = netpbm.read_pgm(t / "alpha.pgm")
+    want = netpbm.quantize_u8(alpha.reshape(H, W))
+    diff = np.abs(got.astype(int) - want.astype(int))
+    # |d alpha| <= 1e-3 can move a value across one quantisation step
+    assert diff.max() <= 1 and (diff > 0).mean() <= 0.02, (diff.max(), (diff > 0).mean())
+    m = re.search(r"Ground truth MSE: ([0-9.e+-]+)", p.stdout)
+    assert m, p.stdout
+    assert float(m.group(1)) == pytest.approx(O.mse_clamped(alpha.reshape(H, W), gt), rel=2e-2)
+
+
+def test_cli_tolerance_mode(gpu, crop):
+    t, W, H, a, b, gt = crop
+    for prec in ("fp32", "fp16"):
+        p = _run(t, "cg", "5000", "im.ppm", "scribs.pgm", "--tol", "1e-7", "--precision", prec, "--no-dump",
+                 "--out", f"a_{prec}.pgm")
+        m = re.search(r"Jacobi-PCG: (\d+) iterations, relative residual ([0-9.e+-]+)", p.stderr)
+        assert m and float(m.group(2)) <= 1e-7, p.stderr
+        x, _, _ = O.pcg_f64(a.astype(np.float64), O.bands25(W), b.astype(np.float64), np.zeros(W * H), O.pad_for(W),
+                            1e-13)
+        got, _ = netpbm.read_pgm(t / f"a_{prec}.pgm")
+        want = netpbm.quantize_u8(x.astype(np.float32).reshape(H, W))
+        assert np.abs(got.astype(int) - want.astype(int)).max() <= 1
+
+
+def test_cli_errors(gpu, crop, tmp_path):
+    t, W, H, a, b, gt = crop
+    p = _run(t, check=False)
+    assert p.returncode == 0 and "Usage: matting" in p.stderr
+    small = np.zeros((H - 1, W), np.uint8)
+    netpbm.write_pgm(tmp_path / "small.pgm", small)
+    p = _run(t, "cg", "10", "im.ppm", str(tmp_path / "small.pgm"), check=False)
+    assert p.returncode == 1 and "scribbles not the same size as the image" in p.stderr
+    p = _run(t, "cg", "10", "missing.ppm", "scribs.pgm", check=False)
+    assert p.returncode == 1 and "missing.ppm" in p.stderr
+    p = _run(t, "cg", "10", "im.ppm", "scribs.pgm", "--bogus", check=False)
+    assert p.returncode == 1 and "unknown option" in p.stderr
+    p = _run(t, "grad", "10", "im.ppm", "scribs.pgm", "--tol", "1e-6", check=False)
+    assert p.returncode == 1
