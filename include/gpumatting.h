@@ -62,7 +62,7 @@ gm_status gm_vec_copy_to_device(float** dx, const float* hx, int length, int lef
 /* vecDeviceFree, Vector.cu:43-46 */
 gm_status gm_vec_device_free(float* dx, int left_pad);
 /* The device-to-host copy the reference's driver makes with cudaMemcpy
- * (main.cu:283, 303): length floats from dx (past any pad) to hx; synchronous. */
+ * (main.cu:257, 261, 282): length floats from dx (past any pad) to hx; synchronous. */
 gm_status gm_vec_copy_to_host(float* hx, const float* dx, int length);
 
 /* ---------------------------------------------------------------------- */
@@ -132,7 +132,8 @@ typedef struct gm_solve_opts_s {
    double tol;          /* stop when ||b - A x|| / ||b|| <= tol; <= 0: run max_iter */
    int restart;         /* > 0: recompute r = A x - b every `restart` iterations */
    int refine_every;    /* FP64 residual replacement period: > 0 that many iterations; 0 = default
-                           (50 for the 16-bit forms, none for FP32); < 0 = none */
+                           (50 for the 16-bit forms and refine_mode 2, none for plain FP32);
+                           < 0 = none */
    int check_every;     /* iterations enqueued between host status polls (0 -> 16) */
    int verify;          /* 1: confirm convergence with the FP64 true residual */
    int refine_mode;     /* 0: FP32 x, residual replacement; 1: iterative refinement with an FP64

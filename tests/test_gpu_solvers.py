@@ -274,7 +274,8 @@ def test_generic_bands_refine_mode2(gpu):
 
 def test_config_1024_alpha_error_per_mode(gpu, capsys):
     """1024x1024, one ellipse, 8 % constraints: plain CG, Jacobi-PCG, BF16 and
-    FP16 mixed to 1e-6, then FP16 and FP32 with the FP64 master x to 2e-9;
+    FP16 mixed to 1e-6, then FP16 and FP32 with the FP64 master x to 2e-9, and
+    the FP64-master solve to 1e-12 that serves as the 4K benchmark's reference;
     max / mean |alpha - alpha_FP64| against the oracle's FP64 Laplacian + FP64
     PCG (1e-12)."""
     p = synth.config_1k()
@@ -287,7 +288,9 @@ def test_config_1024_alpha_error_per_mode(gpu, capsys):
                                       ("bf16-pcg", METHOD_PCG, PREC_BF16, 1e-6, 0),
                                       ("fp16-pcg", METHOD_PCG, PREC_FP16, 1e-6, 0),
                                       ("fp32-pcg-x64", METHOD_PCG, PREC_FP32, 2e-9, 2),
-                                      ("fp16-pcg-x64", METHOD_PCG, PREC_FP16, 2e-9, 2)]:
+                                      ("fp16-pcg-x64", METHOD_PCG, PREC_FP16, 2e-9, 2),
+                                      # the benchmark's reference alpha, formed the same way at 4K
+                                      ("ref-1e-12", METHOD_PCG, PREC_FP32, 1e-12, 1)]:
         x, info = matting.solve(s, method=meth, precision=prec, tol=tol, max_iter=20000, refine_mode=rm)
         d = np.abs(x.numpy().astype(np.float64) - x64)
         rows.append((name, tol, info.iterations, info.true_relres, d.max(), d.mean()))
@@ -301,8 +304,11 @@ def test_config_1024_alpha_error_per_mode(gpu, capsys):
         if tol >= 1e-6:
             # at 1e-6 the error is the tolerance's, not the precision's
             assert mx <= 2e-2 and mean <= 5e-5, name
-        else:
+        elif tol >= 1e-9:
             assert mx <= 1e-4 and mean <= 1e-6, name
+        else:
+            # the FP32 rounding of the matrix moves alpha by far less than 1e-4
+            assert mx <= 1e-5, name
 
 
 @pytest.mark.parametrize("prec", [PREC_FP32, PREC_FP16])
