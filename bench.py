@@ -120,6 +120,85 @@ def bench_batched(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_grad(args):
+    """16 images of 512x512: loss = ||alpha - alpha_gt||^2, gradients with
+    respect to the RGB pixels, the per-pixel weights and the targets through
+    the autograd wrapper (forward solve, adjoint solve, assembly adjoint), to
+    1e-9 with the FP64 master x.  The CPU baseline computes the FP64 host
+    reference gradient of the first image(s); the GPU gradients are checked
+    against it (relative 2-norm error)."""
+    import numpy as np
+    import torch
+
+    from gpumatting_amd import autograd, synth
+    log("generating the differentiation workload (16 x 512x512)")
+    work = synth.config_grad()
+    tol = 1e-9
+    ims = [torch.from_numpy(p.image).cuda() for p in work]
+    lams = [torch.from_numpy(p.lam).cuda() for p in work]
+    tgts = [torch.from_numpy(p.target).cuda() for p in work]
+    gts = [torch.from_numpy(p.alpha_gt).cuda() for p in work]
+    grads = [None] * len(work)
+
+    def step():
+        for i in range(len(work)):
+            im = ims[i].detach().requires_grad_(True)
+            lam = lams[i].detach().requires_grad_(True)
+            tgt = tgts[i].detach().requires_grad_(True)
+            alpha = autograd.matte(im, lam, tgt, work[i].eps, tol)
+            loss = ((alpha - gts[i]) ** 2).sum()
+            loss.backward()
+            grads[i] = (loss.detach(), im.grad, lam.grad, tgt.grad)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(args.steps):
+        step()
+    e1.record()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) * 1e3 / args.steps
+    ms = e0.elapsed_time(e1) / args.steps
+    log(f"grad: {ms:.1f} ms per 16 images forward+backward (wall {wall:.1f})")
+    cpu = None
+    errs = None
+    if not args.no_cpu:
+        from oracle import pyoracle as O
+        nsample = 1
+        t0 = time.perf_counter()
+        errs = []
+        for i in range(nsample):
+            p = work[i]
+            L64, _, gim, glam, gtg = O.matte_loss_grad_f64(p.image, p.lam, p.target, p.alpha_gt, p.eps)
+            loss, g_im, g_lam, g_tgt = grads[i]
+            rel = {}
+            for name, got, ref in (("rgb", g_im, gim), ("lam", g_lam, glam), ("tgt", g_tgt, gtg)):
+                g = got.detach().cpu().numpy().astype(np.float64).ravel()
+                r = np.asarray(ref, np.float64).ravel()
+                rel[name] = float(np.linalg.norm(g - r) / np.linalg.norm(r))
+            rel["loss"] = float(abs(loss.item() - L64) / L64)
+            errs.append(rel)
+        t_img = (time.perf_counter() - t0) / nsample
+        log(f"cpu FP64 reference gradient: {t_img:.2f} s per image; relative errors {errs}")
+        cpu = {"value": round(t_img * 1e3 * len(work), 1), "unit": "ms per 16 images (extrapolated)",
+               "cores": O.num_threads(), "kind": "port",
+               "sample": f"FP64 host gradient (FP64 assembly, forward and adjoint FP64 PCG to 1e-14, assembly "
+                         f"adjoint) of {nsample} image(s), x {len(work)}; {cpu_model()}"}
+    out = {"metric": "differentiable matte forward+backward ms (16 x 512x512, loss ||alpha - alpha_gt||^2)",
+           "value": round(ms, 2), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(ms, 2), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32 / f64 master x and residual", "data": "synthetic (gpumatting_amd/synth.py config_grad seed 5000)",
+           "config": {"workload": "16 images 512x512, one ellipse, 8 % constraints, eps 1e-7, lambda 100; "
+                                  "forward + adjoint Jacobi-PCG to 1e-9 with the FP64 master x, assembly adjoint",
+                      "images": len(work), "width": 512, "height": 512, "tol": tol},
+           "ms_per_image": round(ms / len(work), 3),
+           "grad_rel_err_vs_fp64": errs, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,8 +208,9 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=40)
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default shapes)")
-    ap.add_argument("--workload", choices=["4k", "batched"], default="4k",
-                    help="4k: the headline 3840x2160 solve; batched: 64 uneven images x 4 right-hand sides")
+    ap.add_argument("--workload", choices=["4k", "batched", "grad"], default="4k",
+                    help="4k: the headline 3840x2160 solve; batched: 64 uneven images x 4 right-hand sides; "
+                         "grad: 16 x 512x512 forward + backward")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a 1 GiB device copy (reference kernel for FETCH_SIZE/WRITE_SIZE scaling)")
     args = ap.parse_args()
@@ -148,6 +228,8 @@ def main():
     lib()
     if args.workload == "batched":
         return bench_batched(args)
+    if args.workload == "grad":
+        return bench_grad(args)
     log("generating the 3840x2160 workload")
     p = synth.config_4k()
     W, H, N = p.W, p.H, p.N

@@ -143,6 +143,12 @@ def config_4k(seed: int = 3000) -> MattingProblem:
     return make_problem(3840, 2160, seed, 3, 0.04)
 
 
+def config_grad(n_images: int = 16, seed: int = 5000, W: int = 512, H: int = 512):
+    """Differentiation workload: n_images W x H problems, one ellipse, 8 %
+    constraints each (seed + image index)."""
+    return [make_problem(W, H, seed + i, 1, 0.08) for i in range(n_images)]
+
+
 def config_batched(n_images: int = 64, nrhs: int = 4, seed: int = 4000, wmin=240, wmax=1080, hmin=320, hmax=1920):
     """Batched workload: n_images images with W ~ U[wmin,wmax], H ~ U[hmin,hmax]
     (one ellipse each), and nrhs independent constraint sets per image with

@@ -27,8 +27,12 @@ def test_assembly_adjoint_vs_oracle(gpu, W, H):
     assert np.allclose(gtg.cpu().numpy(), mu * p.lam.ravel(), rtol=1e-6, atol=1e-6)
 
 
-@pytest.mark.parametrize("W,H,seed", [(48, 40, 1), (96, 64, 2)])
+@pytest.mark.parametrize("W,H,seed", [(48, 40, 1), (96, 64, 2), (512, 512, 5000)])
 def test_autograd_vs_fp64_reference_gradient(gpu, W, H, seed):
+    """Gradients of loss = ||alpha - alpha_gt||^2 w.r.t. the RGB pixels, the
+    per-pixel weights and the targets vs the FP64 host reference gradient
+    (whose adjoint the oracle tests pin against central differences); the
+    512x512 case is the first image of the differentiation workload."""
     p = synth.make_problem(W, H, seed, 1, 0.08)
     im = torch.from_numpy(p.image).cuda().requires_grad_(True)
     lam = torch.from_numpy(p.lam).cuda().requires_grad_(True)
