@@ -11,8 +11,11 @@
 // residual-replacement and hand-off conventions (see Scalars).
 //
 // Work lists.  The step and true-residual kernels walk 64x4-pixel tiles (one
-// pixel per thread, any image width); the update kernel walks 1024-pixel
-// chunks.  Each list concatenates all images.  A launch splits its list into
+// pixel per thread, any image width), numbered down each image's 64-pixel
+// columns; the update kernel walks 1024-pixel chunks.  Each list concatenates
+// all images.  Going down a column, a tile's x window (2-pixel halo) shares 4
+// of its 8 rows with the tile above: the window is an LDS ring, and a
+// workgroup that has just done the tile above loads only the 4 new rows.  A launch splits its list into
 // contiguous ranges, one per workgroup, so a workgroup touches few images: it
 // accumulates its partial sums per (image, system) in registers and stores
 // one slot per image it touches.  The consumer kernel folds an image's slots
@@ -43,6 +46,7 @@ constexpr int kTX = 64, kTY = kBlock / kTX;  // step / true-residual tile: 64 x 
 constexpr int kWX = kTX + 4, kWY = kTY + 4;   // staged window (2-pixel halo)
 constexpr int kWin = kWX * kWY;               // 544 cells
 constexpr int kWinPer = (kWin + kBlock - 1) / kBlock;
+static_assert(kWY == 2 * kTY && (kWY & (kWY - 1)) == 0, "window ring: rows advance by kTY mod kWY");
 constexpr int kChunk = 4 * kBlock;            // update-kernel chunk: 1024 pixels, 4 per thread
 static_assert(kTY * kTX == kBlock, "one pixel per thread");
 
@@ -58,7 +62,7 @@ __device__ __forceinline__ int part_begin(const BPart& P, int b) { return (int)(
 
 struct BImg {
     int W, H, N, sys0, nsys;
-    int t0, nt, ntx;  // step tiles: first index in the tile list, count, tiles per image row
+    int t0, nt, nty;  // step tiles: first index in the tile list, count, tiles per image column
     int c0, nc;       // update chunks
     int wf1, wl1;     // first / last workgroup of the tile partition covering the image
     int wf2, wl2;     // same for the chunk partition
@@ -187,6 +191,8 @@ __global__ __launch_bounds__(kBlock) void kb_step(const BImg* __restrict__ imgs,
     int cur = -1;
     unsigned mask = 0;
     BImg I{};
+    // the window in LDS: origin of its tile (wx0 < 0: none) and ring row of window row 0
+    int wx0 = -1, wy0 = 0, top = 0;
     for (int t = tb; t < te; ++t) {
         const int im = P1.img[t];
         if (im != cur) {
@@ -199,13 +205,18 @@ __global__ __launch_bounds__(kBlock) void kb_step(const BImg* __restrict__ imgs,
             for (int j = 0; j < I.nsys; ++j)
                 if (chand[j].mode == kHandRun) mask |= 1u << j;
             cur = im;
+            wx0 = -1;
         }
         if (mask == 0) continue;
         const int lt = t - I.t0;
-        const int x0 = (lt % I.ntx) * kTX, y0 = (lt / I.ntx) * kTY;
+        const int x0 = (lt / I.nty) * kTX, y0 = (lt % I.nty) * kTY;
         const int x = x0 + lane, y = y0 + row;
         const bool valid = x < I.W && y < I.H;
         const int i = y * I.W + x;
+        // the tile directly above is in LDS: its window rows kTY.. are this one's rows 0..
+        const bool carry = x0 == wx0 && y0 == wy0 + kTY;
+        top = carry ? (top + kTY) & (kWY - 1) : 0;
+        const int e0 = carry ? kTY * kWX : 0;  // first window cell to load
         // every global load of the tile first, then LDS
         float a[24];
         if (valid) load_pixel_bands<FMT>(I, i, a);
@@ -218,7 +229,7 @@ __global__ __launch_bounds__(kBlock) void kb_step(const BImg* __restrict__ imgs,
             const float* __restrict__ pold = (chand[j].iters & 1) ? S.p1 : S.p0;
 #pragma unroll
             for (int q = 0; q < kWinPer; ++q) {
-                const int e = (int)threadIdx.x + q * kBlock;
+                const int e = e0 + (int)threadIdx.x + q * kBlock;
                 const int wy = e / kWX, wx = e - wy * kWX;
                 // matrix-row index of the window cell (cells beside the image alias the neighbouring
                 // row, exactly like the row-wise product; their band entries are 0)
@@ -237,25 +248,34 @@ __global__ __launch_bounds__(kBlock) void kb_step(const BImg* __restrict__ imgs,
             if (!((mask >> j) & 1u)) continue;
 #pragma unroll
             for (int q = 0; q < kWinPer; ++q) {
-                const int e = (int)threadIdx.x + q * kBlock;
-                if (e < kWin) win[j][e] = sv[j][q];
+                const int e = e0 + (int)threadIdx.x + q * kBlock;
+                if (e < kWin) {
+                    const int wy = e / kWX;
+                    win[j][((top + wy) & (kWY - 1)) * kWX + (e - wy * kWX)] = sv[j][q];
+                }
             }
         }
         __syncthreads();
+        wx0 = x0;
+        wy0 = y0;
         if (!valid) continue;
+        // LDS offsets of the thread's 5 window rows (cell (row + q, lane) is pixel (y - 2 + q, x - 2))
+        int rb[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) rb[q] = ((top + row + q) & (kWY - 1)) * kWX + lane;
 #pragma unroll
         for (int j = 0; j < kMaxRhs; ++j) {
             if (!((mask >> j) & 1u)) continue;
             const BSys S = sys[I.sys0 + j];
-            const float* __restrict__ w = &win[j][row * kWX + lane];  // cell (row, lane) is pixel (y - 2, x - 2)
+            const float* __restrict__ w = win[j];
             float yv = 0.f;
 #pragma unroll
             for (int k = 0, kk = 0; k < 25; ++k) {
                 if (k == 12) continue;
-                yv = fmaf(a[kk], w[(k / 5) * kWX + (k % 5)], yv);
+                yv = fmaf(a[kk], w[rb[k / 5] + (k % 5)], yv);
                 ++kk;
             }
-            const float pc = w[2 * kWX + 2];
+            const float pc = w[rb[2] + 2];
             yv = fmaf(S.d[i], pc, yv);
             float* __restrict__ pnew = (chand[j].iters & 1) ? S.p0 : S.p1;
             pnew[i] = pc;
@@ -422,6 +442,7 @@ __global__ __launch_bounds__(kBlock) void kb_trueres(const BImg* __restrict__ im
     int cur = -1;
     unsigned mask = 0, dmask = 0;
     BImg I{};
+    int wx0 = -1, wy0 = 0, top = 0;  // window in LDS, as in kb_step
     for (int t = tb; t < te; ++t) {
         const int im = P1.img[t];
         if (im != cur) {
@@ -434,13 +455,17 @@ __global__ __launch_bounds__(kBlock) void kb_trueres(const BImg* __restrict__ im
                 if (op == 1 && s->verify == 0) dmask |= 1u << j;  // periodic replacement: measure the drift
             }
             cur = im;
+            wx0 = -1;
         }
         if (mask == 0) continue;
         const int lt = t - I.t0;
-        const int x0 = (lt % I.ntx) * kTX, y0 = (lt / I.ntx) * kTY;
+        const int x0 = (lt / I.nty) * kTX, y0 = (lt % I.nty) * kTY;
         const int x = x0 + lane, y = y0 + row;
         const bool valid = x < I.W && y < I.H;
         const int i = y * I.W + x;
+        const bool carry = x0 == wx0 && y0 == wy0 + kTY;
+        top = carry ? (top + kTY) & (kWY - 1) : 0;
+        const int e0 = carry ? kTY * kWX : 0;
         float a[24];
         if (valid) load_pixel_bands<kF32>(I, i, a);
         float sv[kMaxRhs][kWinPer];
@@ -450,7 +475,7 @@ __global__ __launch_bounds__(kBlock) void kb_trueres(const BImg* __restrict__ im
             const BSys S = sys[I.sys0 + j];
 #pragma unroll
             for (int q = 0; q < kWinPer; ++q) {
-                const int e = (int)threadIdx.x + q * kBlock;
+                const int e = e0 + (int)threadIdx.x + q * kBlock;
                 const int wy = e / kWX, wx = e - wy * kWX;
                 const long long g = (long long)(y0 - 2 + wy) * I.W + (x0 - 2 + wx);
                 sv[j][q] = (e < kWin && g >= 0 && g < I.N) ? S.x[g] : 0.f;
@@ -462,26 +487,34 @@ __global__ __launch_bounds__(kBlock) void kb_trueres(const BImg* __restrict__ im
             if (!((mask >> j) & 1u)) continue;
 #pragma unroll
             for (int q = 0; q < kWinPer; ++q) {
-                const int e = (int)threadIdx.x + q * kBlock;
-                if (e < kWin) win[j][e] = sv[j][q];
+                const int e = e0 + (int)threadIdx.x + q * kBlock;
+                if (e < kWin) {
+                    const int wy = e / kWX;
+                    win[j][((top + wy) & (kWY - 1)) * kWX + (e - wy * kWX)] = sv[j][q];
+                }
             }
         }
         __syncthreads();
+        wx0 = x0;
+        wy0 = y0;
         if (!valid) continue;
+        int rb[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) rb[q] = ((top + row + q) & (kWY - 1)) * kWX + lane;
 #pragma unroll
         for (int j = 0; j < kMaxRhs; ++j) {
             if (!((mask >> j) & 1u)) continue;
             const BSys S = sys[I.sys0 + j];
-            const float* __restrict__ w = &win[j][row * kWX + lane];
+            const float* __restrict__ w = win[j];
             double s = 0.0;
 #pragma unroll
             for (int k = 0, kk = 0; k < 25; ++k) {
                 if (k == 12) continue;
-                s = fma((double)a[kk], (double)w[(k / 5) * kWX + (k % 5)], s);
+                s = fma((double)a[kk], (double)w[rb[k / 5] + (k % 5)], s);
                 ++kk;
             }
             const float bi = S.b[i];
-            const double tq = fma((double)S.d[i], (double)w[2 * kWX + 2], s) - (double)bi;
+            const double tq = fma((double)S.d[i], (double)w[rb[2] + 2], s) - (double)bi;
             const float out = (float)tq;
             acc[j][0] += (double)(out * (S.dinv[i] * out));
             acc[j][1] += tq * tq;
@@ -707,8 +740,8 @@ gm_status gm_batch_create(gm_batch** out, int nimages, const int* widths, const 
         I.N = N;
         I.sys0 = i * nrhs;
         I.nsys = nrhs;
-        I.ntx = (W + kTX - 1) / kTX;
-        I.nt = I.ntx * ((H + kTY - 1) / kTY);
+        I.nty = (H + kTY - 1) / kTY;
+        I.nt = I.nty * ((W + kTX - 1) / kTX);
         I.t0 = (int)b->p1.img.size();
         I.nc = (N + kChunk - 1) / kChunk;
         I.c0 = (int)b->p2.img.size();
