@@ -74,6 +74,7 @@ struct BImg {
 
 struct BSys {
     float *x, *r, *p0, *p1, *Ap, *d, *dinv, *b;
+    float* z;  // D^-1 r, kept beside r by every kernel that writes r
 };
 
 // Sum an image's slots: slot[(b * K + im - wimg[b]) * stride + off] over the
@@ -178,8 +179,9 @@ __device__ __forceinline__ void load_pixel_bands(const BImg& I, int i, float (&a
 
 // K1: per running system p = -D^-1 r + beta p_old (staged), Ap = (L_off +
 // diag d) p, <p, Ap> partials.
+// (at most 128 VGPRs: 4 waves per SIMD; the FP32 form otherwise takes 129)
 template <int FMT>
-__global__ __launch_bounds__(kBlock) void kb_step(const BImg* __restrict__ imgs, const BSys* __restrict__ sys,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void kb_step(const BImg* __restrict__ imgs, const BSys* __restrict__ sys,
                                                   Scalars* sc, BPart P1, BPart P2, double* __restrict__ slot1,
                                                   const double* __restrict__ slot2, int verify) {
     __shared__ float win[kMaxRhs][kWin];
@@ -236,7 +238,7 @@ __global__ __launch_bounds__(kBlock) void kb_step(const BImg* __restrict__ imgs,
                 const long long g = (long long)(y0 - 2 + wy) * I.W + (x0 - 2 + wx);
                 float v = 0.f;
                 if (e < kWin && g >= 0 && g < I.N) {
-                    const float z = S.dinv[g] * S.r[g];
+                    const float z = S.z[g];
                     v = beta != 0.f ? fmaf(beta, pold[g], -z) : -z;
                 }
                 sv[j][q] = v;
@@ -369,21 +371,26 @@ __global__ __launch_bounds__(kBlock) void kb_update(const BImg* __restrict__ img
                 float xs[4] = {xv.x, xv.y, xv.z, xv.w}, rs[4] = {rv.x, rv.y, rv.z, rv.w};
                 const float ps[4] = {pv.x, pv.y, pv.z, pv.w}, as[4] = {av.x, av.y, av.z, av.w};
                 const float ds[4] = {dv.x, dv.y, dv.z, dv.w};
+                float zs[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     xs[q] = fmaf(al, ps[q], xs[q]);
                     rs[q] = fmaf(al, as[q], rs[q]);
-                    acc[j][0] += (double)(rs[q] * (ds[q] * rs[q]));
+                    zs[q] = ds[q] * rs[q];
+                    acc[j][0] += (double)(rs[q] * zs[q]);
                     acc[j][1] += (double)(rs[q] * rs[q]);
                 }
                 *reinterpret_cast<f32x4*>(S.x + i0) = f32x4{xs[0], xs[1], xs[2], xs[3]};
                 *reinterpret_cast<f32x4*>(S.r + i0) = f32x4{rs[0], rs[1], rs[2], rs[3]};
+                *reinterpret_cast<f32x4*>(S.z + i0) = f32x4{zs[0], zs[1], zs[2], zs[3]};
             } else {
                 for (int i = i0; i < I.N; ++i) {
                     S.x[i] = fmaf(al, p[i], S.x[i]);
                     const float rn = fmaf(al, S.Ap[i], S.r[i]);
+                    const float zn = S.dinv[i] * rn;
                     S.r[i] = rn;
-                    acc[j][0] += (double)(rn * (S.dinv[i] * rn));
+                    S.z[i] = zn;
+                    acc[j][0] += (double)(rn * zn);
                     acc[j][1] += (double)(rn * rn);
                 }
             }
@@ -516,14 +523,18 @@ __global__ __launch_bounds__(kBlock) void kb_trueres(const BImg* __restrict__ im
             const float bi = S.b[i];
             const double tq = fma((double)S.d[i], (double)w[rb[2] + 2], s) - (double)bi;
             const float out = (float)tq;
-            acc[j][0] += (double)(out * (S.dinv[i] * out));
+            const float zo = S.dinv[i] * out;
+            acc[j][0] += (double)(out * zo);
             acc[j][1] += tq * tq;
             if ((dmask >> j) & 1u) {
                 const double dd = (double)out - (double)S.r[i];
                 acc[j][2] += dd * dd;
             }
             acc[j][3] += (double)bi * (double)bi;
-            if (op != 0) S.r[i] = out;
+            if (op != 0) {
+                S.r[i] = out;
+                S.z[i] = zo;
+            }
         }
     }
     if (cur >= 0) flush_slot<4>(acc, slot3 + ((size_t)b * P1.K + (cur - P1.wimg[b])) * 4 * kMaxRhs);
@@ -754,7 +765,7 @@ gm_status gm_batch_create(gm_batch** out, int nimages, const int* widths, const 
         l16off[i] = L16n;
         if (precision == GM_PREC_FP16) L16n += 25 * pitch;
         voff[i] = vfloats;
-        vfloats += (size_t)nrhs * 8 * pitch;
+        vfloats += (size_t)nrhs * 9 * pitch;
         b->himg.push_back(I);
     }
     hipError_t e = hipSuccess;
@@ -779,7 +790,7 @@ gm_status gm_batch_create(gm_batch** out, int nimages, const int* widths, const 
         I.L = b->Lmem + loff[i];
         I.L16 = L16n ? b->L16mem + l16off[i] : nullptr;
         for (int j = 0; j < nrhs; ++j) {
-            float* base = b->vmem + voff[i] + (size_t)j * 8 * I.pitch;
+            float* base = b->vmem + voff[i] + (size_t)j * 9 * I.pitch;
             BSys S;
             S.x = base;
             S.r = base + 1 * I.pitch;
@@ -789,6 +800,7 @@ gm_status gm_batch_create(gm_batch** out, int nimages, const int* widths, const 
             S.d = base + 5 * I.pitch;
             S.dinv = base + 6 * I.pitch;
             S.b = base + 7 * I.pitch;
+            S.z = base + 8 * I.pitch;
             b->hsys.push_back(S);
         }
     }
