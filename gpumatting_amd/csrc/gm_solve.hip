@@ -436,8 +436,9 @@ __global__ __launch_bounds__(kBlock) void k_cgstep_sym(const unsigned short* __r
 // rows below the shared ones.  The first tile of a run, and a tile starting a
 // new column, also load the shared rows.  Every x and U element is then read
 // about once per launch.
+// Held to 168 VGPRs: 3 workgroups per CU (LDS allows 3 at 4-row tiles)
 template <int TY, int FMT>
-__global__ __launch_bounds__(kBlock) void k_cgstep_symw(const unsigned short* __restrict__ U, size_t pitch, int W,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_cgstep_symw(const unsigned short* __restrict__ U, size_t pitch, int W,
                                                         int H, int N, const float* __restrict__ r,
                                                         float* __restrict__ p0, float* __restrict__ p1,
                                                         float* __restrict__ Ap, Scalars* sc,
@@ -483,10 +484,10 @@ __global__ __launch_bounds__(kBlock) void k_cgstep_symw(const unsigned short* __
     constexpr int NX = TY * ST;  // new x cells per tile: window rows 4 .. TY + 3
     constexpr int XE = (NX + kBlock - 1) / kBlock;
     constexpr int NS = 2 * TY;  // side words of the own U rows: words 0 and SW - 1 of window rows 2 .. TY + 1
-    static_assert(NS <= kBlock, "one side word per thread");
+    constexpr int SE = (12 * NS + kBlock - 1) / kBlock;  // side words per thread (band e / NS, word e % NS)
     struct Loads {
         Raw<R, FMT> u[12];
-        unsigned int sv[12];
+        unsigned int sv[SE];
         float xp[XE], xr[XE];
     };
     // tile u: column u / nrow, tile row u % nrow
@@ -505,14 +506,13 @@ __global__ __launch_bounds__(kBlock) void k_cgstep_symw(const unsigned short* __
             if (valid) load_band<R, FMT>(U, (size_t)(13 + m) * pitch + ib, L.u[m]);
             else L.u[m].d[0] = L.u[m].d[1] = 0u;
         }
-        {
-            const int e = (int)threadIdx.x;
-            const int wy = 2 + (e >> 1), ww = (e & 1) ? SW - 1 : 0;
-            const long long gi = (long long)(y0 - 2 + wy) * W + (x0 - 2 + 2 * ww);  // even: W % 4 == 0
-            const bool in = e < NS && gi >= 0 && gi < N;
 #pragma unroll
-            for (int m = 0; m < 12; ++m)
-                L.sv[m] = in ? *reinterpret_cast<const unsigned int*>(U + (size_t)(13 + m) * pitch + (size_t)gi) : 0u;
+        for (int q = 0; q < SE; ++q) {
+            const int e = (int)threadIdx.x + q * kBlock, m = e / NS, f = e - m * NS;
+            const int wy = 2 + (f >> 1), ww = (f & 1) ? SW - 1 : 0;
+            const long long gi = (long long)(y0 - 2 + wy) * W + (x0 - 2 + 2 * ww);  // even: W % 4 == 0
+            const bool in = e < 12 * NS && gi >= 0 && gi < N;
+            L.sv[q] = in ? *reinterpret_cast<const unsigned int*>(U + (size_t)(13 + m) * pitch + (size_t)gi) : 0u;
         }
 #pragma unroll
         for (int q = 0; q < XE; ++q) {
@@ -567,12 +567,15 @@ __global__ __launch_bounds__(kBlock) void k_cgstep_symw(const unsigned short* __
                 ub[m][wr][1 + 2 * tx] = L.u[m].d[0];
                 ub[m][wr][2 + 2 * tx] = L.u[m].d[1];
             }
-            const int e = (int)threadIdx.x;
-            if (e < NS) {
-                const int wy = utop + 2 + (e >> 1), ws = wy < UR ? wy : wy - UR;
-                const int ww = (e & 1) ? SW - 1 : 0;
 #pragma unroll
-                for (int m = 0; m < 12; ++m) ub[m][ws][ww] = L.sv[m];
+            for (int q = 0; q < SE; ++q) {
+                const int e = (int)threadIdx.x + q * kBlock;
+                if (e < 12 * NS) {
+                    const int m = e / NS, f = e - m * NS;
+                    const int wy = utop + 2 + (f >> 1), ws = wy < UR ? wy : wy - UR;
+                    const int ww = (f & 1) ? SW - 1 : 0;
+                    ub[m][ws][ww] = L.sv[q];
+                }
             }
         }
         // x window rows 4 .. TY + 3
