@@ -1739,8 +1739,16 @@ gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, c
             if ((st = launch_resid(S, 1)) != GM_OK) return st;
         }
         long since_restart = 0;
+        // Chunk size: `chunk` until the residual's decay rate (from the last
+        // two polls) predicts convergence within the iterations already
+        // queued; then small chunks, so that few launches find the solve
+        // already stopped.
+        int next = chunk;
+        double rr_a = -1.0, rr_b = -1.0;
+        long it_a = 0, it_b = 0;
         while (!finished) {
-            for (int j = 0; j < chunk && enq < o->max_iter; ++j) {
+            const int this_chunk = next;
+            for (int j = 0; j < this_chunk && enq < o->max_iter; ++j) {
                 if ((st = launch_cg_iter(S)) != GM_OK) return st;
                 ++enq;
                 ++since_restart;
@@ -1758,10 +1766,29 @@ gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, c
             if (polls > 0) {
                 GM_HIP(hipEventSynchronize(w->ev[(polls - 1) & 1]));
                 const Scalars& h = w->hsc[(polls - 1) & 1];
-                if (h.done != kRun) finished = true;
-                else if (h.verify) {
-                    // recursive residual reached the tolerance: check the true one
+                if (h.done != kRun) {
+                    finished = true;
+                } else if (h.verify) {
+                    // recursive residual reached the tolerance: check the true
+                    // one, and wait for the verdict before queueing more
                     if ((st = launch_trueres(S, 1)) != GM_OK) return st;
+                    GM_HIP(hipMemcpyAsync(&w->hsc[polls & 1], w->sc, sizeof(Scalars), hipMemcpyDeviceToHost, S.hs));
+                    GM_HIP(hipStreamSynchronize(S.hs));
+                    if (w->hsc[polls & 1].done != kRun) finished = true;
+                    rr_a = rr_b = -1.0;
+                    next = chunk;
+                } else if (h.tol2 > 0 && h.rr > 0) {
+                    rr_a = rr_b;
+                    it_a = it_b;
+                    rr_b = h.rr;
+                    it_b = h.iters;
+                    next = chunk;
+                    if (rr_a > 0 && rr_b < rr_a && it_b > it_a) {
+                        const double rate = log(rr_b / rr_a) / (double)(it_b - it_a);  // < 0 per iteration
+                        const double left = log(h.tol2 / rr_b) / rate - (double)(enq - it_b);
+                        if (left < chunk) next = left < 1.0 ? 2 : (int)ceil(left) + 1;
+                        if (next > chunk) next = chunk;
+                    }
                 }
             }
             ++polls;
