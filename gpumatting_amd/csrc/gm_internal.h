@@ -615,8 +615,8 @@ struct Geom2D {
         valid = x < W && y < H;
         ib = y * W + x;
     }
-    template <class F>
-    __device__ __forceinline__ void stage(float* __restrict__ lds, int W, int H, int N, F f) const {
+    template <class T, class F>
+    __device__ __forceinline__ void stage(T* __restrict__ lds, int W, int H, int N, F f) const {
         constexpr int NE = (TY + 4) * STRIDE;
         for (int e = threadIdx.x; e < NE; e += kBlock) {
             const int ly = e / STRIDE, lx = e - ly * STRIDE;
@@ -624,10 +624,11 @@ struct Geom2D {
             // image alias the neighbouring image row exactly as the 1-D
             // product does, so both geometries compute the same A x
             const long long g = (long long)(y0 - 2 + ly) * W + (x0 - 2 + lx);
-            lds[e] = (g >= 0 && g < N) ? f((int)g) : 0.f;
+            lds[e] = (g >= 0 && g < N) ? f((int)g) : T(0);
         }
     }
-    __device__ __forceinline__ const float* xs(const float* lds) const {
+    template <class T>
+    __device__ __forceinline__ const T* xs(const T* lds) const {
         return lds + ((int)threadIdx.x / TPR) * STRIDE + R * ((int)threadIdx.x % TPR);
     }
     __device__ __forceinline__ float center(const float* lds, int j) const {

@@ -800,6 +800,71 @@ __global__ __launch_bounds__(kBlock) void k_trueres25(const float* __restrict__ 
     });
 }
 
+// k_trueres25 on 2-D pixel tiles (Geom2D: the FP32 step's staging, which
+// keeps the x halo in L2); the same sums in the same order.
+template <class G, int PM, class XT>
+__global__ __launch_bounds__(kBlock) void k_trueres2d(const float* __restrict__ A, size_t pitch, int W, int H, int N,
+                                                      const XT* __restrict__ x, const float* __restrict__ b,
+                                                      const float* __restrict__ dinv, float* __restrict__ r, int op,
+                                                      Scalars* sc, double* partials, unsigned int* counter) {
+    if (op != 0 && sc->done != kRun) return;
+    constexpr int R = G::R, ST = G::STRIDE;
+    __shared__ __attribute__((aligned(16))) XT lds[G::LDS_FLOATS];
+    const TileRange tr = xcd_tiles(G::ntiles(W, H, N));
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int t = tr.begin; t < tr.end; t += tr.step) {
+        G g;
+        g.at(t, W, H, N);
+        __syncthreads();
+        int nz = 0;
+        g.stage(lds, W, H, N, [&](int i) {
+            const XT xv = x[i];
+            nz |= xv != XT(0);
+            return xv;
+        });
+        // a window of zeros (a zero initial guess) needs no band loads: A x = 0
+        nz = __syncthreads_or(nz);
+        if (!g.valid) continue;
+        double acc[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) acc[j] = 0.0;
+        if (nz) {
+            const XT* xs = g.xs(lds);
+#pragma unroll
+            for (int k = 0; k < 25; ++k) {
+                Raw<R, kF32> a;
+                load_band<R, kF32>(A, (size_t)k * pitch + (size_t)g.ib, a);
+                const int s = k / 5, dx = k % 5;
+#pragma unroll
+                for (int j = 0; j < R; ++j) acc[j] = fma((double)a.get(j), (double)xs[s * ST + j + dx], acc[j]);
+            }
+        }
+        float bb[R], dd[R], out[R], ro[R];
+        load_rows<R>(b, g.ib, N, bb);
+        if (PM != kPlain) load_rows<R>(dinv, g.ib, N, dd);
+        if (op == 1) load_rows<R>(r, g.ib, N, ro);
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const double tj = acc[j] - (double)bb[j];
+            float rj = (float)tj;
+            if (PM == kScaled) rj = (float)(tj * (double)dd[j]);
+            out[j] = rj;
+            if (g.ib + j < N) {
+                v[1] += tj * tj;
+                if (PM == kJacobi) v[0] += (double)(rj * (dd[j] * rj));
+                else v[0] += (double)(rj * rj);
+                if (op == 1) v[2] += (double)(rj - ro[j]) * (double)(rj - ro[j]);
+            }
+        }
+        if (op != 0) store_rows<R>(r, g.ib, N, out);
+    }
+    grid_reduce<3>(v, partials, counter, [&](double* tt) {
+        sc->g0 = tt[1];
+        if (op == 0) return;
+        fin_residual(sc, tt[0], tt[1], op == 2, true, tt[2], PM == kScaled ? tt[0] : tt[1]);
+    });
+}
+
 // Mixed-precision operator: As = S A S with S = diag(1/sqrt(a_ii)) (unit
 // diagonal, band 12 not stored), rounded to FP16 or BF16; s written to svec.
 // SYM: only the upper bands 13..24 (the symmetric step kernel's form).
@@ -1329,7 +1394,23 @@ static gm_status launch_resid(Solver& S, int start) {
 
 template <class XT>
 static void launch_trueres_t(Solver& S, const XT* x, int op) {
-    if (S.tiled) {
+    using G = Geom2D<4, 4>;
+    if (S.tiled && shape_ok(kS2R4Y4, kF32, S.plan.W, S.N)) {
+        // 2 workgroups per CU: its grid-wide reduction fences once per workgroup
+        const int grid = grid_for_tiles(S.dv, shape_tiles(kS2R4Y4, S.plan.W, S.N), 2);
+        if (S.pm == kJacobi)
+            hipLaunchKernelGGL((k_trueres2d<G, kJacobi, XT>), dim3(grid), dim3(kBlock), 0, S.hs, S.A->a, S.A->apitch,
+                               S.plan.W, S.N / S.plan.W, S.N, x, S.b, S.w->dinv, S.w->r, op, S.w->sc, S.rs.partials,
+                               S.rs.counter);
+        else if (S.pm == kScaled)
+            hipLaunchKernelGGL((k_trueres2d<G, kScaled, XT>), dim3(grid), dim3(kBlock), 0, S.hs, S.A->a, S.A->apitch,
+                               S.plan.W, S.N / S.plan.W, S.N, x, S.b, S.w->dinv, S.w->r, op, S.w->sc, S.rs.partials,
+                               S.rs.counter);
+        else
+            hipLaunchKernelGGL((k_trueres2d<G, kPlain, XT>), dim3(grid), dim3(kBlock), 0, S.hs, S.A->a, S.A->apitch,
+                               S.plan.W, S.N / S.plan.W, S.N, x, S.b, S.w->dinv, S.w->r, op, S.w->sc, S.rs.partials,
+                               S.rs.counter);
+    } else if (S.tiled) {
         if (S.pm == kJacobi)
             hipLaunchKernelGGL((k_trueres25<4, kJacobi, XT>), dim3(S.g_tile4), dim3(kBlock), 0, S.hs, S.A->a,
                                S.A->apitch, S.plan.W, S.N, x, S.b, S.w->dinv, S.w->r, op, S.w->sc, S.rs.partials,
