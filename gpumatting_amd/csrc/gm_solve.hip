@@ -1344,7 +1344,7 @@ static TuneVal tune_lookup(int fmt, int W, long N) {
     if (it != g_tune.end()) return it->second;
     // defaults: the shapes the tuner picks for the 3840x2160 Levin matrix
     if (fmt == kF32) return shape_ok(kS2R4Y4, fmt, W, N) ? TuneVal{kS2R4Y4, 2, 0, 0} : TuneVal{kS1R4, 4, 0, 0};
-    return sym16_ok(W, N) ? TuneVal{kS2R4Y8, 4, 0, 0} : TuneVal{kS1R8, 3, 0, 0};
+    return sym16_ok(W, N) ? TuneVal{kS2R4Y4, 3, 0, 0, 1} : TuneVal{kS1R8, 3, 0, 0, 0};
 }
 
 struct Solver {
