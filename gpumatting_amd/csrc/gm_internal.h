@@ -343,6 +343,19 @@ __device__ __forceinline__ TileRange xcd_tiles(int ntiles) {
     return t;
 }
 
+// Raw buffer loads with the hardware's range check: an element outside
+// [0, n) -- a negative index wraps past n -- reads as 0 without touching
+// memory, so window staging needs no branch per cell.  (gfx9 buffer resource:
+// stride 0, num_records in bytes, dword 3 = 0x00020000.)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, long long n_floats) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)(n_floats * 4), 0x00020000);
+}
+// element i (i may be negative or >= n; then 0); `skip` forces 0
+__device__ __forceinline__ float buf_ld(__amdgpu_buffer_rsrc_t r, long long i, bool skip = false) {
+    const unsigned off = skip ? 0xFFFFFFFFu : (unsigned)i * 4u;
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));

@@ -490,6 +490,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         unsigned int sv[SE];
         float xp[XE], xr[XE];
     };
+    const __amdgpu_buffer_rsrc_t bp = buf_rsrc(pold, N), br = buf_rsrc(r, N);
     // tile u: column u / nrow, tile row u % nrow
     auto origin = [&](int u, int& x0, int& y0) {
         x0 = (u / nrow) * TX;
@@ -519,9 +520,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
             const int e = (int)threadIdx.x + q * kBlock;
             const int ly = 4 + e / ST, lx = e % ST;
             const long long gi = (long long)(y0 - 2 + ly) * W + (x0 - 2 + lx);
-            const bool in = e < NX && gi >= 0 && gi < N;
-            L.xp[q] = in ? pold[gi] : 0.f;
-            L.xr[q] = in ? r[gi] : 0.f;
+            L.xp[q] = buf_ld(bp, gi, e >= NX);
+            L.xr[q] = buf_ld(br, gi, e >= NX);
         }
     };
     Loads L;
@@ -550,8 +550,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
             for (int e = (int)threadIdx.x; e < 4 * ST; e += kBlock) {
                 const int ly = e / ST, lx = e - ly * ST;
                 const long long gi = (long long)(y0 - 2 + ly) * W + (x0 - 2 + lx);
-                const bool in = gi >= 0 && gi < N;
-                const float xp = in ? pold[gi] : 0.f, xr = in ? r[gi] : 0.f;
+                const float xp = buf_ld(bp, gi), xr = buf_ld(br, gi);
                 lds[e] = beta != 0.f ? fmaf(beta, xp, -xr) : -xr;
             }
         } else {
@@ -1474,7 +1473,8 @@ static void launch_scale16(Solver& S, int grid) {
 template <int TY, int FMT>
 static void launch_sym_t(Solver& S) {
     Workspace* w = S.w;
-    if (S.walk)
+    // the column walk's range-checked window loads address vectors of < 2^28 floats
+    if (S.walk && S.N < (1 << 28))
         hipLaunchKernelGGL((k_cgstep_symw<TY, FMT>), dim3(S.g_step), dim3(kBlock), 0, S.hs, w->A16, w->pitch16,
                            S.plan.W, S.N / S.plan.W, S.N, w->r, w->p0, w->p1, w->Ap, w->sc, w->dpart,
                            w->dpart + kMaxGrid, S.verify);
