@@ -122,13 +122,16 @@ def bench_batched(args):
 
 def bench_grad(args):
     """16 images of 512x512: loss = ||alpha - alpha_gt||^2, gradients with
-    respect to the RGB pixels, the per-pixel weights and the targets through
-    the autograd wrapper (forward solve, adjoint solve, assembly adjoint), to
-    1e-9 with the FP64 master x.  The CPU baseline computes the FP64 host
-    reference gradient of the first image(s); the GPU gradients are checked
-    against it (relative 2-norm error)."""
+    respect to the RGB pixels, the per-pixel weights and the targets (forward
+    solve, adjoint solve, assembly adjoint -- the calls the autograd wrapper
+    makes, without building a graph), to 1e-9 with the FP64 master x, images
+    spread over --grad-threads streams.  The CPU baseline computes the FP64
+    host reference gradient of the first image(s); the GPU gradients are
+    checked against it (relative 2-norm error)."""
     import numpy as np
     import torch
+
+    import concurrent.futures as cf
 
     from gpumatting_amd import autograd, synth
     log("generating the differentiation workload (16 x 512x512)")
@@ -139,16 +142,26 @@ def bench_grad(args):
     tgts = [torch.from_numpy(p.target).cuda() for p in work]
     gts = [torch.from_numpy(p.alpha_gt).cuda() for p in work]
     grads = [None] * len(work)
+    # A 512x512 solve fills a fraction of the GPU: images run concurrently,
+    # each worker thread on its own stream (the library's workspaces and
+    # reductions are per stream, and its calls release the GIL).
+    nthreads = max(1, args.grad_threads)
+    streams = [torch.cuda.Stream() for _ in range(nthreads)]
+
+    def one(k):
+        with torch.cuda.stream(streams[k]):
+            for i in range(k, len(work), nthreads):
+                loss, _, g_im, g_lam, g_tgt = autograd.matte_loss_and_grad(ims[i], lams[i], tgts[i], gts[i],
+                                                                          work[i].eps, tol)
+                grads[i] = (loss, g_im, g_lam, g_tgt)
+        streams[k].synchronize()
+
+    pool = cf.ThreadPoolExecutor(nthreads)
 
     def step():
-        for i in range(len(work)):
-            im = ims[i].detach().requires_grad_(True)
-            lam = lams[i].detach().requires_grad_(True)
-            tgt = tgts[i].detach().requires_grad_(True)
-            alpha = autograd.matte(im, lam, tgt, work[i].eps, tol)
-            loss = ((alpha - gts[i]) ** 2).sum()
-            loss.backward()
-            grads[i] = (loss.detach(), im.grad, lam.grad, tgt.grad)
+        list(pool.map(one, range(nthreads)))
+
+    torch.cuda.synchronize()  # inputs were copied on the default stream
 
     for _ in range(args.warmup):
         step()
@@ -193,7 +206,7 @@ def bench_grad(args):
            "dtype": "f32 / f64 master x and residual", "data": "synthetic (gpumatting_amd/synth.py config_grad seed 5000)",
            "config": {"workload": "16 images 512x512, one ellipse, 8 % constraints, eps 1e-7, lambda 100; "
                                   "forward + adjoint Jacobi-PCG to 1e-9 with the FP64 master x, assembly adjoint",
-                      "images": len(work), "width": 512, "height": 512, "tol": tol},
+                      "images": len(work), "width": 512, "height": 512, "tol": tol, "streams": nthreads},
            "ms_per_image": round(ms / len(work), 3),
            "grad_rel_err_vs_fp64": errs, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
@@ -211,6 +224,8 @@ def main():
     ap.add_argument("--workload", choices=["4k", "batched", "grad"], default="4k",
                     help="4k: the headline 3840x2160 solve; batched: 64 uneven images x 4 right-hand sides; "
                          "grad: 16 x 512x512 forward + backward")
+    ap.add_argument("--grad-threads", type=int, default=8,
+                    help="grad workload: images processed concurrently, one stream per worker thread")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a 1 GiB device copy (reference kernel for FETCH_SIZE/WRITE_SIZE scaling)")
     args = ap.parse_args()

@@ -51,3 +51,21 @@ class LevinMatte(torch.autograd.Function):
 
 def matte(image, lam, tgt, eps=1e-7, tol=1e-8, max_iter=20000, refine_mode=2):
     return LevinMatte.apply(image, lam, tgt, eps, tol, max_iter, refine_mode)
+
+
+def matte_loss_and_grad(image, lam, tgt, alpha_gt, eps=1e-7, tol=1e-8, max_iter=20000, refine_mode=2):
+    """loss = ||alpha - alpha_gt||^2 and its gradients with respect to the
+    image, lam and tgt without an autograd graph: the same forward solve,
+    adjoint solve and assembly adjoint LevinMatte runs, on the calling thread's
+    current stream.  Returns (loss, alpha (H, W), d_image (H, W, C), d_lam,
+    d_tgt)."""
+    H, W = image.shape[:2]
+    s = matting.assemble(image, lam, tgt, eps, _lib.ASSEMBLY_ACCURATE)
+    alpha, _ = _solve(s, s.b, tol, max_iter, refine_mode)
+    r = alpha - alpha_gt.reshape(-1).to(alpha.dtype)
+    loss = torch.dot(r, r)
+    mu, _ = _solve(s, 2.0 * r, tol, max_iter, refine_mode)
+    gim, glam, gtg = matting.levin_backward(image, mu, alpha, lam, tgt, eps)
+    gi = torch.zeros_like(image)
+    gi[..., :3] = gim.to(image.dtype)
+    return loss, alpha.view(H, W), gi, glam.view_as(lam).to(lam.dtype), gtg.view_as(tgt).to(tgt.dtype)

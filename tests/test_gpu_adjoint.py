@@ -47,3 +47,26 @@ def test_autograd_vs_fp64_reference_gradient(gpu, W, H, seed):
         g = got.detach().cpu().numpy().astype(np.float64)
         rel = np.linalg.norm(g - ref) / np.linalg.norm(ref)
         assert rel <= 1e-4, rel
+
+
+def test_functional_gradient_equals_autograd(gpu):
+    """matte_loss_and_grad makes the autograd wrapper's calls without a graph:
+    the same loss and gradients, also from a side stream."""
+    p = synth.make_problem(96, 64, 9, 1, 0.08)
+    im = torch.from_numpy(p.image).cuda().requires_grad_(True)
+    lam = torch.from_numpy(p.lam).cuda().requires_grad_(True)
+    tgt = torch.from_numpy(p.target).cuda().requires_grad_(True)
+    gt = torch.from_numpy(p.alpha_gt).cuda()
+    alpha = autograd.matte(im, lam, tgt, 1e-7, 1e-9)
+    loss = ((alpha - gt) ** 2).sum()
+    loss.backward()
+    st = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    with torch.cuda.stream(st):
+        out = autograd.matte_loss_and_grad(im.detach(), lam.detach(), tgt.detach(), gt, 1e-7, 1e-9)
+    st.synchronize()
+    l2, a2, g_im, g_lam, g_tgt = out
+    assert torch.equal(a2, alpha.detach())
+    assert abs(l2.item() - loss.item()) <= 1e-6 * loss.item()
+    for got, ref in ((g_im, im.grad), (g_lam, lam.grad), (g_tgt, tgt.grad)):
+        assert torch.equal(got, ref)
