@@ -1,6 +1,6 @@
 """ctypes binding of libgpumatting.so (the C ABI declared in include/gpumatting.h).
 
-The shared library is built in-tree by __graft_entry__.build() (hipcc,
+The shared library is built in-tree by `python -m gpumatting_amd.build` (hipcc,
 gfx950).  Importing this module does not touch the GPU; the first call that
 needs the library loads it and raises GMError if it is missing -- there is no
 CPU fallback.
@@ -127,7 +127,7 @@ def lib():
     global _LIB
     if _LIB is None:
         if not os.path.exists(LIB_PATH):
-            raise GMError(7, f"{LIB_PATH} is not built; run __graft_entry__.build()")
+            raise GMError(7, f"{LIB_PATH} is not built; run python -m gpumatting_amd.build")
         L = C.CDLL(LIB_PATH)
         for name, (args, res) in _SIGS.items():
             f = getattr(L, name)

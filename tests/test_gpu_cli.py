@@ -23,7 +23,7 @@ GOLDEN = os.path.join(ROOT, "tests", "golden", "data05_crop.npz")
 @pytest.fixture(scope="module")
 def crop(tmp_path_factory):
     if not os.path.exists(CLI):
-        pytest.fail(f"{CLI} is not built (run __graft_entry__.build())")
+        pytest.fail(f"{CLI} is not built (make -C gpumatting_amd/cli)")
     d = np.load(GOLDEN)
     rgb, scribs, gt = d["rgb"], d["scribs"], d["gt"]
     maxval = int(d["maxval"])
