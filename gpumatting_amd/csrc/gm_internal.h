@@ -110,6 +110,16 @@ struct Hand {
 };
 enum { kHandRun = 0, kHandStop = 1, kHandIdle = 2 };
 
+// Hand-off between launches of the fused CG iteration (one slot per launch
+// parity: a launch reads slot seq & 1 and writes slot (seq + 1) & 1).
+struct FHand {
+    double rr_prev;  // <r,r> of the residual before the one the state holds
+    float beta;      // beta of the newest direction
+    int iters;       // iterations completed (updates of x)
+    int npart;       // workgroup partial sums left
+    int pad_;
+};
+
 // The two CG kernels have no grid-wide reduction of their own: each
 // workgroup stores its partial sums and the NEXT kernel folds them (every
 // workgroup, in the same fixed order, so all agree bit for bit).  The kernel
@@ -141,6 +151,7 @@ struct Scalars {
     int npart;       // number of those partial sums
     unsigned int pad2_;
     Hand hand;
+    FHand fh[2];
 };
 
 // ---------------------------------------------------------------------------
@@ -347,6 +358,11 @@ __device__ __forceinline__ TileRange xcd_tiles(int ntiles) {
 // [0, n) -- a negative index wraps past n -- reads as 0 without touching
 // memory, so window staging needs no branch per cell.  (gfx9 buffer resource:
 // stride 0, num_records in bytes, dword 3 = 0x00020000.)
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, long long n_floats) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)(n_floats * 4), 0x00020000);
 }
@@ -355,9 +371,23 @@ __device__ __forceinline__ float buf_ld(__amdgpu_buffer_rsrc_t r, long long i, b
     const unsigned off = skip ? 0xFFFFFFFFu : (unsigned)i * 4u;
     return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
 }
+// elements i .. i + 3 := v (i and n multiples of 4); `skip` drops the store
+// (an out-of-range buffer store writes nothing).  Kernels that store this
+// way issue the same memory instructions on every path, so the compiler can
+// wait for a load without waiting for the stores issued after it.
+__device__ __forceinline__ void buf_st4(__amdgpu_buffer_rsrc_t r, long long i, f32x4 v, bool skip = false) {
+    const unsigned off = skip ? 0x80000000u : (unsigned)i * 4u;
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, (int)off, 0, 0);
+}
+// cache policy of a buffer load that streams data read once (the NT bit)
+constexpr int kNonTemporal = 2;
+// elements i .. i + 3 (i and n multiples of 4, so the four are all in range or
+// all 0); `skip` forces 0 with an offset far past any vector of < 2^28 floats
+__device__ __forceinline__ f32x4 buf_ld4(__amdgpu_buffer_rsrc_t r, long long i, bool skip = false) {
+    const unsigned off = skip ? 0x80000000u : (unsigned)i * 4u;
+    return __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0));
+}
 
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
@@ -407,9 +437,6 @@ struct Raw<R, kBF16> {
         return __uint_as_float((j % 2) ? (d[j / 2] & 0xffff0000u) : (d[j / 2] << 16));
     }
 };
-
-typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 // Load R consecutive entries of one band starting at element i (16-byte
 // aligned for FP32 / 16-bit R=8, 8-byte for 16-bit R=4) with a non-temporal

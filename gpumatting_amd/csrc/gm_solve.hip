@@ -615,6 +615,420 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
     step_end(sc, v[0], dpa, in);
 }
 
+// Fused CG iteration on the symmetric 16-bit operator (column walk, 4-row
+// tiles).  One launch finishes iteration j and makes the step of j + 1:
+//   update j   x += alpha_j s p_j,  r_{j+1} = r_j + alpha_j Ap_j
+//   step j+1   p_{j+1} = -r_{j+1} + beta_{j+1} p_j,  Ap_{j+1} = A p_{j+1}
+// alpha_j = <r_j,r_j> / <p_j,Ap_j> uses the previous launch's sums, and so
+// does beta_{j+1} = <r_{j+1},r_{j+1}> / <r_j,r_j> through
+//   <r_{j+1},r_{j+1}> = <r_j,r_j> + 2 alpha_j <r_j,Ap_j> + alpha_j^2 <Ap_j,Ap_j>
+// (the product expanded).  Every launch also sums the exact <r,r> of the
+// residual it stores, which the next expansion starts from, so rounding does
+// not accumulate across iterations.  Sums per workgroup: <p,Ap>, <r,r>,
+// <r,Ap>, <Ap,Ap> in the scaled space and the unscaled ||r/s||^2 (stopping
+// test), all in FP64.
+// x and r follow k_cgupdate's rounding exactly.  r and Ap are read at window
+// cells of neighbouring tiles while their owners write the new ones, so both
+// alternate between two buffers by iteration parity, as p does.
+// Window loads: each thread loads the 4 interior cells of one new window row,
+// the row it outputs now (threads of tile rows 2, 3) or in the next tile
+// (rows 0, 1, held in registers until then), so r_{j+1} and p_j of its own
+// cells never have to be read twice.
+// mode 0: continue from the previous launch (slot seq & 1); 1: first step
+// after a residual from scratch (state from the Scalars, no update: the step
+// j of the current residual); 2: timing (as 0 with alpha = beta = 0.5, no
+// sums folded, no stop test).  The loads of tile u + 1 are in flight while
+// tile u is computed; two workgroups per CU (the tuner's choice: a third, or a
+// second tile of loads in flight, measured slower at 4K).
+// The side words of the symmetric 16-bit operator's column walk, gathered
+// once per operator: for every column of TX-wide tiles, image row and upper
+// band, the two 32-bit words (half pairs) just left (columns x0 - 2, x0 - 1)
+// and right (x0 + TX, x0 + TX + 1) of the column, at
+// E[((column * H + row) * 12 + band) * 2 + side], 0 outside the vector (the
+// indexing of the window cells: rows wrap at the image edge).  A tile's 96
+// side words are then 384 contiguous bytes instead of 96 cache lines that the
+// neighbouring columns' tiles also read.
+__global__ __launch_bounds__(kBlock) void k_uedge(const unsigned short* __restrict__ U, size_t pitch, int W, int H,
+                                                  int N, int TX, unsigned int* __restrict__ E) {
+    const int ncol = (W + TX - 1) / TX;
+    const long long total = (long long)ncol * H * 24;
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < total; i += (long long)gridDim.x * kBlock) {
+        const int rem = (int)(i % 24), m = rem >> 1, side = rem & 1;
+        const long long cy = i / 24;
+        const int y = (int)(cy % H), c = (int)(cy / H);
+        const long long gi = (long long)y * W + (side ? c * TX + TX : c * TX - 2);  // even: W % 4 == 0
+        E[i] = (gi >= 0 && gi < N) ? *reinterpret_cast<const unsigned int*>(U + (size_t)(13 + m) * pitch + gi) : 0u;
+    }
+}
+
+// Vectors of fewer than kFusedMaxN elements: the descriptor over the 12 upper
+// bands spans 24 x pitch bytes, which must stay below 2^31.
+constexpr int kFusedMaxN = 1 << 26;
+template <int FMT>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void k_cgfused_symw(
+    const unsigned short* __restrict__ U, size_t pitch, int W, int H, int N, const float* __restrict__ svec,
+    float* __restrict__ x, float* r0, float* r1, float* p0, float* p1, float* Ap0, float* Ap1,
+    const unsigned int* __restrict__ uedge, Scalars* sc, double* __restrict__ dpf, int seq, int mode, int verify) {
+    constexpr int TY = 4;
+    using SG = SymGeom<TY>;
+    constexpr int R = 4, TPR = SG::TPR, TX = SG::TX, ST = SG::ST, UR = SG::UR, SW = SG::SW, XR = SG::XR;
+    static_assert(TPR == 64 && XR == 2 * TY, "one wave per tile row; the x ring holds two tiles of rows");
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (sc->done != kRun || sc->verify) return;
+    const bool first = mode == 1;
+    FHand hin;
+    if (first) {
+        hin.rr_prev = sc->rz_prev;
+        hin.beta = sc->beta;
+        hin.iters = sc->iters;
+        hin.npart = 0;
+    } else if (mode == 2) {
+        hin.rr_prev = 1.0;
+        hin.beta = 0.5f;
+        hin.iters = 0;
+        hin.npart = 0;
+    } else {
+        hin = sc->fh[seq & 1];
+    }
+    const int j = hin.iters;
+    const int k = first ? j : j + 1;  // the step this launch makes
+    const float* rin = (j & 1) ? r1 : r0;
+    float* rout = (k & 1) ? r1 : r0;
+    const float* Apin = (j & 1) ? Ap1 : Ap0;
+    float* Apout = (k & 1) ? Ap1 : Ap0;
+    const float* pold = (k & 1) ? p1 : p0;
+    float* pnew = (k & 1) ? p0 : p1;
+    __shared__ __attribute__((aligned(16))) float lds[XR * ST];
+    __shared__ __attribute__((aligned(16))) unsigned int ub[12][UR][SW];
+    // tile rows 0, 1: r_{j+1} and p_j of the next tile's output cells, held
+    // from the tile that loaded them (each thread its own slot)
+    __shared__ f32x4 held[2][2 * TPR];
+    const int ncol = (W + TX - 1) / TX, nrow = (H + TY - 1) / TY;
+    int ubeg, uend;  // the run of column-major tiles of this workgroup (as k_cgstep_symw)
+    {
+        const int g = gridDim.x / kNumXcd, xcd = blockIdx.x % kNumXcd, l = blockIdx.x / kNumXcd;
+        if (gridDim.x % kNumXcd == 0 && g >= ncol) {
+            const int sg = g / ncol, nseg = kNumXcd * sg;
+            if (l < ncol * sg) {
+                const int col = l % ncol, seg = xcd * sg + l / ncol;
+                ubeg = col * nrow + (int)((long long)seg * nrow / nseg);
+                uend = col * nrow + (int)((long long)(seg + 1) * nrow / nseg);
+            } else {
+                ubeg = uend = 0;
+            }
+        } else {
+            const long long T = (long long)ncol * nrow;
+            ubeg = (int)(T * blockIdx.x / gridDim.x);
+            uend = (int)(T * (blockIdx.x + 1) / gridDim.x);
+        }
+    }
+    const int tx = (int)threadIdx.x % TPR, ty = (int)threadIdx.x / TPR;
+    // this thread's window row among the 4 a tile loads: 4 + lrow (image row
+    // y0 + 2 + lrow); tile rows 2, 3 load their own output rows (now), rows
+    // 0, 1 the next tile's
+    const int lrow = (ty + 2) & 3;
+    const bool now = ty >= 2;
+    constexpr int NS = 24 * TY;  // side words of the own U rows (row, band, side)
+    constexpr int SE = (NS + kBlock - 1) / kBlock;
+    struct Loads {
+        Raw<R, FMT> u[12];
+        unsigned int sv[SE];
+        f32x4 xr, xa, xp;  // r_j, Ap_j, p_j of the thread's 4 interior window cells
+        float cr, ca, cp;  // ... of one side cell (threads 0 .. 15)
+        f32x4 xo, so;      // x and s of the thread's output cells
+    };
+    // every load and store goes through a range-checked buffer descriptor
+    // (out-of-range: loads 0, stores dropped), so every path issues the same
+    // memory instructions and the wait for a tile's loads never has to wait
+    // for the stores of the tile before
+    const __amdgpu_buffer_rsrc_t brr = buf_rsrc(rin, N), bra = buf_rsrc(Apin, N), brp = buf_rsrc(pold, N),
+                                 brx = buf_rsrc(x, N), brs = buf_rsrc(svec, N), bwr = buf_rsrc(rout, N),
+                                 bwp = buf_rsrc(pnew, N), bwa = buf_rsrc(Apout, N);
+    // one range-checked descriptor over the 12 upper bands (24 pitch bytes <
+    // 2^31): the band loads share one offset register, the band is the
+    // scalar offset
+    const __amdgpu_buffer_rsrc_t bu =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<unsigned short*>(U + 13 * pitch), 0, (int)(24 * pitch), 0x00020000);
+    const __amdgpu_buffer_rsrc_t be = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<unsigned int*>(uedge), 0, (int)(4LL * ((W + TX - 1) / TX) * H * 24), 0x00020000);
+    auto origin = [&](int u, int& x0, int& y0) {
+        x0 = (u / nrow) * TX;
+        y0 = (u % nrow) * TY;
+    };
+    // side cell e (< 16) of window rows w0 .. w0 + 3: row w0 + e / 4, column
+    // 0, 1, TX + 2 or TX + 3
+    auto side_gi = [&](int x0, int y0, int w0) {
+        const int e = (int)threadIdx.x, c = e & 3;
+        return (long long)(y0 - 2 + w0 + (e >> 2)) * W + (x0 - 2 + (c < 2 ? c : TX + c));
+    };
+    auto side_lx = [&]() {
+        const int c = (int)threadIdx.x & 3;
+        return c < 2 ? c : TX + c;
+    };
+    // the loads of tile u (none: the same instructions, all out of range, so
+    // that every pass of the loop issues as many)
+    auto issue = [&](int u, Loads& L, bool none) {
+        int x0, y0;
+        origin(u, x0, y0);
+        const int cx = x0 + R * tx, cy = y0 + ty;
+        const bool valid = cx < W && cy < H && !none;
+        const size_t ib = (size_t)cy * W + cx;
+        const unsigned uoff = valid ? 2u * (unsigned)ib : 0x80000000u;
+#pragma unroll
+        for (int m = 0; m < 12; ++m) {
+            const u32x2 w2 = __builtin_amdgcn_raw_buffer_load_b64(bu, (int)uoff, (int)(2 * m * pitch), kNonTemporal);
+            L.u[m].d[0] = w2.x;
+            L.u[m].d[1] = w2.y;
+        }
+        // side words of the own rows: 96 contiguous words of the edge array
+        // (rows of the image only; a row below it feeds no output cell)
+#pragma unroll
+        for (int q = 0; q < SE; ++q) {
+            const int e = (int)threadIdx.x + q * kBlock;
+            const bool in = e < NS && y0 + e / 24 < H && !none;
+            const unsigned off = in ? 4u * ((unsigned)((x0 / TX) * H + y0) * 24u + (unsigned)e) : 0x80000000u;
+            L.sv[q] = __builtin_amdgcn_raw_buffer_load_b32(be, (int)off, 0, 0);
+        }
+        L.xo = buf_ld4(brx, (long long)ib, !valid || first);
+        L.so = buf_ld4(brs, (long long)ib, !valid);
+        const long long gi = (long long)(y0 + 2 + lrow) * W + cx;
+        L.xr = buf_ld4(brr, gi, none);
+        L.xa = buf_ld4(bra, gi, first || none);
+        L.xp = buf_ld4(brp, gi, none);
+        const long long gs = side_gi(x0, y0, 4);
+        const bool sk = threadIdx.x >= 16 || none;
+        L.cr = buf_ld(brr, gs, sk);
+        L.ca = buf_ld(bra, gs, sk || first);
+        L.cp = buf_ld(brp, gs, sk);
+    };
+    // software pipeline: the loads of tile u + 1 are issued while tile u is
+    // computed (the first tile's while the previous launch's sums are folded)
+    Loads L;
+    if (ubeg < uend) issue(ubeg, L, false);
+    float alpha = 0.f, beta = hin.beta;
+    FHand hout{hin.rr_prev, hin.beta, j, (int)gridDim.x, 0};
+    if (mode == 2) {
+        alpha = 0.5f;
+    } else if (!first) {
+        double t[5];
+        fold_all<5>(dpf + (size_t)(seq & 1) * 5 * kMaxGrid, hin.npart, t);
+        const double pAp = t[0], rr = t[1], rAp = t[2], ApAp = t[3], rru = t[4];
+        int stop = kRun, ver = 0;
+        if (!isfinite(pAp) || !isfinite(rr) || !isfinite(rAp) || !isfinite(ApAp) || !isfinite(rru)) {
+            stop = kNonFinite;
+        } else if (sc->tol2 >= 0 && rru <= sc->tol2) {
+            if (verify) ver = 1;
+            else stop = kConverged;
+        } else if (j >= sc->maxit) {
+            stop = kMaxIt;
+        } else if (!(pAp > 0)) {
+            stop = kBreakdown;
+        }
+        if (stop != kRun || ver) {
+            if (lead) {
+                sc->rz = rr;
+                sc->rz_prev = hin.rr_prev;
+                sc->rr = rru;
+                sc->beta = hin.beta;
+                sc->pAp = pAp;
+                sc->iters = j;
+                sc->pend = 0;
+                sc->verify = ver;
+                sc->done = stop;
+            }
+            return;
+        }
+        alpha = (float)(rr / pAp);
+        const double a = (double)alpha, rr1 = rr + 2.0 * a * rAp + a * a * ApAp;
+        beta = rr1 > 0 ? (float)(rr1 / rr) : 0.f;
+        hout.rr_prev = rr;
+        hout.beta = beta;
+        hout.iters = j + 1;
+    }
+    // r_{j+1} and p_{j+1} of a loaded cell
+    auto advance = [&](float rv, float av, float pv, float& rn) {
+        rn = first ? rv : fmaf(alpha, av, rv);
+        return beta != 0.f ? fmaf(beta, pv, -rn) : -rn;
+    };
+    int xtop = 0, utop = 0;  // LDS rows of window row 0 (x ring of XR rows, U ring of UR rows)
+    double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    // The first tile's loads are complete on every path into the loop, so the
+    // compiler's wait for a tile's loads in the loop only has to cover the
+    // stores issued after them (vmcnt(0) alone: expcnt and lgkmcnt at their
+    // maxima).
+    __builtin_amdgcn_s_waitcnt(0xF70);
+    for (int u = ubeg; u < uend; ++u) {
+        int x0, y0;
+        origin(u, x0, y0);
+        const bool start = u == ubeg || y0 == 0;
+        __syncthreads();  // the previous tile is done with lds and ub
+        if (start) {
+            // the rows shared with the tile above: U window rows 0, 1 and x
+            // window rows 0 .. 3 (as the tile above would have loaded them)
+            xtop = 0;
+            utop = 0;
+            for (int e = (int)threadIdx.x; e < 2 * SW; e += kBlock) {
+                const int wy = e / SW, ww = e - wy * SW;
+                const long long gi = (long long)(y0 - 2 + wy) * W + (x0 - 2 + 2 * ww);
+                const bool in = gi >= 0 && gi < N;
+#pragma unroll
+                for (int m = 0; m < 12; ++m)
+                    ub[m][wy][ww] =
+                        in ? *reinterpret_cast<const unsigned int*>(U + (size_t)(13 + m) * pitch + (size_t)gi) : 0u;
+            }
+            {
+                const long long gi = (long long)(y0 - 2 + lrow) * W + (x0 + R * tx);
+                const f32x4 a = buf_ld4(brr, gi), b = buf_ld4(bra, gi, first), c = buf_ld4(brp, gi);
+                float rn[R], pn[R];
+#pragma unroll
+                for (int q = 0; q < R; ++q) pn[q] = advance(a[q], b[q], c[q], rn[q]);
+                float* d = lds + lrow * ST + 2 + R * tx;
+                *reinterpret_cast<f32x2*>(d) = f32x2{pn[0], pn[1]};
+                *reinterpret_cast<f32x2*>(d + 2) = f32x2{pn[2], pn[3]};
+                if (!now) {
+                    held[0][threadIdx.x] = f32x4{rn[0], rn[1], rn[2], rn[3]};
+                    held[1][threadIdx.x] = c;
+                }
+            }
+            if (threadIdx.x < 16) {
+                const long long gs = side_gi(x0, y0, 0);
+                float rn;
+                const float pn = advance(buf_ld(brr, gs), buf_ld(bra, gs, first), buf_ld(brp, gs), rn);
+                lds[((int)threadIdx.x >> 2) * ST + side_lx()] = pn;
+            }
+        } else {
+            // window rows TY.. of the tile above are rows 0.. of this one
+            xtop = (xtop + TY) & (XR - 1);
+            utop = utop + TY < UR ? utop + TY : utop + TY - UR;
+        }
+        // own U rows (window rows 2 .. TY + 1) and their side words
+        {
+            const int wr = utop + 2 + ty < UR ? utop + 2 + ty : utop + 2 + ty - UR;
+#pragma unroll
+            for (int m = 0; m < 12; ++m) {
+                ub[m][wr][1 + 2 * tx] = L.u[m].d[0];
+                ub[m][wr][2 + 2 * tx] = L.u[m].d[1];
+            }
+#pragma unroll
+            for (int q = 0; q < SE; ++q) {
+                const int e = (int)threadIdx.x + q * kBlock;
+                if (e < NS) {
+                    const int row = e / 24, m = (e % 24) >> 1;
+                    const int wy = utop + 2 + row, ws = wy < UR ? wy : wy - UR;
+                    ub[m][ws][(e & 1) ? SW - 1 : 0] = L.sv[q];
+                }
+            }
+        }
+        // x window rows 4 .. 7: p_{j+1}.  The update of the output cells (x,
+        // r_{j+1} and their squared norms) is done here, so that only r_{j+1}
+        // stays live through the product.
+        const int cx = x0 + R * tx, cy = y0 + ty;
+        const bool valid = cx < W && cy < H;
+        const int ib = cy * W + cx;
+        float cur_r[R];
+        {
+            float rn[R], pn[R], cur_p[R];
+#pragma unroll
+            for (int q = 0; q < R; ++q) pn[q] = advance(L.xr[q], L.xa[q], L.xp[q], rn[q]);
+            float* d = lds + ((xtop + 4 + lrow) & (XR - 1)) * ST + 2 + R * tx;
+            *reinterpret_cast<f32x2*>(d) = f32x2{pn[0], pn[1]};
+            *reinterpret_cast<f32x2*>(d + 2) = f32x2{pn[2], pn[3]};
+            if (now) {
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    cur_r[q] = rn[q];
+                    cur_p[q] = L.xp[q];
+                }
+            } else {
+                const f32x4 hr = held[0][threadIdx.x], hp = held[1][threadIdx.x];
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    cur_r[q] = hr[q];
+                    cur_p[q] = hp[q];
+                }
+                held[0][threadIdx.x] = f32x4{rn[0], rn[1], rn[2], rn[3]};
+                held[1][threadIdx.x] = L.xp;
+            }
+            if (threadIdx.x < 16) {
+                float rs;
+                lds[((xtop + 4 + ((int)threadIdx.x >> 2)) & (XR - 1)) * ST + side_lx()] =
+                    advance(L.cr, L.ca, L.cp, rs);
+            }
+            {
+                f32x4 xn;
+#pragma unroll
+                for (int q = 0; q < R; ++q) xn[q] = fmaf(alpha * L.so[q], cur_p[q], L.xo[q]);
+                buf_st4(brx, ib, xn, !valid || first);
+                buf_st4(bwr, ib, f32x4{cur_r[0], cur_r[1], cur_r[2], cur_r[3]}, !valid || first);
+            }
+            if (valid) {
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    if (ib + q < N) {
+                        const double rd = cur_r[q];
+                        const float un = cur_r[q] / L.so[q];
+                        v[1] += rd * rd;
+                        v[4] += (double)un * (double)un;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        issue(u + 1, L, u + 1 >= uend);
+        float yv[R] = {}, pc[R] = {};
+        if (valid) {
+            int xrow[5], urow[3];
+#pragma unroll
+            for (int q = 0; q < 5; ++q) xrow[q] = ((xtop + ty + q) & (XR - 1)) * ST + R * tx;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const int lr = utop + ty + 2 - d;
+                urow[d] = lr < UR ? lr : lr - UR;
+            }
+            sym_tile<TY, FMT>(lds, xrow, ub, urow, tx, yv, pc);
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                if (ib + q < N) {
+                    const double yd = yv[q];
+                    v[0] += (double)(pc[q] * yv[q]);
+                    v[2] += (double)cur_r[q] * yd;
+                    v[3] += yd * yd;
+                }
+            }
+        }
+        buf_st4(bwp, ib, f32x4{pc[0], pc[1], pc[2], pc[3]}, !valid);
+        buf_st4(bwa, ib, f32x4{yv[0], yv[1], yv[2], yv[3]}, !valid);
+    }
+    store_partials<5>(v, dpf + (size_t)((seq + 1) & 1) * 5 * kMaxGrid);
+    if (lead && mode != 2) sc->fh[(seq + 1) & 1] = hout;
+}
+
+// Publish the fused iteration's live state (slot seq & 1) into the Scalars
+// and run the stopping test (one workgroup; before the host reads the
+// Scalars or replaces the residual).
+__global__ __launch_bounds__(kBlock) void k_fused_fold(Scalars* sc, const double* __restrict__ dpf, int seq,
+                                                       int verify) {
+    if (sc->done != kRun || sc->verify) return;
+    const FHand h = sc->fh[seq & 1];
+    double t[5];
+    fold_all<5>(dpf + (size_t)(seq & 1) * 5 * kMaxGrid, h.npart, t);
+    if (threadIdx.x != 0) return;
+    sc->rz = t[1];
+    sc->rz_prev = h.rr_prev;
+    sc->rr = t[4];
+    sc->beta = h.beta;
+    sc->pAp = t[0];
+    sc->iters = h.iters;
+    sc->pend = 0;
+    if (!isfinite(t[0]) || !isfinite(t[1]) || !isfinite(t[4])) sc->done = kNonFinite;
+    else if (sc->tol2 >= 0 && t[4] <= sc->tol2) {
+        if (verify) sc->verify = 1;
+        else sc->done = kConverged;
+    } else if (h.iters >= sc->maxit) {
+        sc->done = kMaxIt;
+    }
+}
+
 // K2: x += alpha p (scaled: alpha s p), r += alpha Ap, <r,z>, ||r||^2 (scaled:
 // the unscaled residual (r/s)^2).  VEC: every pointer is 16-byte aligned and
 // the body moves float4s (the kernel is a pure stream: wide loads keep enough
@@ -801,12 +1215,16 @@ __global__ __launch_bounds__(kBlock) void k_trueres25(const float* __restrict__ 
 
 // k_trueres25 on 2-D pixel tiles (Geom2D: the FP32 step's staging, which
 // keeps the x halo in L2); the same sums in the same order.
+// r_odd: the fused iteration's second residual buffer, which holds the
+// residual after an odd number of iterations (else null).
 template <class G, int PM, class XT>
 __global__ __launch_bounds__(kBlock) void k_trueres2d(const float* __restrict__ A, size_t pitch, int W, int H, int N,
                                                       const XT* __restrict__ x, const float* __restrict__ b,
-                                                      const float* __restrict__ dinv, float* __restrict__ r, int op,
-                                                      Scalars* sc, double* partials, unsigned int* counter) {
+                                                      const float* __restrict__ dinv, float* __restrict__ r,
+                                                      float* __restrict__ r_odd, int op, Scalars* sc,
+                                                      double* partials, unsigned int* counter) {
     if (op != 0 && sc->done != kRun) return;
+    if (r_odd && (sc->iters & 1)) r = r_odd;
     constexpr int R = G::R, ST = G::STRIDE;
     __shared__ __attribute__((aligned(16))) XT lds[G::LDS_FLOATS];
     const TileRange tr = xcd_tiles(G::ntiles(W, H, N));
@@ -1216,7 +1634,11 @@ struct Workspace {
     long cap64 = 0;
     int pad64 = 0;
     Scalars* sc = nullptr;
-    double* dpart = nullptr;  // deferred partial sums of the CG kernels: 3 x kMaxGrid
+    double* dpart = nullptr;  // deferred partial sums of the CG kernels: 3 x kMaxGrid, then 2 x 5 x kMaxGrid (fused)
+    float *fbase = nullptr, *r2 = nullptr, *Ap2 = nullptr;  // second r and Ap of the fused iteration
+    long capf = 0;
+    unsigned int* uedge = nullptr;  // side words of the 16-bit operator's 256-wide columns (k_uedge)
+    long cape = 0;
     Scalars* hsc = nullptr;  // pinned, 2 slots
     hipEvent_t ev[2] = {nullptr, nullptr};
 };
@@ -1240,8 +1662,8 @@ static gm_status ws_get(gm_stream_t s, long n, int pad, bool need16, Workspace**
         w->dev = dev;
         GM_HIP(hipMalloc(&w->sc, sizeof(Scalars)));
         GM_HIP(hipMemset(w->sc, 0, sizeof(Scalars)));
-        GM_HIP(hipMalloc(&w->dpart, sizeof(double) * 3 * kMaxGrid));
-        GM_HIP(hipMemset(w->dpart, 0, sizeof(double) * 3 * kMaxGrid));
+        GM_HIP(hipMalloc(&w->dpart, sizeof(double) * 13 * kMaxGrid));
+        GM_HIP(hipMemset(w->dpart, 0, sizeof(double) * 13 * kMaxGrid));
         GM_HIP(hipHostMalloc(&w->hsc, 2 * sizeof(Scalars), hipHostMallocDefault));
         GM_HIP(hipEventCreateWithFlags(&w->ev[0], hipEventDisableTiming));
         GM_HIP(hipEventCreateWithFlags(&w->ev[1], hipEventDisableTiming));
@@ -1275,6 +1697,31 @@ static gm_status ws_get(gm_stream_t s, long n, int pad, bool need16, Workspace**
     return GM_OK;
 }
 
+// The fused iteration's second residual and product buffers (N floats each)
+// and the edge array of a W x H operator.
+static gm_status ws_fused(Workspace* w, long N, int W) {
+    const long ne = W > 0 ? (long)((W + 255) / 256) * (N / W) * 24 : 0;
+    if (w->cape < ne) {
+        if (w->uedge) (void)hipFree(w->uedge);
+        w->uedge = nullptr;
+        w->cape = 0;
+        GM_HIP(hipMalloc(&w->uedge, sizeof(unsigned int) * ne));
+        w->cape = ne;
+    }
+    if (w->capf < N) {
+        if (w->fbase) (void)hipFree(w->fbase);
+        w->fbase = w->r2 = w->Ap2 = nullptr;
+        w->capf = 0;
+        const long stride = (N + 63) / 64 * 64;
+        GM_HIP(hipMalloc(&w->fbase, sizeof(float) * stride * 2));
+        GM_HIP(hipMemset(w->fbase, 0, sizeof(float) * stride * 2));
+        w->r2 = w->fbase;
+        w->Ap2 = w->fbase + stride;
+        w->capf = N;
+    }
+    return GM_OK;
+}
+
 // The FP64 master x of the refinement modes: N doubles padded by the largest
 // band offset (the generic residual kernel reads x64[i + off]).  The pad past
 // N is cleared here, since a larger earlier solve may have left values there.
@@ -1303,6 +1750,8 @@ void release_workspaces() {
         ws_free_vectors(w);
         if (w->A16) (void)hipFree(w->A16);
         if (w->x64) (void)hipFree(w->x64 - w->pad64);
+        if (w->fbase) (void)hipFree(w->fbase);
+        if (w->uedge) (void)hipFree(w->uedge);
         if (w->sc) (void)hipFree(w->sc);
         if (w->dpart) (void)hipFree(w->dpart);
         if (w->hsc) (void)hipHostFree(w->hsc);
@@ -1343,7 +1792,7 @@ static TuneVal tune_lookup(int fmt, int W, long N) {
     if (it != g_tune.end()) return it->second;
     // defaults: the shapes the tuner picks for the 3840x2160 Levin matrix
     if (fmt == kF32) return shape_ok(kS2R4Y4, fmt, W, N) ? TuneVal{kS2R4Y4, 2, 0, 0} : TuneVal{kS1R4, 4, 0, 0};
-    return sym16_ok(W, N) ? TuneVal{kS2R4Y4, 3, 0, 0, 1} : TuneVal{kS1R8, 3, 0, 0, 0};
+    return sym16_ok(W, N) ? TuneVal{kS2R4Y4, 2, 0, 0, 2} : TuneVal{kS1R8, 3, 0, 0, 0};
 }
 
 struct Solver {
@@ -1367,6 +1816,9 @@ struct Solver {
     bool sym;           // 16-bit form stored as its 12 upper bands (k_cgstep_sym)
     int walk;           // ... stepped by the column-walk kernel (k_cgstep_symw)
     double* x64;        // refine mode 2: FP64 master x (x then holds the FP32 partial sum since the last fold)
+    bool fused;         // one launch per iteration (k_cgfused_symw)
+    int fseq;           // fused launches so far (hand-off slot parity)
+    int fstate;         // 0: the Scalars hold the state (next launch starts a direction); 1: the fused slots do
 };
 
 static gm_status launch_resid(Solver& S, int start) {
@@ -1399,16 +1851,16 @@ static void launch_trueres_t(Solver& S, const XT* x, int op) {
         const int grid = grid_for_tiles(S.dv, shape_tiles(kS2R4Y4, S.plan.W, S.N), 2);
         if (S.pm == kJacobi)
             hipLaunchKernelGGL((k_trueres2d<G, kJacobi, XT>), dim3(grid), dim3(kBlock), 0, S.hs, S.A->a, S.A->apitch,
-                               S.plan.W, S.N / S.plan.W, S.N, x, S.b, S.w->dinv, S.w->r, op, S.w->sc, S.rs.partials,
-                               S.rs.counter);
+                               S.plan.W, S.N / S.plan.W, S.N, x, S.b, S.w->dinv, S.w->r, S.fused ? S.w->r2 : nullptr, op,
+                               S.w->sc, S.rs.partials, S.rs.counter);
         else if (S.pm == kScaled)
             hipLaunchKernelGGL((k_trueres2d<G, kScaled, XT>), dim3(grid), dim3(kBlock), 0, S.hs, S.A->a, S.A->apitch,
-                               S.plan.W, S.N / S.plan.W, S.N, x, S.b, S.w->dinv, S.w->r, op, S.w->sc, S.rs.partials,
-                               S.rs.counter);
+                               S.plan.W, S.N / S.plan.W, S.N, x, S.b, S.w->dinv, S.w->r, S.fused ? S.w->r2 : nullptr, op,
+                               S.w->sc, S.rs.partials, S.rs.counter);
         else
             hipLaunchKernelGGL((k_trueres2d<G, kPlain, XT>), dim3(grid), dim3(kBlock), 0, S.hs, S.A->a, S.A->apitch,
-                               S.plan.W, S.N / S.plan.W, S.N, x, S.b, S.w->dinv, S.w->r, op, S.w->sc, S.rs.partials,
-                               S.rs.counter);
+                               S.plan.W, S.N / S.plan.W, S.N, x, S.b, S.w->dinv, S.w->r, S.fused ? S.w->r2 : nullptr, op,
+                               S.w->sc, S.rs.partials, S.rs.counter);
     } else if (S.tiled) {
         if (S.pm == kJacobi)
             hipLaunchKernelGGL((k_trueres25<4, kJacobi, XT>), dim3(S.g_tile4), dim3(kBlock), 0, S.hs, S.A->a,
@@ -1439,7 +1891,14 @@ static void launch_trueres_t(Solver& S, const XT* x, int op) {
 // residual is that of the master and the FP32 x restarts from zero: x never
 // carries more than the correction since the last fold, which removes the
 // FP32 rounding floor of the true residual.
+static gm_status fused_sync(Solver& S);
+
 static gm_status launch_trueres(Solver& S, int op) {
+    if (S.fused) {
+        gm_status st = fused_sync(S);
+        if (st != GM_OK) return st;
+        S.fstate = 0;  // the next fused launch starts from this residual
+    }
     if (S.x64) {
         hipLaunchKernelGGL(k_x64, dim3(S.g_elem), dim3(kBlock), 0, S.hs, S.x64, S.x, S.N, 3);
         launch_trueres_t<double>(S, S.x64, op);
@@ -1468,6 +1927,14 @@ static void launch_scale16(Solver& S, int grid) {
             hipLaunchKernelGGL((k_scale16<8, kBF16, false>), dim3(grid), dim3(kBlock), 0, S.hs, A->a, A->apitch,
                                S.plan.W, S.N, w->A16, w->pitch16, w->dinv);
     }
+}
+
+// the fused iteration's edge array of the operator just scaled (256-wide columns)
+static void launch_uedge(Solver& S) {
+    const int H = S.N / S.plan.W;
+    const long total = (long)((S.plan.W + 255) / 256) * H * 24;
+    hipLaunchKernelGGL(k_uedge, dim3(elementwise_grid(S.dv, total, 1)), dim3(kBlock), 0, S.hs, S.w->A16,
+                       S.w->pitch16, S.plan.W, H, S.N, 256, S.w->uedge);
 }
 
 template <int TY, int FMT>
@@ -1576,14 +2043,51 @@ static gm_status launch_update(Solver& S) {
     return GM_OK;
 }
 
+template <int FMT>
+static void launch_fused_t(Solver& S, int mode) {
+    Workspace* w = S.w;
+    hipLaunchKernelGGL((k_cgfused_symw<FMT>), dim3(S.g_step), dim3(kBlock), 0, S.hs, w->A16, w->pitch16, S.plan.W,
+                       S.N / S.plan.W, S.N, w->dinv, S.x, w->r, w->r2, w->p0, w->p1, w->Ap, w->Ap2, w->uedge,
+                       w->sc, w->dpart + 3 * kMaxGrid, S.fseq, mode, S.verify);
+    ++S.fseq;
+}
+
+// one fused launch (mode as k_cgfused_symw)
+static gm_status launch_fused(Solver& S, int mode) {
+    if (S.fmt == kF16) launch_fused_t<kF16>(S, mode);
+    else launch_fused_t<kBF16>(S, mode);
+    GM_LAUNCH_CHECK();
+    return GM_OK;
+}
+
+// Publish the fused iteration's state into the Scalars (nothing to do while
+// they hold it already).
+static gm_status fused_sync(Solver& S) {
+    if (!S.fused || !S.fstate) return GM_OK;
+    hipLaunchKernelGGL(k_fused_fold, dim3(1), dim3(kBlock), 0, S.hs, S.w->sc, S.w->dpart + 3 * kMaxGrid, S.fseq,
+                       S.verify);
+    GM_LAUNCH_CHECK();
+    return GM_OK;
+}
+
 static gm_status launch_cg_iter(Solver& S) {
-    gm_status st = launch_step(S);
-    if (st != GM_OK) return st;
+    gm_status st;
+    if (S.fused) {
+        // a residual from scratch first takes its step (the first direction
+        // or the one kept across a replacement), then one launch per iteration
+        if (!S.fstate) {
+            if ((st = launch_fused(S, 1)) != GM_OK) return st;
+            S.fstate = 1;
+        }
+        return launch_fused(S, 0);
+    }
+    if ((st = launch_step(S)) != GM_OK) return st;
     return launch_update(S);
 }
 
 // fold the last update's pending partial sums into the Scalars
 static gm_status launch_fold(Solver& S) {
+    if (S.fused) return fused_sync(S);
     hipLaunchKernelGGL(k_fold, dim3(1), dim3(kBlock), 0, S.hs, S.w->sc, S.w->dpart + kMaxGrid, S.verify);
     GM_LAUNCH_CHECK();
     return GM_OK;
@@ -1757,9 +2261,14 @@ gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, c
         const TuneVal tv = tune_lookup(S.fmt, S.plan.W, S.N);
         if (S.tiled) set_step_shape(S, tv.shape, tv.per_cu);
         S.walk = tv.walk;
+        // the fused iteration: symmetric 16-bit operator, 4-row column walk,
+        // 16-byte aligned x
+        S.fused = S.sym && tv.walk == 2 && S.shape == kS2R4Y4 && S.N < kFusedMaxN && S.vec &&
+                  o->method != GM_METHOD_GD;
         if (tv.upd_pc > 0) S.g_upd = grid_for_tiles(S.dv, (S.N + 4L * kBlock - 1) / (4L * kBlock), tv.upd_pc);
     }
     Workspace* w = S.w;
+    if (S.fused && (st = ws_fused(w, S.N, S.plan.W)) != GM_OK) return st;
     if (!S.tiled) {
         // generic kernels read r, p and D^-1 at i + offset up to N + pad: clear
         // whatever a larger earlier solve left beyond N in this workspace
@@ -1789,6 +2298,7 @@ gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, c
     } else if (S.pm == kScaled) {
         launch_scale16(S, S.g_tile8);
         GM_LAUNCH_CHECK();
+        if (S.fused) launch_uedge(S);
     }
 
     const int chunk = o->check_every > 0 ? o->check_every : 16;
@@ -1835,13 +2345,17 @@ gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, c
                 ++since_restart;
                 if (o->restart > 0 && since_restart >= o->restart && enq < o->max_iter) {
                     // cgSolve's periodic restart (main.cu:483-493): r from scratch, p = -z
-                    if ((st = S.x64 ? launch_trueres(S, 2) : launch_resid(S, 0)) != GM_OK) return st;
+                    // (the scaled 16-bit system's residual is s (A x - b): the
+                    // true-residual kernel forms it, as at the start)
+                    if ((st = (S.pm == kScaled || S.x64) ? launch_trueres(S, 2) : launch_resid(S, 0)) != GM_OK)
+                        return st;
                     since_restart = 0;
                 }
                 if (refine > 0 && enq % refine == 0) {
                     if ((st = launch_trueres(S, 1)) != GM_OK) return st;
                 }
             }
+            if ((st = fused_sync(S)) != GM_OK) return st;
             GM_HIP(hipMemcpyAsync(&w->hsc[polls & 1], w->sc, sizeof(Scalars), hipMemcpyDeviceToHost, S.hs));
             GM_HIP(hipEventRecord(w->ev[polls & 1], S.hs));
             if (polls > 0) {
@@ -2124,6 +2638,46 @@ gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_re
             best.upd_pc = pc;
         }
     }
+    double fused_us = 0;
+    if (S.sym && best.walk && best.shape == kS2R4Y4 && S.N < kFusedMaxN) {
+        // the fused iteration (timing mode: fixed alpha and beta) against the
+        // best step + update
+        if ((st = ws_fused(w, S.N, S.plan.W)) != GM_OK) return st;
+        launch_uedge(S);
+        hipLaunchKernelGGL(k_fill, dim3(S.g_elem), dim3(kBlock), 0, S.hs, w->Ap, S.N, 0.25f);
+        const int step_pc = best.per_cu;
+        int fused_pc = 0;
+        S.walk = 2;
+        for (int pc : {2, 3}) {
+            set_step_shape(S, kS2R4Y4, pc);
+            double tot = 0;
+            for (int k = 0; k < reps + 1; ++k) {
+                hipLaunchKernelGGL(k_tune_reset, dim3(1), dim3(64), 0, S.hs, w->sc);
+                GM_HIP(hipEventRecord(e0, S.hs));
+                if ((st = launch_fused(S, 2)) != GM_OK) return st;
+                GM_HIP(hipEventRecord(e1, S.hs));
+                GM_HIP(hipEventSynchronize(e1));
+                float ms = 0;
+                GM_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (k > 0) tot += ms;
+            }
+            const double us = tot * 1e3 / reps;
+            snprintf(line, sizeof(line), "cgfused %d %d %d %d %.2f column-walk (step + update %.2f)\n",
+                     (int)kS2R4Y4, 4, 4, pc, us, best.us + best_upd);
+            g_tune_report += line;
+            if (fused_pc == 0 || us < fused_us) {
+                fused_us = us;
+                fused_pc = pc;
+            }
+        }
+        S.walk = best.walk;
+        if (fused_us < best.us + best_upd) {
+            best.walk = 2;
+            best.per_cu = fused_pc;
+        } else {
+            best.per_cu = step_pc;
+        }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     {
@@ -2138,6 +2692,7 @@ gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_re
         out->wg_per_cu = best.per_cu;
         out->step_us = best.us;
         out->step_walk = best.walk;
+        out->fused_us = fused_us;
         out->spmv_rows_per_thread = 0;
         out->spmv_tile_rows = 0;
         out->spmv_direct = 0;
@@ -2177,10 +2732,14 @@ gm_status gm_tune_set_step(int W, int N, int precision, int rows_per_thread, int
         if (kShapeR[k] == rows_per_thread && kShapeTY[k] == tile_rows) sh = k;
     if (sh < 0 || !step_shape_ok(sh, fmt, W, N) || wg_per_cu < 1 || wg_per_cu > 16)
         return fail(GM_ERR_INVALID, "gm_tune_set: unsupported shape");
+    if (column_walk < 0 || column_walk > 2)
+        return fail(GM_ERR_INVALID, "gm_tune_set_step: column_walk must be 0, 1 or 2");
     if (column_walk != 0 && (fmt == kF32 || !sym16_ok(W, N)))
         return fail(GM_ERR_INVALID, "gm_tune_set_step: the column walk is a form of the symmetric 16-bit step");
+    if (column_walk == 2 && sh != kS2R4Y4)
+        return fail(GM_ERR_INVALID, "gm_tune_set_step: the fused iteration walks 4-row tiles");
     std::lock_guard<std::mutex> g(g_tune_mu);
-    g_tune[TuneKey{fmt, W, (long)N}] = TuneVal{sh, wg_per_cu, 0, 0, column_walk != 0};
+    g_tune[TuneKey{fmt, W, (long)N}] = TuneVal{sh, wg_per_cu, 0, 0, column_walk};
     return GM_OK;
 }
 

@@ -351,6 +351,62 @@ def test_every_tile_shape_same_iteration(gpu, small, prec, shapes, width):
     call("gm_tune_set", p.W, p.N, prec, *shapes[0], 4)
 
 
+@pytest.mark.parametrize("W,H", [(256, 192), (1000, 37), (1916, 203), (3840, 64), (16640, 8)])
+def test_fused_iteration_matches_step_and_update(gpu, W, H):
+    """One launch per CG iteration (the update of x and r fused into the next
+    step, beta from the expanded <r,r> recurrence, r and Ap alternating between
+    two buffers) makes the iterates of the two-kernel form: fixed-count runs
+    agree to rounding, and runs to a tolerance take the same iterations and
+    replacements (every 7 iterations here, so that replacements land on odd
+    and even iteration counts) and reach it.  Sizes: ragged right and bottom
+    tiles, 15 columns with runs of one tile, and 65 columns (more than the
+    workgroups per XCD: runs that cross into the next column)."""
+    p = synth.make_problem(W, H, 4321, 1, 0.08)
+    s = _gpu_system(p)
+    res = {}
+    for walk, pc in ((1, 3), (2, 2), (2, 3)):
+        call("gm_tune_set_step", W, p.N, PREC_FP16, 4, 4, pc, walk)
+        x8, _ = matting.solve(s, method=METHOD_PCG, precision=PREC_FP16, tol=0, max_iter=8, verify=False,
+                              refine_every=-1)
+        x, info = matting.solve(s, method=METHOD_PCG, precision=PREC_FP16, tol=1e-6, max_iter=5000, refine_every=7)
+        res[(walk, pc)] = (x8.numpy(), x.numpy(), info)
+    call("gm_tune_set_step", W, p.N, PREC_FP16, 4, 4, 2, 2)
+    x8r, xr, ir = res[(1, 3)]
+    scale = np.abs(x8r).max()
+    assert scale > 0
+    for key in ((2, 2), (2, 3)):
+        x8, x, info = res[key]
+        print(f"\n[{W}x{H} walk {key}] {info.iterations} vs {ir.iterations} iterations, "
+              f"{info.replacements} vs {ir.replacements} replacements, true relres {info.true_relres:.3e}", end="")
+        assert np.abs(x8 - x8r).max() <= 1e-5 * scale
+        assert info.status == 0 and info.true_relres <= 1e-6
+        assert abs(info.iterations - ir.iterations) <= 2
+        assert np.abs(x - xr).max() <= 1e-3
+
+
+def test_fused_iteration_restart_and_iteration_cap(gpu, small):
+    """The fused form through cgSolve's periodic restart (a residual from
+    scratch every 25 iterations, then a fresh first step) and through the
+    iteration cap: the same status, iteration count and final residual as the
+    two-kernel form."""
+    p = small[0]
+    s = _gpu_system(p)
+    out = {}
+    for walk, pc in ((1, 3), (2, 2)):
+        call("gm_tune_set_step", p.W, p.N, PREC_FP16, 4, 4, pc, walk)
+        x, info = matting.solve(s, method=METHOD_PCG, precision=PREC_FP16, tol=1e-6, max_iter=5000, restart=25)
+        xc, ic = matting.solve(s, method=METHOD_PCG, precision=PREC_FP16, tol=1e-12, max_iter=37)
+        out[walk] = (x.numpy(), info, xc.numpy(), ic)
+    call("gm_tune_set_step", p.W, p.N, PREC_FP16, 4, 4, 2, 2)
+    (xa, ia, xca, ica), (xb, ib, xcb, icb) = out[1], out[2]
+    assert ia.status == ib.status == 0 and ib.true_relres <= 1e-6
+    assert abs(ia.iterations - ib.iterations) <= 2
+    assert np.abs(xa - xb).max() <= 1e-3
+    assert ica.status == icb.status == 6 and ica.iterations == icb.iterations == 37  # GM_ERR_NOT_CONVERGED
+    assert abs(ica.relres - icb.relres) <= 1e-3 * ica.relres
+    assert np.abs(xca - xcb).max() <= 1e-4
+
+
 @pytest.mark.parametrize("W,H", [(256, 192), (1916, 1023), (2048, 1030), (4096, 300)])
 @pytest.mark.parametrize("ty", [4, 8, 16])
 def test_column_walk_step_matches_tile_parallel(gpu, W, H, ty):
@@ -377,27 +433,23 @@ def test_column_walk_step_matches_tile_parallel(gpu, W, H, ty):
     assert np.abs(outs[1] - outs[0]).max() <= 1e-5 * scale
 
 
-@pytest.mark.parametrize("width", [256, 250])
-def test_fp16_operator_matches_emulation(gpu, width):
-    """The 16-bit scaled operator the solver applies (the symmetric 12-band
-    layout for W % 4 == 0, the full 24-band one otherwise) is the one a host
-    emulation builds: As = S A S with S = 1/sqrt(a_ii), entries rounded to
-    FP16 (RNE) -- for the symmetric layout the upper entry of each pair serves
-    both.  Checked through k fixed-count CG iterations (no residual
-    replacement): a wrong lower index, sign or missing band moves x by far
-    more than the FP32-vs-FP64 arithmetic difference."""
+def _fp16_emulation(p):
+    """Host FP64 emulation of the scaled 16-bit operator: As = S A S with S =
+    1/sqrt(a_ii), entries rounded to FP16 (RNE) -- for the symmetric layout
+    (W % 4 == 0) the upper entry of each pair serves both -- plus the FP32
+    matrix A itself (as a sparse matrix), S and b."""
     import scipy.sparse as sp
-    p = synth.make_problem(width, 96, 77, 1, 0.08)
     W, N = p.W, p.N
     a, b, _ = O.levin_gather_acc_f32(p.image, p.lam, p.target, 1e-7)
     s32 = (np.float32(1) / np.sqrt(a[12][:N].astype(np.float32))).astype(np.float32)
     offs = O.bands25(W)
-    rows, cols, vals = [], [], []
+    rows, cols, vals, vals32 = [], [], [], []
     for k, o in enumerate(offs):
-        if k == 12:
-            continue
         i = np.arange(N)
         m = (i + o >= 0) & (i + o < N)
+        vals32.append((i[m], i[m] + o, a[k][:N][m].astype(np.float64)))
+        if k == 12:
+            continue
         if W % 4 == 0 and k < 12:
             # lower band: the upper entry of the partner row i + o_k (band 24 - k)
             src = i + o
@@ -410,22 +462,64 @@ def test_fp16_operator_matches_emulation(gpu, width):
         rows.append(i[m]), cols.append(i[m] + o), vals.append(v16[m])
     As = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(N, N))
     As = As + sp.identity(N)
-    sv = s32.astype(np.float64)
+    A = sp.csr_matrix((np.concatenate([v[2] for v in vals32]),
+                       (np.concatenate([v[0] for v in vals32]), np.concatenate([v[1] for v in vals32]))), shape=(N, N))
+    return As, A, s32.astype(np.float64), b.astype(np.float64)
+
+
+def _scaled_cg(As, A, sv, b, k, restart=0):
+    """k iterations of CG on the scaled system from x = 0 (FP64), sign
+    convention r = A x - b, p = -r + beta p; restart: every `restart`
+    iterations r from scratch through the FP32 matrix, scaled, and beta = 0."""
+    x = np.zeros(b.size)
+    r = sv * (A @ x - b)
+    pv = np.zeros(b.size)
+    rz, beta, since = r @ r, 0.0, 0
+    for it in range(k):
+        pv = beta * pv - r
+        Ap = As @ pv
+        al = rz / (pv @ Ap)
+        x += al * sv * pv
+        r += al * Ap
+        rz, beta = r @ r, (r @ r) / rz
+        since += 1
+        if restart and since >= restart and it + 1 < k:
+            r = sv * (A @ x - b)
+            rz, beta, since = r @ r, 0.0, 0
+    return x
+
+
+@pytest.mark.parametrize("width", [256, 250])
+def test_fp16_operator_matches_emulation(gpu, width):
+    """The 16-bit scaled operator the solver applies (the symmetric 12-band
+    layout for W % 4 == 0, the full 24-band one otherwise) is the one a host
+    emulation builds.  Checked through k fixed-count CG iterations (no residual
+    replacement): a wrong lower index, sign or missing band moves x by far
+    more than the FP32-vs-FP64 arithmetic difference."""
+    p = synth.make_problem(width, 96, 77, 1, 0.08)
+    As, A, sv, b = _fp16_emulation(p)
     s = _gpu_system(p)
     for k in (2, 5):
         x, _ = matting.solve(s, method=METHOD_PCG, precision=PREC_FP16, tol=0, max_iter=k, verify=False,
                              refine_every=-1)
-        y = np.zeros(N)
-        r = -sv * b.astype(np.float64)
-        pv = np.zeros(N)
-        rz = r @ r
-        beta = 0.0
-        for _ in range(k):
-            pv = beta * pv - r
-            Ap = As @ pv
-            al = rz / (pv @ Ap)
-            y += al * pv
-            r += al * Ap
-            rz, beta = r @ r, (r @ r) / rz
-        xe = sv * y
+        xe = _scaled_cg(As, A, sv, b, k)
         assert np.abs(x.numpy() - xe).max() <= 1e-5 * np.abs(xe).max(), k
+
+
+@pytest.mark.parametrize("walk", [0, 1, 2])
+def test_fp16_restart_matches_emulation(gpu, walk):
+    """cgSolve's periodic restart on the scaled 16-bit system: the residual is
+    formed from scratch as S (A x - b) and the direction restarts, for the
+    tile-parallel and column-walk steps and the fused iteration.  The host
+    emulation restarts the same way; one iteration after a restart from an
+    unscaled residual already moves x by 1e-2 relative."""
+    p = synth.make_problem(256, 96, 77, 1, 0.08)
+    As, A, sv, b = _fp16_emulation(p)
+    s = _gpu_system(p)
+    call("gm_tune_set_step", p.W, p.N, PREC_FP16, 4, 4, 2 if walk == 2 else 3, walk)
+    for k in (26, 30):
+        x, _ = matting.solve(s, method=METHOD_PCG, precision=PREC_FP16, tol=0, max_iter=k, verify=False,
+                             refine_every=-1, restart=25)
+        xe = _scaled_cg(As, A, sv, b, k, restart=25)
+        assert np.abs(x.numpy() - xe).max() <= 1e-5 * np.abs(xe).max(), k
+    call("gm_tune_set_step", p.W, p.N, PREC_FP16, 4, 4, 2, 2)
