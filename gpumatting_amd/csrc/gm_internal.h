@@ -146,6 +146,7 @@ struct Scalars {
     int maxit;
     int replacements;
     int stall;       // verifications in a row that failed to reduce the true residual 4x
+    int xzero;       // k_bnorm found the initial guess all 0 (its residual is -b: no product)
     unsigned int counter;  // arrival counter of the last-block reduction
     int pend;        // 1: partial sums of <r,z>, ||r||^2 of the last update await folding
     int npart;       // number of those partial sums
@@ -298,12 +299,23 @@ __device__ __forceinline__ void fold_all(const double* __restrict__ part, int n,
     __shared__ double fred[NQ][kWaves];
     __shared__ double ftot[NQ];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    // every quantity's load of one index is issued before any is summed (one
+    // memory latency per kBlock indices, not per quantity); each sum still runs
+    // over the indices in ascending order
+    double s[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) s[q] = 0;
+    for (int i = threadIdx.x; i < n; i += kBlock) {
+        double v[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) v[q] = part[(size_t)q * kMaxGrid + i];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) s[q] += v[q];
+    }
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-        double s = 0;
-        for (int i = threadIdx.x; i < n; i += kBlock) s += part[(size_t)q * kMaxGrid + i];
-        s = wave_sum(s);
-        if (lane == 0) fred[q][wid] = s;
+        const double t = wave_sum(s[q]);
+        if (lane == 0) fred[q][wid] = t;
     }
     __syncthreads();
     if (threadIdx.x < NQ) {

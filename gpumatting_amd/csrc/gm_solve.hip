@@ -59,16 +59,28 @@ __global__ void k_init_scalars(Scalars* sc, int maxit) {
     }
 }
 
-// <b,b> -> bb, tol2 = tol^2 * bb (tol <= 0: -1, never converged)
-__global__ __launch_bounds__(kBlock) void k_bnorm(const float* __restrict__ b, int n, double tol, Scalars* sc,
-                                                  double* partials, unsigned int* counter) {
-    double v[1] = {0.0};
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) v[0] += (double)b[i] * (double)b[i];
-    grid_reduce<1>(v, partials, counter, [&](double* t) {
+// <b,b> -> bb, tol2 = tol^2 * bb (tol <= 0: -1, never converged); x (may be
+// null): xzero = 1 when every x[i] is 0
+__global__ __launch_bounds__(kBlock) void k_bnorm(const float* __restrict__ b, const float* __restrict__ x, int n,
+                                                  double tol, Scalars* sc, double* partials, unsigned int* counter) {
+    double v[2] = {0.0, 0.0};
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        v[0] += (double)b[i] * (double)b[i];
+        if (x) v[1] += (double)x[i] * (double)x[i];
+    }
+    grid_reduce<2>(v, partials, counter, [&](double* t) {
+        sc->xzero = x && t[1] == 0.0 ? 1 : 0;
         sc->bb = t[0];
         const double bb = t[0] > 0 ? t[0] : 1.0;
         sc->tol2 = tol > 0 ? tol * tol * bb : -1.0;
     });
+}
+
+// k_bnorm's grid: every workgroup fences once in the reduction (about 35 ns
+// each, serialised), so at most one workgroup per CU
+static int bnorm_grid(const DevInfo& d, long n) {
+    const int g = elementwise_grid(d, n, 8);
+    return g < d.cus ? g : d.cus;
 }
 
 // ---------------------------------------------------------------------------
@@ -637,7 +649,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
 // mode 0: continue from the previous launch (slot seq & 1); 1: first step
 // after a residual from scratch (state from the Scalars, no update: the step
 // j of the current residual); 2: timing (as 0 with alpha = beta = 0.5, no
-// sums folded, no stop test).  The loads of tile u + 1 are in flight while
+// sums folded, no stop test); 3: the update of iteration j only, when the
+// residual is replaced next (its step would be redone from the new residual):
+// x and r_{j+1} of every cell, <r,r> and ||r/s||^2 of r_{j+1}.  The loads of tile u + 1 are in flight while
 // tile u is computed; two workgroups per CU (the tuner's choice: a third, or a
 // second tile of loads in flight, measured slower at 4K).
 // The side words of the symmetric 16-bit operator's column walk, gathered
@@ -804,7 +818,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
     // software pipeline: the loads of tile u + 1 are issued while tile u is
     // computed (the first tile's while the previous launch's sums are folded)
     Loads L;
-    if (ubeg < uend) issue(ubeg, L, false);
+    if (ubeg < uend && mode != 3) issue(ubeg, L, false);
     float alpha = 0.f, beta = hin.beta;
     FHand hout{hin.rr_prev, hin.beta, j, (int)gridDim.x, 0};
     if (mode == 2) {
@@ -844,6 +858,31 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
         hout.rr_prev = rr;
         hout.beta = beta;
         hout.iters = j + 1;
+    }
+    if (mode == 3) {
+        // own cells only, float4 groups over the whole vector
+        double w[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        const int n4 = N >> 2;
+        for (int i = blockIdx.x * kBlock + (int)threadIdx.x; i < n4; i += gridDim.x * kBlock) {
+            const f32x4 xv = reinterpret_cast<const f32x4*>(x)[i], sv4 = reinterpret_cast<const f32x4*>(svec)[i];
+            const f32x4 pv = reinterpret_cast<const f32x4*>(pold)[i], rv = reinterpret_cast<const f32x4*>(rin)[i];
+            const f32x4 av = reinterpret_cast<const f32x4*>(Apin)[i];
+            f32x4 xn, rn;
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                xn[q] = fmaf(alpha * sv4[q], pv[q], xv[q]);
+                rn[q] = fmaf(alpha, av[q], rv[q]);
+                const double rd = rn[q];
+                const float un = rn[q] / sv4[q];
+                w[1] += rd * rd;
+                w[4] += (double)un * (double)un;
+            }
+            reinterpret_cast<f32x4*>(x)[i] = xn;
+            reinterpret_cast<f32x4*>(rout)[i] = rn;
+        }
+        store_partials<5>(w, dpf + (size_t)((seq + 1) & 1) * 5 * kMaxGrid);
+        if (lead) sc->fh[(seq + 1) & 1] = hout;
+        return;
     }
     // r_{j+1} and p_{j+1} of a loaded cell
     auto advance = [&](float rv, float av, float pv, float& rn) {
@@ -1214,17 +1253,23 @@ __global__ __launch_bounds__(kBlock) void k_trueres25(const float* __restrict__ 
 }
 
 // k_trueres25 on 2-D pixel tiles (Geom2D: the FP32 step's staging, which
-// keeps the x halo in L2); the same sums in the same order.
+// keeps the x halo in L2); the same sums in the same order.  As in the FP32
+// step kernel, a tile's 25 band loads are in flight while its x window is
+// staged, and each workgroup stores its partial sums for k_trueres_fin (no
+// fenced reduction, so the grid can hold 3 workgroups per CU).
 // r_odd: the fused iteration's second residual buffer, which holds the
-// residual after an odd number of iterations (else null).
+// residual after an odd number of iterations (else null).  xzero: the caller
+// knows that k_bnorm saw x = 0 (the first residual of a solve): r = -b, no
+// band loads.
 template <class G, int PM, class XT>
 __global__ __launch_bounds__(kBlock) void k_trueres2d(const float* __restrict__ A, size_t pitch, int W, int H, int N,
                                                       const XT* __restrict__ x, const float* __restrict__ b,
                                                       const float* __restrict__ dinv, float* __restrict__ r,
-                                                      float* __restrict__ r_odd, int op, Scalars* sc,
-                                                      double* partials, unsigned int* counter) {
+                                                      float* __restrict__ r_odd, int op, int xzero, Scalars* sc,
+                                                      double* __restrict__ partials) {
     if (op != 0 && sc->done != kRun) return;
     if (r_odd && (sc->iters & 1)) r = r_odd;
+    const bool zero = xzero && sc->xzero;
     constexpr int R = G::R, ST = G::STRIDE;
     __shared__ __attribute__((aligned(16))) XT lds[G::LDS_FLOATS];
     const TileRange tr = xcd_tiles(G::ntiles(W, H, N));
@@ -1232,28 +1277,23 @@ __global__ __launch_bounds__(kBlock) void k_trueres2d(const float* __restrict__ 
     for (int t = tr.begin; t < tr.end; t += tr.step) {
         G g;
         g.at(t, W, H, N);
+        Bands<R, kF32> av;
+        if (g.valid && !zero) load_bands<R, kF32, kDiagStored>(A, pitch, g.ib, av);
+        __syncthreads();  // the previous tile is done with lds
+        if (!zero) g.stage(lds, W, H, N, [&](int i) { return x[i]; });
         __syncthreads();
-        int nz = 0;
-        g.stage(lds, W, H, N, [&](int i) {
-            const XT xv = x[i];
-            nz |= xv != XT(0);
-            return xv;
-        });
-        // a window of zeros (a zero initial guess) needs no band loads: A x = 0
-        nz = __syncthreads_or(nz);
         if (!g.valid) continue;
         double acc[R];
 #pragma unroll
         for (int j = 0; j < R; ++j) acc[j] = 0.0;
-        if (nz) {
+        if (!zero) {
             const XT* xs = g.xs(lds);
 #pragma unroll
             for (int k = 0; k < 25; ++k) {
-                Raw<R, kF32> a;
-                load_band<R, kF32>(A, (size_t)k * pitch + (size_t)g.ib, a);
                 const int s = k / 5, dx = k % 5;
 #pragma unroll
-                for (int j = 0; j < R; ++j) acc[j] = fma((double)a.get(j), (double)xs[s * ST + j + dx], acc[j]);
+                for (int j = 0; j < R; ++j)
+                    acc[j] = fma((double)av.v[k].get(j), (double)xs[s * ST + j + dx], acc[j]);
             }
         }
         float bb[R], dd[R], out[R], ro[R];
@@ -1275,11 +1315,20 @@ __global__ __launch_bounds__(kBlock) void k_trueres2d(const float* __restrict__ 
         }
         if (op != 0) store_rows<R>(r, g.ib, N, out);
     }
-    grid_reduce<3>(v, partials, counter, [&](double* tt) {
-        sc->g0 = tt[1];
-        if (op == 0) return;
-        fin_residual(sc, tt[0], tt[1], op == 2, true, tt[2], PM == kScaled ? tt[0] : tt[1]);
-    });
+    store_partials<3>(v, partials);
+}
+
+// The reduction of k_trueres2d (one workgroup): g0, and for op 1 / 2 the
+// replacement and stopping test of k_trueres25's finaliser.
+__global__ __launch_bounds__(kBlock) void k_trueres_fin(Scalars* sc, const double* __restrict__ partials, int npart,
+                                                        int op, int scaled) {
+    if (op != 0 && sc->done != kRun) return;
+    double t[3];
+    fold_all<3>(partials, npart, t);
+    if (threadIdx.x != 0) return;
+    sc->g0 = t[1];
+    if (op == 0) return;
+    fin_residual(sc, t[0], t[1], op == 2, true, t[2], scaled ? t[0] : t[1]);
 }
 
 // Mixed-precision operator: As = S A S with S = diag(1/sqrt(a_ii)) (unit
@@ -1843,24 +1892,28 @@ static gm_status launch_resid(Solver& S, int start) {
     return GM_OK;
 }
 
+// xzero: the residual of the initial guess (k_bnorm recorded whether it is 0)
 template <class XT>
-static void launch_trueres_t(Solver& S, const XT* x, int op) {
+static void launch_trueres_t(Solver& S, const XT* x, int op, int xzero) {
     using G = Geom2D<4, 4>;
     if (S.tiled && shape_ok(kS2R4Y4, kF32, S.plan.W, S.N)) {
-        // 2 workgroups per CU: its grid-wide reduction fences once per workgroup
-        const int grid = grid_for_tiles(S.dv, shape_tiles(kS2R4Y4, S.plan.W, S.N), 2);
+        // 3 workgroups per CU (the FP32 step's pick), partial sums folded by k_trueres_fin
+        const int grid = grid_for_tiles(S.dv, shape_tiles(kS2R4Y4, S.plan.W, S.N), 3);
+        float* r_odd = S.fused ? S.w->r2 : nullptr;
         if (S.pm == kJacobi)
             hipLaunchKernelGGL((k_trueres2d<G, kJacobi, XT>), dim3(grid), dim3(kBlock), 0, S.hs, S.A->a, S.A->apitch,
-                               S.plan.W, S.N / S.plan.W, S.N, x, S.b, S.w->dinv, S.w->r, S.fused ? S.w->r2 : nullptr, op,
-                               S.w->sc, S.rs.partials, S.rs.counter);
+                               S.plan.W, S.N / S.plan.W, S.N, x, S.b, S.w->dinv, S.w->r, r_odd, op, xzero, S.w->sc,
+                               S.rs.partials);
         else if (S.pm == kScaled)
             hipLaunchKernelGGL((k_trueres2d<G, kScaled, XT>), dim3(grid), dim3(kBlock), 0, S.hs, S.A->a, S.A->apitch,
-                               S.plan.W, S.N / S.plan.W, S.N, x, S.b, S.w->dinv, S.w->r, S.fused ? S.w->r2 : nullptr, op,
-                               S.w->sc, S.rs.partials, S.rs.counter);
+                               S.plan.W, S.N / S.plan.W, S.N, x, S.b, S.w->dinv, S.w->r, r_odd, op, xzero, S.w->sc,
+                               S.rs.partials);
         else
             hipLaunchKernelGGL((k_trueres2d<G, kPlain, XT>), dim3(grid), dim3(kBlock), 0, S.hs, S.A->a, S.A->apitch,
-                               S.plan.W, S.N / S.plan.W, S.N, x, S.b, S.w->dinv, S.w->r, S.fused ? S.w->r2 : nullptr, op,
-                               S.w->sc, S.rs.partials, S.rs.counter);
+                               S.plan.W, S.N / S.plan.W, S.N, x, S.b, S.w->dinv, S.w->r, r_odd, op, xzero, S.w->sc,
+                               S.rs.partials);
+        hipLaunchKernelGGL(k_trueres_fin, dim3(1), dim3(kBlock), 0, S.hs, S.w->sc, S.rs.partials, grid, op,
+                           S.pm == kScaled ? 1 : 0);
     } else if (S.tiled) {
         if (S.pm == kJacobi)
             hipLaunchKernelGGL((k_trueres25<4, kJacobi, XT>), dim3(S.g_tile4), dim3(kBlock), 0, S.hs, S.A->a,
@@ -1893,7 +1946,7 @@ static void launch_trueres_t(Solver& S, const XT* x, int op) {
 // FP32 rounding floor of the true residual.
 static gm_status fused_sync(Solver& S);
 
-static gm_status launch_trueres(Solver& S, int op) {
+static gm_status launch_trueres(Solver& S, int op, int xzero = 0) {
     if (S.fused) {
         gm_status st = fused_sync(S);
         if (st != GM_OK) return st;
@@ -1901,9 +1954,9 @@ static gm_status launch_trueres(Solver& S, int op) {
     }
     if (S.x64) {
         hipLaunchKernelGGL(k_x64, dim3(S.g_elem), dim3(kBlock), 0, S.hs, S.x64, S.x, S.N, 3);
-        launch_trueres_t<double>(S, S.x64, op);
+        launch_trueres_t<double>(S, S.x64, op, xzero);
     } else {
-        launch_trueres_t<float>(S, S.x, op);
+        launch_trueres_t<float>(S, S.x, op, xzero);
     }
     GM_LAUNCH_CHECK();
     return GM_OK;
@@ -2070,16 +2123,18 @@ static gm_status fused_sync(Solver& S) {
     return GM_OK;
 }
 
-static gm_status launch_cg_iter(Solver& S) {
+// replace_next: a residual from scratch follows this iteration
+static gm_status launch_cg_iter(Solver& S, bool replace_next = false) {
     gm_status st;
     if (S.fused) {
         // a residual from scratch first takes its step (the first direction
-        // or the one kept across a replacement), then one launch per iteration
+        // or the one kept across a replacement), then one launch per
+        // iteration; the iteration before a replacement makes only its update
         if (!S.fstate) {
             if ((st = launch_fused(S, 1)) != GM_OK) return st;
             S.fstate = 1;
         }
-        return launch_fused(S, 0);
+        return launch_fused(S, replace_next ? 3 : 0);
     }
     if ((st = launch_step(S)) != GM_OK) return st;
     return launch_update(S);
@@ -2155,7 +2210,8 @@ static gm_status refine_impl(float* x, const BandedMatrix* A, const float* b, in
     hipLaunchKernelGGL(k_x64, dim3(ge), dim3(kBlock), 0, hs, w->x64, x, N, 0);
     GM_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(64), 0, hs, w->sc, 1);
-    hipLaunchKernelGGL(k_bnorm, dim3(ge), dim3(kBlock), 0, hs, b, N, 0.0, w->sc, rs.partials, rs.counter);
+    hipLaunchKernelGGL(k_bnorm, dim3(bnorm_grid(dv, N)), dim3(kBlock), 0, hs, b, nullptr, N, 0.0, w->sc,
+                       rs.partials, rs.counter);
     GM_LAUNCH_CHECK();
     Scalars h;
     GM_HIP(hipMemcpyAsync(&h, w->sc, sizeof(Scalars), hipMemcpyDeviceToHost, hs));
@@ -2279,7 +2335,8 @@ gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, c
 
     hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(64), 0, S.hs, w->sc, o->max_iter);
     GM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_bnorm, dim3(S.g_elem), dim3(kBlock), 0, S.hs, b, S.N, o->tol, w->sc, S.rs.partials,
+    hipLaunchKernelGGL(k_bnorm, dim3(bnorm_grid(S.dv, S.N)), dim3(kBlock), 0, S.hs, b, x, S.N, o->tol, w->sc,
+                       S.rs.partials,
                        S.rs.counter);
     GM_LAUNCH_CHECK();
     if (o->refine_mode == 2) {
@@ -2325,7 +2382,7 @@ gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, c
         }
     } else {
         if (S.pm == kScaled || S.x64) {
-            if ((st = launch_trueres(S, 2)) != GM_OK) return st;
+            if ((st = launch_trueres(S, 2, 1)) != GM_OK) return st;
         } else {
             if ((st = launch_resid(S, 1)) != GM_OK) return st;
         }
@@ -2340,7 +2397,9 @@ gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, c
         while (!finished) {
             const int this_chunk = next;
             for (int j = 0; j < this_chunk && enq < o->max_iter; ++j) {
-                if ((st = launch_cg_iter(S)) != GM_OK) return st;
+                const bool restart_next = o->restart > 0 && since_restart + 1 >= o->restart && enq + 1 < o->max_iter;
+                const bool refine_next = refine > 0 && (enq + 1) % refine == 0;
+                if ((st = launch_cg_iter(S, restart_next || refine_next)) != GM_OK) return st;
                 ++enq;
                 ++since_restart;
                 if (o->restart > 0 && since_restart >= o->restart && enq < o->max_iter) {
@@ -2523,7 +2582,8 @@ gm_status gm_true_relres(const BandedMatrix* A, const float* x, const float* b, 
     S.g_gen = elementwise_grid(S.dv, S.N, 1);
     hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(64), 0, S.hs, S.w->sc, 1);
     GM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_bnorm, dim3(S.g_elem), dim3(kBlock), 0, S.hs, b, S.N, 0.0, S.w->sc, S.rs.partials,
+    hipLaunchKernelGGL(k_bnorm, dim3(bnorm_grid(S.dv, S.N)), dim3(kBlock), 0, S.hs, b, nullptr, S.N, 0.0, S.w->sc,
+                       S.rs.partials,
                        S.rs.counter);
     GM_LAUNCH_CHECK();
     if ((st = launch_trueres(S, 0)) != GM_OK) return st;
