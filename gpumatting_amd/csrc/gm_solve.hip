@@ -817,8 +817,47 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
     };
     // software pipeline: the loads of tile u + 1 are issued while tile u is
     // computed (the first tile's while the previous launch's sums are folded)
+    // The rows a run's first tile shares with the tile above (U window rows
+    // 0, 1; x window rows 0 .. 3, one row per thread as the tile above would
+    // have loaded them).  The first tile's are loaded with its other loads, so
+    // they are in flight while the previous launch's sums are folded.
+    constexpr int PE = (2 * SW + kBlock - 1) / kBlock;
+    struct Pre {
+        unsigned int u[PE][12];
+        f32x4 a, b, c;
+        float sr, sa, sp;
+    };
+    auto load_pre = [&](int u, Pre& P) {
+        int x0, y0;
+        origin(u, x0, y0);
+#pragma unroll
+        for (int k = 0; k < PE; ++k) {
+            const int e = (int)threadIdx.x + k * kBlock;
+            const int wy = e / SW, ww = e - wy * SW;
+            const long long gi = (long long)(y0 - 2 + wy) * W + (x0 - 2 + 2 * ww);
+            const bool in = e < 2 * SW && gi >= 0 && gi < N;
+#pragma unroll
+            for (int m = 0; m < 12; ++m) {
+                const unsigned off = in ? 2u * ((unsigned)m * (unsigned)pitch + (unsigned)gi) : 0x80000000u;
+                P.u[k][m] = __builtin_amdgcn_raw_buffer_load_b32(bu, (int)off, 0, 0);
+            }
+        }
+        const long long gi = (long long)(y0 - 2 + lrow) * W + (x0 + R * tx);
+        P.a = buf_ld4(brr, gi);
+        P.b = buf_ld4(bra, gi, first);
+        P.c = buf_ld4(brp, gi);
+        const long long gs = side_gi(x0, y0, 0);
+        const bool sk = threadIdx.x >= 16;
+        P.sr = buf_ld(brr, gs, sk);
+        P.sa = buf_ld(bra, gs, sk || first);
+        P.sp = buf_ld(brp, gs, sk);
+    };
     Loads L;
-    if (ubeg < uend && mode != 3) issue(ubeg, L, false);
+    Pre P;
+    if (ubeg < uend && mode != 3) {
+        issue(ubeg, L, false);
+        load_pre(ubeg, P);
+    }
     float alpha = 0.f, beta = hin.beta;
     FHand hout{hin.rr_prev, hin.beta, j, (int)gridDim.x, 0};
     if (mode == 2) {
@@ -906,18 +945,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
             // window rows 0 .. 3 (as the tile above would have loaded them)
             xtop = 0;
             utop = 0;
-            for (int e = (int)threadIdx.x; e < 2 * SW; e += kBlock) {
-                const int wy = e / SW, ww = e - wy * SW;
-                const long long gi = (long long)(y0 - 2 + wy) * W + (x0 - 2 + 2 * ww);
-                const bool in = gi >= 0 && gi < N;
+            if (u != ubeg) load_pre(u, P);  // a run that enters the next column
 #pragma unroll
-                for (int m = 0; m < 12; ++m)
-                    ub[m][wy][ww] =
-                        in ? *reinterpret_cast<const unsigned int*>(U + (size_t)(13 + m) * pitch + (size_t)gi) : 0u;
+            for (int k = 0; k < PE; ++k) {
+                const int e = (int)threadIdx.x + k * kBlock;
+                if (e < 2 * SW) {
+                    const int wy = e / SW, ww = e - wy * SW;
+#pragma unroll
+                    for (int m = 0; m < 12; ++m) ub[m][wy][ww] = P.u[k][m];
+                }
             }
             {
-                const long long gi = (long long)(y0 - 2 + lrow) * W + (x0 + R * tx);
-                const f32x4 a = buf_ld4(brr, gi), b = buf_ld4(bra, gi, first), c = buf_ld4(brp, gi);
+                const f32x4 a = P.a, b = P.b, c = P.c;
                 float rn[R], pn[R];
 #pragma unroll
                 for (int q = 0; q < R; ++q) pn[q] = advance(a[q], b[q], c[q], rn[q]);
@@ -930,9 +969,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
                 }
             }
             if (threadIdx.x < 16) {
-                const long long gs = side_gi(x0, y0, 0);
                 float rn;
-                const float pn = advance(buf_ld(brr, gs), buf_ld(bra, gs, first), buf_ld(brp, gs), rn);
+                const float pn = advance(P.sr, P.sa, P.sp, rn);
                 lds[((int)threadIdx.x >> 2) * ST + side_lx()] = pn;
             }
         } else {
