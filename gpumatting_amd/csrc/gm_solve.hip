@@ -59,28 +59,35 @@ __global__ void k_init_scalars(Scalars* sc, int maxit) {
     }
 }
 
-// <b,b> -> bb, tol2 = tol^2 * bb (tol <= 0: -1, never converged); x (may be
-// null): xzero = 1 when every x[i] is 0
+// <b,b> and <x,x> partial sums per workgroup (x may be null), for k_bnorm_fin
 __global__ __launch_bounds__(kBlock) void k_bnorm(const float* __restrict__ b, const float* __restrict__ x, int n,
-                                                  double tol, Scalars* sc, double* partials, unsigned int* counter) {
+                                                  double* __restrict__ partials) {
     double v[2] = {0.0, 0.0};
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
         v[0] += (double)b[i] * (double)b[i];
         if (x) v[1] += (double)x[i] * (double)x[i];
     }
-    grid_reduce<2>(v, partials, counter, [&](double* t) {
-        sc->xzero = x && t[1] == 0.0 ? 1 : 0;
-        sc->bb = t[0];
-        const double bb = t[0] > 0 ? t[0] : 1.0;
-        sc->tol2 = tol > 0 ? tol * tol * bb : -1.0;
-    });
+    store_partials<2>(v, partials);
 }
 
-// k_bnorm's grid: every workgroup fences once in the reduction (about 35 ns
-// each, serialised), so at most one workgroup per CU
-static int bnorm_grid(const DevInfo& d, long n) {
-    const int g = elementwise_grid(d, n, 8);
-    return g < d.cus ? g : d.cus;
+// bb = <b,b>, tol2 = tol^2 * bb (tol <= 0: -1, never converged), xzero = 1
+// when x was given and every x[i] is 0 (one workgroup)
+__global__ __launch_bounds__(kBlock) void k_bnorm_fin(const double* __restrict__ partials, int npart, int has_x,
+                                                      double tol, Scalars* sc) {
+    double t[2];
+    fold_all<2>(partials, npart, t);
+    if (threadIdx.x != 0) return;
+    sc->xzero = has_x && t[1] == 0.0 ? 1 : 0;
+    sc->bb = t[0];
+    const double bb = t[0] > 0 ? t[0] : 1.0;
+    sc->tol2 = tol > 0 ? tol * tol * bb : -1.0;
+}
+
+static void launch_bnorm(const DevInfo& d, hipStream_t hs, const float* b, const float* x, int n, double tol,
+                         Scalars* sc, double* partials) {
+    const int g = elementwise_grid(d, n, 4);
+    hipLaunchKernelGGL(k_bnorm, dim3(g), dim3(kBlock), 0, hs, b, x, n, partials);
+    hipLaunchKernelGGL(k_bnorm_fin, dim3(1), dim3(kBlock), 0, hs, partials, g, x ? 1 : 0, tol, sc);
 }
 
 // ---------------------------------------------------------------------------
@@ -2248,8 +2255,7 @@ static gm_status refine_impl(float* x, const BandedMatrix* A, const float* b, in
     hipLaunchKernelGGL(k_x64, dim3(ge), dim3(kBlock), 0, hs, w->x64, x, N, 0);
     GM_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(64), 0, hs, w->sc, 1);
-    hipLaunchKernelGGL(k_bnorm, dim3(bnorm_grid(dv, N)), dim3(kBlock), 0, hs, b, nullptr, N, 0.0, w->sc,
-                       rs.partials, rs.counter);
+    launch_bnorm(dv, hs, b, nullptr, N, 0.0, w->sc, rs.partials);
     GM_LAUNCH_CHECK();
     Scalars h;
     GM_HIP(hipMemcpyAsync(&h, w->sc, sizeof(Scalars), hipMemcpyDeviceToHost, hs));
@@ -2373,9 +2379,7 @@ gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, c
 
     hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(64), 0, S.hs, w->sc, o->max_iter);
     GM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_bnorm, dim3(bnorm_grid(S.dv, S.N)), dim3(kBlock), 0, S.hs, b, x, S.N, o->tol, w->sc,
-                       S.rs.partials,
-                       S.rs.counter);
+    launch_bnorm(S.dv, S.hs, b, x, S.N, o->tol, w->sc, S.rs.partials);
     GM_LAUNCH_CHECK();
     if (o->refine_mode == 2) {
         // the initial guess moves into the FP64 master; x accumulates from zero
@@ -2620,9 +2624,7 @@ gm_status gm_true_relres(const BandedMatrix* A, const float* x, const float* b, 
     S.g_gen = elementwise_grid(S.dv, S.N, 1);
     hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(64), 0, S.hs, S.w->sc, 1);
     GM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_bnorm, dim3(bnorm_grid(S.dv, S.N)), dim3(kBlock), 0, S.hs, b, nullptr, S.N, 0.0, S.w->sc,
-                       S.rs.partials,
-                       S.rs.counter);
+    launch_bnorm(S.dv, S.hs, b, nullptr, S.N, 0.0, S.w->sc, S.rs.partials);
     GM_LAUNCH_CHECK();
     if ((st = launch_trueres(S, 0)) != GM_OK) return st;
     Scalars h;
