@@ -164,16 +164,24 @@ __device__ __forceinline__ void b_step_enter(const BImg& I, int im, const Scalar
 // The 24 off-diagonal band values of pixel i (band order, 12 skipped).
 template <int FMT>
 __device__ __forceinline__ void load_pixel_bands(const BImg& I, int i, float (&a)[24]) {
+    if constexpr (FMT == kF32) {
 #pragma unroll
-    for (int k = 0, kk = 0; k < 25; ++k) {
-        if (k == 12) continue;
-        if constexpr (FMT == kF32) {
-            a[kk] = __builtin_nontemporal_load(I.L + (size_t)k * I.pitch + i);
-        } else {
-            const unsigned short h = __builtin_nontemporal_load(I.L16 + (size_t)k * I.pitch16 + i);
-            a[kk] = (float)__builtin_bit_cast(_Float16, h);
+        for (int k = 0, kk = 0; k < 25; ++k) {
+            if (k == 12) continue;
+            a[kk++] = __builtin_nontemporal_load(I.L + (size_t)k * I.pitch + i);
         }
-        ++kk;
+    } else {
+        // the 16-bit bands in groups of 8 per pixel (kb_to16): three 16-byte loads
+        const u32x4* g = reinterpret_cast<const u32x4*>(I.L16);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const u32x4 v = __builtin_nontemporal_load(g + (size_t)q * I.pitch16 + i);
+#pragma unroll
+            for (int h = 0; h < 8; ++h) {
+                const unsigned short u = (unsigned short)(v[h >> 1] >> (16 * (h & 1)));
+                a[8 * q + h] = (float)__builtin_bit_cast(_Float16, u);
+            }
+        }
     }
 }
 
@@ -585,12 +593,27 @@ __global__ __launch_bounds__(kBlock) void kb_set_rhs(const float* __restrict__ L
     }
 }
 
+// The off-diagonal bands in FP16, 8 per pixel in each of three groups: group q
+// of pixel i is the 16 bytes at element q * pitch16 + i, halves in band order
+// (bands 0..11, 13..24), so a pixel's 24 entries are three aligned 16-byte
+// loads (a lane per pixel: 1 KB contiguous per wave and load).
 __global__ __launch_bounds__(kBlock) void kb_to16(const float* __restrict__ L, size_t pitch, int N,
                                                   unsigned short* __restrict__ L16, size_t pitch16) {
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < N; i += gridDim.x * kBlock)
+    u32x4* g = reinterpret_cast<u32x4*>(L16);
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < N; i += gridDim.x * kBlock) {
 #pragma unroll
-        for (int k = 0; k < 25; ++k)
-            if (k != 12) L16[(size_t)k * pitch16 + i] = f32_to_f16_rne(L[(size_t)k * pitch + i]);
+        for (int q = 0; q < 3; ++q) {
+            u32x4 v;
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                const int kk0 = 8 * q + 2 * w, kk1 = kk0 + 1;  // off-diagonal band indices
+                const int k0 = kk0 < 12 ? kk0 : kk0 + 1, k1 = kk1 < 12 ? kk1 : kk1 + 1;
+                v[w] = (unsigned)f32_to_f16_rne(L[(size_t)k0 * pitch + i]) |
+                       ((unsigned)f32_to_f16_rne(L[(size_t)k1 * pitch + i]) << 16);
+            }
+            g[(size_t)q * pitch16 + i] = v;
+        }
+    }
 }
 
 __global__ void kb_init(Scalars* sc, int nsys, int maxit) {
