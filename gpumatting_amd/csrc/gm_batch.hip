@@ -231,10 +231,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         float a[24];
         if (valid) load_pixel_bands<FMT>(I, i, a);
         float sv[kMaxRhs][kWinPer];
+        float dd[kMaxRhs];  // each system's diagonal at the thread's pixel, loaded with the rest
 #pragma unroll
         for (int j = 0; j < kMaxRhs; ++j) {
             if (!((mask >> j) & 1u)) continue;
             const BSys S = sys[I.sys0 + j];
+            dd[j] = valid ? S.d[i] : 0.f;
             const float beta = chand[j].beta;
             const float* __restrict__ pold = (chand[j].iters & 1) ? S.p1 : S.p0;
 #pragma unroll
@@ -286,7 +288,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 ++kk;
             }
             const float pc = w[rb[2] + 2];
-            yv = fmaf(S.d[i], pc, yv);
+            yv = fmaf(dd[j], pc, yv);
             float* __restrict__ pnew = (chand[j].iters & 1) ? S.p0 : S.p1;
             pnew[i] = pc;
             S.Ap[i] = yv;
