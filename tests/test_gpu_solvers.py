@@ -351,7 +351,7 @@ def test_every_tile_shape_same_iteration(gpu, small, prec, shapes, width):
     call("gm_tune_set", p.W, p.N, prec, *shapes[0], 4)
 
 
-@pytest.mark.parametrize("W,H", [(256, 192), (1000, 37), (1916, 203), (3840, 64), (16640, 8)])
+@pytest.mark.parametrize("W,H", [(256, 192), (1000, 37), (1916, 203), (3840, 64), (16640, 8), (512, 3), (8, 64)])
 def test_fused_iteration_matches_step_and_update(gpu, W, H):
     """One launch per CG iteration (the update of x and r fused into the next
     step, beta from the expanded <r,r> recurrence, r and Ap alternating between
@@ -359,8 +359,9 @@ def test_fused_iteration_matches_step_and_update(gpu, W, H):
     agree to rounding, and runs to a tolerance take the same iterations and
     replacements (every 7 iterations here, so that replacements land on odd
     and even iteration counts) and reach it.  Sizes: ragged right and bottom
-    tiles, 15 columns with runs of one tile, and 65 columns (more than the
-    workgroups per XCD: runs that cross into the next column)."""
+    tiles, 15 columns with runs of one tile, 65 columns (more than the
+    workgroups per XCD: runs that cross into the next column), an image of 3
+    rows (one partial tile per column) and one 8 pixels wide."""
     p = synth.make_problem(W, H, 4321, 1, 0.08)
     s = _gpu_system(p)
     res = {}
