@@ -10,12 +10,14 @@
 // lam_ij.  Jacobi-PCG per system, with the single-system kernels' sign,
 // residual-replacement and hand-off conventions (see Scalars).
 //
-// Work lists.  The step and true-residual kernels walk 64x4-pixel tiles (one
-// pixel per thread, any image width), numbered down each image's 64-pixel
+// Work lists.  The step and true-residual kernels walk 128x2-pixel tiles (one
+// pixel per thread, any image width), numbered down each image's 128-pixel
 // columns; the update kernel walks 1024-pixel chunks.  Each list concatenates
 // all images.  Going down a column, a tile's x window (2-pixel halo) shares 4
-// of its 8 rows with the tile above: the window is an LDS ring, and a
-// workgroup that has just done the tile above loads only the 4 new rows.  A launch splits its list into
+// of its 6 rows with the tile above: the window is an LDS ring, and a
+// workgroup that has just done the tile above loads only the 2 new rows.
+// (Against 64x4 tiles, the 128-wide window rows touch fewer cache lines
+// outside the tile per pixel.)  A launch splits its list into
 // contiguous ranges, one per workgroup, so a workgroup touches few images: it
 // accumulates its partial sums per (image, system) in registers and stores
 // one slot per image it touches.  The consumer kernel folds an image's slots
@@ -42,13 +44,17 @@ gm_status levin_assemble_raw(float* A, size_t apitch, float* b, float* dinv, con
                              int mode, int W, int H, gm_stream_t s);
 
 constexpr int kMaxRhs = 8;
-constexpr int kTX = 64, kTY = kBlock / kTX;  // step / true-residual tile: 64 x 4 pixels
-constexpr int kWX = kTX + 4, kWY = kTY + 4;   // staged window (2-pixel halo)
-constexpr int kWin = kWX * kWY;               // 544 cells
-constexpr int kWinPer = (kWin + kBlock - 1) / kBlock;
-static_assert(kWY == 2 * kTY && (kWY & (kWY - 1)) == 0, "window ring: rows advance by kTY mod kWY");
+constexpr int kTX = 128, kTY = kBlock / kTX;  // step / true-residual tile: 128 x 2 pixels
+constexpr int kWX = kTX + 4, kWY = kTY + 4;   // staged window (2-pixel halo), kWY rows
+constexpr int kKeep = 4;                      // window rows a tile shares with the tile directly above
+constexpr int kNew = (kWY - kKeep) * kWX;     // cells of the window rows below those
+constexpr int kNewPer = (kNew + kBlock - 1) / kBlock;
+static_assert(kTY * kTX == kBlock && kTY <= kKeep, "one pixel per thread; a tile shares 4 window rows");
 constexpr int kChunk = 4 * kBlock;            // update-kernel chunk: 1024 pixels, 4 per thread
-static_assert(kTY * kTX == kBlock, "one pixel per thread");
+// The window is a ring of kWY LDS rows: window row wy of a tile is ring row
+// ring_row(top + wy); the tile below starts at ring_row(top + kTY), so the
+// kKeep rows it shares stay where they are.  (s < 2 kWY)
+__device__ __forceinline__ int ring_row(int s) { return s >= kWY ? s - kWY : s; }
 
 // Static split of a work list over the workgroups of a launch.
 struct BPart {
@@ -161,38 +167,133 @@ __device__ __forceinline__ void b_step_enter(const BImg& I, int im, const Scalar
     __syncthreads();
 }
 
-// The 24 off-diagonal band values of pixel i (band order, 12 skipped).
+// Element at a byte offset from a vector's base, in global memory (the
+// pointers come from device tables, so the compiler cannot tell by itself and
+// would use flat accesses, which also count against the LDS wait counter).
+// Images have at most 2^30 pixels, so offsets fit 32 bits: the access takes
+// the base from scalar registers and one 32-bit offset register, which the
+// systems' vectors share.
+typedef __attribute__((address_space(1))) char gchar;
+__device__ __forceinline__ float at_off(const float* base, unsigned off) {
+    return *reinterpret_cast<const __attribute__((address_space(1))) float*>((const gchar*)base + off);
+}
+__device__ __forceinline__ void store_off(float* base, unsigned off, float v) {
+    *reinterpret_cast<__attribute__((address_space(1))) float*>((gchar*)base + off) = v;
+}
+__device__ __forceinline__ f32x4 at_off4(const float* base, unsigned off) {
+    return *reinterpret_cast<const __attribute__((address_space(1))) f32x4*>((const gchar*)base + off);
+}
+__device__ __forceinline__ void store_off4(float* base, unsigned off, f32x4 v) {
+    *reinterpret_cast<__attribute__((address_space(1))) f32x4*>((gchar*)base + off) = v;
+}
+template <class T>
+__device__ __forceinline__ T nt_at_off(const T* base, unsigned off) {
+    return __builtin_nontemporal_load(reinterpret_cast<const __attribute__((address_space(1))) T*>((const gchar*)base + off));
+}
+
+// The 24 off-diagonal band values of pixel i (band order, 12 skipped), as
+// loaded: FP32 values, or the 16-bit bands in groups of 8 per pixel (kb_to16).
 template <int FMT>
-__device__ __forceinline__ void load_pixel_bands(const BImg& I, int i, float (&a)[24]) {
+struct PixBands {
+    float a[24];
+};
+template <>
+struct PixBands<kF16> {
+    u32x4 g[3];
+};
+template <int FMT>
+__device__ __forceinline__ void load_pixel_bands(const BImg& I, int i, PixBands<FMT>& pb) {
     if constexpr (FMT == kF32) {
 #pragma unroll
         for (int k = 0, kk = 0; k < 25; ++k) {
             if (k == 12) continue;
-            a[kk++] = __builtin_nontemporal_load(I.L + (size_t)k * I.pitch + i);
+            pb.a[kk++] = nt_at_off(I.L + (size_t)k * I.pitch, 4u * (unsigned)i);
         }
     } else {
-        // the 16-bit bands in groups of 8 per pixel (kb_to16): three 16-byte loads
         const u32x4* g = reinterpret_cast<const u32x4*>(I.L16);
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            const u32x4 v = __builtin_nontemporal_load(g + (size_t)q * I.pitch16 + i);
+        for (int q = 0; q < 3; ++q) pb.g[q] = nt_at_off(g + (size_t)q * I.pitch16, 16u * (unsigned)i);
+    }
+}
+template <int FMT>
+__device__ __forceinline__ void pixel_bands(const PixBands<FMT>& pb, float (&a)[24]) {
+    if constexpr (FMT == kF32) {
+#pragma unroll
+        for (int k = 0; k < 24; ++k) a[k] = pb.a[k];
+    } else {
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
 #pragma unroll
             for (int h = 0; h < 8; ++h) {
-                const unsigned short u = (unsigned short)(v[h >> 1] >> (16 * (h & 1)));
+                const unsigned short u = (unsigned short)(pb.g[q][h >> 1] >> (16 * (h & 1)));
                 a[8 * q + h] = (float)__builtin_bit_cast(_Float16, u);
+            }
+    }
+}
+
+// Window staging of the step and true-residual kernels.  cell(j, o) is
+// system j's staged value at the matrix row of byte offset o; cells beside
+// the image alias the neighbouring row, exactly like the row-wise product
+// (their band entries are 0), and cells outside the vector are 0.
+// window_new_load: the cells of window rows kKeep.. into registers (issued
+// before the previous tile is done with the LDS ring).
+template <class Cell>
+__device__ __forceinline__ void window_new_load(const BImg& I, unsigned mask, int x0, int y0, Cell cell,
+                                                float (&sv)[kMaxRhs][kNewPer]) {
+#pragma unroll
+    for (int j = 0; j < kMaxRhs; ++j) {
+        if (!((mask >> j) & 1u)) continue;
+#pragma unroll
+        for (int q = 0; q < kNewPer; ++q) {
+            const int e = (int)threadIdx.x + q * kBlock;
+            const int wy = kKeep + e / kWX, wx = e % kWX;
+            const long long g = (long long)(y0 - 2 + wy) * I.W + (x0 - 2 + wx);
+            sv[j][q] = (e < kNew && g >= 0 && g < I.N) ? cell(j, 4u * (unsigned)g) : 0.f;
+        }
+    }
+}
+// window_store: the ring rows of the tile at (x0, y0) with its first row at
+// ring row top: the registers of window_new_load, and, for a tile that is not
+// directly below the previous one (carry false), its first kKeep rows loaded
+// here.  Every thread calls it (two barriers).
+template <class Cell>
+__device__ __forceinline__ void window_store(const BImg& I, unsigned mask, int x0, int y0, bool carry, int top,
+                                             Cell cell, const float (&sv)[kMaxRhs][kNewPer],
+                                             float (*win)[kWY * kWX]) {
+    __syncthreads();  // the previous tile is done with win
+    if (!carry) {
+        for (int j = 0; j < kMaxRhs; ++j) {
+            if (!((mask >> j) & 1u)) continue;
+            for (int e = (int)threadIdx.x; e < kKeep * kWX; e += kBlock) {
+                const int wy = e / kWX, wx = e - wy * kWX;
+                const long long g = (long long)(y0 - 2 + wy) * I.W + (x0 - 2 + wx);
+                win[j][ring_row(top + wy) * kWX + wx] = (g >= 0 && g < I.N) ? cell(j, 4u * (unsigned)g) : 0.f;
             }
         }
     }
+#pragma unroll
+    for (int j = 0; j < kMaxRhs; ++j) {
+        if (!((mask >> j) & 1u)) continue;
+#pragma unroll
+        for (int q = 0; q < kNewPer; ++q) {
+            const int e = (int)threadIdx.x + q * kBlock;
+            if (e < kNew) {
+                const int r = e / kWX;
+                win[j][ring_row(top + kKeep + r) * kWX + (e - r * kWX)] = sv[j][q];
+            }
+        }
+    }
+    __syncthreads();
 }
 
 // K1: per running system p = -D^-1 r + beta p_old (staged), Ap = (L_off +
 // diag d) p, <p, Ap> partials.
-// (at most 128 VGPRs: 4 waves per SIMD; the FP32 form otherwise takes 129)
+// (at most 128 VGPRs: 4 waves per SIMD)
 template <int FMT>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void kb_step(const BImg* __restrict__ imgs, const BSys* __restrict__ sys,
                                                   Scalars* sc, BPart P1, BPart P2, double* __restrict__ slot1,
                                                   const double* __restrict__ slot2, int verify) {
-    __shared__ float win[kMaxRhs][kWin];
+    __shared__ float win[kMaxRhs][kWY * kWX];
     __shared__ Hand chand[kMaxRhs];
     const int b = blockIdx.x;
     const int tb = part_begin(P1, b), te = part_begin(P1, b + 1);
@@ -203,6 +304,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     BImg I{};
     // the window in LDS: origin of its tile (wx0 < 0: none) and ring row of window row 0
     int wx0 = -1, wy0 = 0, top = 0;
+    auto cell = [&](int j, unsigned o) {
+        const BSys& S = sys[I.sys0 + j];
+        const float beta = chand[j].beta;
+        const float* __restrict__ pold = (chand[j].iters & 1) ? S.p1 : S.p0;
+        const float z = at_off(S.z, o);
+        return beta != 0.f ? fmaf(beta, at_off(pold, o), -z) : -z;
+    };
     for (int t = tb; t < te; ++t) {
         const int im = P1.img[t];
         if (im != cur) {
@@ -225,56 +333,26 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         const int i = y * I.W + x;
         // the tile directly above is in LDS: its window rows kTY.. are this one's rows 0..
         const bool carry = x0 == wx0 && y0 == wy0 + kTY;
-        top = carry ? (top + kTY) & (kWY - 1) : 0;
-        const int e0 = carry ? kTY * kWX : 0;  // first window cell to load
+        top = carry ? ring_row(top + kTY) : 0;
         // every global load of the tile first, then LDS
-        float a[24];
-        if (valid) load_pixel_bands<FMT>(I, i, a);
-        float sv[kMaxRhs][kWinPer];
-        float dd[kMaxRhs];  // each system's diagonal at the thread's pixel, loaded with the rest
+        PixBands<FMT> pb;
+        if (valid) load_pixel_bands<FMT>(I, i, pb);
+        float dd[kMaxRhs];  // each system's diagonal at the thread's pixel
 #pragma unroll
-        for (int j = 0; j < kMaxRhs; ++j) {
-            if (!((mask >> j) & 1u)) continue;
-            const BSys S = sys[I.sys0 + j];
-            dd[j] = valid ? S.d[i] : 0.f;
-            const float beta = chand[j].beta;
-            const float* __restrict__ pold = (chand[j].iters & 1) ? S.p1 : S.p0;
-#pragma unroll
-            for (int q = 0; q < kWinPer; ++q) {
-                const int e = e0 + (int)threadIdx.x + q * kBlock;
-                const int wy = e / kWX, wx = e - wy * kWX;
-                // matrix-row index of the window cell (cells beside the image alias the neighbouring
-                // row, exactly like the row-wise product; their band entries are 0)
-                const long long g = (long long)(y0 - 2 + wy) * I.W + (x0 - 2 + wx);
-                float v = 0.f;
-                if (e < kWin && g >= 0 && g < I.N) {
-                    const float z = S.z[g];
-                    v = beta != 0.f ? fmaf(beta, pold[g], -z) : -z;
-                }
-                sv[j][q] = v;
-            }
-        }
-        __syncthreads();  // the previous tile is done with win
-#pragma unroll
-        for (int j = 0; j < kMaxRhs; ++j) {
-            if (!((mask >> j) & 1u)) continue;
-#pragma unroll
-            for (int q = 0; q < kWinPer; ++q) {
-                const int e = e0 + (int)threadIdx.x + q * kBlock;
-                if (e < kWin) {
-                    const int wy = e / kWX;
-                    win[j][((top + wy) & (kWY - 1)) * kWX + (e - wy * kWX)] = sv[j][q];
-                }
-            }
-        }
-        __syncthreads();
+        for (int j = 0; j < kMaxRhs; ++j)
+            if ((mask >> j) & 1u) dd[j] = valid ? at_off(sys[I.sys0 + j].d, 4u * (unsigned)i) : 0.f;
+        float sv[kMaxRhs][kNewPer];
+        window_new_load(I, mask, x0, y0, cell, sv);
+        window_store(I, mask, x0, y0, carry, top, cell, sv, win);
         wx0 = x0;
         wy0 = y0;
         if (!valid) continue;
+        float a[24];
+        pixel_bands<FMT>(pb, a);
         // LDS offsets of the thread's 5 window rows (cell (row + q, lane) is pixel (y - 2 + q, x - 2))
         int rb[5];
 #pragma unroll
-        for (int q = 0; q < 5; ++q) rb[q] = ((top + row + q) & (kWY - 1)) * kWX + lane;
+        for (int q = 0; q < 5; ++q) rb[q] = ring_row(top + row + q) * kWX + lane;
 #pragma unroll
         for (int j = 0; j < kMaxRhs; ++j) {
             if (!((mask >> j) & 1u)) continue;
@@ -290,8 +368,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             const float pc = w[rb[2] + 2];
             yv = fmaf(dd[j], pc, yv);
             float* __restrict__ pnew = (chand[j].iters & 1) ? S.p0 : S.p1;
-            pnew[i] = pc;
-            S.Ap[i] = yv;
+            store_off(pnew, 4u * (unsigned)i, pc);
+            store_off(S.Ap, 4u * (unsigned)i, yv);
             acc[j][0] += (double)(pc * yv);
         }
     }
@@ -373,11 +451,9 @@ __global__ __launch_bounds__(kBlock) void kb_update(const BImg* __restrict__ img
             const float al = calpha[j];
             const float* __restrict__ p = (chand[j].iters & 1) ? S.p0 : S.p1;  // the p_new K1 wrote
             if (i0 + 4 <= I.N) {  // per-system vectors are 256-byte aligned and i0 % 4 == 0
-                f32x4 xv = *reinterpret_cast<const f32x4*>(S.x + i0);
-                f32x4 rv = *reinterpret_cast<const f32x4*>(S.r + i0);
-                const f32x4 pv = *reinterpret_cast<const f32x4*>(p + i0);
-                const f32x4 av = *reinterpret_cast<const f32x4*>(S.Ap + i0);
-                const f32x4 dv = *reinterpret_cast<const f32x4*>(S.dinv + i0);
+                const unsigned o = 4u * (unsigned)i0;
+                const f32x4 xv = at_off4(S.x, o), rv = at_off4(S.r, o), pv = at_off4(p, o), av = at_off4(S.Ap, o),
+                            dv = at_off4(S.dinv, o);
                 float xs[4] = {xv.x, xv.y, xv.z, xv.w}, rs[4] = {rv.x, rv.y, rv.z, rv.w};
                 const float ps[4] = {pv.x, pv.y, pv.z, pv.w}, as[4] = {av.x, av.y, av.z, av.w};
                 const float ds[4] = {dv.x, dv.y, dv.z, dv.w};
@@ -390,9 +466,9 @@ __global__ __launch_bounds__(kBlock) void kb_update(const BImg* __restrict__ img
                     acc[j][0] += (double)(rs[q] * zs[q]);
                     acc[j][1] += (double)(rs[q] * rs[q]);
                 }
-                *reinterpret_cast<f32x4*>(S.x + i0) = f32x4{xs[0], xs[1], xs[2], xs[3]};
-                *reinterpret_cast<f32x4*>(S.r + i0) = f32x4{rs[0], rs[1], rs[2], rs[3]};
-                *reinterpret_cast<f32x4*>(S.z + i0) = f32x4{zs[0], zs[1], zs[2], zs[3]};
+                store_off4(S.x, o, f32x4{xs[0], xs[1], xs[2], xs[3]});
+                store_off4(S.r, o, f32x4{rs[0], rs[1], rs[2], rs[3]});
+                store_off4(S.z, o, f32x4{zs[0], zs[1], zs[2], zs[3]});
             } else {
                 for (int i = i0; i < I.N; ++i) {
                     S.x[i] = fmaf(al, p[i], S.x[i]);
@@ -451,7 +527,7 @@ __device__ __forceinline__ bool tr_selected(const Scalars* s, int op) {
 // replace r for systems whose recursive residual asked for verification.
 __global__ __launch_bounds__(kBlock) void kb_trueres(const BImg* __restrict__ imgs, const BSys* __restrict__ sys,
                                                      const Scalars* sc, BPart P1, int op, double* __restrict__ slot3) {
-    __shared__ float win[kMaxRhs][kWin];
+    __shared__ float win[kMaxRhs][kWY * kWX];
     const int b = blockIdx.x;
     const int tb = part_begin(P1, b), te = part_begin(P1, b + 1);
     const int lane = (int)threadIdx.x % kTX, row = (int)threadIdx.x / kTX;
@@ -481,43 +557,21 @@ __global__ __launch_bounds__(kBlock) void kb_trueres(const BImg* __restrict__ im
         const bool valid = x < I.W && y < I.H;
         const int i = y * I.W + x;
         const bool carry = x0 == wx0 && y0 == wy0 + kTY;
-        top = carry ? (top + kTY) & (kWY - 1) : 0;
-        const int e0 = carry ? kTY * kWX : 0;
-        float a[24];
-        if (valid) load_pixel_bands<kF32>(I, i, a);
-        float sv[kMaxRhs][kWinPer];
-#pragma unroll
-        for (int j = 0; j < kMaxRhs; ++j) {
-            if (!((mask >> j) & 1u)) continue;
-            const BSys S = sys[I.sys0 + j];
-#pragma unroll
-            for (int q = 0; q < kWinPer; ++q) {
-                const int e = e0 + (int)threadIdx.x + q * kBlock;
-                const int wy = e / kWX, wx = e - wy * kWX;
-                const long long g = (long long)(y0 - 2 + wy) * I.W + (x0 - 2 + wx);
-                sv[j][q] = (e < kWin && g >= 0 && g < I.N) ? S.x[g] : 0.f;
-            }
-        }
-        __syncthreads();  // the previous tile is done with win
-#pragma unroll
-        for (int j = 0; j < kMaxRhs; ++j) {
-            if (!((mask >> j) & 1u)) continue;
-#pragma unroll
-            for (int q = 0; q < kWinPer; ++q) {
-                const int e = e0 + (int)threadIdx.x + q * kBlock;
-                if (e < kWin) {
-                    const int wy = e / kWX;
-                    win[j][((top + wy) & (kWY - 1)) * kWX + (e - wy * kWX)] = sv[j][q];
-                }
-            }
-        }
-        __syncthreads();
+        top = carry ? ring_row(top + kTY) : 0;
+        PixBands<kF32> pb;
+        if (valid) load_pixel_bands<kF32>(I, i, pb);
+        auto cell = [&](int j, unsigned o) { return at_off(sys[I.sys0 + j].x, o); };
+        float sv[kMaxRhs][kNewPer];
+        window_new_load(I, mask, x0, y0, cell, sv);
+        window_store(I, mask, x0, y0, carry, top, cell, sv, win);
         wx0 = x0;
         wy0 = y0;
         if (!valid) continue;
+        float a[24];
+        pixel_bands<kF32>(pb, a);
         int rb[5];
 #pragma unroll
-        for (int q = 0; q < 5; ++q) rb[q] = ((top + row + q) & (kWY - 1)) * kWX + lane;
+        for (int q = 0; q < 5; ++q) rb[q] = ring_row(top + row + q) * kWX + lane;
 #pragma unroll
         for (int j = 0; j < kMaxRhs; ++j) {
             if (!((mask >> j) & 1u)) continue;
@@ -530,20 +584,21 @@ __global__ __launch_bounds__(kBlock) void kb_trueres(const BImg* __restrict__ im
                 s = fma((double)a[kk], (double)w[rb[k / 5] + (k % 5)], s);
                 ++kk;
             }
-            const float bi = S.b[i];
-            const double tq = fma((double)S.d[i], (double)w[rb[2] + 2], s) - (double)bi;
+            const unsigned o = 4u * (unsigned)i;
+            const float bi = at_off(S.b, o);
+            const double tq = fma((double)at_off(S.d, o), (double)w[rb[2] + 2], s) - (double)bi;
             const float out = (float)tq;
-            const float zo = S.dinv[i] * out;
+            const float zo = at_off(S.dinv, o) * out;
             acc[j][0] += (double)(out * zo);
             acc[j][1] += tq * tq;
             if ((dmask >> j) & 1u) {
-                const double dd = (double)out - (double)S.r[i];
+                const double dd = (double)out - (double)at_off(S.r, o);
                 acc[j][2] += dd * dd;
             }
             acc[j][3] += (double)bi * (double)bi;
             if (op != 0) {
-                S.r[i] = out;
-                S.z[i] = zo;
+                store_off(S.r, o, out);
+                store_off(S.z, o, zo);
             }
         }
     }
