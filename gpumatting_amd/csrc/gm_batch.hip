@@ -471,11 +471,12 @@ __global__ __launch_bounds__(kBlock) void kb_update(const BImg* __restrict__ img
                 store_off4(S.z, o, f32x4{zs[0], zs[1], zs[2], zs[3]});
             } else {
                 for (int i = i0; i < I.N; ++i) {
-                    S.x[i] = fmaf(al, p[i], S.x[i]);
-                    const float rn = fmaf(al, S.Ap[i], S.r[i]);
-                    const float zn = S.dinv[i] * rn;
-                    S.r[i] = rn;
-                    S.z[i] = zn;
+                    const unsigned o = 4u * (unsigned)i;
+                    store_off(S.x, o, fmaf(al, at_off(p, o), at_off(S.x, o)));
+                    const float rn = fmaf(al, at_off(S.Ap, o), at_off(S.r, o));
+                    const float zn = at_off(S.dinv, o) * rn;
+                    store_off(S.r, o, rn);
+                    store_off(S.z, o, zn);
                     acc[j][0] += (double)(rn * zn);
                     acc[j][1] += (double)(rn * rn);
                 }
