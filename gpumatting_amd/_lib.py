@@ -34,6 +34,18 @@ class BandedMatrix(C.Structure):
                 ("nbands", C.c_int), ("rows", C.c_int), ("cols", C.c_int)]
 
 
+class CompressedMatrix(C.Structure):
+    """Same layout as the reference's CompressedMatrix (CompressedMatrix.h:26-37)."""
+    _fields_ = [("p", C.c_void_p), ("j", C.c_void_p), ("k", C.c_void_p), ("rows", C.c_int), ("cols", C.c_int),
+                ("nnz", C.c_size_t), ("length", C.c_size_t)]
+
+
+class CoordMatrix(C.Structure):
+    """Same layout as the reference's CoordMatrix (CoordMatrix.h:25-36)."""
+    _fields_ = [("i", C.c_void_p), ("j", C.c_void_p), ("k", C.c_void_p), ("rows", C.c_int), ("cols", C.c_int),
+                ("nnz", C.c_size_t), ("length", C.c_size_t)]
+
+
 class SolveOpts(C.Structure):
     _fields_ = [("method", C.c_int), ("precision", C.c_int), ("max_iter", C.c_int), ("tol", C.c_double),
                 ("restart", C.c_int), ("refine_every", C.c_int), ("check_every", C.c_int), ("verify", C.c_int),
@@ -57,6 +69,8 @@ _LIB = None
 
 vp, ci, cf, cd, sz = C.c_void_p, C.c_int, C.c_float, C.c_double, C.c_size_t
 BMp = C.POINTER(BandedMatrix)
+CSMp = C.POINTER(CompressedMatrix)
+COOp = C.POINTER(CoordMatrix)
 
 _SIGS = {
     "gm_last_error": ([], C.c_char_p),
@@ -116,6 +130,20 @@ _SIGS = {
     "gm_clamped_mse": ([vp, vp, ci, C.POINTER(cd)], ci),
     "gm_bm_dump": ([BMp, C.c_char_p], ci),
     "gm_bm_load": ([BMp, C.c_char_p], ci),
+    "gm_csm_init": ([CSMp, ci, ci, sz], ci),
+    "gm_csm_free": ([CSMp], ci),
+    "gm_coo_init": ([COOp, ci, ci, sz], ci),
+    "gm_coo_release": ([COOp], ci),
+    "gm_coo_append": ([COOp, vp, vp, vp, sz], ci),
+    "gm_coo_to_csm": ([CSMp, COOp, vp], ci),
+    "gm_csm_ax": ([vp, CSMp, vp, vp], ci),
+    "gm_csm_axpy": ([vp, CSMp, vp, vp, ci, vp], ci),
+    "gm_coo_axpy": ([vp, COOp, vp, vp, vp], ci),
+    "gm_sp_restriction": ([CSMp, vp, ci, ci, vp], ci),
+    "gm_sp_prolong": ([vp, vp, vp, ci, ci, vp], ci),
+    "gm_sp_galerkin": ([CSMp, BMp, vp, ci, vp], ci),
+    "gm_sp_merge_levels": ([vp, C.POINTER(ci), vp, ci, sz, ci, ci, ci, cf], ci),
+    "gm_slic": ([vp, C.POINTER(ci), vp, ci, sz, ci, ci, ci, cf, ci, vp], ci),
 }
 
 

@@ -16,6 +16,8 @@
 
 #include <stddef.h>
 #include "BandedMatrix.h"
+#include "CompressedMatrix.h"
+#include "CoordMatrix.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -292,6 +294,89 @@ int gm_batch_tiles(gm_batch* batch);
 
 /* Release cached solver workspaces (device memory held between solves). */
 gm_status gm_release_workspaces(void);
+
+/* ---------------------------------------------------------------------- */
+/* Sparse matrices and superpixel multilevel operators                     */
+/* (CompressedMatrix.cu, CoordMatrix.cu, gridoperators.h, SLIC.h)          */
+/*                                                                         */
+/* The reference's sparse layer is unfinished and never called (element    */
+/* counts passed as byte counts, a row bound off by one, y[0] added to     */
+/* every row, a float atomicAdd product, host-to-host copies); these entry */
+/* points give the operations it names their evident meaning.  Every       */
+/* product is deterministic (fixed summation order, FP64 accumulation of   */
+/* FP32 products).                                                         */
+
+/* csmInit, CompressedMatrix.cu:20-31: rows x cols, room for `length`
+ * entries, p zeroed, nnz = 0 (device arrays). */
+gm_status gm_csm_init(CompressedMatrix* m, int rows, int cols, size_t length);
+/* Free what gm_csm_init (or a builder below) allocated; zeroes *m. */
+gm_status gm_csm_free(CompressedMatrix* m);
+/* cooInit, CoordMatrix.cu:20-29 */
+gm_status gm_coo_init(CoordMatrix* m, int rows, int cols, size_t length);
+/* cooRelease, CoordMatrix.cu:31-40 */
+gm_status gm_coo_release(CoordMatrix* m);
+/* cooAppend, CoordMatrix.cu:42-48: append n host entries (i, j, k) after the
+ * nnz already held; indices are checked against rows / cols and the total
+ * against length (nothing is appended on error).  Synchronous. */
+gm_status gm_coo_append(CoordMatrix* m, const int* i, const int* j, const float* k, size_t n);
+/* CSR form of a COO matrix (no reference counterpart): entries sorted by
+ * (row, column), repeats kept in their COO order.  Allocates *out (free with
+ * gm_csm_free).  Synchronous. */
+gm_status gm_coo_to_csm(CompressedMatrix* out, const CoordMatrix* in, gm_stream_t stream);
+/* csmAx, CompressedMatrix.cu:118-165: b = A x (b: rows, x: cols floats). */
+gm_status gm_csm_ax(float* b, const CompressedMatrix* A, const float* x, gm_stream_t stream);
+/* csmAxpy, CompressedMatrix.cu:45-85: b = A x + y (add) or A x - y. */
+gm_status gm_csm_axpy(float* b, const CompressedMatrix* A, const float* x, const float* y, int add,
+                      gm_stream_t stream);
+/* cooAxpy, CoordMatrix.cu:50-87: b += A x + y (the reference accumulates onto
+ * b).  Sorts a copy of the entries on every call: convert once with
+ * gm_coo_to_csm for repeated products.  Synchronous. */
+gm_status gm_coo_axpy(float* b, const CoordMatrix* A, const float* x, const float* y, gm_stream_t stream);
+
+/* First level of downsampleOps, gridoperators.h:18-57: the restriction P
+ * (nlabels x npix, P[labels[u], u] = 1) of a superpixel labelling, as CSR
+ * (each row's pixels ascending).  labels: npix device ints in [0, nlabels)
+ * (GM_ERR_INVALID otherwise).  Allocates *P.  Synchronous.  P x sums a pixel
+ * vector over each superpixel (gm_csm_ax); gm_sp_prolong is P^T. */
+gm_status gm_sp_restriction(CompressedMatrix* P, const int* labels, int npix, int nlabels, gm_stream_t stream);
+/* e[u] = xc[labels[u]] for u < npix: the prolongation P^T xc (no reference
+ * counterpart: the reference stops at building P); NaN where a label is
+ * outside [0, nlabels). */
+gm_status gm_sp_prolong(float* e, const float* xc, const int* labels, int npix, int nlabels, gm_stream_t stream);
+/* The Galerkin coarse operator Ac = P A P^T (nlabels x nlabels CSR, repeats
+ * merged: one entry per (row, column), summed in FP64 in the order of the
+ * fine entries (pixel, band)) of a device banded matrix (any band count;
+ * entries stored as 0 are skipped) and a labelling as above -- the operator
+ * the reference's multilevel direction coarsens with.  Allocates *Ac.
+ * Synchronous. */
+gm_status gm_sp_galerkin(CompressedMatrix* Ac, const BandedMatrix* A, const int* labels, int nlabels,
+                         gm_stream_t stream);
+/* Later levels of downsampleOps, gridoperators.h:59-138 (there unfinished):
+ * merge superpixels pairwise, greedily by ascending squared distance of their
+ * mean colours, while that distance is at most `threshold` (the reference:
+ * 0.0111); only superpixels that share a 4-neighbour pixel edge merge (the
+ * reference's "need to check ... spatially adjacent"), and each merges at most
+ * once per level.  Each level maps label c to map[c] (merged pairs take new
+ * labels in merge order, the rest follow in label order) and the means are
+ * recomputed from the pixels (its "need to update means").  im: host RGB
+ * (nch floats per pixel, pitch pixels per row); labels: host, updated in
+ * place to the last level's labels; *nlabels in/out.  Host computation. */
+gm_status gm_sp_merge_levels(int* labels, int* nlabels, const float* im, int nch, size_t pitch, int W, int H,
+                             int nlevels, float threshold);
+/* SLIC superpixels (slicSegmentation, SLIC.h:43; algorithm of Achanta et al.,
+ * "SLIC Superpixels Compared to State-of-the-Art Superpixel Methods", TPAMI
+ * 2012): CIELAB k-means over (l, a, b, x, y) with grid step S = round(sqrt(W H
+ * / K)), seeds moved to the lowest gradient of their 3x3 neighbourhood,
+ * `iters` assignment / update rounds (distance (dc/m)^2 + (ds/S)^2 over the
+ * seeds whose 2S x 2S window holds the pixel), then connected components:
+ * every 4-connected piece of a label becomes a superpixel, and pieces smaller
+ * than a quarter of S^2 join the superpixel of the pixel left of (or else
+ * above) their first pixel in raster order.  Final labels are consecutive
+ * from 0 in raster order of first appearance.  im: device RGB in [0, 1] (nch
+ * floats per pixel, pitch pixels per row), labels: W*H device ints.
+ * Synchronous. */
+gm_status gm_slic(int* labels, int* nlabels, const float* im, int nch, size_t pitch, int W, int H, int K,
+                  float compactness, int iters, gm_stream_t stream);
 
 /* ---- host file I/O of the matting driver (no device work) ---------------
  * Netpbm: binary or ASCII, header tokens separated by whitespace with '#'

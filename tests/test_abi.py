@@ -69,10 +69,10 @@ def test_cpp_dropin_headers_compile():
 
 
 def test_c_abi_header_is_plain_c():
-    """include/gpumatting.h and BandedMatrix.h compile as C99 with every
-    warning an error, so C, cgo-style and ctypes callers can include them."""
+    """include/gpumatting.h and the container headers compile as C99 with
+    every warning an error, so C, cgo-style and ctypes callers can include them."""
     inc = os.path.join(ROOT, "include")
-    for h in ("gpumatting.h", "BandedMatrix.h"):
+    for h in ("gpumatting.h", "BandedMatrix.h", "CompressedMatrix.h", "CoordMatrix.h"):
         r = subprocess.run(["gcc", "-x", "c", "-std=c99", "-Wall", "-Wextra", "-Werror", "-pedantic", "-fsyntax-only",
                             "-I" + inc, os.path.join(inc, h)], capture_output=True, text=True)
         assert r.returncode == 0, (h, r.stderr)
@@ -91,3 +91,14 @@ def test_ctypes_structs_match_the_header():
     for struct, cls in (("gm_solve_opts_s", _lib.SolveOpts), ("gm_solve_info_s", _lib.SolveInfo),
                         ("gm_tune_result_s", _lib.TuneResult)):
         assert fields(struct) == [f for f, _ in cls._fields_], struct
+
+
+def test_sparse_structs_match_the_headers():
+    """The ctypes mirrors of CompressedMatrix / CoordMatrix have the headers'
+    field names and order (the reference's, CompressedMatrix.h:26-37,
+    CoordMatrix.h:25-36)."""
+    for h, cls in (("CompressedMatrix.h", _lib.CompressedMatrix), ("CoordMatrix.h", _lib.CoordMatrix)):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        body = re.search(r"typedef struct \w+\s*\{(.*?)\}", text, re.S).group(1)
+        names = [m.group(1) for m in re.finditer(r"(\w+)\s*;", body)]
+        assert names == [f for f, _ in cls._fields_], h

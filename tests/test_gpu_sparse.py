@@ -1,0 +1,172 @@
+"""Sparse matrices and superpixel operators on the GPU against the CPU oracle
+(oracle/sporacle.py, itself pinned to scipy.sparse and closed forms in
+test_sparse_oracle.py).  Integer structure (row pointers, columns, labels) is
+compared exactly; FP32 values to 2e-6 relative (FP64 sums of FP32 products in
+a different but fixed order)."""
+import numpy as np
+import pytest
+import torch
+
+from gpumatting_amd import matting, sparse
+from gpumatting_amd._lib import GMError
+from oracle import pyoracle as O
+from oracle import sporacle as SP
+
+pytestmark = pytest.mark.gpu
+
+
+def _coo(rng, rows, cols, n):
+    i = rng.integers(0, rows, n).astype(np.int32)
+    j = rng.integers(0, cols, n).astype(np.int32)
+    k = rng.standard_normal(n).astype(np.float32)
+    return i, j, k
+
+
+@pytest.mark.parametrize("rows,cols,n", [(1, 1, 1), (7, 5, 40), (300, 200, 5000), (64, 4000, 20000), (5, 9, 0)])
+def test_coo_csr_products(gpu, rows, cols, n):
+    print(f"[sparse] coo {rows}x{cols} nnz {n}", flush=True)
+    rng = np.random.default_rng(rows + n)
+    i, j, k = _coo(rng, rows, cols, n)
+    A = sparse.CoordMatrix(rows, cols, n + 3)
+    A.append(i[: n // 2], j[: n // 2], k[: n // 2])
+    A.append(i[n // 2:], j[n // 2:], k[n // 2:])
+    assert A.nnz == n
+    C = A.to_csr()
+    p, jj, kk = C.to_numpy()
+    pw, jw, kw = SP.coo_to_csr(rows, cols, i, j, k)
+    assert np.array_equal(p, pw) and np.array_equal(jj, jw) and np.array_equal(kk, kw)
+    x = rng.standard_normal(cols).astype(np.float32)
+    y = rng.standard_normal(rows).astype(np.float32)
+    xt, yt = torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda()
+    tol = dict(rtol=2e-6, atol=1e-6)
+    np.testing.assert_allclose(C.ax(xt).cpu().numpy(), SP.csr_ax(pw, jw, kw, x), **tol)
+    np.testing.assert_allclose(C.axpy(xt, yt, True).cpu().numpy(), SP.csr_ax(pw, jw, kw, x, y, 1.0), **tol)
+    np.testing.assert_allclose(C.axpy(xt, yt, False).cpu().numpy(), SP.csr_ax(pw, jw, kw, x, y, -1.0), **tol)
+    b0 = rng.standard_normal(rows).astype(np.float32)
+    bt = torch.from_numpy(b0).cuda()
+    A.axpy_(bt, xt, yt)
+    np.testing.assert_allclose(bt.cpu().numpy(), SP.csr_ax(pw, jw, kw, x, y, 1.0, b0), **tol)
+    C.free()
+    A.release()
+
+
+def test_coo_append_checks(gpu):
+    A = sparse.CoordMatrix(4, 3, 5)
+    A.append([0, 3], [2, 0], [1.0, 2.0])
+    with pytest.raises(GMError):
+        A.append([4], [0], [1.0])  # row out of range
+    with pytest.raises(GMError):
+        A.append([0], [-1], [1.0])  # column out of range
+    with pytest.raises(GMError):
+        A.append([0, 1, 2, 3], [0, 0, 0, 0], [1, 1, 1, 1])  # past length
+    assert A.nnz == 2
+    A.release()
+
+
+def test_restriction_and_prolongation(gpu):
+    rng = np.random.default_rng(11)
+    H, W, nl = 37, 53, 41
+    labels = rng.integers(0, nl - 1, (H, W)).astype(np.int32)  # label nl-1 empty
+    P = sparse.restriction(labels, nl)
+    p, j, k = P.to_numpy()
+    pw, jw, kw = SP.restriction(labels, nl)
+    assert np.array_equal(p, pw) and np.array_equal(j, jw) and np.array_equal(k, kw)
+    x = rng.standard_normal(H * W).astype(np.float32)
+    np.testing.assert_allclose(P.ax(torch.from_numpy(x).cuda()).cpu().numpy(), SP.csr_ax(pw, jw, kw, x),
+                               rtol=2e-6, atol=1e-6)
+    xc = rng.standard_normal(nl).astype(np.float32)
+    e = sparse.prolong(torch.from_numpy(xc).cuda(), labels, nl).cpu().numpy()
+    assert np.array_equal(e, SP.prolong(xc, labels, nl))
+    bad = labels.copy()
+    bad[3, 4] = nl
+    with pytest.raises(GMError):
+        sparse.restriction(bad, nl)
+    e = sparse.prolong(torch.from_numpy(xc).cuda(), bad, nl).cpu().numpy()
+    assert np.isnan(e[3 * W + 4]) and np.isfinite(np.delete(e, 3 * W + 4)).all()
+
+
+@pytest.mark.parametrize("W,H,cell", [(16, 12, 4), (61, 45, 7)])
+def test_galerkin_matches_oracle(gpu, W, H, cell):
+    print(f"[sparse] galerkin {W}x{H} cell {cell}", flush=True)
+    rng = np.random.default_rng(W)
+    image = rng.random((H, W, 3)).astype(np.float32)
+    lam = np.where(rng.random(W * H) < 0.1, 100.0, 0.0).astype(np.float32)
+    s = matting.assemble(image, lam, np.zeros(W * H, np.float32), 1e-5)
+    a = s.L.to_numpy()
+    labels = (np.arange(H)[:, None] // cell) * ((W + cell - 1) // cell) + (np.arange(W)[None, :] // cell)
+    nl = int(labels.max()) + 1
+    Ac = sparse.galerkin(s.L, labels, nl)
+    p, j, k = Ac.to_numpy()
+    pw, jw, kw = SP.galerkin(a, O.bands25(W), labels, nl)
+    assert np.array_equal(p, pw) and np.array_equal(j, jw)
+    np.testing.assert_allclose(k, kw, rtol=2e-6, atol=1e-6)
+    # coarse row sums are the restricted fine row sums (the data term)
+    D = Ac.to_scipy().toarray().astype(np.float64)
+    rs = np.bincount(labels.ravel(), weights=lam.astype(np.float64), minlength=nl)
+    np.testing.assert_allclose(D.sum(axis=1), rs, rtol=1e-4, atol=1e-3 * max(1.0, rs.max()))
+    np.testing.assert_allclose(D, D.T, rtol=1e-5, atol=1e-5)
+    Ac.free()
+
+
+def _agreement(a, b):
+    """Fraction of pixels on which partition a agrees with partition b under
+    the best label matching of a onto b."""
+    n = 0
+    for c in np.unique(a):
+        n += np.bincount(b[a == c]).max()
+    return n / a.size
+
+
+def test_slic_flat_blocks_exact(gpu):
+    rng = np.random.default_rng(2)
+    S, nbx, nby = 9, 7, 5
+    W, H = S * nbx, S * nby
+    pal = rng.random((nbx * nby, 3)).astype(np.float32) * 0.8 + 0.1
+    blocks = (np.arange(H)[:, None] // S) * nbx + (np.arange(W)[None, :] // S)
+    labels, n = sparse.slic(pal[blocks], nbx * nby)
+    assert n == nbx * nby and np.array_equal(labels.cpu().numpy(), blocks)
+
+
+@pytest.mark.parametrize("W,H,K", [(64, 48, 30), (97, 61, 50), (33, 200, 20)])
+def test_slic_matches_oracle(gpu, W, H, K):
+    print(f"[sparse] slic {W}x{H} K {K}", flush=True)
+    rng = np.random.default_rng(W)
+    # a smooth image with edges: blurred random blocks plus noise
+    base = rng.random((H // 8 + 2, W // 8 + 2, 3))
+    im = base[np.arange(H)[:, None] // 8, np.arange(W)[None, :] // 8]
+    im = np.clip(im + 0.03 * rng.standard_normal((H, W, 3)), 0, 1).astype(np.float32)
+    got, n = sparse.slic(im, K, 10.0, 10)
+    got = got.cpu().numpy()
+    want, nw = SP.slic(im, K, 10.0, 10)
+    assert _agreement(got, want) >= 0.99 and _agreement(want, got) >= 0.99, (_agreement(got, want), n, nw)
+    # the library's own output satisfies the connectivity contract exactly
+    assert got.min() == 0 and got.max() == n - 1
+    again, n2 = SP.connect(got, 1)
+    assert n2 == n and np.array_equal(again, got)
+    S, _, _ = SP.slic_grid(W, H, K)
+    sizes = np.bincount(got.ravel())
+    assert sizes.min() >= 1
+
+
+def test_multilevel_pipeline(gpu):
+    """SLIC -> merged level -> restriction / Galerkin operator on a matte
+    system; the two-level operators satisfy P A P^T 1 = P (A 1) and
+    x^T Ac x = (P^T x)^T A (P^T x)."""
+    W, H = 96, 64
+    rng = np.random.default_rng(9)
+    image = rng.random((H // 16 + 2, W // 16 + 2, 3))[np.arange(H)[:, None] // 16, np.arange(W)[None, :] // 16]
+    image = np.clip(image + 0.01 * rng.standard_normal((H, W, 3)), 0, 1).astype(np.float32)
+    lam = np.where(rng.random(W * H) < 0.05, 100.0, 0.0).astype(np.float32)
+    s = matting.assemble(image, lam, np.zeros(W * H, np.float32), 1e-5)
+    labels, n = sparse.slic(image, 40)
+    lab2, n2 = sparse.merge_levels(labels.cpu().numpy(), n, image, 2)
+    assert n2 <= n
+    Ac = sparse.galerkin(s.L, lab2, n2)
+    xc = torch.from_numpy(rng.standard_normal(n2).astype(np.float32)).cuda()
+    e = sparse.prolong(xc, lab2, n2)
+    pad = s.pad
+    ep = matting.Padded(W * H, pad, init=e)
+    Ae = matting.bm_ax(s.L, ep)
+    lhs = float(torch.dot(xc.double(), Ac.ax(xc).double()))
+    rhs = float(torch.dot(e.double(), Ae.double()))
+    assert abs(lhs - rhs) <= 1e-4 * max(1.0, abs(rhs)), (lhs, rhs)
