@@ -212,6 +212,89 @@ def bench_grad(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_superpixel(args):
+    """Superpixel multilevel setup on the 3840x2160 workload: GPU SLIC (K =
+    4000, 10 rounds), the restriction P, the Galerkin coarse operator P A P^T
+    of the assembled Laplacian, one merge level (host), and the CSR products
+    P x and Ac x (HIP events on the launch stream).  The CPU baseline is the
+    oracle's numpy SLIC on a 960x540 crop (K = 250), scaled by pixel count."""
+    import numpy as np
+    import torch
+
+    from gpumatting_amd import ASSEMBLY_ACCURATE, matting, sparse, synth
+    log("generating the 3840x2160 workload")
+    p = synth.config_4k()
+    W, H, N = p.W, p.H, p.N
+    img = torch.from_numpy(p.image).cuda()
+    s = matting.assemble(img, torch.from_numpy(p.lam).cuda(), torch.from_numpy(p.target).cuda(), p.eps,
+                         ASSEMBLY_ACCURATE)
+    K = 4000
+
+    def timed(fn, reps):
+        out, ts = None, []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            out = fn()
+            torch.cuda.synchronize()
+            ts.append((time.perf_counter() - t0) * 1e3)
+        return out, sorted(ts)[len(ts) // 2]
+
+    reps = max(3, args.steps)
+    sparse.slic(img, K)  # warm-up
+    (labels, nl), t_slic = timed(lambda: sparse.slic(img, K), reps)
+    P, t_res = timed(lambda: sparse.restriction(labels, nl), reps)
+    Ac, t_gal = timed(lambda: sparse.galerkin(s.L, labels, nl), reps)
+    lab_h = labels.cpu().numpy()
+    (lab2, nl2), t_merge = timed(lambda: sparse.merge_levels(lab_h, nl, p.image, 2), 1)
+    log(f"SLIC {t_slic:.1f} ms ({nl} superpixels), restriction {t_res:.2f} ms, Galerkin {t_gal:.1f} ms "
+        f"({Ac.nnz} coarse entries), merge level {t_merge:.0f} ms ({nl2} superpixels)")
+    x = torch.rand(N, device="cuda")
+    xc = torch.rand(nl, device="cuda")
+    st = torch.cuda.current_stream()
+
+    def event_us(fn, n=50):
+        fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(n):
+            fn()
+        e1.record(st)
+        e1.synchronize()
+        return e0.elapsed_time(e1) * 1e3 / n
+
+    us_p = event_us(lambda: P.ax(x))
+    us_c = event_us(lambda: Ac.ax(xc))
+    bytes_p = 4 * (nl + 1) + 8 * N + 4 * N + 4 * nl  # p, (j, k), x, out
+    log(f"P x {us_p:.1f} us ({bytes_p / us_p / 1e3:.0f} GB/s), Ac x {us_c:.1f} us")
+    cpu = None
+    if not args.no_cpu:
+        from oracle import sporacle as SP
+        crop = p.image[:540, :960, :3]
+        t0 = time.perf_counter()
+        SP.slic(crop, 250, 10.0, 10)
+        t_crop = time.perf_counter() - t0
+        cpu = {"value": round(t_crop * 1e3 * N / crop[..., 0].size, 1), "unit": "ms per 4K SLIC (scaled)",
+               "cores": 1, "kind": "port",
+               "sample": f"oracle numpy SLIC of a 960x540 crop, K=250, 10 rounds ({t_crop * 1e3:.0f} ms) x "
+                         f"{N / crop[..., 0].size:.0f}; {cpu_model()}"}
+    total = t_slic + t_res + t_gal
+    out = {"metric": "superpixel multilevel setup ms at 3840x2160 (SLIC K=4000 + restriction + Galerkin P A P^T)",
+           "value": round(total, 2), "unit": "ms", "n_gpus": 1, "steps": reps, "warmup": 1,
+           "ms_per_step": round(total, 2), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32 (FP64 sums)", "data": "synthetic (gpumatting_amd/synth.py config_4k)",
+           "config": {"workload": "3840x2160 image, SLIC K=4000 compactness 10, 10 rounds; Galerkin operator of "
+                                  "the assembled 25-band Laplacian"},
+           "detail": {"slic_ms": round(t_slic, 2), "superpixels": nl, "restriction_ms": round(t_res, 3),
+                      "galerkin_ms": round(t_gal, 2), "coarse_nnz": int(Ac.nnz),
+                      "merge_level_ms_host": round(t_merge, 1), "merged_superpixels": nl2,
+                      "restriction_product_us": round(us_p, 1),
+                      "restriction_product_GBps": round(bytes_p / us_p / 1e3, 1),
+                      "coarse_product_us": round(us_c, 1)},
+           "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -221,9 +304,9 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=40)
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default shapes)")
-    ap.add_argument("--workload", choices=["4k", "batched", "grad"], default="4k",
+    ap.add_argument("--workload", choices=["4k", "batched", "grad", "superpixel"], default="4k",
                     help="4k: the headline 3840x2160 solve; batched: 64 uneven images x 4 right-hand sides; "
-                         "grad: 16 x 512x512 forward + backward")
+                         "grad: 16 x 512x512 forward + backward; superpixel: SLIC + multilevel operators at 4K")
     ap.add_argument("--grad-threads", type=int, default=8,
                     help="grad workload: images processed concurrently, one stream per worker thread")
     ap.add_argument("--copy-calib", action="store_true",
@@ -245,6 +328,8 @@ def main():
         return bench_batched(args)
     if args.workload == "grad":
         return bench_grad(args)
+    if args.workload == "superpixel":
+        return bench_superpixel(args)
     log("generating the 3840x2160 workload")
     p = synth.config_4k()
     W, H, N = p.W, p.H, p.N

@@ -20,15 +20,14 @@
 //        update: each seed moves to the mean (l, a, b, x, y) of the pixels it
 //        took this round (one workgroup per seed over its window, FP64 sums in
 //        a fixed order); a seed that took none stays.
-//   4. Connectivity, on the host: every 4-connected piece of a label is a
-//      superpixel; a piece smaller than max(1, S^2 / 4) pixels joins the
-//      superpixel of the pixel left of its first pixel in raster order (or
-//      else above it; the first piece of the image always stays).  Labels are
-//      numbered from 0 in raster order of first appearance.
+//   4. Connectivity: every 4-connected piece of a label is a superpixel; a
+//      piece smaller than max(1, S^2 / 4) pixels joins the superpixel of the
+//      pixel left of its first pixel in raster order (or else above it; the
+//      first piece of the image always stays).  Labels are numbered from 0 in
+//      raster order of first appearance.
+#include <hipcub/hipcub.hpp>
 #include <math.h>
 #include <string.h>
-
-#include <vector>
 
 #include "gm_internal.h"
 
@@ -193,40 +192,91 @@ __global__ __launch_bounds__(kBlock) void k_slic_update(const float* __restrict_
     }
 }
 
-// Connectivity and final numbering (host); labels in/out, returns the count.
-static int slic_connect(std::vector<int>& lab, int W, int H, int minsize) {
-    const long N = (long)W * H;
-    std::vector<int> out(N, -1), stack, piece;
-    int next = 0;
-    for (long u0 = 0; u0 < N; ++u0) {
-        if (out[u0] >= 0) continue;
-        const int c = lab[u0];
-        piece.clear();
-        stack.clear();
-        stack.push_back((int)u0);
-        out[u0] = -2;  // queued
-        while (!stack.empty()) {
-            const int u = stack.back();
-            stack.pop_back();
-            piece.push_back(u);
-            const int y = u / W, x = u - y * W;
-            const int nb[4] = {x > 0 ? u - 1 : -1, x + 1 < W ? u + 1 : -1, y > 0 ? u - W : -1, y + 1 < H ? u + W : -1};
-            for (int v : nb)
-                if (v >= 0 && out[v] == -1 && lab[v] == c) {
-                    out[v] = -2;
-                    stack.push_back(v);
-                }
+// Connectivity.  Union-find over the 4-neighbour edges that join equal
+// labels: a union links the larger root under the smaller (atomicMin; a link
+// that lost a race continues with the parent it found, the merge of Playne &
+// Hawick, "A new algorithm for parallel connected-component labelling on GPUs",
+// IEEE TPDS 2018), so every component's root is its first pixel in raster
+// order whatever the interleaving.  Then sizes, the piece a small component
+// joins (the component left of, or else above, its root), chains of joins
+// resolved, and survivors numbered by a scan over their roots.
+__device__ __forceinline__ int cc_parent(const int* parent, int u) {
+    return __hip_atomic_load(parent + u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int cc_find(const int* parent, int u) {
+    for (int p = cc_parent(parent, u); p != u; p = cc_parent(parent, u)) u = p;
+    return u;
+}
+__device__ void cc_unite(int* parent, int a, int b) {
+    for (;;) {
+        a = cc_find(parent, a);
+        b = cc_find(parent, b);
+        if (a == b) return;
+        if (a > b) {
+            const int t = a;
+            a = b;
+            b = t;
         }
-        const int y0 = (int)(u0 / W), x0 = (int)(u0 - (long)y0 * W);
-        int tgt;
-        if ((int)piece.size() < minsize && (x0 > 0 || y0 > 0))
-            tgt = x0 > 0 ? out[u0 - 1] : out[u0 - W];
-        else
-            tgt = next++;
-        for (int u : piece) out[u] = tgt;
+        const int old = atomicMin(parent + b, a);  // parent[v] <= v always
+        if (old == b) return;
+        b = old;
     }
-    lab.swap(out);
-    return next;
+}
+
+__global__ __launch_bounds__(kBlock) void k_cc_init(int* __restrict__ parent, int* __restrict__ size, long N) {
+    for (long u = (long)blockIdx.x * kBlock + threadIdx.x; u < N; u += (long)gridDim.x * kBlock) {
+        parent[u] = (int)u;
+        size[u] = 0;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_cc_union(const int* __restrict__ labels, int W, long N, int* parent) {
+    for (long u = (long)blockIdx.x * kBlock + threadIdx.x; u < N; u += (long)gridDim.x * kBlock) {
+        const int x = (int)(u % W);
+        if (x > 0 && labels[u - 1] == labels[u]) cc_unite(parent, (int)u, (int)u - 1);
+        if (u >= W && labels[u - W] == labels[u]) cc_unite(parent, (int)u, (int)(u - W));
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_cc_flatten(int* parent, int* __restrict__ size, long N) {
+    for (long u = (long)blockIdx.x * kBlock + threadIdx.x; u < N; u += (long)gridDim.x * kBlock) {
+        const int r = cc_find(parent, (int)u);
+        parent[u] = r;
+        atomicAdd(size + r, 1);
+    }
+}
+
+// roots: the root whose component a small one joins (itself when it stays)
+__global__ __launch_bounds__(kBlock) void k_cc_target(const int* __restrict__ parent, const int* __restrict__ size,
+                                                      int W, long N, int minsize, int* __restrict__ tgt) {
+    for (long u = (long)blockIdx.x * kBlock + threadIdx.x; u < N; u += (long)gridDim.x * kBlock) {
+        if (parent[u] != u) continue;
+        int t = (int)u;
+        if (u > 0 && size[u] < minsize) t = parent[(u % W) > 0 ? u - 1 : u - W];
+        tgt[u] = t;
+    }
+}
+
+// chains of joins (each step goes to a smaller root), survivors flagged
+__global__ __launch_bounds__(kBlock) void k_cc_resolve(const int* __restrict__ parent, const int* __restrict__ tgt,
+                                                       long N, int* __restrict__ fin, unsigned* __restrict__ flag) {
+    for (long u = (long)blockIdx.x * kBlock + threadIdx.x; u < N; u += (long)gridDim.x * kBlock) {
+        unsigned f = 0;
+        if (parent[u] == u) {
+            int r = (int)u;
+            while (tgt[r] != r) r = tgt[r];
+            fin[u] = r;
+            f = tgt[u] == u;
+        }
+        flag[u] = f;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_cc_label(const int* __restrict__ parent, const int* __restrict__ fin,
+                                                     const unsigned* __restrict__ num, long N,
+                                                     int* __restrict__ labels) {
+    for (long u = (long)blockIdx.x * kBlock + threadIdx.x; u < N; u += (long)gridDim.x * kBlock)
+        labels[u] = (int)num[fin[parent[u]]];
 }
 
 }  // namespace gm
@@ -275,19 +325,37 @@ extern "C" gm_status gm_slic(int* labels, int* nlabels, const float* im, int nch
         hipLaunchKernelGGL(k_slic_update, dim3(ugrid), dim3(kBlock), 0, s, lab, g, labels, s0, s1);
         std::swap(s0, s1);
     }
+    // connectivity: parent, size, tgt, fin as ints; flag, num as unsigned
+    int* cc = nullptr;
+    void* tmp = nullptr;
+    size_t ntmp = 0;
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess) {
-        cleanup();
-        return hip_fail(e, "gm_slic launch", __FILE__, __LINE__);
+    if (e == hipSuccess) e = hipMalloc(&cc, sizeof(int) * 6 * (size_t)N);
+    int *parent = cc, *size = cc + N, *tgt = cc + 2 * N, *fin = cc + 3 * N;
+    unsigned *flag = reinterpret_cast<unsigned*>(cc + 4 * N), *num = reinterpret_cast<unsigned*>(cc + 5 * N);
+    if (e == hipSuccess) e = hipcub::DeviceScan::ExclusiveSum(nullptr, ntmp, flag, num, (int)N, s);
+    if (e == hipSuccess) e = hipMalloc(&tmp, ntmp ? ntmp : 1);
+    if (e == hipSuccess) {
+        const int minsize = g.S * g.S / 4 > 1 ? g.S * g.S / 4 : 1;
+        hipLaunchKernelGGL(k_cc_init, dim3(grid), dim3(kBlock), 0, s, parent, size, N);
+        hipLaunchKernelGGL(k_cc_union, dim3(grid), dim3(kBlock), 0, s, labels, W, N, parent);
+        hipLaunchKernelGGL(k_cc_flatten, dim3(grid), dim3(kBlock), 0, s, parent, size, N);
+        hipLaunchKernelGGL(k_cc_target, dim3(grid), dim3(kBlock), 0, s, parent, size, W, N, minsize, tgt);
+        hipLaunchKernelGGL(k_cc_resolve, dim3(grid), dim3(kBlock), 0, s, parent, tgt, N, fin, flag);
+        e = hipcub::DeviceScan::ExclusiveSum(tmp, ntmp, flag, num, (int)N, s);
     }
-    std::vector<int> h(N);
-    e = hipMemcpyAsync(h.data(), labels, sizeof(int) * N, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_cc_label, dim3(grid), dim3(kBlock), 0, s, parent, fin, num, N, labels);
+        e = hipGetLastError();
+    }
+    unsigned last[2] = {0, 0};
+    if (e == hipSuccess) e = hipMemcpyAsync(&last[0], num + N - 1, sizeof(unsigned), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(&last[1], flag + N - 1, sizeof(unsigned), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (cc) (void)hipFree(cc);
+    if (tmp) (void)hipFree(tmp);
     cleanup();
-    if (e != hipSuccess) return hip_fail(e, "gm_slic copy", __FILE__, __LINE__);
-    const int minsize = g.S * g.S / 4 > 1 ? g.S * g.S / 4 : 1;
-    *nlabels = slic_connect(h, W, H, minsize);
-    e = hipMemcpy(labels, h.data(), sizeof(int) * N, hipMemcpyHostToDevice);
-    if (e != hipSuccess) return hip_fail(e, "gm_slic copy", __FILE__, __LINE__);
+    if (e != hipSuccess) return hip_fail(e, "gm_slic", __FILE__, __LINE__);
+    *nlabels = (int)(last[0] + last[1]);
     return GM_OK;
 }

@@ -143,25 +143,31 @@ __global__ __launch_bounds__(kBlock) void k_run_starts(const unsigned* __restric
         if (head[e]) start[rid[e]] = (unsigned)e;
 }
 
-// one thread per run: the run's values summed in FP64 in sorted (= slot) order
+// one wave per run: lane l sums the run's entries l, l + 64, ... (sorted =
+// slot order) in FP64, then a fixed butterfly
 __global__ __launch_bounds__(kBlock) void k_gal_sum(const float* __restrict__ a, size_t apitch, int nb,
                                                     const unsigned long long* __restrict__ key,
                                                     const unsigned* __restrict__ slot,
                                                     const unsigned* __restrict__ start, long nruns, long nv, int nl,
                                                     unsigned long long* __restrict__ rkey, int* __restrict__ oj,
                                                     float* __restrict__ ok) {
-    for (long r = (long)blockIdx.x * kBlock + threadIdx.x; r < nruns; r += (long)gridDim.x * kBlock) {
+    const int lane = (int)threadIdx.x & 63;
+    const long nw = ((long)gridDim.x * kBlock) >> 6;
+    for (long r = ((long)blockIdx.x * kBlock + threadIdx.x) >> 6; r < nruns; r += nw) {
         const long e0 = start[r], e1 = r + 1 < nruns ? (long)start[r + 1] : nv;
         double s = 0.0;
-        for (long e = e0; e < e1; ++e) {
+        for (long e = e0 + lane; e < e1; e += 64) {
             const unsigned sl = slot[e];
             const unsigned i = sl / (unsigned)nb, kb = sl - i * (unsigned)nb;
             s += (double)a[(size_t)kb * apitch + i];
         }
-        const unsigned long long kk = key[e0];
-        rkey[r] = kk;
-        oj[r] = (int)(kk % (unsigned long long)nl);
-        ok[r] = (float)s;
+        s = wave_sum(s);
+        if (lane == 0) {
+            const unsigned long long kk = key[e0];
+            rkey[r] = kk;
+            oj[r] = (int)(kk % (unsigned long long)nl);
+            ok[r] = (float)s;
+        }
     }
 }
 
@@ -463,8 +469,13 @@ gm_status gm_sp_galerkin(CompressedMatrix* Ac, const BandedMatrix* A, const int*
     GM_LAUNCH_CHECK();
     GM_TRY(gm_csm_init(Ac, nlabels, nlabels, (size_t)nruns));
     Ac->nnz = (size_t)nruns;
-    hipLaunchKernelGGL(k_gal_sum, dim3(grid_for(nruns)), dim3(kBlock), 0, s, A->a, A->apitch, plan.args.nb, k1, v1,
-                       start, nruns, nv, nlabels, rkey, Ac->j, Ac->k);
+    {
+        DevInfo d;
+        GM_TRY(dev_info(&d));
+        const long blocks = std::min((nruns + kWaves - 1) / kWaves, (long)d.cus * 16);
+        hipLaunchKernelGGL(k_gal_sum, dim3((unsigned)blocks), dim3(kBlock), 0, s, A->a, A->apitch, plan.args.nb, k1,
+                           v1, start, nruns, nv, nlabels, rkey, Ac->j, Ac->k);
+    }
     GM_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_rowptr<unsigned long long>, dim3(grid_for(nl + 1)), dim3(kBlock), 0, s, rkey, nruns, nlabels,
                        nl, Ac->p);

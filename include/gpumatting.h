@@ -344,8 +344,8 @@ gm_status gm_sp_restriction(CompressedMatrix* P, const int* labels, int npix, in
  * outside [0, nlabels). */
 gm_status gm_sp_prolong(float* e, const float* xc, const int* labels, int npix, int nlabels, gm_stream_t stream);
 /* The Galerkin coarse operator Ac = P A P^T (nlabels x nlabels CSR, repeats
- * merged: one entry per (row, column), summed in FP64 in the order of the
- * fine entries (pixel, band)) of a device banded matrix (any band count;
+ * merged: one entry per (row, column), the fine entries (pixel, band) summed
+ * in FP64 in a fixed order) of a device banded matrix (any band count;
  * entries stored as 0 are skipped) and a labelling as above -- the operator
  * the reference's multilevel direction coarsens with.  Allocates *Ac.
  * Synchronous. */

@@ -112,6 +112,79 @@ inline void hostLevinLaplacian(BandedMatrix L, float* b, float lambda, Float4* i
                                                       imH, imPitch));
 }
 
+// ---- sparse matrices and superpixels (CompressedMatrix.cu, CoordMatrix.cu,
+// SLIC.h, ppm.h) -------------------------------------------------------------
+// The reference's CSR/COO routines are device functions over device-side
+// structs; here the structs live on the host and hold device arrays.
+// CompressedMatrix.cu:20-31
+inline void csmInit(CompressedMatrix* m, int rows, int cols, size_t length) {
+    gpumatting_detail::record(gm_csm_init(m, rows, cols, length));
+}
+// csmAx, CompressedMatrix.cu:118-165: b = A x
+inline void csmAx(float* b, CompressedMatrix const* a, float const* x, gm_stream_t s = 0) {
+    gpumatting_detail::record(gm_csm_ax(b, a, x, s));
+}
+// csmAxpy, CompressedMatrix.cu:45-85: b = A x + y
+inline void csmAxpy(float* b, CompressedMatrix const* a, float const* x, float const* y, gm_stream_t s = 0) {
+    gpumatting_detail::record(gm_csm_axpy(b, a, x, y, 1, s));
+}
+// CoordMatrix.cu:20-29
+inline void cooInit(CoordMatrix* m, int rows, int cols, size_t length) {
+    gpumatting_detail::record(gm_coo_init(m, rows, cols, length));
+}
+// CoordMatrix.cu:31-40
+inline void cooRelease(CoordMatrix* m) { gpumatting_detail::record(gm_coo_release(m)); }
+// CoordMatrix.cu:42-48 (host entries; the reference's `int* k` holds the
+// values, which are float)
+inline void cooAppend(CoordMatrix* m, int* i, int* j, float* k, size_t length) {
+    gpumatting_detail::record(gm_coo_append(m, i, j, k, length));
+}
+// cooAxpy, CoordMatrix.cu:50-87: b += A x + y
+inline void cooAxpy(float* b, CoordMatrix const* a, float const* x, float const* y, gm_stream_t s = 0) {
+    gpumatting_detail::record(gm_coo_axpy(b, a, x, y, s));
+}
+// ppm.h:151-165: 8-bit RGB to 4 bytes per pixel A, R, G, B (A = 0xff); malloc'd
+inline unsigned char* rgbToArgb(unsigned char* rgb, int w, int h) {
+    unsigned char* ret = (unsigned char*)malloc((size_t)4 * w * h);
+    if (!ret) return NULL;
+    for (size_t p = 0; p < (size_t)w * h; ++p) {
+        ret[4 * p] = 0xff;
+        ret[4 * p + 1] = rgb[3 * p];
+        ret[4 * p + 2] = rgb[3 * p + 1];
+        ret[4 * p + 3] = rgb[3 * p + 2];
+    }
+    return ret;
+}
+// SLIC.h:43: host pixels laid out as rgbToArgb makes them, host labels out
+// (width * height ints); returns the number of superpixels (0 on failure,
+// status in gmLastStatus()).  nseg is the requested count K, and
+// spatialConsistency the compactness m.
+inline int slicSegmentation(int* labels, const unsigned int* in, int width, int height, int nseg,
+                            double spatialConsistency = 10.0) {
+    const int n = width * height;
+    float* h = (float*)malloc(sizeof(float) * 3 * (size_t)(n > 0 ? n : 1));
+    if (!h) {
+        gpumatting_detail::record(GM_ERR_OOM);
+        return 0;
+    }
+    const unsigned char* px = reinterpret_cast<const unsigned char*>(in);
+    for (int p = 0; p < n; ++p)
+        for (int c = 0; c < 3; ++c) h[3 * p + c] = px[4 * p + 1 + c] / 255.0f;
+    float *dim = NULL, *dlab = NULL;
+    int nl = 0;
+    gm_status st = gm_vec_copy_to_device(&dim, h, 3 * n, 0, 0);
+    if (st == GM_OK) st = gm_vec_device_malloc(&dlab, n, 0, 0);
+    if (st == GM_OK)
+        st = gm_slic(reinterpret_cast<int*>(dlab), &nl, dim, 3, (size_t)width, width, height, nseg,
+                     (float)spatialConsistency, 10, 0);
+    if (st == GM_OK) st = gm_vec_copy_to_host(reinterpret_cast<float*>(labels), dlab, n);
+    if (dim) gm_vec_device_free(dim, 0);
+    if (dlab) gm_vec_device_free(dlab, 0);
+    free(h);
+    gpumatting_detail::record(st);
+    return st == GM_OK ? nl : 0;
+}
+
 // ---- driver I/O (ppm.h, main.cu) ------------------------------------------
 // Like the reference, readers return malloc'd buffers (release with free())
 // or NULL on failure (message in gm_last_error()), writers return 0 / -1.

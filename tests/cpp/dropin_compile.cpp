@@ -1,5 +1,8 @@
 // Compile-only check: every reference-named wrapper resolves against the C ABI.
 #include "BandedMatrix.h"
+#include "CompressedMatrix.h"
+#include "CoordMatrix.h"
+#include "SLIC.h"
 #include "Matting.h"
 #include "Solve.h"
 #include "Vector.h"
@@ -30,5 +33,20 @@ void use_everything(BandedMatrix hA, float* x, float* b, unsigned char* s, F4* i
     hostLevinLaplacian(hA, b, 1e-2f, im, s, 4, 4, 4);
     vecDeviceFree(dx, 2);
     bmDeviceFree(&dA);
+    CoordMatrix coo;
+    cooInit(&coo, 4, 4, 8);
+    int ii[2] = {0, 1}, jj[2] = {1, 2};
+    float kk[2] = {1.f, 2.f};
+    cooAppend(&coo, ii, jj, kk, 2);
+    cooAxpy(dx, &coo, dx, dx);
+    CompressedMatrix csm;
+    csmInit(&csm, 4, 4, 8);
+    csmAx(dx, &csm, dx);
+    csmAxpy(dx, &csm, dx, dx);
+    cooRelease(&coo);
+    unsigned char* argb = rgbToArgb(s, 4, 4);
+    int labels[16];
+    (void)slicSegmentation(labels, reinterpret_cast<const unsigned int*>(argb), 4, 4, 2, 10.0);
+    free(argb);
     (void)gmLastStatus();
 }
