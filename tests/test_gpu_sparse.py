@@ -170,3 +170,30 @@ def test_multilevel_pipeline(gpu):
     lhs = float(torch.dot(xc.double(), Ac.ax(xc).double()))
     rhs = float(torch.dot(e.double(), Ae.double()))
     assert abs(lhs - rhs) <= 1e-4 * max(1.0, abs(rhs)), (lhs, rhs)
+
+
+@pytest.mark.parametrize("W,H,K", [(1, 1, 1), (3, 2, 4), (5, 40, 1000), (17, 9, 1)])
+def test_slic_edge_sizes(gpu, W, H, K):
+    """Tiny images, K above the pixel count, a single superpixel."""
+    rng = np.random.default_rng(W * H)
+    im = rng.random((H, W, 3)).astype(np.float32)
+    got, n = sparse.slic(im, K, 10.0, 10)
+    got = got.cpu().numpy()
+    want, nw = SP.slic(im, K, 10.0, 10)
+    assert got.min() == 0 and got.max() == n - 1
+    again, n2 = SP.connect(got, 1)
+    assert n2 == n and np.array_equal(again, got)
+    assert _agreement(got, want) >= 0.9
+
+
+def test_galerkin_single_label_is_total_sum(gpu):
+    W, H = 20, 11
+    rng = np.random.default_rng(3)
+    image = rng.random((H, W, 3)).astype(np.float32)
+    lam = np.where(rng.random(W * H) < 0.2, 10.0, 0.0).astype(np.float32)
+    s = matting.assemble(image, lam, np.zeros(W * H, np.float32), 1e-4)
+    Ac = sparse.galerkin(s.L, np.zeros(W * H, np.int32), 1)
+    p, j, k = Ac.to_numpy()
+    assert list(p) == [0, 1] and list(j) == [0]
+    # 1^T A 1 = sum of the data term
+    assert abs(float(k[0]) - float(lam.astype(np.float64).sum())) <= 1e-3 * max(1.0, float(lam.sum()))
