@@ -26,8 +26,8 @@
 // have converged are skipped.  Iterations are captured in a hipGraph (kernel
 // nodes only) and replayed.
 //
-// Mixed precision: the off-diagonal bands are also kept in FP16 (unscaled;
-// |L_ij| < 20), the per-system diagonals stay FP32, and the residual is
+// Mixed precision: the off-diagonal bands are also kept in FP16 or BF16
+// (unscaled; |L_ij| < 20), the per-system diagonals stay FP32, and the residual is
 // replaced by the FP64-accumulated true residual (FP32 L) every 50
 // iterations (see kDrift2 for when the search direction is kept).
 #include <string.h>
@@ -201,6 +201,10 @@ template <>
 struct PixBands<kF16> {
     u32x4 g[3];
 };
+template <>
+struct PixBands<kBF16> {
+    u32x4 g[3];
+};
 template <int FMT>
 __device__ __forceinline__ void load_pixel_bands(const BImg& I, int i, PixBands<FMT>& pb) {
     if constexpr (FMT == kF32) {
@@ -226,7 +230,7 @@ __device__ __forceinline__ void pixel_bands(const PixBands<FMT>& pb, float (&a)[
 #pragma unroll
             for (int h = 0; h < 8; ++h) {
                 const unsigned short u = (unsigned short)(pb.g[q][h >> 1] >> (16 * (h & 1)));
-                a[8 * q + h] = (float)__builtin_bit_cast(_Float16, u);
+                a[8 * q + h] = FMT == kBF16 ? bf16_to_f32(u) : (float)__builtin_bit_cast(_Float16, u);
             }
     }
 }
@@ -651,12 +655,14 @@ __global__ __launch_bounds__(kBlock) void kb_set_rhs(const float* __restrict__ L
     }
 }
 
-// The off-diagonal bands in FP16, 8 per pixel in each of three groups: group q
+// The off-diagonal bands in FP16 or BF16, 8 per pixel in each of three groups: group q
 // of pixel i is the 16 bytes at element q * pitch16 + i, halves in band order
 // (bands 0..11, 13..24), so a pixel's 24 entries are three aligned 16-byte
 // loads (a lane per pixel: 1 KB contiguous per wave and load).
+template <int FMT>
 __global__ __launch_bounds__(kBlock) void kb_to16(const float* __restrict__ L, size_t pitch, int N,
                                                   unsigned short* __restrict__ L16, size_t pitch16) {
+    auto cvt = [](float v) { return (unsigned)(FMT == kBF16 ? f32_to_bf16_rne(v) : f32_to_f16_rne(v)); };
     u32x4* g = reinterpret_cast<u32x4*>(L16);
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < N; i += gridDim.x * kBlock) {
 #pragma unroll
@@ -666,8 +672,7 @@ __global__ __launch_bounds__(kBlock) void kb_to16(const float* __restrict__ L, s
             for (int w = 0; w < 4; ++w) {
                 const int kk0 = 8 * q + 2 * w, kk1 = kk0 + 1;  // off-diagonal band indices
                 const int k0 = kk0 < 12 ? kk0 : kk0 + 1, k1 = kk1 < 12 ? kk1 : kk1 + 1;
-                v[w] = (unsigned)f32_to_f16_rne(L[(size_t)k0 * pitch + i]) |
-                       ((unsigned)f32_to_f16_rne(L[(size_t)k1 * pitch + i]) << 16);
+                v[w] = cvt(L[(size_t)k0 * pitch + i]) | (cvt(L[(size_t)k1 * pitch + i]) << 16);
             }
             g[(size_t)q * pitch16 + i] = v;
         }
@@ -809,7 +814,7 @@ extern "C" {
 gm_status gm_batch_create(gm_batch** out, int nimages, const int* widths, const int* heights, int nrhs, int precision) {
     if (!out || nimages <= 0 || !widths || !heights || nrhs <= 0 || nrhs > kMaxRhs)
         return fail(GM_ERR_INVALID, "gm_batch_create: bad arguments (nrhs must be 1..%d)", kMaxRhs);
-    if (precision != GM_PREC_FP32 && precision != GM_PREC_FP16)
+    if (precision != GM_PREC_FP32 && precision != GM_PREC_FP16 && precision != GM_PREC_BF16)
         return fail(GM_ERR_UNSUPPORTED, "gm_batch_create: batched forms are FP32 and FP16");
     gm_batch* b = new gm_batch();
     b->nimg = nimages;
@@ -844,7 +849,7 @@ gm_status gm_batch_create(gm_batch** out, int nimages, const int* widths, const 
         loff[i] = Lfloats;
         Lfloats += 25 * pitch;
         l16off[i] = L16n;
-        if (precision == GM_PREC_FP16) L16n += 25 * pitch;
+        if (precision != GM_PREC_FP32) L16n += 25 * pitch;
         voff[i] = vfloats;
         vfloats += (size_t)nrhs * 9 * pitch;
         b->himg.push_back(I);
@@ -915,11 +920,15 @@ gm_status gm_batch_assemble(gm_batch* b, int image, const float* im, int nch, si
     gm_status st = levin_assemble_raw(const_cast<float*>(I.L), I.pitch, nullptr, nullptr, im, nch, im_pitch, nullptr,
                                       nullptr, nullptr, 0.f, eps, GM_ASSEMBLY_ACCURATE, I.W, I.H, s);
     if (st != GM_OK) return st;
-    if (b->precision == GM_PREC_FP16) {
+    if (b->precision != GM_PREC_FP32) {
         DevInfo d;
         if ((st = dev_info(&d)) != GM_OK) return st;
-        hipLaunchKernelGGL(kb_to16, dim3(elementwise_grid(d, I.N, 1)), dim3(kBlock), 0, (hipStream_t)s, I.L, I.pitch,
-                           I.N, const_cast<unsigned short*>(I.L16), I.pitch16);
+        if (b->precision == GM_PREC_BF16)
+            hipLaunchKernelGGL(kb_to16<kBF16>, dim3(elementwise_grid(d, I.N, 1)), dim3(kBlock), 0, (hipStream_t)s,
+                               I.L, I.pitch, I.N, const_cast<unsigned short*>(I.L16), I.pitch16);
+        else
+            hipLaunchKernelGGL(kb_to16<kF16>, dim3(elementwise_grid(d, I.N, 1)), dim3(kBlock), 0, (hipStream_t)s,
+                               I.L, I.pitch, I.N, const_cast<unsigned short*>(I.L16), I.pitch16);
         GM_LAUNCH_CHECK();
     }
     return GM_OK;
@@ -973,6 +982,9 @@ static gm_status batch_launch_iter(gm_batch* b, int verify, hipStream_t hs) {
     if (b->precision == GM_PREC_FP16)
         hipLaunchKernelGGL(kb_step<kF16>, dim3(P1.G), dim3(kBlock), 0, hs, b->dimg, b->dsys, b->dsc, P1, P2, slot1(b),
                            slot2(b), verify);
+    else if (b->precision == GM_PREC_BF16)
+        hipLaunchKernelGGL(kb_step<kBF16>, dim3(P1.G), dim3(kBlock), 0, hs, b->dimg, b->dsys, b->dsc, P1, P2,
+                           slot1(b), slot2(b), verify);
     else
         hipLaunchKernelGGL(kb_step<kF32>, dim3(P1.G), dim3(kBlock), 0, hs, b->dimg, b->dsys, b->dsc, P1, P2, slot1(b),
                            slot2(b), verify);
@@ -1036,7 +1048,7 @@ gm_status gm_batch_solve(gm_batch* b, const gm_solve_opts* opts, gm_stream_t s) 
     hipStream_t hs = (hipStream_t)s;
     const int verify = (o.tol > 0 && o.verify) ? 1 : 0;
     const int chunk = o.check_every > 0 ? o.check_every : 10;
-    const bool mixed = b->precision == GM_PREC_FP16;
+    const bool mixed = b->precision != GM_PREC_FP32;
     int refine = o.refine_every > 0 ? o.refine_every : (o.refine_every == 0 && mixed ? 50 : 0);
     if (refine > 0 && refine % chunk) refine = (refine / chunk + 1) * chunk;  // replacement at chunk boundaries
 

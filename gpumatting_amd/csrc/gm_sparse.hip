@@ -216,6 +216,16 @@ struct DevBufs {
         if (st_ != GM_OK) return st_;  \
     } while (0)
 
+// frees an output matrix that a builder allocated unless the build finished
+struct OutGuard {
+    CompressedMatrix* m;
+    bool done = false;
+    explicit OutGuard(CompressedMatrix* m_) : m(m_) {}
+    ~OutGuard() {
+        if (!done && m && m->p) gm_csm_free(m);
+    }
+};
+
 // stable sort of (key, value) pairs over bits [0, end_bit)
 template <class K>
 gm_status sort_pairs(DevBufs& B, const K* kin, K* kout, const unsigned* vin, unsigned* vout, long n, int end_bit,
@@ -324,6 +334,7 @@ gm_status gm_coo_to_csm(CompressedMatrix* out, const CoordMatrix* in, gm_stream_
     out->nnz = (size_t)n;
     const unsigned long long cols = in->cols > 0 ? (unsigned long long)in->cols : 1ull;
     if (n == 0) return GM_OK;
+    OutGuard guard(out);
     DevBufs B;
     unsigned long long *k0, *k1;
     unsigned *v0, *v1;
@@ -340,6 +351,7 @@ gm_status gm_coo_to_csm(CompressedMatrix* out, const CoordMatrix* in, gm_stream_
                        in->rows, cols, out->p);
     GM_LAUNCH_CHECK();
     GM_HIP(hipStreamSynchronize(s));
+    guard.done = true;
     return GM_OK;
 }
 
@@ -388,6 +400,7 @@ gm_status gm_sp_restriction(CompressedMatrix* P, const int* labels, int npix, in
     GM_TRY(gm_csm_init(P, nlabels, npix, (size_t)npix));
     P->nnz = (size_t)npix;
     if (npix == 0) return GM_OK;
+    OutGuard guard(P);
     unsigned *k0, *k1, *v0;
     GM_TRY(B.alloc(&k0, npix));
     GM_TRY(B.alloc(&k1, npix));
@@ -402,6 +415,7 @@ gm_status gm_sp_restriction(CompressedMatrix* P, const int* labels, int npix, in
                        nlabels, 1ull, P->p);
     GM_LAUNCH_CHECK();
     GM_HIP(hipStreamSynchronize(s));
+    guard.done = true;
     return GM_OK;
 }
 
@@ -469,6 +483,7 @@ gm_status gm_sp_galerkin(CompressedMatrix* Ac, const BandedMatrix* A, const int*
     GM_LAUNCH_CHECK();
     GM_TRY(gm_csm_init(Ac, nlabels, nlabels, (size_t)nruns));
     Ac->nnz = (size_t)nruns;
+    OutGuard guard(Ac);
     {
         DevInfo d;
         GM_TRY(dev_info(&d));
@@ -481,6 +496,7 @@ gm_status gm_sp_galerkin(CompressedMatrix* Ac, const BandedMatrix* A, const int*
                        nl, Ac->p);
     GM_LAUNCH_CHECK();
     GM_HIP(hipStreamSynchronize(s));
+    guard.done = true;
     return GM_OK;
 }
 

@@ -268,8 +268,9 @@ int gm_levin_pad(int W);
 typedef struct gm_batch_s gm_batch;
 
 /* Allocate a batch: nimages images (widths[i] x heights[i]) with nrhs <= 8
- * constraint sets each; precision GM_PREC_FP32 or GM_PREC_FP16 (FP16
- * off-diagonal bands, FP32 diagonals and vectors, FP64 residual replacement). */
+ * constraint sets each; precision GM_PREC_FP32, GM_PREC_FP16 or GM_PREC_BF16
+ * (16-bit off-diagonal bands, FP32 diagonals and vectors, FP64 residual
+ * replacement). */
 gm_status gm_batch_create(gm_batch** batch, int nimages, const int* widths, const int* heights, int nrhs,
                           int precision);
 gm_status gm_batch_destroy(gm_batch* batch);

@@ -2,7 +2,7 @@
 import numpy as np
 import pytest
 
-from gpumatting_amd import PREC_FP16, PREC_FP32, synth
+from gpumatting_amd import PREC_BF16, PREC_FP16, PREC_FP32, synth
 from gpumatting_amd.batch import Batch
 from oracle import pyoracle as O
 
@@ -19,7 +19,7 @@ def _sparse(a64, W, N):
     return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(N, N))
 
 
-@pytest.mark.parametrize("prec", [PREC_FP32, PREC_FP16])
+@pytest.mark.parametrize("prec", [PREC_FP32, PREC_FP16, PREC_BF16])
 def test_batched_matches_fp64_per_system(gpu, prec):
     """Every system reaches the tolerance, checked twice: by the solver's own
     FP64 true residual and by an independent FP64 residual of the returned
