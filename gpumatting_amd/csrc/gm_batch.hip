@@ -722,7 +722,7 @@ struct gm_batch_s {
     hipGraphExec_t gexec = nullptr;
     int graph_iters = 0, graph_verify = -1;
     hipEvent_t ev = nullptr;
-    int per_cu = 4;
+    int per_cu = 16;  // workgroups per CU of the batched launches (gm_batch_set_grid)
 };
 
 static void free_part(HostPart& p) {
@@ -776,13 +776,23 @@ static void split(HostPart& p, int G, std::vector<BImg>& imgs, bool chunks) {
     }
 }
 
+// Workgroups of a batched launch: per_cu per CU (more than fit at once: a
+// workgroup whose range lies in converged images ends at once and the
+// hardware starts the next, which balances the uneven work), at most one per
+// item, a multiple of the XCD count.
+static int batch_grid(const DevInfo& d, long items, int per_cu) {
+    long g = std::min((long)d.cus * per_cu, items);
+    g = std::max(g, 1L);
+    return (int)((g + kNumXcd - 1) / kNumXcd * kNumXcd);
+}
+
 // (Re)build both partitions for the batch's grid and upload them.
 static gm_status batch_partition(gm_batch* b) {
     DevInfo d;
     gm_status st = dev_info(&d);
     if (st != GM_OK) return st;
-    const int G1 = grid_for_tiles(d, (long)b->p1.img.size(), b->per_cu);
-    const int G2 = grid_for_tiles(d, (long)b->p2.img.size(), 2 * b->per_cu);
+    const int G1 = batch_grid(d, (long)b->p1.img.size(), b->per_cu);
+    const int G2 = batch_grid(d, (long)b->p2.img.size(), 2 * b->per_cu);
     split(b->p1, G1, b->himg, false);
     split(b->p2, G2, b->himg, true);
     for (HostPart* p : {&b->p1, &b->p2}) {
@@ -1096,7 +1106,7 @@ gm_status gm_batch_solve(gm_batch* b, const gm_solve_opts* opts, gm_stream_t s) 
 }
 
 gm_status gm_batch_set_grid(gm_batch* b, int per_cu) {
-    if (!b || per_cu < 1 || per_cu > 16) return fail(GM_ERR_INVALID, "gm_batch_set_grid: per_cu 1..16");
+    if (!b || per_cu < 1 || per_cu > 64) return fail(GM_ERR_INVALID, "gm_batch_set_grid: per_cu 1..64");
     b->per_cu = per_cu;
     return batch_partition(b);
 }

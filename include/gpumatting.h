@@ -289,7 +289,10 @@ gm_status gm_batch_solve(gm_batch* batch, const gm_solve_opts* opts, gm_stream_t
 gm_status gm_batch_alpha(gm_batch* batch, int image, int rhs, float** alpha);
 /* Per-system result of the last gm_batch_solve. */
 gm_status gm_batch_info(gm_batch* batch, int image, int rhs, gm_solve_info* info);
-/* Persistent grid size in workgroups per CU (autotuned value or manual). */
+/* Workgroups per CU of the batched launches, 1..64 (default 16: more than fit
+ * at once, so workgroups whose ranges lie in converged images end early and
+ * the hardware balances the rest; 4 per CU, one resident wave, is 10 % slower
+ * on uneven batches). */
 gm_status gm_batch_set_grid(gm_batch* batch, int per_cu);
 int gm_batch_tiles(gm_batch* batch);
 
