@@ -17,6 +17,16 @@ from ._lib import CompressedMatrix as _CSM, CoordMatrix as _COO, call, lib
 from .matting import DeviceMatrix, _p, _stream
 
 
+def _vec(x, n: int, what: str) -> torch.Tensor:
+    """x as a contiguous CUDA float32 vector of n elements (the library reads
+    exactly n; a shorter buffer would be read past its end)."""
+    t = torch.as_tensor(np.ascontiguousarray(x, np.float32)) if isinstance(x, np.ndarray) else x
+    t = t.to(device="cuda", dtype=torch.float32).contiguous().reshape(-1)
+    if t.numel() != n:
+        raise ValueError(f"{what}: {t.numel()} elements, expected {n}")
+    return t
+
+
 def _host_copy(ptr, n: int, dtype) -> np.ndarray:
     out = np.empty(n, dtype)
     if n:
@@ -42,12 +52,14 @@ class CompressedMatrix:
 
     def ax(self, x: torch.Tensor) -> torch.Tensor:
         """csmAx: A x."""
+        x = _vec(x, self.cols, "x")
         b = torch.empty(self.rows, dtype=torch.float32, device="cuda")
         call("gm_csm_ax", _p(b), self.ref, _p(x), _stream())
         return b
 
     def axpy(self, x: torch.Tensor, y: torch.Tensor, add: bool = True) -> torch.Tensor:
         """csmAxpy: A x + y (add) or A x - y."""
+        x, y = _vec(x, self.cols, "x"), _vec(y, self.rows, "y")
         b = torch.empty(self.rows, dtype=torch.float32, device="cuda")
         call("gm_csm_axpy", _p(b), self.ref, _p(x), _p(y), int(add), _stream())
         return b
@@ -106,7 +118,10 @@ class CoordMatrix:
         return out
 
     def axpy_(self, b: torch.Tensor, x: torch.Tensor, y: torch.Tensor) -> torch.Tensor:
-        """cooAxpy: b += A x + y, in place."""
+        """cooAxpy: b += A x + y, in place (b: a contiguous CUDA float32 vector)."""
+        if not (b.is_cuda and b.dtype == torch.float32 and b.is_contiguous() and b.numel() == self.rows):
+            raise ValueError("b must be a contiguous CUDA float32 vector of `rows` elements")
+        x, y = _vec(x, self.cols, "x"), _vec(y, self.rows, "y")
         call("gm_coo_axpy", _p(b), self.ref, _p(x), _p(y), _stream())
         return b
 
@@ -138,14 +153,17 @@ def restriction(labels, nlabels: int) -> CompressedMatrix:
 def prolong(xc: torch.Tensor, labels, nlabels: int) -> torch.Tensor:
     """P^T xc: each pixel takes its superpixel's value."""
     lt = _labels(labels)
+    xc = _vec(xc, nlabels, "xc")
     e = torch.empty(lt.numel(), dtype=torch.float32, device="cuda")
-    call("gm_sp_prolong", _p(e), _p(xc.contiguous()), _p(lt), lt.numel(), nlabels, _stream())
+    call("gm_sp_prolong", _p(e), _p(xc), _p(lt), lt.numel(), nlabels, _stream())
     return e
 
 
 def galerkin(A: DeviceMatrix, labels, nlabels: int) -> CompressedMatrix:
     """The coarse operator P A P^T of a device banded matrix."""
     lt = _labels(labels)
+    if lt.numel() != A.rows:
+        raise ValueError(f"labels: {lt.numel()} elements, the matrix has {A.rows} rows")
     out = CompressedMatrix(_raw=_CSM())
     call("gm_sp_galerkin", out.ref, A.ref, _p(lt), nlabels, _stream())
     return out
@@ -156,6 +174,8 @@ def merge_levels(labels, nlabels: int, image, nlevels: int, threshold: float = 0
     im = np.ascontiguousarray(image, np.float32)
     H, W, nch = im.shape
     lab = np.ascontiguousarray(labels, np.int32).reshape(-1).copy()
+    if lab.size != H * W:
+        raise ValueError(f"labels: {lab.size} elements for a {W}x{H} image")
     n = C.c_int(nlabels)
     call("gm_sp_merge_levels", lab.ctypes.data, C.byref(n), im.ctypes.data, nch, W, W, H, nlevels, threshold)
     return lab.reshape(H, W), n.value
