@@ -24,7 +24,7 @@ def dptr(t):
 
 
 def test_cpp_bandedtest_and_vectortest(gpu):
-    for prog in ("bandedtest", "vectortest"):
+    for prog in ("bandedtest", "vectortest", "sparsetest"):
         exe = os.path.join(ROOT, "tests", "cpp", "bin", prog)
         r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
         assert r.returncode == 0 and "Test PASSED" in r.stdout, r.stdout + r.stderr
