@@ -76,6 +76,7 @@ struct BImg {
     size_t pitch;
     const unsigned short* L16;
     size_t pitch16;
+    const float* Lg;  // FP32 form: the off-diagonal bands 4 per pixel in 6 groups (kb_group32)
 };
 
 struct BSys {
@@ -205,6 +206,12 @@ template <>
 struct PixBands<kBF16> {
     u32x4 g[3];
 };
+// the FP32 bands in groups of 4 per pixel (kb_group32): six 16-byte loads
+constexpr int kF32G = 3;
+template <>
+struct PixBands<kF32G> {
+    f32x4 g[6];
+};
 template <int FMT>
 __device__ __forceinline__ void load_pixel_bands(const BImg& I, int i, PixBands<FMT>& pb) {
     if constexpr (FMT == kF32) {
@@ -213,6 +220,10 @@ __device__ __forceinline__ void load_pixel_bands(const BImg& I, int i, PixBands<
             if (k == 12) continue;
             pb.a[kk++] = nt_at_off(I.L + (size_t)k * I.pitch, 4u * (unsigned)i);
         }
+    } else if constexpr (FMT == kF32G) {
+        const f32x4* g = reinterpret_cast<const f32x4*>(I.Lg);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) pb.g[q] = nt_at_off(g + (size_t)q * I.pitch, 16u * (unsigned)i);
     } else {
         const u32x4* g = reinterpret_cast<const u32x4*>(I.L16);
 #pragma unroll
@@ -224,6 +235,11 @@ __device__ __forceinline__ void pixel_bands(const PixBands<FMT>& pb, float (&a)[
     if constexpr (FMT == kF32) {
 #pragma unroll
         for (int k = 0; k < 24; ++k) a[k] = pb.a[k];
+    } else if constexpr (FMT == kF32G) {
+#pragma unroll
+        for (int q = 0; q < 6; ++q)
+#pragma unroll
+            for (int h = 0; h < 4; ++h) a[4 * q + h] = pb.g[q][h];
     } else {
 #pragma unroll
         for (int q = 0; q < 3; ++q)
@@ -679,6 +695,26 @@ __global__ __launch_bounds__(kBlock) void kb_to16(const float* __restrict__ L, s
     }
 }
 
+// The off-diagonal FP32 bands 4 per pixel in six groups: group q of pixel i is
+// the 16 bytes at element q * pitch + i (f32x4 units), bands in order (0..11,
+// 13..24), so a pixel's 24 entries are six aligned 16-byte loads.
+__global__ __launch_bounds__(kBlock) void kb_group32(const float* __restrict__ L, size_t pitch, int N,
+                                                     float* __restrict__ Lg) {
+    f32x4* g = reinterpret_cast<f32x4*>(Lg);
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < N; i += gridDim.x * kBlock) {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            f32x4 v;
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                const int kk = 4 * q + h, k = kk < 12 ? kk : kk + 1;
+                v[h] = L[(size_t)k * pitch + i];
+            }
+            g[(size_t)q * pitch + i] = v;
+        }
+    }
+}
+
 __global__ void kb_init(Scalars* sc, int nsys, int maxit) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nsys) return;
@@ -716,6 +752,7 @@ struct gm_batch_s {
     size_t slot1_n = 0, slot2_n = 0, slot3_n = 0;
     float* Lmem = nullptr;
     unsigned short* L16mem = nullptr;
+    float* Lgmem = nullptr;  // grouped FP32 bands (FP32 batches)
     float* vmem = nullptr;
     hipStream_t cap = nullptr;
     hipGraph_t graph = nullptr;
@@ -740,7 +777,7 @@ static void batch_free(gm_batch* b) {
     free_part(b->p1);
     free_part(b->p2);
     for (void* p : {(void*)b->dimg, (void*)b->dsys, (void*)b->dsc, (void*)b->dslot, (void*)b->Lmem,
-                    (void*)b->L16mem, (void*)b->vmem})
+                    (void*)b->L16mem, (void*)b->Lgmem, (void*)b->vmem})
         if (p) (void)hipFree(p);
     if (b->hsc) (void)hipHostFree(b->hsc);
     delete b;
@@ -831,8 +868,8 @@ gm_status gm_batch_create(gm_batch** out, int nimages, const int* widths, const 
     b->nrhs = nrhs;
     b->nsys = nimages * nrhs;
     b->precision = precision;
-    size_t Lfloats = 0, L16n = 0, vfloats = 0;
-    std::vector<size_t> loff(nimages), l16off(nimages), voff(nimages);
+    size_t Lfloats = 0, L16n = 0, Lgn = 0, vfloats = 0;
+    std::vector<size_t> loff(nimages), l16off(nimages), lgoff(nimages), voff(nimages);
     for (int i = 0; i < nimages; ++i) {
         const int W = widths[i], H = heights[i];
         if (W < 1 || H < 1 || (long long)W * H > (1LL << 30)) {
@@ -860,6 +897,8 @@ gm_status gm_batch_create(gm_batch** out, int nimages, const int* widths, const 
         Lfloats += 25 * pitch;
         l16off[i] = L16n;
         if (precision != GM_PREC_FP32) L16n += 25 * pitch;
+        lgoff[i] = Lgn;
+        if (precision == GM_PREC_FP32) Lgn += 24 * pitch;
         voff[i] = vfloats;
         vfloats += (size_t)nrhs * 9 * pitch;
         b->himg.push_back(I);
@@ -879,12 +918,17 @@ gm_status gm_batch_create(gm_batch** out, int nimages, const int* widths, const 
         BTRY(hipMalloc(&b->L16mem, sizeof(unsigned short) * L16n + 256));
         BTRY(hipMemset(b->L16mem, 0, sizeof(unsigned short) * L16n + 256));
     }
+    if (Lgn) {
+        BTRY(hipMalloc(&b->Lgmem, sizeof(float) * Lgn + 256));
+        BTRY(hipMemset(b->Lgmem, 0, sizeof(float) * Lgn + 256));
+    }
     BTRY(hipMalloc(&b->vmem, sizeof(float) * vfloats + 256));
     BTRY(hipMemset(b->vmem, 0, sizeof(float) * vfloats + 256));
     for (int i = 0; i < nimages; ++i) {
         BImg& I = b->himg[i];
         I.L = b->Lmem + loff[i];
         I.L16 = L16n ? b->L16mem + l16off[i] : nullptr;
+        I.Lg = Lgn ? b->Lgmem + lgoff[i] : nullptr;
         for (int j = 0; j < nrhs; ++j) {
             float* base = b->vmem + voff[i] + (size_t)j * 9 * I.pitch;
             BSys S;
@@ -930,7 +974,13 @@ gm_status gm_batch_assemble(gm_batch* b, int image, const float* im, int nch, si
     gm_status st = levin_assemble_raw(const_cast<float*>(I.L), I.pitch, nullptr, nullptr, im, nch, im_pitch, nullptr,
                                       nullptr, nullptr, 0.f, eps, GM_ASSEMBLY_ACCURATE, I.W, I.H, s);
     if (st != GM_OK) return st;
-    if (b->precision != GM_PREC_FP32) {
+    if (b->precision == GM_PREC_FP32) {
+        DevInfo d;
+        if ((st = dev_info(&d)) != GM_OK) return st;
+        hipLaunchKernelGGL(kb_group32, dim3(elementwise_grid(d, I.N, 1)), dim3(kBlock), 0, (hipStream_t)s, I.L,
+                           I.pitch, I.N, const_cast<float*>(I.Lg));
+        GM_LAUNCH_CHECK();
+    } else {
         DevInfo d;
         if ((st = dev_info(&d)) != GM_OK) return st;
         if (b->precision == GM_PREC_BF16)
@@ -996,8 +1046,8 @@ static gm_status batch_launch_iter(gm_batch* b, int verify, hipStream_t hs) {
         hipLaunchKernelGGL(kb_step<kBF16>, dim3(P1.G), dim3(kBlock), 0, hs, b->dimg, b->dsys, b->dsc, P1, P2,
                            slot1(b), slot2(b), verify);
     else
-        hipLaunchKernelGGL(kb_step<kF32>, dim3(P1.G), dim3(kBlock), 0, hs, b->dimg, b->dsys, b->dsc, P1, P2, slot1(b),
-                           slot2(b), verify);
+        hipLaunchKernelGGL(kb_step<kF32G>, dim3(P1.G), dim3(kBlock), 0, hs, b->dimg, b->dsys, b->dsc, P1, P2,
+                           slot1(b), slot2(b), verify);
     GM_LAUNCH_CHECK();
     hipLaunchKernelGGL(kb_update, dim3(P2.G), dim3(kBlock), 0, hs, b->dimg, b->dsys, b->dsc, P1, P2, slot1(b),
                        slot2(b));
