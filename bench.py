@@ -10,7 +10,9 @@ Inputs (image, weights, targets) are resident in HBM before timing starts.
 `value` is the mixed-precision (FP16 diagonals, FP32 accumulate, FP64 residual
 replacement) ms per solve; the FP32 solve is timed the same way beside it.
 The roofline object is the FP32 banded SpMV (y = A x, 25 diagonals), timed
-with HIP events on its launch stream; the CPU baseline is the oracle's OpenMP
+with HIP events on its launch stream; `roofline_solve_kernel` is the same
+figure for the kernel that dominates the timed mixed solve, the fused CG
+iteration; the CPU baseline is the oracle's OpenMP
 restatement of the same assembly + Jacobi-PCG on a bounded sample.
 `alpha_err_vs_fp64` gives max / mean |alpha - alpha_FP64| of both 1e-6 solves
 against the same system solved to 1e-12 with an FP64 master x;
@@ -460,6 +462,29 @@ def main():
         log("cpu baseline (oracle, OpenMP)")
         cpu = cpu_baseline(p, results["fp32"]["iterations"], args.cpu_iters)
 
+    # the dominant kernel of the timed mixed solve: the fused CG iteration
+    # (about 86 % of the solve), timed by the autotuner with HIP events on its
+    # stream in this process; 60 B per row (12 x 2 B bands, r, Ap, p, x, s in;
+    # x, r, p, Ap out)
+    fused = None
+    mt = tuned.get("mixed")
+    if mt and mt.get("fused_us", 0) > 0:
+        fused_bytes = 60 * N
+        fused_gbs = fused_bytes / (mt["fused_us"] * 1e-6) / 1e9
+        ftraffic = None
+        try:
+            with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+                t = json.load(f).get("gm::k_cgfused_symw<1>")
+            if t and t.get("workload", "").startswith(f"{W}x{H}"):
+                ftraffic = t["bytes_per_launch"]
+        except (OSError, ValueError):
+            pass
+        fused = {"bound": "hbm", "kernel": "gm::k_cgfused_symw<1> (fused mixed CG iteration, FP16 bands)",
+                 "achieved": round(fused_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": round(fused_gbs / HBM_PEAK_GBS, 4), "traffic": ftraffic, "algorithmic_bytes": fused_bytes,
+                 "us_per_launch": mt["fused_us"]}
+        log(f"fused iteration: {mt['fused_us']:.1f} us/launch, {fused_gbs:.0f} GB/s algorithmic")
+
     mixed, fp32 = results["mixed"], results["fp32"]
     out = {
         "metric": METRIC,
@@ -492,6 +517,7 @@ def main():
                      "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes": spmv_bytes},
+        "roofline_solve_kernel": fused,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
