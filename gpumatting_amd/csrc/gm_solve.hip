@@ -1377,52 +1377,59 @@ __global__ __launch_bounds__(kBlock) void k_trueres_fin(Scalars* sc, const doubl
 }
 
 // Mixed-precision operator: As = S A S with S = diag(1/sqrt(a_ii)) (unit
-// diagonal, band 12 not stored), rounded to FP16 or BF16; s written to svec.
+// diagonal, band 12 not stored), rounded to FP16 or BF16, in two streaming
+// passes.  k_svec: s (1 where a_ii <= 0).  k_scale16: each thread 4
+// consecutive rows, every band's 16-byte chunk and the 4 neighbour scales of
+// each band issued before any conversion; the neighbour scales (s at i + o,
+// 0 outside the vector, where the band entry is 0 too) come from L2, since s
+// was just written.  (a_ik s_i) s_{i+o} in that order.
 // SYM: only the upper bands 13..24 (the symmetric step kernel's form).
-template <int R, int FMT, bool SYM>
+__global__ __launch_bounds__(kBlock) void k_svec(const float* __restrict__ A, size_t pitch, int N,
+                                                 float* __restrict__ svec) {
+    for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long)gridDim.x * kBlock) {
+        const float d = A[(size_t)12 * pitch + i];
+        svec[i] = d > 0.f ? 1.0f / sqrtf(d) : 1.0f;
+    }
+}
+
+template <int FMT, bool SYM>
 __global__ __launch_bounds__(kBlock) void k_scale16(const float* __restrict__ A, size_t pitch, int W, int N,
                                                     unsigned short* __restrict__ A16, size_t pitch16,
-                                                    float* __restrict__ svec) {
-    static_assert(R % 2 == 0, "half pairs");
-    __shared__ __attribute__((aligned(16))) float lds[Tile<R>::LDS_FLOATS];
-    constexpr int SEG = Tile<R>::SEG;
-    const int ntiles = (N + Tile<R>::ROWS - 1) / Tile<R>::ROWS;
-    const TileRange tr = xcd_tiles(ntiles);
-    for (int t = tr.begin; t < tr.end; t += tr.step) {
-        const int i0 = t * Tile<R>::ROWS;
-        __syncthreads();
-        stage5<R>(lds, i0, W, N, [&](int g) {
-            const float d = A[(size_t)12 * pitch + g];
-            return d > 0.f ? 1.0f / sqrtf(d) : 1.0f;
-        });
-        __syncthreads();
-        const int ib = i0 + R * threadIdx.x;
-        if (ib >= N) continue;
-        const float* xs = lds + R * threadIdx.x;
+                                                    const float* __restrict__ svec) {
+    constexpr int K0 = SYM ? 13 : 0, NB = 25 - K0 - (SYM ? 0 : 1);
+    const long nq = ((long)N + 3) / 4;
+    for (long q = (long)blockIdx.x * kBlock + threadIdx.x; q < nq; q += (long)gridDim.x * kBlock) {
+        const int ib = (int)(4 * q);
+        float si[4], a[NB][4], sn[NB][4];
 #pragma unroll
-        for (int k = SYM ? 13 : 0; k < 25; ++k) {
-            if (k == 12) continue;
-            const int s = k / 5, dx = k % 5;
-            Raw<R, kF32> a;
-            load_band<R, kF32>(A, (size_t)k * pitch + (size_t)ib, a);
-            unsigned int h[R / 2];
+        for (int j = 0; j < 4; ++j) si[j] = ib + j < N ? svec[ib + j] : 0.f;
 #pragma unroll
-            for (int j = 0; j < R; j += 2) {
-                const float v0 = (a.get(j) * xs[2 * SEG + j + 2]) * xs[s * SEG + j + dx];
-                const float v1 = (a.get(j + 1) * xs[2 * SEG + j + 3]) * xs[s * SEG + j + 1 + dx];
+        for (int m = 0; m < NB; ++m) {
+            const int k = K0 + m + (!SYM && K0 + m >= 12 ? 1 : 0);
+            const int o = (k / 5 - 2) * W + (k % 5 - 2);
+            // rows ib .. ib + 3 lie inside the padded pitch; 16-byte aligned
+            const f32x4 v = *reinterpret_cast<const f32x4*>(A + (size_t)k * pitch + ib);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                a[m][j] = v[j];
+                const long n = (long)ib + j + o;
+                sn[m][j] = (n >= 0 && n < N) ? svec[n] : 0.f;
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < NB; ++m) {
+            const int k = K0 + m + (!SYM && K0 + m >= 12 ? 1 : 0);
+            unsigned int h[2];
+#pragma unroll
+            for (int j = 0; j < 4; j += 2) {
+                const float v0 = (a[m][j] * si[j]) * sn[m][j];
+                const float v1 = (a[m][j + 1] * si[j + 1]) * sn[m][j + 1];
                 const unsigned int h0 = FMT == kF16 ? f32_to_f16_rne(v0) : f32_to_bf16_rne(v0);
                 const unsigned int h1 = FMT == kF16 ? f32_to_f16_rne(v1) : f32_to_bf16_rne(v1);
                 h[j / 2] = h0 | (h1 << 16);
             }
-            // rows ib .. ib + R - 1 lie inside the padded pitch; 4R-byte aligned
-            unsigned int* dst = reinterpret_cast<unsigned int*>(A16 + (size_t)k * pitch16 + ib);
-#pragma unroll
-            for (int j = 0; j < R / 2; ++j) dst[j] = h[j];
+            *reinterpret_cast<u32x2*>(A16 + (size_t)k * pitch16 + ib) = u32x2{h[0], h[1]};
         }
-        float sc[R];
-#pragma unroll
-        for (int j = 0; j < R; ++j) sc[j] = xs[2 * SEG + j + 2];
-        store_rows<R>(svec, ib, N, sc);
     }
 }
 
@@ -2007,22 +2014,25 @@ static gm_status launch_trueres(Solver& S, int op, int xzero = 0) {
     return GM_OK;
 }
 
-static void launch_scale16(Solver& S, int grid) {
+static void launch_scale16(Solver& S) {
     Workspace* w = S.w;
     const BandedMatrix* A = S.A;
+    hipLaunchKernelGGL(k_svec, dim3(elementwise_grid(S.dv, S.N, 1)), dim3(kBlock), 0, S.hs, A->a, A->apitch, S.N,
+                       w->dinv);
+    const int grid = elementwise_grid(S.dv, ((long)S.N + 3) / 4, 1);
     if (S.fmt == kF16) {
         if (S.sym)
-            hipLaunchKernelGGL((k_scale16<8, kF16, true>), dim3(grid), dim3(kBlock), 0, S.hs, A->a, A->apitch,
+            hipLaunchKernelGGL((k_scale16<kF16, true>), dim3(grid), dim3(kBlock), 0, S.hs, A->a, A->apitch,
                                S.plan.W, S.N, w->A16, w->pitch16, w->dinv);
         else
-            hipLaunchKernelGGL((k_scale16<8, kF16, false>), dim3(grid), dim3(kBlock), 0, S.hs, A->a, A->apitch,
+            hipLaunchKernelGGL((k_scale16<kF16, false>), dim3(grid), dim3(kBlock), 0, S.hs, A->a, A->apitch,
                                S.plan.W, S.N, w->A16, w->pitch16, w->dinv);
     } else {
         if (S.sym)
-            hipLaunchKernelGGL((k_scale16<8, kBF16, true>), dim3(grid), dim3(kBlock), 0, S.hs, A->a, A->apitch,
+            hipLaunchKernelGGL((k_scale16<kBF16, true>), dim3(grid), dim3(kBlock), 0, S.hs, A->a, A->apitch,
                                S.plan.W, S.N, w->A16, w->pitch16, w->dinv);
         else
-            hipLaunchKernelGGL((k_scale16<8, kBF16, false>), dim3(grid), dim3(kBlock), 0, S.hs, A->a, A->apitch,
+            hipLaunchKernelGGL((k_scale16<kBF16, false>), dim3(grid), dim3(kBlock), 0, S.hs, A->a, A->apitch,
                                S.plan.W, S.N, w->A16, w->pitch16, w->dinv);
     }
 }
@@ -2395,7 +2405,7 @@ gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, c
                            w->dinv, S.N);
         GM_LAUNCH_CHECK();
     } else if (S.pm == kScaled) {
-        launch_scale16(S, S.g_tile8);
+        launch_scale16(S);
         GM_LAUNCH_CHECK();
         if (S.fused) launch_uedge(S);
     }
@@ -2668,7 +2678,7 @@ gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_re
     hipLaunchKernelGGL(k_fill, dim3(S.g_elem), dim3(kBlock), 0, S.hs, w->p1, S.N, 0.5f);
     if (mixed) {
         S.sym = sym16_ok(S.plan.W, S.N);
-        launch_scale16(S, grid_for_tiles(S.dv, (S.N + 2047) / 2048, 3));
+        launch_scale16(S);
     } else {
         hipLaunchKernelGGL(k_diag_inv, dim3(S.g_elem), dim3(kBlock), 0, S.hs,
                            L->a + (size_t)S.plan.args.center * L->apitch, w->dinv, S.N);
