@@ -144,6 +144,10 @@ _SIGS = {
     "gm_sp_galerkin": ([CSMp, BMp, vp, ci, vp], ci),
     "gm_sp_merge_levels": ([vp, C.POINTER(ci), vp, ci, sz, ci, ci, ci, cf], ci),
     "gm_slic": ([vp, C.POINTER(ci), vp, ci, sz, ci, ci, ci, cf, ci, vp], ci),
+    "gm_slic_grid": ([ci, ci, ci, C.POINTER(ci), C.POINTER(ci), C.POINTER(ci)], ci),
+    "gm_slic_lab": ([vp, vp, ci, sz, ci, ci, vp], ci),
+    "gm_slic_raw": ([vp, vp, ci, vp, ci, ci, ci, cf, ci, vp], ci),
+    "gm_slic_connect": ([vp, C.POINTER(ci), ci, ci, ci, vp], ci),
 }
 
 

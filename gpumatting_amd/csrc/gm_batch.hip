@@ -171,7 +171,8 @@ __device__ __forceinline__ void b_step_enter(const BImg& I, int im, const Scalar
 // Element at a byte offset from a vector's base, in global memory (the
 // pointers come from device tables, so the compiler cannot tell by itself and
 // would use flat accesses, which also count against the LDS wait counter).
-// Images have at most 2^30 pixels, so offsets fit 32 bits: the access takes
+// Images have fewer than 2^28 pixels, so byte offsets of up to 16 bytes per
+// pixel fit 32 bits: the access takes
 // the base from scalar registers and one 32-bit offset register, which the
 // systems' vectors share.
 typedef __attribute__((address_space(1))) char gchar;
@@ -872,9 +873,11 @@ gm_status gm_batch_create(gm_batch** out, int nimages, const int* widths, const 
     std::vector<size_t> loff(nimages), l16off(nimages), lgoff(nimages), voff(nimages);
     for (int i = 0; i < nimages; ++i) {
         const int W = widths[i], H = heights[i];
-        if (W < 1 || H < 1 || (long long)W * H > (1LL << 30)) {
+        // below 2^28 pixels: the grouped band loads address 16 bytes per pixel
+        // with a 32-bit byte offset
+        if (W < 1 || H < 1 || (long long)W * H >= (1LL << 28)) {
             batch_free(b);
-            return fail(GM_ERR_INVALID, "gm_batch_create: image %d is %dx%d", i, W, H);
+            return fail(GM_ERR_INVALID, "gm_batch_create: image %d is %dx%d (1 to 2^28-1 pixels)", i, W, H);
         }
         const int N = W * H;
         const size_t pitch = (size_t)(N + 63) / 64 * 64;

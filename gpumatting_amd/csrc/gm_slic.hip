@@ -5,7 +5,10 @@
 // 34(11), 2012, Algorithm 1).
 //
 //   1. RGB in [0, 1] -> CIELAB (sRGB transfer curve, D65 white; IEC 61966-2-1
-//      and CIE 15 formulas).
+//      and CIE 15 formulas), evaluated in FP64 and rounded to the nearest
+//      multiple of 2^-16.  |L|, |a|, |b| < 128, so these values are exact in
+//      float, and every FP64 sum of fewer than 2^29 of them is exact in any
+//      order: the seed means of step 3 do not depend on the summation order.
 //   2. Seeds on a grid of step S = round(sqrt(W H / K)): nsx x nsy cells
 //      (nsx = round(W / S), nsy = round(H / S), at least 1), one seed at each
 //      cell centre, moved to the lowest gradient position of its 3 x 3
@@ -25,11 +28,17 @@
 //      pixel left of its first pixel in raster order (or else above it; the
 //      first piece of the image always stays).  Labels are numbered from 0 in
 //      raster order of first appearance.
+//
+// Every float operation of steps 2-3 is a single IEEE operation (no FMA
+// contraction), so from the same Lab planes the labels and seeds are exactly
+// those of a float32 evaluation in the order written here.
 #include <hipcub/hipcub.hpp>
 #include <math.h>
 #include <string.h>
 
 #include "gm_internal.h"
+
+#pragma clang fp contract(off)
 
 namespace gm {
 
@@ -38,27 +47,29 @@ struct SlicGrid {
     float wd;  // m^2 / S^2
 };
 
+__device__ __forceinline__ float lab_grid(double v) { return (float)(rint(v * 65536.0) / 65536.0); }
+
 __global__ __launch_bounds__(kBlock) void k_rgb_to_lab(const float* __restrict__ im, int nch, size_t pitch, int W,
                                                        int H, float* __restrict__ lab) {
     const long N = (long)W * H;
     for (long u = (long)blockIdx.x * kBlock + threadIdx.x; u < N; u += (long)gridDim.x * kBlock) {
         const int y = (int)(u / W), x = (int)(u - (long)y * W);
         const float* px = im + ((size_t)y * pitch + x) * nch;
-        float c[3];
+        double c[3];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-            const float v = px[q];
-            c[q] = v <= 0.04045f ? v / 12.92f : powf((v + 0.055f) / 1.055f, 2.4f);
+            const double v = (double)px[q];
+            c[q] = v <= 0.04045 ? v / 12.92 : pow((v + 0.055) / 1.055, 2.4);
         }
         // linear sRGB -> XYZ (D65), divided by the white point
-        const float X = (0.4124564f * c[0] + 0.3575761f * c[1] + 0.1804375f * c[2]) / 0.95047f;
-        const float Y = 0.2126729f * c[0] + 0.7151522f * c[1] + 0.0721750f * c[2];
-        const float Z = (0.0193339f * c[0] + 0.1191920f * c[1] + 0.9503041f * c[2]) / 1.08883f;
-        auto f = [](float t) { return t > 0.008856f ? cbrtf(t) : 7.787f * t + 16.f / 116.f; };
-        const float fx = f(X), fy = f(Y), fz = f(Z);
-        lab[u] = 116.f * fy - 16.f;
-        lab[N + u] = 500.f * (fx - fy);
-        lab[2 * N + u] = 200.f * (fy - fz);
+        const double X = (0.4124564 * c[0] + 0.3575761 * c[1] + 0.1804375 * c[2]) / 0.95047;
+        const double Y = 0.2126729 * c[0] + 0.7151522 * c[1] + 0.0721750 * c[2];
+        const double Z = (0.0193339 * c[0] + 0.1191920 * c[1] + 0.9503041 * c[2]) / 1.08883;
+        auto f = [](double t) { return t > 0.008856 ? cbrt(t) : 7.787 * t + 16.0 / 116.0; };
+        const double fx = f(X), fy = f(Y), fz = f(Z);
+        lab[u] = lab_grid(116.0 * fy - 16.0);
+        lab[N + u] = lab_grid(500.0 * (fx - fy));
+        lab[2 * N + u] = lab_grid(200.0 * (fy - fz));
     }
 }
 
@@ -283,14 +294,9 @@ __global__ __launch_bounds__(kBlock) void k_cc_label(const int* __restrict__ par
 
 using namespace gm;
 
-extern "C" gm_status gm_slic(int* labels, int* nlabels, const float* im, int nch, size_t pitch, int W, int H, int K,
-                             float compactness, int iters, gm_stream_t stream) {
-    if (!labels || !nlabels || !im || nch < 3 || W <= 0 || H <= 0 || pitch < (size_t)W || K <= 0 ||
-        !(compactness > 0.f) || iters < 0 || (long)W * H >= (1L << 31))
-        return fail(GM_ERR_INVALID, "gm_slic: arguments");
-    hipStream_t s = (hipStream_t)stream;
-    const long N = (long)W * H;
+static SlicGrid slic_grid(int W, int H, int K, float compactness) {
     SlicGrid g;
+    const long N = (long)W * H;
     g.W = W;
     g.H = H;
     g.S = (int)(sqrt((double)N / K) + 0.5);
@@ -300,23 +306,23 @@ extern "C" gm_status gm_slic(int* labels, int* nlabels, const float* im, int nch
     if (g.nsx < 1) g.nsx = 1;
     if (g.nsy < 1) g.nsy = 1;
     g.wd = compactness * compactness / (float)(g.S * g.S);
+    return g;
+}
+
+// steps 2-3 from Lab planes: raw labels (seed indices), final seeds into sout
+// when given (5 planes of ns floats)
+static gm_status slic_raw(int* labels, float* sout, const float* lab, const SlicGrid& g, int iters, hipStream_t s) {
+    const long N = (long)g.W * g.H;
     const int ns = g.nsx * g.nsy;
     DevInfo d;
     gm_status st = dev_info(&d);
     if (st != GM_OK) return st;
-    float *lab = nullptr, *s0 = nullptr, *s1 = nullptr;
-    auto cleanup = [&]() {
-        if (lab) (void)hipFree(lab);
+    float *s0 = nullptr, *s1 = nullptr;
+    if (hipMalloc(&s0, sizeof(float) * 5 * ns) != hipSuccess || hipMalloc(&s1, sizeof(float) * 5 * ns) != hipSuccess) {
         if (s0) (void)hipFree(s0);
-        if (s1) (void)hipFree(s1);
-    };
-    if (hipMalloc(&lab, sizeof(float) * 3 * N) != hipSuccess || hipMalloc(&s0, sizeof(float) * 5 * ns) != hipSuccess ||
-        hipMalloc(&s1, sizeof(float) * 5 * ns) != hipSuccess) {
-        cleanup();
-        return fail(GM_ERR_OOM, "gm_slic: %dx%d", W, H);
+        return fail(GM_ERR_OOM, "gm_slic: %d seeds", ns);
     }
     const int grid = elementwise_grid(d, N, 1);
-    hipLaunchKernelGGL(k_rgb_to_lab, dim3(grid), dim3(kBlock), 0, s, im, nch, pitch, W, H, lab);
     hipLaunchKernelGGL(k_slic_seeds, dim3((ns + kBlock - 1) / kBlock), dim3(kBlock), 0, s, lab, g, s0);
     hipLaunchKernelGGL(k_slic_init_labels, dim3(grid), dim3(kBlock), 0, s, g, labels);
     const int ugrid = ns < d.cus * 8 ? ns : d.cus * 8;
@@ -325,18 +331,32 @@ extern "C" gm_status gm_slic(int* labels, int* nlabels, const float* im, int nch
         hipLaunchKernelGGL(k_slic_update, dim3(ugrid), dim3(kBlock), 0, s, lab, g, labels, s0, s1);
         std::swap(s0, s1);
     }
-    // connectivity: parent, size, tgt, fin as ints; flag, num as unsigned
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && sout) e = hipMemcpyAsync(sout, s0, sizeof(float) * 5 * ns, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(s0);
+    (void)hipFree(s1);
+    if (e != hipSuccess) return hip_fail(e, "gm_slic", __FILE__, __LINE__);
+    return GM_OK;
+}
+
+// step 4 in place
+static gm_status slic_connect(int* labels, int* nlabels, int W, int H, int minsize, hipStream_t s) {
+    const long N = (long)W * H;
+    DevInfo d;
+    gm_status st = dev_info(&d);
+    if (st != GM_OK) return st;
+    const int grid = elementwise_grid(d, N, 1);
+    // parent, size, tgt, fin as ints; flag, num as unsigned
     int* cc = nullptr;
     void* tmp = nullptr;
     size_t ntmp = 0;
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipMalloc(&cc, sizeof(int) * 6 * (size_t)N);
+    hipError_t e = hipMalloc(&cc, sizeof(int) * 6 * (size_t)N);
     int *parent = cc, *size = cc + N, *tgt = cc + 2 * N, *fin = cc + 3 * N;
     unsigned *flag = reinterpret_cast<unsigned*>(cc + 4 * N), *num = reinterpret_cast<unsigned*>(cc + 5 * N);
     if (e == hipSuccess) e = hipcub::DeviceScan::ExclusiveSum(nullptr, ntmp, flag, num, (int)N, s);
     if (e == hipSuccess) e = hipMalloc(&tmp, ntmp ? ntmp : 1);
     if (e == hipSuccess) {
-        const int minsize = g.S * g.S / 4 > 1 ? g.S * g.S / 4 : 1;
         hipLaunchKernelGGL(k_cc_init, dim3(grid), dim3(kBlock), 0, s, parent, size, N);
         hipLaunchKernelGGL(k_cc_union, dim3(grid), dim3(kBlock), 0, s, labels, W, N, parent);
         hipLaunchKernelGGL(k_cc_flatten, dim3(grid), dim3(kBlock), 0, s, parent, size, N);
@@ -354,8 +374,67 @@ extern "C" gm_status gm_slic(int* labels, int* nlabels, const float* im, int nch
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (cc) (void)hipFree(cc);
     if (tmp) (void)hipFree(tmp);
-    cleanup();
-    if (e != hipSuccess) return hip_fail(e, "gm_slic", __FILE__, __LINE__);
+    if (e != hipSuccess) return hip_fail(e, "gm_slic_connect", __FILE__, __LINE__);
     *nlabels = (int)(last[0] + last[1]);
     return GM_OK;
 }
+
+static bool slic_size_ok(int W, int H) { return W > 0 && H > 0 && (long)W * H < (1L << 31); }
+
+extern "C" {
+
+gm_status gm_slic_grid(int W, int H, int K, int* S, int* nsx, int* nsy) {
+    if (!slic_size_ok(W, H) || K <= 0) return fail(GM_ERR_INVALID, "gm_slic_grid: arguments");
+    const SlicGrid g = slic_grid(W, H, K, 1.f);
+    if (S) *S = g.S;
+    if (nsx) *nsx = g.nsx;
+    if (nsy) *nsy = g.nsy;
+    return GM_OK;
+}
+
+gm_status gm_slic_lab(float* lab, const float* im, int nch, size_t pitch, int W, int H, gm_stream_t stream) {
+    if (!lab || !im || nch < 3 || !slic_size_ok(W, H) || pitch < (size_t)W)
+        return fail(GM_ERR_INVALID, "gm_slic_lab: arguments");
+    DevInfo d;
+    gm_status st = dev_info(&d);
+    if (st != GM_OK) return st;
+    hipLaunchKernelGGL(k_rgb_to_lab, dim3(elementwise_grid(d, (long)W * H, 1)), dim3(kBlock), 0, (hipStream_t)stream,
+                       im, nch, pitch, W, H, lab);
+    GM_LAUNCH_CHECK();
+    return GM_OK;
+}
+
+gm_status gm_slic_raw(int* labels, float* seeds, int maxseeds, const float* lab, int W, int H, int K,
+                      float compactness, int iters, gm_stream_t stream) {
+    if (!labels || !lab || !slic_size_ok(W, H) || K <= 0 || !(compactness > 0.f) || iters < 0)
+        return fail(GM_ERR_INVALID, "gm_slic_raw: arguments");
+    const SlicGrid g = slic_grid(W, H, K, compactness);
+    if (seeds && maxseeds < g.nsx * g.nsy)
+        return fail(GM_ERR_INVALID, "gm_slic_raw: %d seeds do not fit %d", g.nsx * g.nsy, maxseeds);
+    return slic_raw(labels, seeds, lab, g, iters, (hipStream_t)stream);
+}
+
+gm_status gm_slic_connect(int* labels, int* nlabels, int W, int H, int minsize, gm_stream_t stream) {
+    if (!labels || !nlabels || !slic_size_ok(W, H) || minsize < 1)
+        return fail(GM_ERR_INVALID, "gm_slic_connect: arguments");
+    return slic_connect(labels, nlabels, W, H, minsize, (hipStream_t)stream);
+}
+
+gm_status gm_slic(int* labels, int* nlabels, const float* im, int nch, size_t pitch, int W, int H, int K,
+                  float compactness, int iters, gm_stream_t stream) {
+    if (!labels || !nlabels || !im || nch < 3 || !slic_size_ok(W, H) || pitch < (size_t)W || K <= 0 ||
+        !(compactness > 0.f) || iters < 0)
+        return fail(GM_ERR_INVALID, "gm_slic: arguments");
+    hipStream_t s = (hipStream_t)stream;
+    const long N = (long)W * H;
+    const SlicGrid g = slic_grid(W, H, K, compactness);
+    float* lab = nullptr;
+    if (hipMalloc(&lab, sizeof(float) * 3 * N) != hipSuccess) return fail(GM_ERR_OOM, "gm_slic: %dx%d", W, H);
+    gm_status st = gm_slic_lab(lab, im, nch, pitch, W, H, stream);
+    if (st == GM_OK) st = slic_raw(labels, nullptr, lab, g, iters, s);
+    (void)hipFree(lab);
+    if (st != GM_OK) return st;
+    return slic_connect(labels, nlabels, W, H, g.S * g.S / 4 > 1 ? g.S * g.S / 4 : 1, s);
+}
+
+}  // extern "C"

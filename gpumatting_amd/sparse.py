@@ -191,3 +191,44 @@ def slic(image, K: int, compactness: float = 10.0, iters: int = 10):
     n = C.c_int(0)
     call("gm_slic", _p(labels), C.byref(n), _p(im), nch, W, W, H, K, compactness, iters, _stream())
     return labels, n.value
+
+
+def slic_grid(W: int, H: int, K: int):
+    """(S, nsx, nsy): the grid step and seed counts SLIC uses for K superpixels."""
+    S, nx, ny = C.c_int(), C.c_int(), C.c_int()
+    call("gm_slic_grid", W, H, K, C.byref(S), C.byref(nx), C.byref(ny))
+    return S.value, nx.value, ny.value
+
+
+def slic_lab(image) -> torch.Tensor:
+    """The CIELAB planes SLIC clusters: (3, H, W) float32 CUDA tensor."""
+    t = torch.as_tensor(np.ascontiguousarray(image, np.float32)) if isinstance(image, np.ndarray) else image
+    im = t.to(device="cuda", dtype=torch.float32).contiguous()
+    H, W, nch = im.shape
+    lab = torch.empty((3, H, W), dtype=torch.float32, device="cuda")
+    call("gm_slic_lab", _p(lab), _p(im), nch, W, W, H, _stream())
+    return lab
+
+
+def slic_raw(lab, K: int, compactness: float = 10.0, iters: int = 10):
+    """Seeds and `iters` rounds from Lab planes (3, H, W): (raw labels (H, W)
+    int32, seeds (5, nseeds) float32: l, a, b, x, y), both CUDA tensors."""
+    t = torch.as_tensor(np.ascontiguousarray(lab, np.float32)) if isinstance(lab, np.ndarray) else lab
+    lab = t.to(device="cuda", dtype=torch.float32).contiguous()
+    _, H, W = lab.shape
+    _, nx, ny = slic_grid(W, H, K)
+    labels = torch.empty((H, W), dtype=torch.int32, device="cuda")
+    seeds = torch.empty((5, nx * ny), dtype=torch.float32, device="cuda")
+    call("gm_slic_raw", _p(labels), _p(seeds), nx * ny, _p(lab), W, H, K, compactness, iters, _stream())
+    return labels, seeds
+
+
+def slic_connect(labels, minsize: int):
+    """Connectivity of raw labels (H, W): (labels (H, W) int32 CUDA tensor,
+    number of superpixels)."""
+    t = torch.as_tensor(np.ascontiguousarray(labels, np.int32)) if isinstance(labels, np.ndarray) else labels
+    out = t.to(device="cuda", dtype=torch.int32).contiguous().clone()
+    H, W = out.shape
+    n = C.c_int(0)
+    call("gm_slic_connect", _p(out), C.byref(n), W, H, minsize, _stream())
+    return out, n.value

@@ -61,11 +61,10 @@ def _bilinear_field(rng: np.random.Generator, W: int, H: int, gw: int = 8, gh: i
     x0 = np.minimum(np.floor(xs).astype(np.int64), gw - 2)
     fy = (ys - y0)[:, None, None]
     fx = (xs - x0)[None, :, None]
-    g00 = grid[y0][:, x0]
-    g01 = grid[y0][:, x0 + 1]
-    g10 = grid[y0 + 1][:, x0]
-    g11 = grid[y0 + 1][:, x0 + 1]
-    return (1 - fy) * ((1 - fx) * g00 + fx * g01) + fy * ((1 - fx) * g10 + fx * g11)
+    # (1-fy)*((1-fx)*g00 + fx*g01) + fy*((1-fx)*g10 + fx*g11), with the inner
+    # x interpolation done once per grid row (same operations per element)
+    rows = (1 - fx) * grid[:, x0] + fx * grid[:, x0 + 1]  # (gh, W, 3)
+    return (1 - fy) * rows[y0] + fy * rows[y0 + 1]
 
 
 def _ellipses_alpha(rng: np.random.Generator, W: int, H: int, n: int, width: float) -> np.ndarray:
@@ -149,16 +148,34 @@ def config_grad(n_images: int = 16, seed: int = 5000, W: int = 512, H: int = 512
     return [make_problem(W, H, seed + i, 1, 0.08) for i in range(n_images)]
 
 
-def config_batched(n_images: int = 64, nrhs: int = 4, seed: int = 4000, wmin=240, wmax=1080, hmin=320, hmax=1920):
-    """Batched workload: n_images images with W ~ U[wmin,wmax], H ~ U[hmin,hmax]
-    (one ellipse each), and nrhs independent constraint sets per image with
-    densities ~ U[2 %, 10 %].  Returns a list of (problem, [(lam, tgt), ...])."""
+def config_batched_plan(n_images: int = 64, nrhs: int = 4, seed: int = 4000, wmin=240, wmax=1080, hmin=320,
+                        hmax=1920):
+    """The draws of config_batched without building any image: a list of
+    (W, H, [density of each constraint set]) per image."""
     rng = np.random.default_rng(seed)
-    out = []
-    for i in range(n_images):
+    plan = []
+    for _ in range(n_images):
         W = int(rng.integers(wmin, wmax + 1))
         H = int(rng.integers(hmin, hmax + 1))
-        p = make_problem(W, H, seed + 1 + i, 1, 0.0)
-        rhs = [extra_rhs(p, seed * 100 + i * nrhs + j, float(rng.uniform(0.02, 0.10))) for j in range(nrhs)]
-        out.append((p, rhs))
-    return out
+        plan.append((W, H, [float(rng.uniform(0.02, 0.10)) for _ in range(nrhs)]))
+    return plan
+
+
+def batched_image(i: int, W: int, H: int, densities, seed: int = 4000):
+    """Image i of a batched workload (seeded as config_batched does): the
+    problem and its constraint sets [(lam, tgt), ...]."""
+    nrhs = len(densities)
+    p = make_problem(W, H, seed + 1 + i, 1, 0.0)
+    return p, [extra_rhs(p, seed * 100 + i * nrhs + j, d) for j, d in enumerate(densities)]
+
+
+def config_batched(n_images: int = 64, nrhs: int = 4, seed: int = 4000, wmin=240, wmax=1080, hmin=320, hmax=1920,
+                   pick=None):
+    """Batched workload: n_images images with W ~ U[wmin,wmax], H ~ U[hmin,hmax]
+    (one ellipse each), and nrhs independent constraint sets per image with
+    densities ~ U[2 %, 10 %].  Returns a list of (problem, [(lam, tgt), ...]);
+    with `pick` (image indices) only those images, in that order, each exactly
+    as in the full workload."""
+    plan = config_batched_plan(n_images, nrhs, seed, wmin, wmax, hmin, hmax)
+    idx = range(n_images) if pick is None else pick
+    return [batched_image(i, *plan[i], seed=seed) for i in idx]

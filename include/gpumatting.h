@@ -270,7 +270,8 @@ typedef struct gm_batch_s gm_batch;
 /* Allocate a batch: nimages images (widths[i] x heights[i]) with nrhs <= 8
  * constraint sets each; precision GM_PREC_FP32, GM_PREC_FP16 or GM_PREC_BF16
  * (16-bit off-diagonal bands, FP32 diagonals and vectors, FP64 residual
- * replacement). */
+ * replacement).  Each image has 1 to 2^28 - 1 pixels (GM_ERR_INVALID
+ * otherwise, before any device call). */
 gm_status gm_batch_create(gm_batch** batch, int nimages, const int* widths, const int* heights, int nrhs,
                           int precision);
 gm_status gm_batch_destroy(gm_batch* batch);
@@ -381,6 +382,24 @@ gm_status gm_sp_merge_levels(int* labels, int* nlabels, const float* im, int nch
  * Synchronous. */
 gm_status gm_slic(int* labels, int* nlabels, const float* im, int nch, size_t pitch, int W, int H, int K,
                   float compactness, int iters, gm_stream_t stream);
+/* The stages of gm_slic, for callers that keep the colour planes or the raw
+ * clustering (no reference counterpart: slicSegmentation, SLIC.h:43, runs
+ * them as one call).
+ * gm_slic_grid: the grid gm_slic uses for K superpixels: step S and nsx x nsy
+ * seeds (host computation).
+ * gm_slic_lab: CIELAB planes (L, a, b; 3*W*H device floats) of the RGB image,
+ * evaluated in FP64 and rounded to multiples of 2^-16.
+ * gm_slic_raw: seeds and `iters` assignment / update rounds from Lab planes:
+ * labels are seed indices before connectivity; seeds (optional; 5 planes of
+ * maxseeds >= nsx*nsy floats: l, a, b, x, y) receives the final seeds.
+ * Synchronous.
+ * gm_slic_connect: connectivity in place (pieces below minsize pixels join
+ * their left / upper neighbour; consecutive numbering).  Synchronous. */
+gm_status gm_slic_grid(int W, int H, int K, int* S, int* nsx, int* nsy);
+gm_status gm_slic_lab(float* lab, const float* im, int nch, size_t pitch, int W, int H, gm_stream_t stream);
+gm_status gm_slic_raw(int* labels, float* seeds, int maxseeds, const float* lab, int W, int H, int K,
+                      float compactness, int iters, gm_stream_t stream);
+gm_status gm_slic_connect(int* labels, int* nlabels, int W, int H, int minsize, gm_stream_t stream);
 
 /* ---- host file I/O of the matting driver (no device work) ---------------
  * Netpbm: binary or ASCII, header tokens separated by whitespace with '#'

@@ -545,14 +545,24 @@ void orc_bm_axpy_f32(float* out, const float* a, size_t apitch, const int* bands
     }
 }
 
+/* Rows in blocks of 256, bands outer within a block: every row still sums its
+ * products in band order from 0.0, so the result is that of the plain row
+ * loop, and the inner loop vectorizes. */
+__attribute__((optimize("tree-vectorize")))
 void orc_bm_ax_f64(double* y, const double* a, size_t apitch, const int* bands, int nbands,
                    int rows, const double* x)
 {
 #pragma omp parallel for schedule(static)
-    for (int i = 0; i < rows; ++i) {
-        double s = 0.0;
-        for (int k = 0; k < nbands; ++k) s += a[(size_t)i + (size_t)k * apitch] * x[(ptrdiff_t)i + bands[k]];
-        y[i] = s;
+    for (int i0 = 0; i0 < rows; i0 += 256) {
+        const int n = rows - i0 < 256 ? rows - i0 : 256;
+        double s[256];
+        for (int t = 0; t < n; ++t) s[t] = 0.0;
+        for (int k = 0; k < nbands; ++k) {
+            const double* ak = a + (size_t)k * apitch + i0;
+            const double* xk = x + (ptrdiff_t)i0 + bands[k];
+            for (int t = 0; t < n; ++t) s[t] += ak[t] * xk[t];
+        }
+        for (int t = 0; t < n; ++t) y[i0 + t] = s[t];
     }
 }
 
@@ -566,10 +576,22 @@ double orc_dot_f32(const float* x, const float* y, int n)
     return s;
 }
 
+/* FP64 sums over fixed chunks (ORC_NCHUNK of them, partial sums added in chunk
+ * order), so the result does not depend on the thread count. */
+#define ORC_NCHUNK 256
+static size_t orc_chunk_lo(size_t n, int c) { return n * (size_t)c / ORC_NCHUNK; }
+
 static double orc_dot_f64(const double* x, const double* y, int n)
 {
+    double part[ORC_NCHUNK];
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < ORC_NCHUNK; ++c) {
+        double s = 0;
+        for (size_t i = orc_chunk_lo((size_t)n, c); i < orc_chunk_lo((size_t)n, c + 1); ++i) s += x[i] * y[i];
+        part[c] = s;
+    }
     double s = 0;
-    for (int i = 0; i < n; ++i) s += x[i] * y[i];
+    for (int c = 0; c < ORC_NCHUNK; ++c) s += part[c];
     return s;
 }
 
@@ -724,21 +746,33 @@ int orc_pcg_f64(double* x, const double* a, size_t apitch, const int* bands, int
         p[i] = -z;
     }
     int it = 0;
+    double prz[ORC_NCHUNK], prr[ORC_NCHUNK];
     while (sqrt(rr) / bn > tol && it < maxit) {
         orc_bm_ax_f64(Ap, a, apitch, bands, nbands, rows, p);
         double pAp = orc_dot_f64(p, Ap, rows);
         if (!(pAp > 0)) break;
         double al = rz / pAp, rzn = 0;
         rr = 0;
-        for (size_t i = 0; i < n; ++i) {
-            x[i] += al * p[i];
-            r[i] += al * Ap[i];
-            rzn += r[i] * dinv[i] * r[i];
-            rr += r[i] * r[i];
+#pragma omp parallel for schedule(static)
+        for (int c = 0; c < ORC_NCHUNK; ++c) {
+            double sz = 0, sr = 0;
+            for (size_t i = orc_chunk_lo(n, c); i < orc_chunk_lo(n, c + 1); ++i) {
+                x[i] += al * p[i];
+                r[i] += al * Ap[i];
+                sz += r[i] * dinv[i] * r[i];
+                sr += r[i] * r[i];
+            }
+            prz[c] = sz;
+            prr[c] = sr;
+        }
+        for (int c = 0; c < ORC_NCHUNK; ++c) {
+            rzn += prz[c];
+            rr += prr[c];
         }
         double be = rzn / rz;
         rz = rzn;
-        for (size_t i = 0; i < n; ++i) p[i] = -dinv[i] * r[i] + be * p[i];
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < rows; ++i) p[i] = -dinv[i] * r[i] + be * p[i];
         ++it;
     }
     if (relres) *relres = sqrt(rr) / bn;
@@ -762,6 +796,40 @@ double orc_true_relres_f32(const float* a, size_t apitch, const int* bands, int 
         den += (double)b[i] * (double)b[i];
     }
     return den > 0 ? sqrt(num / den) : sqrt(num);
+}
+
+/* Squared FP64 residual of one batched system, as the FP32 system a
+ * BandedMatrix holds (Matting.cu:205-216 adds the constraint weight to the
+ * diagonal in float): A = L except a_ii = fl(L_ii + lam_i), b_i = fl(lam_i *
+ * tgt_i).  L has no data term; x is padded by the band reach.  out[0] =
+ * ||A x - b||^2, out[1] = ||b||^2, both summed in FP64 over fixed chunks. */
+void orc_resid_data_f64(double* out, const float* a, size_t apitch, const int* bands, int nbands, int rows,
+                        const float* lam, const float* tgt, const float* x)
+{
+    int cb = orc_center_band(bands, nbands);
+    double pn[ORC_NCHUNK], pd[ORC_NCHUNK];
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < ORC_NCHUNK; ++c) {
+        double sn = 0, sd = 0;
+        for (size_t i = orc_chunk_lo((size_t)rows, c); i < orc_chunk_lo((size_t)rows, c + 1); ++i) {
+            double s = 0;
+            for (int k = 0; k < nbands; ++k) {
+                float v = a[i + (size_t)k * apitch];
+                if (k == cb) v = v + lam[i];
+                s += (double)v * (double)x[(ptrdiff_t)i + bands[k]];
+            }
+            double bi = (double)(lam[i] * tgt[i]);
+            sn += (s - bi) * (s - bi);
+            sd += bi * bi;
+        }
+        pn[c] = sn;
+        pd[c] = sd;
+    }
+    out[0] = out[1] = 0;
+    for (int c = 0; c < ORC_NCHUNK; ++c) {
+        out[0] += pn[c];
+        out[1] += pd[c];
+    }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -817,6 +885,17 @@ double orc_mse_clamped(const float* alpha, const float* gt, int W, int H)
             else ssd += (a - g) * (a - g);
         }
     return ssd / ((double)W * H);
+}
+
+/* OpenMP team size for parallel regions the calling thread starts (a
+ * caller running several oracle calls on its own threads sets 1). */
+void orc_set_num_threads(int n)
+{
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+#else
+    (void)n;
+#endif
 }
 
 int orc_num_threads(void)

@@ -62,11 +62,13 @@ def lib():
         L.orc_pcg_f64.restype = ci
         L.orc_true_relres_f32.argtypes = [vp, sz, i32p, ci, ci, vp, vp]
         L.orc_true_relres_f32.restype = cd
+        L.orc_resid_data_f64.argtypes = [f64p, vp, sz, i32p, ci, ci, vp, vp, vp]
         L.orc_gs_f32.argtypes = [vp, vp, sz, i32p, ci, ci, vp, sz]
         L.orc_jacobi_f32.argtypes = [vp, vp, sz, i32p, ci, ci, vp, sz, ci, cf]
         L.orc_mse_clamped.argtypes = [f32p, f32p, ci, ci]
         L.orc_mse_clamped.restype = cd
         L.orc_num_threads.restype = ci
+        L.orc_set_num_threads.argtypes = [ci]
         L.orc_levin_backward_f64.argtypes = [f64p, f32p, ci, sz, f64p, f64p, cd, ci, ci]
         L.orc_baseline_pcg_f32.argtypes = [vp, vp, sz, i32p, ci, ci, vp, ci, ci]
         L.orc_baseline_pcg_f32.restype = cd
@@ -240,6 +242,19 @@ def true_relres_f32(a, bands, b, x, pad):
                                      len(bands), b.size, _ptr(b), _ptr(xp, pad))
 
 
+def resid_data_f64(a, bands, lam, tgt, x, pad):
+    """(||A x - b||, ||b||) in FP64 for the FP32 system with constraint weights
+    lam and targets tgt on a Laplacian a without data term (a_ii + lam_i and
+    lam_i * tgt_i rounded to float, as a BandedMatrix stores them)."""
+    xp = padded(np.asarray(x, np.float32).ravel(), pad)
+    la = np.ascontiguousarray(lam, np.float32).ravel()
+    tg = np.ascontiguousarray(tgt, np.float32).ravel()
+    out = np.zeros(2, np.float64)
+    lib().orc_resid_data_f64(out, _ptr(a), a.shape[1], np.ascontiguousarray(bands, np.int32), len(bands), la.size,
+                             _ptr(la), _ptr(tg), _ptr(xp, pad))
+    return float(np.sqrt(out[0])), float(np.sqrt(out[1]))
+
+
 def gs_f32(a, bands, b, x0, pad, iterations):
     N = b.size
     xp = padded(np.asarray(x0, np.float32), pad)
@@ -381,3 +396,55 @@ def matte_loss_f64(image, lam, tgt, alpha_gt, eps=1e-7, tol=1e-14):
     al, _, _ = pcg_f64(a64, bands25(W), b64, np.zeros(W * H), pad_for(W), tol, 1000000)
     r = al - np.asarray(alpha_gt, np.float64).ravel()
     return float(r @ r)
+
+
+def matte_loss_fd_f64(image, lam, tgt, alpha_gt, coords, h=1e-4, eps=1e-7, tol=1e-11, workers=8):
+    """FP64 central finite differences of loss = ||alpha - alpha_gt||^2, the
+    loss of matte_loss_grad_f64, at `coords`: ("rgb", y, x, c) for an image
+    channel, ("lam", i) for a constraint weight.  For each coordinate the
+    parameter moves to theta +- h (the image is float32, so the step is the
+    float32 difference actually taken).  The change of the solution, delta =
+    alpha(theta +- h) - alpha, is solved for directly, A(theta +- h) delta =
+    b(theta +- h) - A(theta +- h) alpha, and the loss change is 2 (alpha -
+    alpha_gt) . delta + delta . delta: the same central difference as two full
+    solves and a subtraction of losses, without the cancellation.  Returns the
+    derivatives in the order of coords."""
+    import concurrent.futures as cf
+    image = np.ascontiguousarray(image, np.float32)
+    H, W = image.shape[:2]
+    N = W * H
+    bands = bands25(W)
+    pad = pad_for(W)
+    lam = np.ascontiguousarray(lam, np.float32).ravel()
+    tgt = np.ascontiguousarray(tgt, np.float32).ravel()
+    a64, b64 = levin_f64(image, lam, tgt, eps)
+    al, _, _ = pcg_f64(a64, bands, b64, np.zeros(N), pad, 1e-14, 1000000)
+    r = al - np.asarray(alpha_gt, np.float64).ravel()
+    alp = padded(al, pad)
+
+    def dloss(img, la):
+        a, b = levin_f64(img, la, tgt, eps)
+        rhs = np.zeros(N)
+        lib().orc_bm_ax_f64(_ptr(rhs), _ptr(a), N, np.ascontiguousarray(bands, np.int32), 25, N, _ptr(alp, pad))
+        rhs = b - rhs
+        d, _, _ = pcg_f64(a, bands, rhs, np.zeros(N), pad, tol, 1000000)
+        return 2.0 * float(r @ d) + float(d @ d)
+
+    def one(cd):
+        lib().orc_set_num_threads(1)
+        if cd[0] == "rgb":
+            _, y, x, c = cd
+            up, dn = image.copy(), image.copy()
+            up[y, x, c] = np.float32(float(image[y, x, c]) + h)
+            dn[y, x, c] = np.float32(float(image[y, x, c]) - h)
+            return (dloss(up, lam) - dloss(dn, lam)) / (float(up[y, x, c]) - float(dn[y, x, c]))
+        _, i = cd
+        up, dn = lam.copy(), lam.copy()
+        up[i] = np.float32(float(lam[i]) + h)
+        dn[i] = np.float32(float(lam[i]) - h)
+        return (dloss(image, up) - dloss(image, dn)) / (float(up[i]) - float(dn[i]))
+
+    with cf.ThreadPoolExecutor(max_workers=workers) as ex:
+        out = list(ex.map(one, coords))
+    lib().orc_set_num_threads(num_threads())
+    return np.array(out)

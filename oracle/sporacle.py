@@ -195,8 +195,10 @@ def merge_levels(labels, nlabels, im, nlevels, threshold=0.0111, brute=False):
 # SLIC
 
 def rgb_to_lab(im):
-    """(H, W, >=3) RGB in [0, 1] -> (3, H, W) float32 CIELAB (sRGB, D65)."""
-    c = np.asarray(im, np.float64)[..., :3]
+    """(H, W, >=3) RGB in [0, 1] -> (3, H, W) float32 CIELAB (sRGB, D65),
+    evaluated in FP64 and rounded to the nearest multiple of 2^-16 (half to
+    even), which float32 holds exactly for |value| < 128."""
+    c = np.asarray(im, np.float32)[..., :3].astype(np.float64)
     c = np.where(c <= 0.04045, c / 12.92, ((c + 0.055) / 1.055) ** 2.4)
     X = (0.4124564 * c[..., 0] + 0.3575761 * c[..., 1] + 0.1804375 * c[..., 2]) / 0.95047
     Y = 0.2126729 * c[..., 0] + 0.7151522 * c[..., 1] + 0.0721750 * c[..., 2]
@@ -206,7 +208,8 @@ def rgb_to_lab(im):
         return np.where(t > 0.008856, np.cbrt(t), 7.787 * t + 16.0 / 116.0)
 
     fx, fy, fz = f(X), f(Y), f(Z)
-    return np.stack([116.0 * fy - 16.0, 500.0 * (fx - fy), 200.0 * (fy - fz)]).astype(F)
+    lab = np.stack([116.0 * fy - 16.0, 500.0 * (fx - fy), 200.0 * (fy - fz)])
+    return (np.rint(lab * 65536.0) / 65536.0).astype(F)
 
 
 def slic_grid(W, H, K):
@@ -221,11 +224,17 @@ def _cell(v, n, extent):
     return np.minimum(n - 1, (np.asarray(v, np.int64) * n) // extent)
 
 
-def slic_raw(im, K, m=10.0, iters=10):
-    """SLIC before the connectivity step: (labels (H, W), S)."""
-    im = np.asarray(im, F)
-    H, W = im.shape[:2]
-    lab = rgb_to_lab(im)
+def slic_raw(im, K, m=10.0, iters=10, lab=None, return_seeds=False):
+    """SLIC before the connectivity step: (labels (H, W), S), and the final
+    seeds (5, nseeds) with return_seeds.  lab: start from these (3, H, W)
+    planes instead of converting im.  Every float operation is one float32
+    operation in the order written (numpy float32 scalars and arrays), so the
+    result is exactly that of any evaluation with the same rounding steps."""
+    if lab is None:
+        im = np.asarray(im, F)
+        lab = rgb_to_lab(im)
+    lab = np.asarray(lab, F)
+    H, W = lab.shape[1:]
     S, nsx, nsy = slic_grid(W, H, K)
     ns = nsx * nsy
     seeds = np.zeros((5, ns), F)
@@ -240,11 +249,11 @@ def slic_raw(im, K, m=10.0, iters=10):
                 x, y = sx + dx, sy + dy
                 if x < 1 or x > W - 2 or y < 1 or y > H - 2:
                     continue
-                g = 0.0
+                g = F(0)
                 for q in range(3):
-                    ddx = float(lab[q, y, x + 1]) - float(lab[q, y, x - 1])
-                    ddy = float(lab[q, y + 1, x]) - float(lab[q, y - 1, x])
-                    g += ddx * ddx + ddy * ddy
+                    ddx = F(lab[q, y, x + 1] - lab[q, y, x - 1])
+                    ddy = F(lab[q, y + 1, x] - lab[q, y - 1, x])
+                    g = F(g + F(F(ddx * ddx) + F(ddy * ddy)))
                 if g < best:
                     best, bx, by = g, x, y
         seeds[:, k] = (lab[0, by, bx], lab[1, by, bx], lab[2, by, bx], bx, by)
@@ -283,6 +292,8 @@ def slic_raw(im, K, m=10.0, iters=10):
             t = np.bincount(lk[inwin], weights=feats[q][inwin].astype(np.float64), minlength=ns)
             new[q] = np.where(cnt > 0, (t / np.maximum(cnt, 1)).astype(F), seeds[q])
         seeds = new
+    if return_seeds:
+        return labels.astype(np.int32), S, seeds
     return labels.astype(np.int32), S
 
 

@@ -1,11 +1,13 @@
-"""Generate tests/golden/data05_crop.npz from the reference's sample data.
+"""Generate tests/golden/data05_crop.npz and data05_full.npz from the
+reference's sample data.
 
 Reads /root/reference/data/05.ppm, 05_scribs.pgm and 05_gt.pgm (the input of
 the reference's only end-to-end run, README.md:27), keeps a 96x64 crop that
 holds foreground scribbles, background scribbles and unknown pixels, and
 stores the raw 8-bit crop plus the digest of the oracle's reference-rounding
 Laplacian for it (eps 1e-2, gamma 1: the reference's own call, main.cu:222),
-so that later oracle changes are caught.
+so that later oracle changes are caught.  data05_full.npz holds the three
+full 800x552 images as raw 8-bit arrays (the input of the end-to-end CLI test).
 
     python tests/golden/make_golden.py
 """
@@ -48,6 +50,8 @@ def main():
                         b_sha256=np.frombuffer(hashlib.sha256(b.tobytes()).digest(), np.uint8),
                         L_row_samples=a[:, ::97].copy(), b_sum=np.float64(b.astype(np.float64).sum()))
     print("wrote data05_crop.npz", rgb.shape, np.unique(s, return_counts=True))
+    np.savez_compressed(os.path.join(HERE, "data05_full.npz"), rgb=im, maxval=np.int32(mv), scribs=sc, gt=gt)
+    print("wrote data05_full.npz", im.shape)
 
 
 if __name__ == "__main__":

@@ -70,3 +70,34 @@ def test_functional_gradient_equals_autograd(gpu):
     assert abs(l2.item() - loss.item()) <= 1e-6 * loss.item()
     for got, ref in ((g_im, im.grad), (g_lam, lam.grad), (g_tgt, tgt.grad)):
         assert torch.equal(got, ref)
+
+
+def test_autograd_vs_central_differences_512(gpu):
+    """GPU gradients of loss = ||alpha - alpha_gt||^2 on the first 512x512
+    image of the differentiation workload against FP64 central differences
+    (h = 1e-4) at 256 coordinates: 192 RGB channels and 64 constraint weights
+    (tests/golden/fd_grad_512.npz, written by tests/golden/make_fd.py from the
+    oracle).  Relative 2-norm error per group and per-coordinate agreement."""
+    import os
+    T = torch
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "fd_grad_512.npz"))
+    p = synth.config_grad(1)[0]
+    assert (p.W, p.H) == (int(g["W"]), int(g["H"]))
+    im = T.from_numpy(p.image).cuda().requires_grad_(True)
+    lam = T.from_numpy(p.lam).cuda().requires_grad_(True)
+    tgt = T.from_numpy(p.target).cuda()
+    gt = T.from_numpy(p.alpha_gt).cuda()
+    alpha = autograd.matte(im, lam, tgt, 1e-7, 1e-9)
+    ((alpha - gt) ** 2).sum().backward()
+    gim = im.grad.cpu().numpy().astype(np.float64)
+    glam = lam.grad.cpu().numpy().astype(np.float64).ravel()
+    rgb = g["rgb"]
+    got_rgb = gim[rgb[:, 0], rgb[:, 1], rgb[:, 2]]
+    got_lam = glam[g["lam_idx"]]
+    for name, got, fd in (("rgb", got_rgb, g["fd_rgb"]), ("lam", got_lam, g["fd_lam"])):
+        rel = np.linalg.norm(got - fd) / np.linalg.norm(fd)
+        rms = np.sqrt(np.mean(fd ** 2))
+        worst = np.max(np.abs(got - fd) / np.maximum(np.abs(fd), 0.01 * rms))
+        print(f"\n[fd 512^2] {name}: {fd.size} coordinates, relative 2-norm error {rel:.2e}, worst {worst:.2e}")
+        assert rel <= 1e-4, (name, rel)
+        assert worst <= 2e-3, (name, worst)

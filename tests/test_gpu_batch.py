@@ -19,15 +19,39 @@ def _sparse(a64, W, N):
     return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(N, N))
 
 
-@pytest.mark.parametrize("prec", [PREC_FP32, PREC_FP16, PREC_BF16])
-def test_batched_matches_fp64_per_system(gpu, prec):
+@pytest.fixture(scope="module")
+def small_work():
+    """5 images of at most 130x90, 3 right-hand sides, with the FP64 matrix,
+    FP64 solution and smallest eigenvalue of every system."""
+    import scipy.sparse.linalg as sla
+    work = synth.config_batched(n_images=5, nrhs=3, seed=77, wmin=20, wmax=130, hmin=15, hmax=90)
+    ref = {}
+    for i, (p, rhs) in enumerate(work):
+        for j, (lam, tgt) in enumerate(rhs):
+            a, bb, _ = O.levin_gather_acc_f32(p.image, lam, tgt, 1e-7)
+            a64, b64 = a.astype(np.float64), bb.astype(np.float64)
+            xe, _, _ = O.pcg_f64(a64, O.bands25(p.W), b64, np.zeros(p.N), O.pad_for(p.W), 1e-13)
+            A = _sparse(a64, p.W, p.N)
+            lmin = sla.eigsh(A, k=1, sigma=0, which="LM", return_eigenvectors=False)[0]
+            assert lmin > 0
+            ref[i, j] = (A, b64, xe, lmin)
+    return work, ref
+
+
+# per_cu 16 (the default: at most one tile per workgroup on this workload);
+# 1 and 2 put several tiles and several images in one workgroup's range, so the
+# LDS ring carry and the multi-image slot index run.
+@pytest.mark.parametrize("prec,per_cu", [(PREC_FP32, None), (PREC_FP16, None), (PREC_BF16, None),
+                                         (PREC_FP32, 1), (PREC_FP16, 1), (PREC_FP32, 2), (PREC_BF16, 2)])
+def test_batched_matches_fp64_per_system(gpu, small_work, prec, per_cu):
     """Every system reaches the tolerance, checked twice: by the solver's own
     FP64 true residual and by an independent FP64 residual of the returned
     alpha on the host; the alpha error against the FP64 solution is within
     what that residual allows (||x - x*|| <= ||A x - b|| / lambda_min(A))."""
-    import scipy.sparse.linalg as sla
-    work = synth.config_batched(n_images=5, nrhs=3, seed=77, wmin=20, wmax=130, hmin=15, hmax=90)
+    work, ref = small_work
     b = Batch([(p.W, p.H) for p, _ in work], 3, prec)
+    if per_cu is not None:
+        b.set_grid(per_cu)
     for i, (p, rhs) in enumerate(work):
         b.assemble(i, p.image, 1e-7)
         for j, (lam, tgt) in enumerate(rhs):
@@ -37,19 +61,38 @@ def test_batched_matches_fp64_per_system(gpu, prec):
         for j, (lam, tgt) in enumerate(rhs):
             inf = b.info(i, j)
             assert inf.status == 0 and inf.true_relres <= 1e-7, (i, j, inf.true_relres)
-            a, bb, _ = O.levin_gather_acc_f32(p.image, lam, tgt, 1e-7)
-            a64, b64 = a.astype(np.float64), bb.astype(np.float64)
-            xe, _, _ = O.pcg_f64(a64, O.bands25(p.W), b64, np.zeros(p.N), O.pad_for(p.W), 1e-13)
+            A, b64, xe, lmin = ref[i, j]
             x = b.alpha(i, j).cpu().numpy().astype(np.float64)
-            A = _sparse(a64, p.W, p.N)
             res = np.linalg.norm(A @ x - b64)
             assert res <= 1.01e-7 * np.linalg.norm(b64), (i, j, res)
-            lmin = sla.eigsh(A, k=1, sigma=0, which="LM", return_eigenvectors=False)[0]
-            assert lmin > 0
             err = np.linalg.norm(x - xe)
             assert err <= res / lmin + 1e-6 * np.linalg.norm(xe), (i, j, err, res / lmin)
             assert np.abs(x - xe).max() <= 1e-2, (i, j)
     b.close()
+
+
+def test_batched_partition_covers_every_tile_once(gpu, small_work):
+    """The same batch solved at three grids gives the same iterates up to the
+    order of the FP64 partial sums: equal iteration counts (within 2) and
+    alphas within the tolerance's reach."""
+    work, _ = small_work
+    outs = []
+    for per_cu in (16, 1, 2):
+        b = Batch([(p.W, p.H) for p, _ in work], 3, PREC_FP32)
+        b.set_grid(per_cu)
+        for i, (p, rhs) in enumerate(work):
+            b.assemble(i, p.image, 1e-7)
+            for j, (lam, tgt) in enumerate(rhs):
+                b.set_rhs(i, j, lam, tgt)
+        b.solve(tol=1e-7, max_iter=20000, check_every=5)
+        outs.append({(i, j): (b.info(i, j).iterations, b.alpha(i, j).cpu().numpy())
+                     for i in range(len(work)) for j in range(3)})
+        b.close()
+    for o in outs[1:]:
+        for key, (it, x) in o.items():
+            it0, x0 = outs[0][key]
+            assert abs(it - it0) <= 2, (key, it, it0)
+            assert np.abs(x - x0).max() <= 1e-3, key
 
 
 def test_batched_tolerance_and_iterations_match_single_solver(gpu):

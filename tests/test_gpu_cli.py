@@ -110,3 +110,63 @@ def test_cli_errors(gpu, crop, tmp_path):
     assert p.returncode == 1 and "unknown option" in p.stderr
     p = _run(t, "grad", "10", "im.ppm", "scribs.pgm", "--tol", "1e-6", check=False)
     assert p.returncode == 1
+
+
+@pytest.fixture(scope="module")
+def full05(tmp_path_factory):
+    """The reference's own end-to-end input, data/05 (800x552), as the CLI
+    reads it (README.md:27)."""
+    if not os.path.exists(CLI):
+        pytest.fail(f"{CLI} is not built (make -C gpumatting_amd/cli)")
+    d = np.load(os.path.join(ROOT, "tests", "golden", "data05_full.npz"))
+    rgb, scribs, gt = d["rgb"], d["scribs"], d["gt"]
+    maxval = int(d["maxval"])
+    H, W = scribs.shape
+    assert (W, H) == (800, 552)
+    t = tmp_path_factory.mktemp("cli05")
+    (t / "05.ppm").write_bytes(f"P6\n{W} {H}\n{maxval}\n".encode() + rgb.tobytes())
+    netpbm.write_pgm(t / "05_scribs.pgm", scribs)
+    netpbm.write_pgm(t / "05_gt.pgm", gt)
+    image = rgb.astype(np.float32) / np.float32(maxval)
+    a, b = O.levin_scatter_u8(image, scribs, 1e-2, 1.0)
+    return t, W, H, a, b, gt.astype(np.float32) / np.float32(255)
+
+
+@pytest.mark.parametrize("solver,iters", [("cg", 200), ("grad", 200)])
+def test_cli_reference_example_full_image(gpu, full05, solver, iters, capsys):
+    """`matting <solver> 200 data/05.ppm data/05_scribs.pgm data/05_gt.pgm` on
+    the full 800x552 images against the oracle at equal iteration count: the
+    dumped Laplacian bit for bit, the matte within one 8-bit step, and the
+    printed ground-truth MSE."""
+    t, W, H, a, b, gt = full05
+    p = _run(t, solver, str(iters), "05.ppm", "05_scribs.pgm", "05_gt.pgm")
+    assert f"Image Size: {W} x {H}" in p.stdout
+    A = np.fromfile(t / "A.bin", np.float32).reshape(25, W * H)
+    assert np.array_equal(A.view(np.uint32), a.view(np.uint32))
+    alpha = _oracle_alpha(solver, a, b, W, iters)
+    got, _ = netpbm.read_pgm(t / "alpha.pgm")
+    want = netpbm.quantize_u8(alpha.reshape(H, W))
+    diff = np.abs(got.astype(int) - want.astype(int))
+    m = re.search(r"Ground truth MSE: ([0-9.e+-]+)", p.stdout)
+    assert m, p.stdout
+    mse_ref = O.mse_clamped(alpha.reshape(H, W), gt)
+    with capsys.disabled():
+        print(f"\n[data/05 800x552] {solver} {iters}: MSE gpu {float(m.group(1)):.4e} oracle {mse_ref:.4e}, "
+              f"alpha.pgm differs from the oracle's at {(diff > 0).mean() * 100:.3f} % of pixels (max {diff.max()})")
+    assert diff.max() <= 1 and (diff > 0).mean() <= 0.02, (diff.max(), (diff > 0).mean())
+    assert float(m.group(1)) == pytest.approx(mse_ref, rel=2e-2)
+
+
+def test_cli_readme_run(gpu, full05, capsys):
+    """The README's documented run, `matting grad 10000 data/05.ppm
+    data/05_scribs.pgm data/05_gt.pgm`: it completes, writes a full-size
+    alpha.pgm and prints a finite MSE no worse than 200 iterations give."""
+    t, W, H, a, b, gt = full05
+    p = _run(t, "grad", "10000", "05.ppm", "05_scribs.pgm", "05_gt.pgm", "--no-dump")
+    got, _ = netpbm.read_pgm(t / "alpha.pgm")
+    assert got.shape == (H, W)
+    mse = float(re.search(r"Ground truth MSE: ([0-9.e+-]+)", p.stdout).group(1))
+    mse200 = O.mse_clamped(_oracle_alpha("grad", a, b, W, 200).reshape(H, W), gt)
+    with capsys.disabled():
+        print(f"\n[data/05 800x552] grad 10000: MSE {mse:.4e} (200 iterations: {mse200:.4e})")
+    assert np.isfinite(mse) and mse <= mse200 * 1.001

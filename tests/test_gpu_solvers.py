@@ -57,6 +57,39 @@ def test_gradsolve_matches_reference_recurrence(gpu, small):
     assert np.abs(x.numpy() - xr).max() <= 1e-3
 
 
+@pytest.fixture(scope="module")
+def k1024():
+    p = synth.config_1k()
+    a, b, d = O.levin_gather_acc_f32(p.image, p.lam, p.target, 1e-7)
+    return p, a, b, O.bands25(p.W), O.pad_for(p.W)
+
+
+@pytest.mark.parametrize("iters,restart", [(200, 101), (120, 40)])
+def test_cgsolve_matches_reference_recurrence_1024(gpu, k1024, iters, restart):
+    """cgSolve at equal iteration count on the 1024x1024 workload."""
+    p, a, b, bands, pad = k1024
+    s = _gpu_system(p)
+    x = Padded(p.N, pad, init=np.full(p.N, 0.5, np.float32))
+    call("gm_cg_solve", x.ptr, s.L.ref, dptr(s.b), pad, iters, restart, S0)
+    torch.cuda.synchronize()
+    xr = O.cg_ref_f32(a, bands, b, np.full(p.N, 0.5, np.float32), pad, iters, restart)
+    d = np.abs(x.numpy() - xr)
+    print(f"\n[1024^2] cgSolve {iters} iterations restart {restart}: max |da| {d.max():.2e}")
+    assert d.max() <= 1e-3
+
+
+def test_gradsolve_matches_reference_recurrence_1024(gpu, k1024):
+    p, a, b, bands, pad = k1024
+    s = _gpu_system(p)
+    x = Padded(p.N, pad, init=np.full(p.N, 0.5, np.float32))
+    call("gm_grad_solve", x.ptr, s.L.ref, dptr(s.b), 60, pad, S0)
+    torch.cuda.synchronize()
+    xr = O.gd_ref_f32(a, bands, b, np.full(p.N, 0.5, np.float32), pad, 60)
+    d = np.abs(x.numpy() - xr)
+    print(f"\n[1024^2] gradSolve 60 iterations: max |da| {d.max():.2e}")
+    assert d.max() <= 1e-3
+
+
 @pytest.mark.parametrize("method", [METHOD_CG, METHOD_PCG])
 def test_tolerance_solve_vs_fp64(gpu, small, method):
     p, a, b, bands, pad, x_exact = small

@@ -248,6 +248,49 @@ def test_cg_reference_recurrence_converges_and_a_norm_decreases():
     assert np.abs(x - xd).max() <= 1e-3
 
 
+@pytest.mark.parametrize("k", [1, 3, 7])
+def test_cg_restart_schedule(k):
+    """main.cu:481-493: innerIter starts at 0, so r and p are formed from
+    scratch before iterations 1, k+1, 2k+1, ...  With restart k the first k
+    iterations are the unrestarted ones bit for bit; iteration k+1 is not."""
+    p, a, b = small_system()
+    pad, bands = O.pad_for(p.W), O.bands25(p.W)
+    x0 = np.full(p.N, 0.5, np.float32)
+    for m in range(1, k + 2):
+        xr = O.cg_ref_f32(a, bands, b, x0, pad, m, restart=k)
+        xi = O.cg_ref_f32(a, bands, b, x0, pad, m, restart=10 ** 6)
+        if m <= k:
+            assert np.array_equal(xr, xi), m
+        else:
+            assert not np.array_equal(xr, xi), m
+    # two restart periods: the second period starts from the first's result
+    x2 = O.cg_ref_f32(a, bands, b, x0, pad, 2 * k, restart=k)
+    xk = O.cg_ref_f32(a, bands, b, x0, pad, k, restart=k)
+    assert np.array_equal(x2, O.cg_ref_f32(a, bands, b, xk, pad, k, restart=k))
+
+
+def test_cg_restart_every_iteration_is_steepest_descent():
+    """restart = 1 makes every cgSolve iteration a fresh r = L alpha - b,
+    p = -r and an exact line search: steepest descent, the same method as
+    gradSolve (main.cu:424-449, whose k = (<d,b> - <Ld,alpha>)/<Ld,d> equals
+    -<d,d>/<d,Ld> for symmetric L).  Both follow an FP64 dense steepest-descent
+    trajectory on the same matrix to FP32 rounding."""
+    p, a, b = small_system()
+    pad, bands = O.pad_for(p.W), O.bands25(p.W)
+    D = banded_to_dense(a.astype(np.float64), bands, p.N)
+    b64 = b.astype(np.float64)
+    x = np.full(p.N, 0.5)
+    x0 = x.astype(np.float32)
+    for m in range(1, 16):
+        d = D @ x - b64
+        x = x - (d @ d) / (d @ D @ d) * d
+        scale = np.abs(x).max()
+        xc = O.cg_ref_f32(a, bands, b, x0, pad, m, restart=1)
+        xg = O.gd_ref_f32(a, bands, b, x0, pad, m)
+        assert np.abs(xc - x).max() <= 2e-5 * scale, m
+        assert np.abs(xg - x).max() <= 2e-5 * scale, m
+
+
 def test_gd_reference_exact_line_search():
     """After one steepest-descent step with exact line search the new gradient
     is orthogonal to the old one (main.cu:424-449)."""
@@ -259,6 +302,30 @@ def test_gd_reference_exact_line_search():
     x1 = O.gd_ref_f32(a, bands, b, x0, pad, 1)
     d1 = O.bm_ax_f32(a, bands, O.padded(x1, pad), pad) - b
     assert abs(float(d0.astype(np.float64) @ d1)) <= 1e-3 * float(d0.astype(np.float64) @ d0)
+
+
+def test_resid_data_f64_matches_scipy_and_assembled_system():
+    """The batched systems' host residual: the FP32 system with the weights on
+    the diagonal, as scipy.sparse in FP64, and as true_relres_f32 on the
+    oracle's own assembly with the data term (Matting.cu:205-216)."""
+    import scipy.sparse as sp
+    p = synth.make_problem(41, 29, 12, 1, 0.1)
+    bands = O.bands25(p.W)
+    pad = O.pad_for(p.W)
+    a0, _, _ = O.levin_gather_acc_f32(p.image, None, None, 1e-7)
+    a, b, _ = O.levin_gather_acc_f32(p.image, p.lam, p.target, 1e-7)
+    x = np.random.default_rng(3).random(p.N).astype(np.float32)
+    res, bn = O.resid_data_f64(a0, bands, p.lam, p.target, x, pad)
+    rows, cols, vals = [], [], []
+    for k, o in enumerate(bands):
+        i = np.arange(p.N)
+        m = (i + o >= 0) & (i + o < p.N)
+        rows.append(i[m]), cols.append(i[m] + o), vals.append(a[k][:p.N][m].astype(np.float64))
+    A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(p.N, p.N))
+    r64 = A @ x.astype(np.float64) - b.astype(np.float64)
+    assert res == pytest.approx(np.linalg.norm(r64), rel=1e-12)
+    assert bn == pytest.approx(np.linalg.norm(b.astype(np.float64)), rel=1e-12)
+    assert res / bn == pytest.approx(O.true_relres_f32(a, bands, b, x, pad), rel=1e-12)
 
 
 def test_pcg_f32_reaches_tolerance():
@@ -395,3 +462,20 @@ def test_full_gradient_matches_finite_differences():
             fd = (O.matte_loss_f64(im, *args_up, p.alpha_gt, eps) -
                   O.matte_loss_f64(im, *args_dn, p.alpha_gt, eps)) / (float(up.ravel()[i]) - float(dn.ravel()[i]))
             assert abs(fd - g[i]) <= 1e-3 * max(abs(fd), 1e-6)
+
+
+def test_fd_fixture_matches_fp64_adjoint_and_oracle():
+    """tests/golden/fd_grad_512.npz (FP64 central differences, h = 1e-4, at
+    256 coordinates of the first 512x512 differentiation image) against the
+    FP64 adjoint gradient, and four of its entries recomputed live."""
+    g = np.load(os.path.join(HERE, "golden", "fd_grad_512.npz"))
+    p = synth.config_grad(1)[0]
+    loss, al, gim, glam, gtg = O.matte_loss_grad_f64(p.image, p.lam, p.target, p.alpha_gt, 1e-7)
+    rgb = g["rgb"]
+    for got, fd in ((gim[rgb[:, 0], rgb[:, 1], rgb[:, 2]], g["fd_rgb"]), (glam[g["lam_idx"]], g["fd_lam"])):
+        assert np.linalg.norm(got - fd) <= 1e-4 * np.linalg.norm(fd)
+    pick = [0, 9, 100]
+    coords = [("rgb", int(rgb[i, 0]), int(rgb[i, 1]), int(rgb[i, 2])) for i in pick] + [("lam", int(g["lam_idx"][40]))]
+    live = O.matte_loss_fd_f64(p.image, p.lam, p.target, p.alpha_gt, coords, h=float(g["h"]))
+    want = np.concatenate([g["fd_rgb"][pick], g["fd_lam"][[40]]])
+    np.testing.assert_allclose(live, want, rtol=1e-9, atol=1e-15)
