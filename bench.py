@@ -13,7 +13,8 @@ The roofline object is the FP32 banded SpMV (y = A x, 25 diagonals), timed
 with HIP events on its launch stream; `roofline_solve_kernel` is the same
 figure for the kernel that dominates the timed mixed solve, the fused CG
 iteration; the CPU baseline is the oracle's OpenMP
-restatement of the same assembly + Jacobi-PCG on a bounded sample.
+restatement of the same assembly + Jacobi-PCG, built -O3 -march=native for
+the host, median of 3 full 4K solves.
 `alpha_err_vs_fp64` gives max / mean |alpha - alpha_FP64| of both 1e-6 solves
 against the same system solved to 1e-12 with an FP64 master x;
 `accuracy_matched` times both forms at the tolerance where their max alpha
@@ -26,6 +27,7 @@ import argparse
 import json
 import os
 import platform
+import subprocess
 import sys
 import time
 
@@ -51,24 +53,25 @@ def cpu_model():
     return platform.processor()
 
 
-def cpu_baseline(p, iters_needed, sample_iters):
-    """Oracle OpenMP assembly + Jacobi-PCG on the host: times the full 4K
-    assembly and `sample_iters` PCG iterations, extrapolates to the iteration
-    count the solve needs."""
-    import numpy as np
-
-    from oracle import pyoracle as O
-    t0 = time.perf_counter()
-    a, b, _ = O.levin_gather_acc_f32(p.image, p.lam, p.target, p.eps)
-    t_asm = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    O.baseline_pcg_f32(a, O.bands25(p.W), b, np.zeros(p.N, np.float32), O.pad_for(p.W), sample_iters)
-    t_it = (time.perf_counter() - t0) / sample_iters
-    ms = 1e3 * (t_asm + t_it * iters_needed)
-    return {"value": round(ms, 1), "unit": "ms per 4K solve (extrapolated)", "cores": O.num_threads(),
-            "kind": "port",
-            "sample": f"full 4K assembly ({t_asm * 1e3:.0f} ms) + {sample_iters} Jacobi-PCG iterations "
-                      f"({t_it * 1e3:.1f} ms each) x {iters_needed} iterations; {cpu_model()}"}
+def cpu_baseline(runs=3):
+    """The same assembly + Jacobi-PCG to 1e-6 on the host (oracle/baseline_run.py:
+    oracle.c built -O3 -march=native for this host, OpenMP with
+    OMP_PROC_BIND=close), median of `runs` full 4K solves, in a child process
+    so that the OpenMP settings apply before its runtime starts."""
+    env = dict(os.environ)
+    threads = int(env.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    env.update(OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    out = subprocess.run([sys.executable, "-m", "oracle.baseline_run", "--runs", str(runs)], cwd=ROOT, env=env,
+                         capture_output=True, text=True, timeout=600)
+    if out.returncode != 0:
+        raise RuntimeError("cpu baseline failed:\n" + out.stderr[-2000:])
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    return {"value": round(r["ms"], 1), "unit": "ms per 4K solve (assembly + Jacobi-PCG to 1e-6)",
+            "cores": r["threads"], "kind": "port",
+            "sample": f"median of {runs} full 4K solves {r['runs']} ms: assembly {r['asm_ms']:.0f} ms + "
+                      f"{r['iterations']} Jacobi-PCG iterations {r['pcg_ms']:.0f} ms (relres {r['relres']:.2e}); "
+                      f"oracle.c gcc -O3 -march=native, OpenMP {r['threads']} threads, OMP_PROC_BIND=close; "
+                      f"{r['cpu']}"}
 
 
 def bench_batched(args):
@@ -303,7 +306,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-iters", type=int, default=40)
+    ap.add_argument("--cpu-runs", type=int, default=3, help="full CPU-baseline solves (median reported)")
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default shapes)")
     ap.add_argument("--workload", choices=["4k", "batched", "grad", "superpixel"], default="4k",
@@ -459,8 +462,8 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        log("cpu baseline (oracle, OpenMP)")
-        cpu = cpu_baseline(p, results["fp32"]["iterations"], args.cpu_iters)
+        log("cpu baseline (oracle assembly + Jacobi-PCG, -O3 -march=native, OpenMP): median of 3 full solves")
+        cpu = cpu_baseline(args.cpu_runs)
 
     # the dominant kernel of the timed mixed solve: the fused CG iteration
     # (about 86 % of the solve), timed by the autotuner with HIP events on its

@@ -910,20 +910,21 @@ int orc_num_threads(void)
 /* ------------------------------------------------------------------------ */
 /* CPU baseline (timed by bench.py, not a checker): the same Jacobi-PCG as
  * orc_pcg_f32 with every vector loop OpenMP-parallel (static schedule, one
- * fused pass per step), FP32 vectors, double reductions.  Runs exactly `iters`
- * iterations from x (no tolerance test) and returns the final ||r||/||b||. */
+ * fused pass per step), FP32 vectors, double reductions.  Iterates from x
+ * until the recursive ||r||/||b|| <= tol (tol <= 0: exactly maxit iterations)
+ * or maxit iterations; *iters gets the count.  Returns the final ||r||/||b||. */
 double orc_baseline_pcg_f32(float* x, const float* a, size_t apitch, const int* bands, int nbands, int rows,
-                            const float* b, int pad, int iters)
+                            const float* b, int pad, int maxit, double tol, int* iters)
 {
     size_t n = (size_t)rows, len = n + 2 * (size_t)pad;
     int cb = orc_center_band(bands, nbands);
     float* rb = (float*)calloc(len, sizeof(float));
     float* pb = (float*)calloc(len, sizeof(float));
-    float* Ap = (float*)calloc(n, sizeof(float));
+    float* Ap = (float*)calloc(n > 0 ? n : 1, sizeof(float));
     float *r = rb + pad, *p = pb + pad;
     const float* dg = a + (size_t)cb * apitch;
     double bb = 0, rz = 0, rr = 0;
-#pragma omp parallel for reduction(+ : bb, rz) schedule(static)
+#pragma omp parallel for reduction(+ : bb, rz, rr) schedule(static)
     for (int i = 0; i < rows; ++i) {
         float s = 0.f;
         for (int k = 0; k < nbands; ++k) s += a[(size_t)i + (size_t)k * apitch] * x[(ptrdiff_t)i + bands[k]];
@@ -932,9 +933,12 @@ double orc_baseline_pcg_f32(float* x, const float* a, size_t apitch, const int* 
         float z = dg[i] != 0.f ? s / dg[i] : s;
         p[i] = -z;
         rz += (double)(s * z);
+        rr += (double)(s * s);
         bb += (double)b[i] * b[i];
     }
-    for (int it = 0; it < iters; ++it) {
+    const double tol2 = tol > 0 ? tol * tol * (bb > 0 ? bb : 1.0) : -1.0;
+    int it = 0;
+    while (it < maxit && !(rr <= tol2)) {
         double pAp = 0;
 #pragma omp parallel for reduction(+ : pAp) schedule(static)
         for (int i = 0; i < rows; ++i) {
@@ -943,6 +947,7 @@ double orc_baseline_pcg_f32(float* x, const float* a, size_t apitch, const int* 
             Ap[i] = s;
             pAp += (double)(p[i] * s);
         }
+        if (!(pAp > 0)) break;
         float al = (float)(rz / pAp);
         double rzn = 0;
         rr = 0;
@@ -962,7 +967,9 @@ double orc_baseline_pcg_f32(float* x, const float* a, size_t apitch, const int* 
             float z = dg[i] != 0.f ? r[i] / dg[i] : r[i];
             p[i] = -z + be * p[i];
         }
+        ++it;
     }
+    if (iters) *iters = it;
     free(rb); free(pb); free(Ap);
     return bb > 0 ? sqrt(rr / bb) : sqrt(rr);
 }

@@ -38,42 +38,51 @@ def lib():
         path = os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             build()
-        L = C.CDLL(path)
-        L.orc_bands25.argtypes = [ci, i32p]
-        L.orc_pad.argtypes = [ci]
-        L.orc_pad.restype = ci
-        L.orc_levin_scatter_f32.argtypes = [f32p, sz, f32p, f32p, ci, sz, vp, vp, cf, ci, ci]
-        L.orc_levin_scatter_u8.argtypes = [f32p, sz, f32p, f32p, ci, sz, u8p, cf, cf, ci, ci]
-        L.orc_levin_gather_f32.argtypes = [f32p, sz, f32p, vp, f32p, ci, sz, vp, vp, cf, ci, ci]
-        L.orc_levin_gather_acc_f32.argtypes = [f32p, sz, f32p, vp, f32p, ci, sz, vp, vp, cf, ci, ci]
-        L.orc_levin_f64.argtypes = [f64p, sz, f64p, f32p, ci, sz, vp, vp, cd, ci, ci]
-        L.orc_levin_dense_f64.argtypes = [f64p, f32p, ci, sz, cd, ci, ci]
-        for nm in ("orc_bm_ax_f32",):
-            getattr(L, nm).argtypes = [vp, vp, sz, i32p, ci, ci, vp]
-        L.orc_bm_axpy_f32.argtypes = [vp, vp, sz, i32p, ci, ci, vp, vp, ci]
-        L.orc_bm_ax_f64.argtypes = [vp, vp, sz, i32p, ci, ci, vp]
-        L.orc_dot_f32.argtypes = [f32p, f32p, ci]
-        L.orc_dot_f32.restype = cd
-        L.orc_cg_ref_f32.argtypes = [vp, vp, sz, i32p, ci, ci, vp, ci, ci, ci]
-        L.orc_gd_ref_f32.argtypes = [vp, vp, sz, i32p, ci, ci, vp, ci, ci]
-        L.orc_pcg_f32.argtypes = [vp, vp, sz, i32p, ci, ci, vp, ci, cd, ci, ci, ci, C.POINTER(cd)]
-        L.orc_pcg_f32.restype = ci
-        L.orc_pcg_f64.argtypes = [vp, vp, sz, i32p, ci, ci, vp, ci, cd, ci, C.POINTER(cd)]
-        L.orc_pcg_f64.restype = ci
-        L.orc_true_relres_f32.argtypes = [vp, sz, i32p, ci, ci, vp, vp]
-        L.orc_true_relres_f32.restype = cd
-        L.orc_resid_data_f64.argtypes = [f64p, vp, sz, i32p, ci, ci, vp, vp, vp]
-        L.orc_gs_f32.argtypes = [vp, vp, sz, i32p, ci, ci, vp, sz]
-        L.orc_jacobi_f32.argtypes = [vp, vp, sz, i32p, ci, ci, vp, sz, ci, cf]
-        L.orc_mse_clamped.argtypes = [f32p, f32p, ci, ci]
-        L.orc_mse_clamped.restype = cd
-        L.orc_num_threads.restype = ci
-        L.orc_set_num_threads.argtypes = [ci]
-        L.orc_levin_backward_f64.argtypes = [f64p, f32p, ci, sz, f64p, f64p, cd, ci, ci]
-        L.orc_baseline_pcg_f32.argtypes = [vp, vp, sz, i32p, ci, ci, vp, ci, ci]
-        L.orc_baseline_pcg_f32.restype = cd
-        _LIB = L
+        _LIB = _bind(C.CDLL(path))
     return _LIB
+
+
+def baseline_lib(path):
+    """The CPU-baseline build of oracle.c (oracle/Makefile target baseline:
+    -O3 -march=native, FP contraction allowed) at `path`."""
+    return _bind(C.CDLL(path))
+
+
+def _bind(L):
+    L.orc_bands25.argtypes = [ci, i32p]
+    L.orc_pad.argtypes = [ci]
+    L.orc_pad.restype = ci
+    L.orc_levin_scatter_f32.argtypes = [f32p, sz, f32p, f32p, ci, sz, vp, vp, cf, ci, ci]
+    L.orc_levin_scatter_u8.argtypes = [f32p, sz, f32p, f32p, ci, sz, u8p, cf, cf, ci, ci]
+    L.orc_levin_gather_f32.argtypes = [f32p, sz, f32p, vp, f32p, ci, sz, vp, vp, cf, ci, ci]
+    L.orc_levin_gather_acc_f32.argtypes = [f32p, sz, f32p, vp, f32p, ci, sz, vp, vp, cf, ci, ci]
+    L.orc_levin_f64.argtypes = [f64p, sz, f64p, f32p, ci, sz, vp, vp, cd, ci, ci]
+    L.orc_levin_dense_f64.argtypes = [f64p, f32p, ci, sz, cd, ci, ci]
+    for nm in ("orc_bm_ax_f32",):
+        getattr(L, nm).argtypes = [vp, vp, sz, i32p, ci, ci, vp]
+    L.orc_bm_axpy_f32.argtypes = [vp, vp, sz, i32p, ci, ci, vp, vp, ci]
+    L.orc_bm_ax_f64.argtypes = [vp, vp, sz, i32p, ci, ci, vp]
+    L.orc_dot_f32.argtypes = [f32p, f32p, ci]
+    L.orc_dot_f32.restype = cd
+    L.orc_cg_ref_f32.argtypes = [vp, vp, sz, i32p, ci, ci, vp, ci, ci, ci]
+    L.orc_gd_ref_f32.argtypes = [vp, vp, sz, i32p, ci, ci, vp, ci, ci]
+    L.orc_pcg_f32.argtypes = [vp, vp, sz, i32p, ci, ci, vp, ci, cd, ci, ci, ci, C.POINTER(cd)]
+    L.orc_pcg_f32.restype = ci
+    L.orc_pcg_f64.argtypes = [vp, vp, sz, i32p, ci, ci, vp, ci, cd, ci, C.POINTER(cd)]
+    L.orc_pcg_f64.restype = ci
+    L.orc_true_relres_f32.argtypes = [vp, sz, i32p, ci, ci, vp, vp]
+    L.orc_true_relres_f32.restype = cd
+    L.orc_resid_data_f64.argtypes = [f64p, vp, sz, i32p, ci, ci, vp, vp, vp]
+    L.orc_gs_f32.argtypes = [vp, vp, sz, i32p, ci, ci, vp, sz]
+    L.orc_jacobi_f32.argtypes = [vp, vp, sz, i32p, ci, ci, vp, sz, ci, cf]
+    L.orc_mse_clamped.argtypes = [f32p, f32p, ci, ci]
+    L.orc_mse_clamped.restype = cd
+    L.orc_num_threads.restype = ci
+    L.orc_set_num_threads.argtypes = [ci]
+    L.orc_levin_backward_f64.argtypes = [f64p, f32p, ci, sz, f64p, f64p, cd, ci, ci]
+    L.orc_baseline_pcg_f32.argtypes = [vp, vp, sz, i32p, ci, ci, vp, ci, ci, cd, C.POINTER(ci)]
+    L.orc_baseline_pcg_f32.restype = cd
+    return L
 
 
 def _ptr(a: np.ndarray, off: int = 0) -> int:
@@ -277,13 +286,16 @@ def mse_clamped(alpha, gt):
                                  np.ascontiguousarray(gt, np.float32).ravel(), W, H)
 
 
-def baseline_pcg_f32(a, bands, b, x0, pad, iters):
-    """OpenMP Jacobi-PCG, exactly `iters` iterations (the CPU baseline)."""
+def baseline_pcg_f32(a, bands, b, x0, pad, maxit, tol=0.0, L=None):
+    """OpenMP Jacobi-PCG (the CPU baseline) until ||r||/||b|| <= tol or maxit
+    iterations (tol 0: exactly maxit): (x, iterations, relres).  L: the
+    library to run it in (default the oracle's)."""
     N = b.size
     xp = padded(np.asarray(x0, np.float32), pad)
-    rr = lib().orc_baseline_pcg_f32(_ptr(xp, pad), _ptr(a), a.shape[1], np.ascontiguousarray(bands, np.int32),
-                                    len(bands), N, _ptr(b), pad, iters)
-    return xp[pad:pad + N].copy(), rr
+    it = C.c_int(0)
+    rr = (L or lib()).orc_baseline_pcg_f32(_ptr(xp, pad), _ptr(a), a.shape[1], np.ascontiguousarray(bands, np.int32),
+                                           len(bands), N, _ptr(b), pad, maxit, tol, C.byref(it))
+    return xp[pad:pad + N].copy(), it.value, rr
 
 
 def levin_backward_f64(image, mu, alpha, eps=1e-7):

@@ -96,8 +96,11 @@ def test_autograd_vs_central_differences_512(gpu):
     got_lam = glam[g["lam_idx"]]
     for name, got, fd in (("rgb", got_rgb, g["fd_rgb"]), ("lam", got_lam, g["fd_lam"])):
         rel = np.linalg.norm(got - fd) / np.linalg.norm(fd)
-        rms = np.sqrt(np.mean(fd ** 2))
-        worst = np.max(np.abs(got - fd) / np.maximum(np.abs(fd), 0.01 * rms))
-        print(f"\n[fd 512^2] {name}: {fd.size} coordinates, relative 2-norm error {rel:.2e}, worst {worst:.2e}")
+        # per coordinate: 2e-3 relative, above an absolute floor of 3e-4 of the
+        # largest derivative (FP32 noise of the small ones: the lam gradient of
+        # a constrained pixel is mu (t - alpha) with alpha ~ t)
+        excess = np.abs(got - fd) / (2e-3 * np.abs(fd) + 3e-4 * np.abs(fd).max())
+        print(f"\n[fd 512^2] {name}: {fd.size} coordinates, relative 2-norm error {rel:.2e}, "
+              f"worst |got - fd| / bound {excess.max():.2f}")
         assert rel <= 1e-4, (name, rel)
-        assert worst <= 2e-3, (name, worst)
+        assert excess.max() <= 1.0, (name, int(np.argmax(excess)), excess.max())
