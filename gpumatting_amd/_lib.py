@@ -91,6 +91,7 @@ _SIGS = {
     "gm_vec_scale_const": ([vp, vp, cf, ci, vp], ci),
     "gm_vec_scale": ([vp, vp, vp, ci, vp], ci),
     "gm_inner_prod": ([vp, vp, vp, ci, vp], ci),
+    "gm_inner_prod_noalloc": ([vp, vp, vp, ci, vp], ci),
     "gm_grad_solve": ([vp, BMp, vp, ci, ci, vp], ci),
     "gm_cg_solve": ([vp, BMp, vp, ci, ci, ci, vp], ci),
     "gm_jacobi": ([vp, BMp, vp, cf, ci, ci, vp], ci),

@@ -1,6 +1,7 @@
 // gm_blas1.hip -- BLAS-1 style vector kernels (Vector.cu:124-265 roles) and
 // the per-stream reduction scratch shared by every reducing kernel.
 #include <map>
+#include <unordered_map>
 #include <utility>
 
 #include "gm_internal.h"
@@ -69,6 +70,46 @@ __global__ __launch_bounds__(kBlock) void k_dot(const float* __restrict__ x, con
     grid_reduce<1>(v, partials, counter, [&](double* t) { *result = (float)t[0]; });
 }
 
+// Scratch of gm_inner_prod_noalloc: one FP64 partial sum per workgroup (the
+// grid is at most kMaxGrid workgroups) followed by the last-block arrival
+// counter, 33 KB.  Every GPU the library runs on holds its own copy until the
+// process ends; host code reaches it only through the address
+// hipGetSymbolAddress returns for the current device.
+constexpr size_t kNoAllocPartials = kMaxGrid;
+constexpr size_t kNoAllocBytes = sizeof(double) * kNoAllocPartials + 256;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+__global__ void k_clear_u32(unsigned int* p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
+}
+
+static std::mutex g_na_mu;
+static std::unordered_map<int, unsigned char*> g_na_ptr;  // device -> its copy of scratch
+
+// The current device's scratch: its address looked up once per device, and
+// the arrival counter cleared then (by a kernel on stream s, ordered before
+// the caller's reduction); every completed reduction leaves it at zero.
+static gm_status noalloc_scratch(gm_stream_t s, RedScratch* out) {
+    int dev = 0;
+    GM_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> g(g_na_mu);
+    auto it = g_na_ptr.find(dev);
+    if (it == g_na_ptr.end()) {
+        void* ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+        if (!ptr) return fail(GM_ERR_HIP, "hipGetSymbolAddress(scratch) returned no address");
+        unsigned char* base = static_cast<unsigned char*>(ptr);
+        hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(64), 0, (hipStream_t)s,
+                           reinterpret_cast<unsigned int*>(base + sizeof(double) * kNoAllocPartials), 1);
+        GM_LAUNCH_CHECK();
+        it = g_na_ptr.emplace(dev, base).first;
+    }
+    out->partials = reinterpret_cast<double*>(it->second);
+    out->counter = reinterpret_cast<unsigned int*>(it->second + sizeof(double) * kNoAllocPartials);
+    return GM_OK;
+}
+
 }  // namespace gm
 
 using namespace gm;
@@ -125,6 +166,11 @@ gm_status gm_vec_scale(float* r, const float* x, const float* sp, int len, gm_st
     return GM_OK;
 }
 
+static int dot_grid(const DevInfo& d, int len) {
+    int g = elementwise_grid(d, len > 0 ? len : 1, 8);
+    return g > kMaxGrid ? kMaxGrid : g;
+}
+
 gm_status gm_inner_prod(float* result, const float* x, const float* y, int len, gm_stream_t s) {
     gm_status st = vec_check(result, x, y, len);
     if (st != GM_OK) return st;
@@ -133,10 +179,28 @@ gm_status gm_inner_prod(float* result, const float* x, const float* y, int len, 
     if ((st = dev_info(&d)) != GM_OK) return st;
     RedScratch rs;
     if ((st = red_scratch(s, &rs)) != GM_OK) return st;
-    int g = elementwise_grid(d, len > 0 ? len : 1, 8);
-    if (g > kMaxGrid) g = kMaxGrid;
-    hipLaunchKernelGGL(k_dot, dim3(g), dim3(kBlock), 0, (hipStream_t)s, x, y, len, result, rs.partials,
-                       rs.counter);
+    hipLaunchKernelGGL(k_dot, dim3(dot_grid(d, len)), dim3(kBlock), 0, (hipStream_t)s, x, y, len, result,
+                       rs.partials, rs.counter);
+    GM_LAUNCH_CHECK();
+    return GM_OK;
+}
+
+// gm_inner_prod with its partial sums and counter in the library's static
+// device scratch: no allocation, same kernel and grid (so the same result
+// bit for bit).  Calls that overlap on one GPU share that scratch and race.
+gm_status gm_inner_prod_noalloc(float* result, const float* x, const float* y, int len, gm_stream_t s) {
+    gm_status st = vec_check(result, x, y, len);
+    if (st != GM_OK) return st;
+    if (!result) return fail(GM_ERR_INVALID, "null result");
+    DevInfo d;
+    if ((st = dev_info(&d)) != GM_OK) return st;
+    const int g = dot_grid(d, len);
+    if ((size_t)g > kNoAllocPartials)
+        return fail(GM_ERR_INVALID, "gm_inner_prod_noalloc: %d workgroups exceed the scratch's %zu partial sums", g,
+                    kNoAllocPartials);
+    RedScratch rs;
+    if ((st = noalloc_scratch(s, &rs)) != GM_OK) return st;
+    hipLaunchKernelGGL(k_dot, dim3(g), dim3(kBlock), 0, (hipStream_t)s, x, y, len, result, rs.partials, rs.counter);
     GM_LAUNCH_CHECK();
     return GM_OK;
 }

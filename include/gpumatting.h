@@ -91,6 +91,16 @@ gm_status gm_vec_scale(float* r, const float* x, const float* s, int len, gm_str
 /* innerProd_k, Vector.cu:195-265: *result (device) = <x,y>; deterministic
  * two-stage sum (float products, double accumulation), no atomics on data. */
 gm_status gm_inner_prod(float* result, const float* x, const float* y, int len, gm_stream_t stream);
+/* gm_inner_prod without allocating: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc): with it, bmAx, the
+ * vector ops and this call run the reference's CG loop (main.cu:481-518) with
+ * no allocation at all.  Its partial sums and counter live in a static 33 KB
+ * device array that every GPU holds until the process ends; any len >= 0
+ * fits (the grid is capped at 4096 workgroups).  Same kernel and grid as
+ * gm_inner_prod, so the same result bit for bit.  Calls that overlap on one
+ * GPU (several streams or threads at once) share that array and race. */
+gm_status gm_inner_prod_noalloc(float* result, const float* x, const float* y, int len, gm_stream_t stream);
 
 /* ---------------------------------------------------------------------- */
 /* Solvers                                                                 */
