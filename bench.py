@@ -91,13 +91,25 @@ def bench_batched(args):
         imgs = [torch.from_numpy(p.image).cuda() for p, _ in work]
         rhs = [[(torch.from_numpy(l).cuda(), torch.from_numpy(t).cuda()) for (l, t) in r] for _, r in work]
 
-        def step():
+        def setup():
             for i in range(len(work)):
                 b.assemble(i, imgs[i], 1e-7)
                 for j in range(4):
                     b.set_rhs(i, j, *rhs[i][j])
+
+        def step():
+            setup()
             b.solve(tol=1e-6, max_iter=20000)
 
+        tuned = None
+        if not args.no_tune:
+            # the fused iteration's tile width and grid, timed on this batch by
+            # whole solves (before the timed steps)
+            setup()
+            us = b.autotune(reps=1, solve=dict(tol=1e-6, max_iter=20000))
+            tw, pc, _ = b.config()
+            tuned = {"tile_width": tw, "wg_per_cu": pc, "tuner_solve_ms": round(us / 1e3, 1)}
+            log(f"batched {name}: tuned {tw}-wide tiles, {pc} workgroups per CU ({us / 1e3:.1f} ms per solve)")
         for _ in range(args.warmup):
             step()
         torch.cuda.synchronize()
@@ -109,7 +121,7 @@ def bench_batched(args):
         its = [b.info(i, j).iterations for i in range(len(work)) for j in range(4)]
         rr = max(b.info(i, j).true_relres for i in range(len(work)) for j in range(4))
         res[name] = {"ms": ms, "max_iterations": max(its), "mean_iterations": sum(its) / len(its),
-                     "max_true_relres": rr}
+                     "max_true_relres": rr, "autotuned": tuned}
         log(f"batched {name}: {ms:.1f} ms, iterations max {max(its)} mean {sum(its) / len(its):.0f}, "
             f"max true relres {rr:.2e}")
         b.close()

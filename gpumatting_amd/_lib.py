@@ -120,6 +120,9 @@ _SIGS = {
     "gm_batch_alpha": ([vp, ci, ci, C.POINTER(vp)], ci),
     "gm_batch_info": ([vp, ci, ci, C.POINTER(SolveInfo)], ci),
     "gm_batch_set_grid": ([vp, ci], ci),
+    "gm_batch_set_tiles": ([vp, ci], ci),
+    "gm_batch_autotune": ([vp, C.POINTER(SolveOpts), ci, C.POINTER(C.c_double), vp], ci),
+    "gm_batch_get_config": ([vp, C.POINTER(ci), C.POINTER(ci), C.POINTER(C.c_double)], ci),
     "gm_batch_tiles": ([vp], ci),
     "gm_vec_copy_to_host": ([vp, vp, ci], ci),
     "gm_free": ([vp], None),

@@ -44,6 +44,29 @@ class Batch:
     def set_grid(self, per_cu: int):
         call("gm_batch_set_grid", self.h, per_cu)
 
+    def set_tiles(self, tile_width: int):
+        call("gm_batch_set_tiles", self.h, tile_width)
+
+    def autotune(self, reps: int = 3, solve: dict | None = None) -> float:
+        """Pick the fused iteration's tile width and grid by timing them on this
+        batch: full-load launches (solve None; returns microseconds per launch)
+        or whole solves with the given solve() options (returns microseconds
+        per solve)."""
+        us = C.c_double(0)
+        o = None
+        if solve is not None:
+            kw = dict(tol=1e-6, max_iter=20000, check_every=10, refine_every=0, verify=True)
+            kw.update(solve)
+            o = C.byref(solve_opts(method=_lib.METHOD_PCG, **kw))
+        call("gm_batch_autotune", self.h, o, reps, C.byref(us), _stream())
+        return us.value
+
+    def config(self):
+        """(tile width, workgroups per CU, tuned microseconds per launch)."""
+        tw, pc, us = C.c_int(0), C.c_int(0), C.c_double(0)
+        call("gm_batch_get_config", self.h, C.byref(tw), C.byref(pc), C.byref(us))
+        return tw.value, pc.value, us.value
+
     def solve(self, tol=1e-6, max_iter=20000, check_every=10, refine_every=0, verify=True, raise_on_error=True):
         o = solve_opts(method=_lib.METHOD_PCG, tol=tol, max_iter=max_iter, check_every=check_every,
                        refine_every=refine_every, verify=verify)

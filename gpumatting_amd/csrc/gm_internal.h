@@ -150,7 +150,7 @@ struct Scalars {
     unsigned int counter;  // arrival counter of the last-block reduction
     int pend;        // 1: partial sums of <r,z>, ||r||^2 of the last update await folding
     int npart;       // number of those partial sums
-    unsigned int pad2_;
+    int fresh;       // batched solver: the residual was formed from scratch; the next fused launch starts from it
     Hand hand;
     FHand fh[2];
 };

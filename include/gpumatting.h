@@ -293,8 +293,10 @@ gm_status gm_batch_assemble(gm_batch* batch, int image, const float* im, int nch
  * none), initial guess x0 (device or NULL = 0). */
 gm_status gm_batch_set_rhs(gm_batch* batch, int image, int rhs, const float* lam, const float* tgt, const float* x0,
                            gm_stream_t stream);
-/* Jacobi-PCG on every system at once (one launch per phase for all images,
- * hipGraph-captured iteration chunks); per-system convergence. */
+/* Jacobi-PCG on every system at once: one launch per CG iteration for all
+ * images and right-hand sides (the update of iteration j and the step of
+ * j + 1 fused), hipGraph-captured chunks of check_every (rounded up to even)
+ * iterations; per-system convergence. */
 gm_status gm_batch_solve(gm_batch* batch, const gm_solve_opts* opts, gm_stream_t stream);
 /* Device pointer of system (image, rhs)'s solution (valid until destroy). */
 gm_status gm_batch_alpha(gm_batch* batch, int image, int rhs, float** alpha);
@@ -305,6 +307,20 @@ gm_status gm_batch_info(gm_batch* batch, int image, int rhs, gm_solve_info* info
  * the hardware balances the rest; 4 per CU, one resident wave, is 10 % slower
  * on uneven batches). */
 gm_status gm_batch_set_grid(gm_batch* batch, int per_cu);
+/* Tile width of the fused iteration: 60 (60 x 4 tiles), 124 (124 x 2, the
+ * default) or 252 (252 x 1; batches of up to 4 right-hand sides). */
+gm_status gm_batch_set_tiles(gm_batch* batch, int tile_width);
+/* Time the fused iteration at every tile width and 4, 8, 16 and 32
+ * workgroups per CU on this batch's own matrices and keep the fastest.  With
+ * opts NULL each candidate runs reps full-load launches (*best_us: time per
+ * launch); with solve options, reps whole solves with them (*best_us: time
+ * per solve), which also weighs a solve's tail, where only the slowest images
+ * still run.  Assemble and set the right-hand sides first; the initial
+ * guesses are kept.  Synchronous. */
+gm_status gm_batch_autotune(gm_batch* batch, const gm_solve_opts* opts, int reps, double* best_us, gm_stream_t stream);
+/* The tile width and workgroups per CU in use, and the tuned time per launch
+ * (0 before gm_batch_autotune). */
+gm_status gm_batch_get_config(gm_batch* batch, int* tile_width, int* per_cu, double* tuned_us);
 int gm_batch_tiles(gm_batch* batch);
 
 /* Release cached solver workspaces (device memory held between solves). */

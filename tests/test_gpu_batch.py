@@ -40,10 +40,13 @@ def small_work():
 
 # per_cu 16 (the default: at most one tile per workgroup on this workload);
 # 1 and 2 put several tiles and several images in one workgroup's range, so the
-# LDS ring carry and the multi-image slot index run.
-@pytest.mark.parametrize("prec,per_cu", [(PREC_FP32, None), (PREC_FP16, None), (PREC_BF16, None),
-                                         (PREC_FP32, 1), (PREC_FP16, 1), (PREC_FP32, 2), (PREC_BF16, 2)])
-def test_batched_matches_fp64_per_system(gpu, small_work, prec, per_cu):
+# LDS ring carry and the multi-image slot index run; tile widths 64 and 256
+# beside the default 128.
+@pytest.mark.parametrize("prec,per_cu,tile_w", [(PREC_FP32, None, None), (PREC_FP16, None, None),
+                                                (PREC_BF16, None, None), (PREC_FP32, 1, None), (PREC_FP16, 1, 60),
+                                                (PREC_FP32, 2, 252), (PREC_BF16, 2, None), (PREC_FP16, None, 252),
+                                                (PREC_FP32, None, 60)])
+def test_batched_matches_fp64_per_system(gpu, small_work, prec, per_cu, tile_w):
     """Every system reaches the tolerance, checked twice: by the solver's own
     FP64 true residual and by an independent FP64 residual of the returned
     alpha on the host; the alpha error against the FP64 solution is within
@@ -52,6 +55,8 @@ def test_batched_matches_fp64_per_system(gpu, small_work, prec, per_cu):
     b = Batch([(p.W, p.H) for p, _ in work], 3, prec)
     if per_cu is not None:
         b.set_grid(per_cu)
+    if tile_w is not None:
+        b.set_tiles(tile_w)
     for i, (p, rhs) in enumerate(work):
         b.assemble(i, p.image, 1e-7)
         for j, (lam, tgt) in enumerate(rhs):
@@ -72,14 +77,15 @@ def test_batched_matches_fp64_per_system(gpu, small_work, prec, per_cu):
 
 
 def test_batched_partition_covers_every_tile_once(gpu, small_work):
-    """The same batch solved at three grids gives the same iterates up to the
-    order of the FP64 partial sums: equal iteration counts (within 2) and
-    alphas within the tolerance's reach."""
+    """The same batch solved at several grids and tile widths gives the same
+    iterates up to the order of the FP64 partial sums: equal iteration counts
+    (within 2) and alphas within the tolerance's reach."""
     work, _ = small_work
     outs = []
-    for per_cu in (16, 1, 2):
+    for per_cu, tw in ((16, 124), (1, 124), (2, 124), (16, 60), (1, 252), (4, 252)):
         b = Batch([(p.W, p.H) for p, _ in work], 3, PREC_FP32)
         b.set_grid(per_cu)
+        b.set_tiles(tw)
         for i, (p, rhs) in enumerate(work):
             b.assemble(i, p.image, 1e-7)
             for j, (lam, tgt) in enumerate(rhs):
@@ -111,4 +117,34 @@ def test_batched_tolerance_and_iterations_match_single_solver(gpu):
             inf = b.info(i, j)
             assert inf.true_relres <= 1e-6
             assert abs(inf.iterations - info1.iterations) <= max(6, 0.05 * info1.iterations)
+    b.close()
+
+
+def test_batched_autotune_then_solve(gpu):
+    """The tuner times every tile width and grid on the batch itself, keeps a
+    candidate, leaves the initial guesses as they were, and the solve that
+    follows converges every system."""
+    import torch
+    work = synth.config_batched(n_images=4, nrhs=2, seed=11, wmin=40, wmax=300, hmin=30, hmax=200)
+    b = Batch([(p.W, p.H) for p, _ in work], 2, PREC_FP16)
+    rng = np.random.default_rng(3)
+    x0s = {}
+    for i, (p, rhs) in enumerate(work):
+        b.assemble(i, p.image, 1e-7)
+        for j, (lam, tgt) in enumerate(rhs):
+            x0 = rng.random(p.N).astype(np.float32)
+            x0s[i, j] = x0
+            b.set_rhs(i, j, lam, tgt, x0)
+    for solve in (None, dict(tol=1e-6, check_every=6)):  # launch timing, then whole-solve timing
+        us = b.autotune(reps=2, solve=solve)
+        tw, pc, tuned = b.config()
+        assert tw in (60, 124, 252) and pc in (4, 8, 16, 32) and us > 0 and tuned == us
+        torch.cuda.synchronize()
+        for (i, j), x0 in x0s.items():
+            assert np.array_equal(b.alpha(i, j).cpu().numpy(), x0)
+    b.solve(tol=1e-6, check_every=6)
+    for i, (p, rhs) in enumerate(work):
+        for j in range(2):
+            inf = b.info(i, j)
+            assert inf.status == 0 and inf.true_relres <= 1e-6, (i, j, inf.true_relres)
     b.close()
