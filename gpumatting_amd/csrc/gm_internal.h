@@ -15,6 +15,7 @@
 #include <string>
 
 #include "gpumatting.h"
+#include "gm_half.h"
 
 namespace gm {
 
@@ -411,17 +412,7 @@ enum Fmt { kF32 = 0, kF16 = 1, kBF16 = 2 };
 __device__ __forceinline__ float bf16_to_f32(unsigned short h) {
     return __uint_as_float(((unsigned int)h) << 16);
 }
-__device__ __forceinline__ unsigned short f32_to_bf16_rne(float f) {
-    unsigned int u = __float_as_uint(f);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40u);  // quiet NaN
-    unsigned int lsb = (u >> 16) & 1u;
-    u += 0x7fffu + lsb;
-    return (unsigned short)(u >> 16);
-}
-__device__ __forceinline__ unsigned short f32_to_f16_rne(float f) {
-    _Float16 h = (_Float16)f;  // round to nearest even
-    return *reinterpret_cast<unsigned short*>(&h);
-}
+// f32_to_f16_rne / f32_to_bf16_rne: gm_half.h
 
 // R consecutive matrix entries of one band, kept packed in registers until use
 // (16-bit formats hold two entries per dword).

@@ -9,6 +9,15 @@
 // float additions in the same order as the reference's scatter loop, which is
 // what makes GM_ASSEMBLY_REFERENCE bit-identical to it; no atomics, no races.
 //
+// GM_ASSEMBLY_ACCURATE keeps the same gather but evaluates each window's
+// contribution from FP64 statistics: per window S' = -(C + eps/9 I)^-1 / 9
+// and the mean mu = mh + ml (float head and tail); per pixel i and window w
+// the whitened vector v = S' (x_i - mu) (double, rounded to float once) and
+// c = -1/9 - v.ml, so that each other member j adds
+// c + v.(x_j - mh) = -1/9 + v.(x_j - mu) -- one subtraction and one FMA per
+// channel -- to L[i, j].  The operations (and their FMAs) are fixed, so the
+// result is deterministic and restated exactly by the CPU oracle.
+//
 // Workgroup tile: 64 x 4 pixels.  LDS holds the 68 x 8 pixel halo and the
 // statistics (mean, FP64 inverse covariance) of the 66 x 6 windows touching
 // the tile; each window's statistics are computed once per workgroup.
@@ -21,6 +30,8 @@ constexpr int kTX = 64, kTY = 4;              // pixels per workgroup
 constexpr int kWX = kTX + 2, kWY = kTY + 2;   // windows touching the tile
 constexpr int kPX = kTX + 4, kPY = kTY + 4;   // pixels those windows read
 constexpr int kNW = kWX * kWY;
+constexpr double kInv9 = 1.0 / 9.0;
+constexpr float kInv9f = 1.0f / 9.0f;
 
 // MODE: GM_ASSEMBLY_REFERENCE or GM_ASSEMBLY_ACCURATE
 template <int MODE>
@@ -34,9 +45,9 @@ __global__ __launch_bounds__(kBlock) void k_levin(const float* __restrict__ im, 
     // per-window statistics, structure of arrays:
     //   REFERENCE: ws[0..5] = adjugate d11 d12 d13 d22 d23 d33, ws[6] = det,
     //              ws[7..9] = float mean (held exactly as double)
-    //   ACCURATE : ws[0..5] = inverse S11 S12 S13 S22 S23 S33, ws[7..9] = mean
+    //   ACCURATE : ws[0..5] = -inverse/9: S'11 S'12 S'13 S'22 S'23 S'33, ws[7..9] = mean
     __shared__ double ws[10][kNW];
-    __shared__ float wm[6][kNW];  // ACCURATE: float hi/lo split of the mean
+    __shared__ float wm[6][kNW];  // ACCURATE: mean = head wm[0..2] + tail wm[3..5]
     __shared__ unsigned char wvalid[kNW];
 
     const int x0 = blockIdx.x * kTX, y0 = blockIdx.y * kTY;
@@ -97,7 +108,9 @@ __global__ __launch_bounds__(kBlock) void k_levin(const float* __restrict__ im, 
             ws[5][e] = c22 * c11 - c12 * c12;
             ws[7][e] = (double)mx; ws[8][e] = (double)my; ws[9][e] = (double)mz;
         } else {
-            // exact double mean and covariance sums, FP64 adjugate inverse
+            // exact double sums (float products are exact in double), the
+            // 1/9 scalings as multiplications, FMA-contracted adjugate and
+            // determinant, one division
             double m0 = 0, m1 = 0, m2 = 0, s00 = 0, s01 = 0, s02 = 0, s11 = 0, s12 = 0, s22 = 0;
 #pragma unroll
             for (int v1 = 0; v1 < 3; ++v1)
@@ -105,26 +118,28 @@ __global__ __launch_bounds__(kBlock) void k_levin(const float* __restrict__ im, 
                 for (int u1 = 0; u1 < 3; ++u1) {
                     const double x = px[0][wy + v1][wx + u1], y = px[1][wy + v1][wx + u1], z = px[2][wy + v1][wx + u1];
                     m0 += x; m1 += y; m2 += z;
-                    s00 += x * x; s01 += x * y; s02 += x * z;
-                    s11 += y * y; s12 += y * z; s22 += z * z;
+                    // products of floats are exact in double: fma == add of the product
+                    s00 = __builtin_fma(x, x, s00); s01 = __builtin_fma(x, y, s01); s02 = __builtin_fma(x, z, s02);
+                    s11 = __builtin_fma(y, y, s11); s12 = __builtin_fma(y, z, s12); s22 = __builtin_fma(z, z, s22);
                 }
-            m0 /= 9.0; m1 /= 9.0; m2 /= 9.0;
-            const double reg = (double)eps / 9.0;
-            const double c11 = s00 / 9.0 - m0 * m0 + reg;
-            const double c12 = s01 / 9.0 - m0 * m1;
-            const double c13 = s02 / 9.0 - m0 * m2;
-            const double c22 = s11 / 9.0 - m1 * m1 + reg;
-            const double c23 = s12 / 9.0 - m1 * m2;
-            const double c33 = s22 / 9.0 - m2 * m2 + reg;
-            const double d11 = c22 * c33 - c23 * c23;
-            const double d12 = c13 * c23 - c12 * c33;
-            const double d13 = c12 * c23 - c13 * c22;
-            const double d22 = c11 * c33 - c13 * c13;
-            const double d23 = c12 * c13 - c11 * c23;
-            const double d33 = c11 * c22 - c12 * c12;
-            const double det = c11 * d11 + c12 * d12 + c13 * d13;
-            ws[0][e] = d11 / det; ws[1][e] = d12 / det; ws[2][e] = d13 / det;
-            ws[3][e] = d22 / det; ws[4][e] = d23 / det; ws[5][e] = d33 / det;
+            m0 *= kInv9; m1 *= kInv9; m2 *= kInv9;
+            const double reg = (double)eps * kInv9;
+            const double c11 = __builtin_fma(-m0, m0, s00 * kInv9) + reg;
+            const double c12 = __builtin_fma(-m0, m1, s01 * kInv9);
+            const double c13 = __builtin_fma(-m0, m2, s02 * kInv9);
+            const double c22 = __builtin_fma(-m1, m1, s11 * kInv9) + reg;
+            const double c23 = __builtin_fma(-m1, m2, s12 * kInv9);
+            const double c33 = __builtin_fma(-m2, m2, s22 * kInv9) + reg;
+            const double d11 = __builtin_fma(c22, c33, -(c23 * c23));
+            const double d12 = __builtin_fma(c13, c23, -(c12 * c33));
+            const double d13 = __builtin_fma(c12, c23, -(c13 * c22));
+            const double d22 = __builtin_fma(c11, c33, -(c13 * c13));
+            const double d23 = __builtin_fma(c12, c13, -(c11 * c23));
+            const double d33 = __builtin_fma(c11, c22, -(c12 * c12));
+            const double det = __builtin_fma(c11, d11, __builtin_fma(c12, d12, c13 * d13));
+            const double q = -1.0 / (9.0 * det);
+            ws[0][e] = d11 * q; ws[1][e] = d12 * q; ws[2][e] = d13 * q;
+            ws[3][e] = d22 * q; ws[4][e] = d23 * q; ws[5][e] = d33 * q;
             ws[7][e] = m0; ws[8][e] = m1; ws[9][e] = m2;
             const float h0 = (float)m0, h1 = (float)m1, h2 = (float)m2;
             wm[0][e] = h0; wm[1][e] = h1; wm[2][e] = h2;
@@ -144,59 +159,74 @@ __global__ __launch_bounds__(kBlock) void k_levin(const float* __restrict__ im, 
     for (int k = 0; k < 25; ++k) acc[k] = 0.f;
     const float xi0 = px[0][ty + 2][tx + 2], xi1 = px[1][ty + 2][tx + 2], xi2 = px[2][ty + 2][tx + 2];
 
+    if constexpr (MODE == GM_ASSEMBLY_REFERENCE) {
 #pragma unroll
-    for (int wy2 = 0; wy2 < 3; ++wy2)
+        for (int wy2 = 0; wy2 < 3; ++wy2)
 #pragma unroll
-        for (int wx2 = 0; wx2 < 3; ++wx2) {
-            const int e = (ty + wy2) * kWX + (tx + wx2);
-            if (!wvalid[e]) continue;
-            float wx, wy, wz;
-            if constexpr (MODE == GM_ASSEMBLY_REFERENCE) {
+            for (int wx2 = 0; wx2 < 3; ++wx2) {
+                const int e = (ty + wy2) * kWX + (tx + wx2);
+                if (!wvalid[e]) continue;
                 // Matting.cu:146-153
                 const float mx = (float)ws[7][e], my = (float)ws[8][e], mz = (float)ws[9][e];
                 const float rx = xi0 - mx, ry = xi1 - my, rz = xi2 - mz;
                 const double det = ws[6][e];
-                wx = (float)((ws[0][e] * (double)rx + ws[1][e] * (double)ry + ws[2][e] * (double)rz) / det);
-                wy = (float)((ws[1][e] * (double)rx + ws[3][e] * (double)ry + ws[4][e] * (double)rz) / det);
-                wz = (float)((ws[2][e] * (double)rx + ws[4][e] * (double)ry + ws[5][e] * (double)rz) / det);
-            } else {
-                const double rx = (double)xi0 - ws[7][e], ry = (double)xi1 - ws[8][e], rz = (double)xi2 - ws[9][e];
-                wx = (float)(ws[0][e] * rx + ws[1][e] * ry + ws[2][e] * rz);
-                wy = (float)(ws[1][e] * rx + ws[3][e] * ry + ws[4][e] * rz);
-                wz = (float)(ws[2][e] * rx + ws[4][e] * ry + ws[5][e] * rz);
-            }
-            float mh0, mh1, mh2, ml0 = 0.f, ml1 = 0.f, ml2 = 0.f;
-            if constexpr (MODE == GM_ASSEMBLY_REFERENCE) {
-                mh0 = (float)ws[7][e]; mh1 = (float)ws[8][e]; mh2 = (float)ws[9][e];
-            } else {
-                mh0 = wm[0][e]; mh1 = wm[1][e]; mh2 = wm[2][e];
-                ml0 = wm[3][e]; ml1 = wm[4][e]; ml2 = wm[5][e];
-            }
+                const float wx = (float)((ws[0][e] * (double)rx + ws[1][e] * (double)ry + ws[2][e] * (double)rz) / det);
+                const float wy = (float)((ws[1][e] * (double)rx + ws[3][e] * (double)ry + ws[4][e] * (double)rz) / det);
+                const float wz = (float)((ws[2][e] * (double)rx + ws[4][e] * (double)ry + ws[5][e] * (double)rz) / det);
 #pragma unroll
-            for (int vy = 0; vy < 3; ++vy)
+                for (int vy = 0; vy < 3; ++vy)
 #pragma unroll
-                for (int vx = 0; vx < 3; ++vx) {
-                    const int lx = tx + wx2 + vx, ly = ty + wy2 + vy;  // pixel j in the halo
-                    float rx, ry, rz;
-                    if constexpr (MODE == GM_ASSEMBLY_REFERENCE) {
-                        rx = px[0][ly][lx] - mh0;
-                        ry = px[1][ly][lx] - mh1;
-                        rz = px[2][ly][lx] - mh2;
-                    } else {
-                        rx = (px[0][ly][lx] - mh0) - ml0;
-                        ry = (px[1][ly][lx] - mh1) - ml1;
-                        rz = (px[2][ly][lx] - mh2) - ml2;
-                    }
-                    // Matting.cu:177
-                    float s = 1.0f + wx * rx;
-                    s = s + wy * ry;
-                    s = s + wz * rz;
-                    if constexpr (MODE == GM_ASSEMBLY_REFERENCE)
+                    for (int vx = 0; vx < 3; ++vx) {
+                        const int lx = tx + wx2 + vx, ly = ty + wy2 + vy;  // pixel j in the halo
+                        const float qx = px[0][ly][lx] - mx, qy = px[1][ly][lx] - my, qz = px[2][ly][lx] - mz;
+                        // Matting.cu:177
+                        float s = 1.0f + wx * qx;
+                        s = s + wy * qy;
+                        s = s + wz * qz;
                         acc[12 + (wx2 - 1 + vx - 1) + 5 * (wy2 - 1 + vy - 1)] += -s / 9.0f;
-                    else  // window sums of (1 + w.r); scaled by -1/9 once below
-                        acc[12 + (wx2 - 1 + vx - 1) + 5 * (wy2 - 1 + vy - 1)] += s;
-                }
+                    }
+            }
+    } else {
+        // per window n (raster order; centre offset (n % 3 - 1, n / 3 - 1)):
+        // v = S' (x_i - mu), c = -1/9 - v.ml.  An invalid window gets
+        // v = c = 0: its terms are exactly +0 and leave the sums unchanged.
+        float vc[9], v0[9], v1[9], v2[9], h0[9], h1[9], h2[9];
+        const double xd0 = xi0, xd1 = xi1, xd2 = xi2;
+#pragma unroll
+        for (int n = 0; n < 9; ++n) {
+            const int e = (ty + n / 3) * kWX + (tx + n % 3);
+            vc[n] = v0[n] = v1[n] = v2[n] = h0[n] = h1[n] = h2[n] = 0.f;
+            if (!wvalid[e]) continue;
+            const double r0 = xd0 - ws[7][e], r1 = xd1 - ws[8][e], r2 = xd2 - ws[9][e];
+            const double a00 = ws[0][e], a01 = ws[1][e], a02 = ws[2][e];
+            const double a11 = ws[3][e], a12 = ws[4][e], a22 = ws[5][e];
+            v0[n] = (float)__builtin_fma(a02, r2, __builtin_fma(a01, r1, a00 * r0));
+            v1[n] = (float)__builtin_fma(a12, r2, __builtin_fma(a11, r1, a01 * r0));
+            v2[n] = (float)__builtin_fma(a22, r2, __builtin_fma(a12, r1, a02 * r0));
+            vc[n] = __builtin_fmaf(-v2[n], wm[5][e],
+                                   __builtin_fmaf(-v1[n], wm[4][e], __builtin_fmaf(-v0[n], wm[3][e], -kInv9f)));
+            h0[n] = wm[0][e]; h1[n] = wm[1][e]; h2[n] = wm[2][e];
         }
+        // member j = i + (dx, dy) lies in window n iff |dx - cx_n| <= 1 and
+        // |dy - cy_n| <= 1; L[i, j] sums its windows in raster order.  The
+        // diagonal is set from the row sum below.
+#pragma unroll
+        for (int dy = -2; dy <= 2; ++dy)
+#pragma unroll
+            for (int dx = -2; dx <= 2; ++dx) {
+                if (dx == 0 && dy == 0) continue;
+                const int lx = tx + 2 + dx, ly = ty + 2 + dy;
+                const float p0 = px[0][ly][lx], p1 = px[1][ly][lx], p2 = px[2][ly][lx];
+#pragma unroll
+                for (int n = 0; n < 9; ++n) {
+                    const int cx = n % 3 - 1, cy = n / 3 - 1;
+                    if (dx - cx < -1 || dx - cx > 1 || dy - cy < -1 || dy - cy > 1) continue;
+                    const float t = __builtin_fmaf(
+                        v2[n], p2 - h2[n], __builtin_fmaf(v1[n], p1 - h1[n], __builtin_fmaf(v0[n], p0 - h0[n], vc[n])));
+                    acc[12 + dx + 5 * dy] += t;
+                }
+            }
+    }
 
     const size_t i = (size_t)gy * (size_t)W + (size_t)gx;
     float l = 0.f, tg = 0.f;
@@ -218,10 +248,7 @@ __global__ __launch_bounds__(kBlock) void k_levin(const float* __restrict__ im, 
         double off = 0;
 #pragma unroll
         for (int k = 0; k < 25; ++k)
-            if (k != 12) {
-                acc[k] = -acc[k] / 9.0f;
-                off += (double)acc[k];
-            }
+            if (k != 12) off += (double)acc[k];
         acc[12] = (float)(-off);
     }
     float bi = 0.f;

@@ -1396,6 +1396,11 @@ template <int FMT, bool SYM>
 __global__ __launch_bounds__(kBlock) void k_scale16(const float* __restrict__ A, size_t pitch, int W, int N,
                                                     unsigned short* __restrict__ A16, size_t pitch16,
                                                     const float* __restrict__ svec) {
+    // both products rounded to float, then the float to 16 bits: without
+    // this the compiler fuses the second product and the conversion into one
+    // mixed FMA (a single rounding), which differs where the float product
+    // lands on a 16-bit rounding tie
+#pragma clang fp contract(off)
     constexpr int K0 = SYM ? 13 : 0, NB = 25 - K0 - (SYM ? 0 : 1);
     const long nq = ((long)N + 3) / 4;
     for (long q = (long)blockIdx.x * kBlock + threadIdx.x; q < nq; q += (long)gridDim.x * kBlock) {

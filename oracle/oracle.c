@@ -253,18 +253,25 @@ void orc_levin_gather_f32(float* a, size_t apitch, float* b, float* dinv, const 
 /* "Accurate" FP32 assembly (not the reference's rounding; same definition).
  * The reference forms the window covariance from float means and float
  * products (Matting.cu:96-120), which loses ~4 digits when the colour variance
- * is ~1e-4 and eps is 1e-7.  Here the window mean and covariance sums use
- * exact double products and the inverse is adjugate/det in double; the
- * whitened pixel S(x_i - mu) is rounded to float once; the window mean is
- * split into float hi + lo parts and (x_j - mu) = (x_j - hi) - lo in float;
- * per entry the window terms (1 + w.r) are summed in float in window-raster
- * order and the sum is scaled by -1/9 once (a division by 9).  The diagonal
- * is -sum of the off-diagonal entries in double, then + lam_i. */
+ * is ~1e-4 and eps is 1e-7.  Here, per window: exact double sums of the
+ * colours and of their products (float products are exact in double), the
+ * 1/9 scalings as multiplications by the double 1/9, the covariance as
+ * fma(-mu_a, mu_b, s_ab/9) (+ eps/9 on the diagonal), FMA-contracted adjugate
+ * and determinant, and S' = -inverse/9 = adjugate * (-1 / (9 det)).  Per pixel
+ * i and window w (raster order): r = x_i - mu (double), v = S' r and
+ * c = -1/9 + v.r (double, FMA chains), rounded to float once; every other
+ * member j = i + (dx, dy) adds c + v.(x_j - x_i) (float FMAs, x_j - x_i in
+ * float) to L[i, j], windows in raster order.  The diagonal is -sum of the
+ * off-diagonal entries in double, then + lam_i.  Every fused operation is an
+ * explicit fma()/fmaf() (the file is built without contraction), so this is
+ * the GPU kernel's arithmetic operation for operation. */
 typedef struct {
     double m[3];
-    double S[6]; /* S11 S12 S13 S22 S23 S33 */
-    float mh[3], ml[3];
+    double S[6]; /* S'11 S'12 S'13 S'22 S'23 S'33 (= -inverse / 9) */
+    float mh[3], ml[3]; /* mean = mh + ml: mh = float(mean), ml = float(mean - mh) */
 } orc_wacc_t;
+
+static const double ORC_INV9 = 1.0 / 9.0;
 
 static void orc_window_stats_acc(orc_wacc_t* w, const float* im, int nch, size_t pitch,
                                  int u, int v, float eps)
@@ -275,26 +282,27 @@ static void orc_window_stats_acc(orc_wacc_t* w, const float* im, int nch, size_t
             const float* p = im + ((size_t)v1 * pitch + u1) * nch;
             double x = p[0], y = p[1], z = p[2];
             m0 += x; m1 += y; m2 += z;
-            s00 += x * x; s01 += x * y; s02 += x * z;
-            s11 += y * y; s12 += y * z; s22 += z * z;
+            s00 = fma(x, x, s00); s01 = fma(x, y, s01); s02 = fma(x, z, s02);
+            s11 = fma(y, y, s11); s12 = fma(y, z, s12); s22 = fma(z, z, s22);
         }
-    m0 /= 9.0; m1 /= 9.0; m2 /= 9.0;
-    double reg = (double)eps / 9.0;
-    double c11 = s00 / 9.0 - m0 * m0 + reg;
-    double c12 = s01 / 9.0 - m0 * m1;
-    double c13 = s02 / 9.0 - m0 * m2;
-    double c22 = s11 / 9.0 - m1 * m1 + reg;
-    double c23 = s12 / 9.0 - m1 * m2;
-    double c33 = s22 / 9.0 - m2 * m2 + reg;
-    double d11 = c22 * c33 - c23 * c23;
-    double d12 = c13 * c23 - c12 * c33;
-    double d13 = c12 * c23 - c13 * c22;
-    double d22 = c11 * c33 - c13 * c13;
-    double d23 = c12 * c13 - c11 * c23;
-    double d33 = c11 * c22 - c12 * c12;
-    double det = c11 * d11 + c12 * d12 + c13 * d13;
-    w->S[0] = d11 / det; w->S[1] = d12 / det; w->S[2] = d13 / det;
-    w->S[3] = d22 / det; w->S[4] = d23 / det; w->S[5] = d33 / det;
+    m0 *= ORC_INV9; m1 *= ORC_INV9; m2 *= ORC_INV9;
+    double reg = (double)eps * ORC_INV9;
+    double c11 = fma(-m0, m0, s00 * ORC_INV9) + reg;
+    double c12 = fma(-m0, m1, s01 * ORC_INV9);
+    double c13 = fma(-m0, m2, s02 * ORC_INV9);
+    double c22 = fma(-m1, m1, s11 * ORC_INV9) + reg;
+    double c23 = fma(-m1, m2, s12 * ORC_INV9);
+    double c33 = fma(-m2, m2, s22 * ORC_INV9) + reg;
+    double d11 = fma(c22, c33, -(c23 * c23));
+    double d12 = fma(c13, c23, -(c12 * c33));
+    double d13 = fma(c12, c23, -(c13 * c22));
+    double d22 = fma(c11, c33, -(c13 * c13));
+    double d23 = fma(c12, c13, -(c11 * c23));
+    double d33 = fma(c11, c22, -(c12 * c12));
+    double det = fma(c11, d11, fma(c12, d12, c13 * d13));
+    double q = -1.0 / (9.0 * det);
+    w->S[0] = d11 * q; w->S[1] = d12 * q; w->S[2] = d13 * q;
+    w->S[3] = d22 * q; w->S[4] = d23 * q; w->S[5] = d33 * q;
     w->m[0] = m0; w->m[1] = m1; w->m[2] = m2;
     for (int c = 0; c < 3; ++c) {
         w->mh[c] = (float)w->m[c];
@@ -323,30 +331,26 @@ void orc_levin_gather_acc_f32(float* a, size_t apitch, float* b, float* dinv, co
                 for (int cx = px - 1; cx <= px + 1; ++cx) {
                     if (cy < 1 || cy > H - 2 || cx < 1 || cx > W - 2) continue;
                     const orc_wacc_t* w = &wst[(size_t)(cy - py + 1) * W + cx];
-                    double x = (double)pi[0] - w->m[0], y = (double)pi[1] - w->m[1],
-                           z = (double)pi[2] - w->m[2];
-                    float wx = (float)(w->S[0] * x + w->S[1] * y + w->S[2] * z);
-                    float wy = (float)(w->S[1] * x + w->S[3] * y + w->S[4] * z);
-                    float wz = (float)(w->S[2] * x + w->S[4] * y + w->S[5] * z);
+                    const double* S = w->S;
+                    double r0 = (double)pi[0] - w->m[0], r1 = (double)pi[1] - w->m[1],
+                           r2 = (double)pi[2] - w->m[2];
+                    double u0 = fma(S[2], r2, fma(S[1], r1, S[0] * r0));
+                    double u1 = fma(S[4], r2, fma(S[3], r1, S[1] * r0));
+                    double u2 = fma(S[5], r2, fma(S[4], r1, S[2] * r0));
+                    float v0 = (float)u0, v1 = (float)u1, v2 = (float)u2;
+                    float c = fmaf(-v2, w->ml[2], fmaf(-v1, w->ml[1], fmaf(-v0, w->ml[0], -1.0f / 9.0f)));
                     for (int vy = cy - 1; vy <= cy + 1; ++vy)
                         for (int vx = cx - 1; vx <= cx + 1; ++vx) {
+                            if (vx == px && vy == py) continue; /* the diagonal comes from the row sum */
                             const float* pj = im + ((size_t)vy * pitch + vx) * nch;
-                            float rx = (pj[0] - w->mh[0]) - w->ml[0];
-                            float ry = (pj[1] - w->mh[1]) - w->ml[1];
-                            float rz = (pj[2] - w->mh[2]) - w->ml[2];
-                            float s = 1.0f + wx * rx;
-                            s = s + wy * ry;
-                            s = s + wz * rz;
-                            acc[CB + (vx - px) + 5 * (vy - py)] += s;
+                            float q0 = pj[0] - w->mh[0], q1 = pj[1] - w->mh[1], q2 = pj[2] - w->mh[2];
+                            acc[CB + (vx - px) + 5 * (vy - py)] += fmaf(v2, q2, fmaf(v1, q1, fmaf(v0, q0, c)));
                         }
                 }
             size_t i = (size_t)px + (size_t)py * W;
             double off = 0;
             for (int k = 0; k < NB; ++k)
-                if (k != CB) {
-                    acc[k] = -acc[k] / 9.0f;
-                    off += (double)acc[k];
-                }
+                if (k != CB) off += (double)acc[k];
             acc[CB] = (float)(-off);
             float l = lam ? lam[i] : 0.f;
             if (l != 0.f) {

@@ -156,7 +156,9 @@ def levin_gather_f32(image, lam=None, tgt=None, eps=1e-7, apitch=None):
 
 
 def levin_gather_acc_f32(image, lam=None, tgt=None, eps=1e-7, apitch=None):
-    """Accurate-rounding FP32 assembly (double covariance sums); returns (a, b, dinv)."""
+    """Accurate-rounding FP32 assembly (FP64 window statistics, FMA-contracted
+    whitening, float centring on the head of the window mean); returns
+    (a, b, dinv).  The GPU's GM_ASSEMBLY_ACCURATE operation for operation."""
     im, W, H, nch = _img(image)
     N = W * H
     ap = apitch or N
