@@ -392,6 +392,16 @@ __device__ __forceinline__ void buf_st4(__amdgpu_buffer_rsrc_t r, long long i, f
     const unsigned off = skip ? 0x80000000u : (unsigned)i * 4u;
     __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, (int)off, 0, 0);
 }
+// elements i, i + 1 := v (i and n even); `skip` drops the store
+__device__ __forceinline__ void buf_st2(__amdgpu_buffer_rsrc_t r, long long i, f32x2 v, bool skip = false) {
+    const unsigned off = skip ? 0x80000000u : (unsigned)i * 4u;
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)off, 0, 0);
+}
+// elements i, i + 1 (i and n even: both in range or both 0); `skip` forces 0
+__device__ __forceinline__ f32x2 buf_ld2(__amdgpu_buffer_rsrc_t r, long long i, bool skip = false) {
+    const unsigned off = skip ? 0x80000000u : (unsigned)i * 4u;
+    return __builtin_bit_cast(f32x2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
 // cache policy of a buffer load that streams data read once (the NT bit)
 constexpr int kNonTemporal = 2;
 // elements i .. i + 3 (i and n multiples of 4, so the four are all in range or

@@ -22,6 +22,7 @@
 
 #include <map>
 #include <string>
+#include <type_traits>
 #include <utility>
 
 #include "gm_internal.h"
@@ -264,31 +265,54 @@ __device__ __forceinline__ float h2f(unsigned int h) {
 // U holds 0 for pairs outside the image or wrapping a row, so the product
 // equals the full 24-band one wherever As is symmetric.  16-bit bands move
 // 24 B per row instead of 48.
-template <int TY>
+// FP32 upper bands (the fused FP32 iteration) take 2 cells per thread, so that
+// a thread's band words, and a staged U row's 32-bit words, are as many as
+// for 16-bit bands with 4 cells per thread.
+template <int TY, int FMT = kF16>
 struct SymGeom {
-    using G = Geom2D<4, TY>;
-    static constexpr int R = 4, TPR = G::TPR, TX = G::TX, ST = G::STRIDE;
-    static constexpr int UR = TY + 2;      // staged U rows: 2 above the tile, then the tile's
-    static constexpr int SW = TX / 2 + 2;  // 32-bit words (half pairs) per staged U row: TX + 4 halves
-    static constexpr int XR = TY + 4;      // staged x rows
+    static constexpr int R = FMT == kF32 ? 2 : 4;    // cells per thread
+    static constexpr int EPW = FMT == kF32 ? 1 : 2;  // band entries per 32-bit word
+    using G = Geom2D<R, TY>;
+    static constexpr int TPR = G::TPR, TX = G::TX, ST = G::STRIDE;
+    static constexpr int UR = TY + 2;          // staged U rows: 2 above the tile, then the tile's
+    static constexpr int SW = (TX + 4) / EPW;  // 32-bit words per staged U row: TX + 4 entries
+    static constexpr int XR = TY + 4;          // staged x rows
 };
 
+// Entry q of a run of band words (16-bit: two per word, low half first).
+template <int FMT>
+__device__ __forceinline__ float band_entry(const unsigned int* w, int q) {
+    if constexpr (FMT == kF32) return __uint_as_float(w[q]);
+    else return h2f<FMT>(w[q >> 1] >> (16 * (q & 1)));
+}
+
 // One tile's product from the staged windows.  xrow[q]: LDS offset of the
-// thread's 8 x cells (columns 4 tx .. 4 tx + 7) in window row ty + q;
+// thread's R + 4 x cells (columns R tx .. R tx + R + 3) in window row ty + q;
 // urow[d]: LDS row of U window row ty + 2 - d.  Sums in band order, so every
 // staging scheme gives bit-identical results.
 template <int TY, int FMT>
 __device__ __forceinline__ void sym_tile(const float* __restrict__ lds, const int (&xrow)[5],
-                                         const unsigned int (*ub)[SymGeom<TY>::UR][SymGeom<TY>::SW],
-                                         const int (&urow)[3], int tx, float (&y)[4], float (&pc)[4]) {
-    constexpr int R = 4;
-    float xw[5][8];
+                                         const unsigned int (*ub)[SymGeom<TY, FMT>::UR][SymGeom<TY, FMT>::SW],
+                                         const int (&urow)[3], int tx, float (&y)[SymGeom<TY, FMT>::R],
+                                         float (&pc)[SymGeom<TY, FMT>::R]) {
+    using SG = SymGeom<TY, FMT>;
+    constexpr int R = SG::R, NC = R + 4, NW = NC / SG::EPW;  // window cells, and their band words
+    float xw[5][NC];
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
-        const f32x4 a0 = *reinterpret_cast<const f32x4*>(lds + xrow[q]);
-        const f32x4 a1 = *reinterpret_cast<const f32x4*>(lds + xrow[q] + 4);
-        xw[q][0] = a0.x; xw[q][1] = a0.y; xw[q][2] = a0.z; xw[q][3] = a0.w;
-        xw[q][4] = a1.x; xw[q][5] = a1.y; xw[q][6] = a1.z; xw[q][7] = a1.w;
+        if constexpr (R == 4) {
+            const f32x4 a0 = *reinterpret_cast<const f32x4*>(lds + xrow[q]);
+            const f32x4 a1 = *reinterpret_cast<const f32x4*>(lds + xrow[q] + 4);
+            xw[q][0] = a0.x; xw[q][1] = a0.y; xw[q][2] = a0.z; xw[q][3] = a0.w;
+            xw[q][4] = a1.x; xw[q][5] = a1.y; xw[q][6] = a1.z; xw[q][7] = a1.w;
+        } else {
+#pragma unroll
+            for (int h = 0; h < NC / 2; ++h) {
+                const f32x2 a = *reinterpret_cast<const f32x2*>(lds + xrow[q] + 2 * h);
+                xw[q][2 * h] = a.x;
+                xw[q][2 * h + 1] = a.y;
+            }
+        }
     }
 #pragma unroll
     for (int j = 0; j < R; ++j) {
@@ -298,26 +322,30 @@ __device__ __forceinline__ void sym_tile(const float* __restrict__ lds, const in
 #pragma unroll
     for (int m = 0; m < 12; ++m) {
         const int k = 13 + m, dy = k / 5 - 2, dx = k % 5 - 2;
-        // halves 4 tx .. 4 tx + 7 of window rows ty + 2 (own) and ty + 2 - dy
-        const unsigned int* wo = &ub[m][urow[0]][2 * tx];
-        const uint2 o01 = *reinterpret_cast<const uint2*>(wo);
-        const uint2 o23 = *reinterpret_cast<const uint2*>(wo + 2);
-        const unsigned int wn[4] = {o01.x, o01.y, o23.x, o23.y};
-        unsigned int wl[4] = {o01.x, o01.y, o23.x, o23.y};
+        // words of cells R tx .. R tx + R + 3 of window rows ty + 2 (own) and ty + 2 - dy
+        unsigned int wn[NW], wl[NW];
+        const unsigned int* wo = &ub[m][urow[0]][R * tx / SG::EPW];
+#pragma unroll
+        for (int h = 0; h < NW / 2; ++h) {
+            const uint2 o = *reinterpret_cast<const uint2*>(wo + 2 * h);
+            wn[2 * h] = wl[2 * h] = o.x;
+            wn[2 * h + 1] = wl[2 * h + 1] = o.y;
+        }
         if (dy != 0) {
-            const unsigned int* wr = &ub[m][urow[dy]][2 * tx];
-            const uint2 w01 = *reinterpret_cast<const uint2*>(wr);
-            const uint2 w23 = *reinterpret_cast<const uint2*>(wr + 2);
-            wl[0] = w01.x; wl[1] = w01.y; wl[2] = w23.x; wl[3] = w23.y;
+            const unsigned int* wr = &ub[m][urow[dy]][R * tx / SG::EPW];
+#pragma unroll
+            for (int h = 0; h < NW / 2; ++h) {
+                const uint2 o = *reinterpret_cast<const uint2*>(wr + 2 * h);
+                wl[2 * h] = o.x;
+                wl[2 * h + 1] = o.y;
+            }
         }
 #pragma unroll
         for (int j = 0; j < R; ++j) {
-            // upper entry of this row (half 2 + j): x at (+dy, +dx)
-            const int qo = 2 + j;
-            y[j] = fmaf(h2f<FMT>(wn[qo >> 1] >> (16 * (qo & 1))), xw[2 + dy][2 + dx + j], y[j]);
-            // lower entry: U_k of the row at (-dy, -dx) (half 2 + j - dx): x there
-            const int ql = 2 + j - dx;
-            y[j] = fmaf(h2f<FMT>(wl[ql >> 1] >> (16 * (ql & 1))), xw[2 - dy][2 - dx + j], y[j]);
+            // upper entry of this row (cell 2 + j): x at (+dy, +dx)
+            y[j] = fmaf(band_entry<FMT>(wn, 2 + j), xw[2 + dy][2 + dx + j], y[j]);
+            // lower entry: U_k of the row at (-dy, -dx) (cell 2 + j - dx): x there
+            y[j] = fmaf(band_entry<FMT>(wl, 2 + j - dx), xw[2 - dy][2 - dx + j], y[j]);
         }
     }
 }
@@ -682,17 +710,67 @@ __global__ __launch_bounds__(kBlock) void k_uedge(const unsigned short* __restri
     }
 }
 
-// Vectors of fewer than kFusedMaxN elements: the descriptor over the 12 upper
-// bands spans 24 x pitch bytes, which must stay below 2^31.
+// The same side words of the FP32 upper bands (128-wide columns, 2 cells per
+// side): E[(((column * H + row) * 12 + band) * 2 + side) * 2 + w] =
+// U_band[row * W + column * TX - 2 + w] (left) or U_band[... + TX + w]
+// (right), 0 outside the vector.  U holds band 13 + m at U + m * pitch.
+__global__ __launch_bounds__(kBlock) void k_uedge32(const float* __restrict__ U, size_t pitch, int W, int H, int N,
+                                                    int TX, unsigned int* __restrict__ E) {
+    const int ncol = (W + TX - 1) / TX;
+    const long long total = (long long)ncol * H * 48;
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < total; i += (long long)gridDim.x * kBlock) {
+        const int rem = (int)(i % 48), m = rem >> 2, side = (rem >> 1) & 1, wd = rem & 1;
+        const long long cy = i / 48;
+        const int y = (int)(cy % H), c = (int)(cy / H);
+        const long long gi = (long long)y * W + (side ? c * TX + TX : c * TX - 2) + wd;
+        E[i] = (gi >= 0 && gi < N) ? __float_as_uint(U[(size_t)m * pitch + gi]) : 0u;
+    }
+}
+
+// Vectors of fewer than fused_max_n(FMT) elements: the descriptor over the 12
+// upper bands spans 12 x pitch entries (24 or 48 x pitch bytes), which must
+// stay below 2^31.
 constexpr int kFusedMaxN = 1 << 26;
+constexpr int kFused32MaxN = 1 << 25;
+inline int fused_max_n(int fmt) { return fmt == kF32 ? kFused32MaxN : kFusedMaxN; }
+// Words of the fused iteration's side-word array of a W x H operator.
+inline long fused_edge_words(int fmt, int W, long N) {
+    if (W <= 0) return 0;
+    const int TX = fmt == kF32 ? 128 : 256;
+    return (long)((W + TX - 1) / TX) * (N / W) * (fmt == kF32 ? 48 : 24);
+}
+// Uup: band 13 of the scaled operator (16-bit: bands at k * pitch of the
+// 25-band array, so Uup = A16 + 13 pitch; FP32: the 12 upper bands at m * pitch).
 template <int FMT>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void k_cgfused_symw(
-    const unsigned short* __restrict__ U, size_t pitch, int W, int H, int N, const float* __restrict__ svec,
+    const void* __restrict__ Uup, size_t pitch, int W, int H, int N, const float* __restrict__ svec,
     float* __restrict__ x, float* r0, float* r1, float* p0, float* p1, float* Ap0, float* Ap1,
     const unsigned int* __restrict__ uedge, Scalars* sc, double* __restrict__ dpf, int seq, int mode, int verify) {
     constexpr int TY = 4;
-    using SG = SymGeom<TY>;
-    constexpr int R = 4, TPR = SG::TPR, TX = SG::TX, ST = SG::ST, UR = SG::UR, SW = SG::SW, XR = SG::XR;
+    using SG = SymGeom<TY, FMT>;
+    constexpr int R = SG::R, TPR = SG::TPR, TX = SG::TX, ST = SG::ST, UR = SG::UR, SW = SG::SW, XR = SG::XR;
+    constexpr int EPW = SG::EPW, EB = FMT == kF32 ? 4 : 2;  // band entries per word, bytes per entry
+    constexpr int SPB = 2 / EPW;                             // side words per band and side
+    using VR = std::conditional_t<R == 4, f32x4, f32x2>;
+    auto ldR = [](__amdgpu_buffer_rsrc_t rs, long long i, bool skip = false) -> VR {
+        if constexpr (R == 4) return buf_ld4(rs, i, skip);
+        else return buf_ld2(rs, i, skip);
+    };
+    auto stR = [](__amdgpu_buffer_rsrc_t rs, long long i, VR v, bool skip = false) {
+        if constexpr (R == 4) buf_st4(rs, i, v, skip);
+        else buf_st2(rs, i, v, skip);
+    };
+    auto mkR = [](const float (&a)[R]) {
+        VR v;
+#pragma unroll
+        for (int q = 0; q < R; ++q) v[q] = a[q];
+        return v;
+    };
+    // p values of R consecutive window cells into the x ring (8-byte stores)
+    auto put_p = [](float* d, const float (&pn)[R]) {
+#pragma unroll
+        for (int h = 0; h < R / 2; ++h) *reinterpret_cast<f32x2*>(d + 2 * h) = f32x2{pn[2 * h], pn[2 * h + 1]};
+    };
     static_assert(TPR == 64 && XR == 2 * TY, "one wave per tile row; the x ring holds two tiles of rows");
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     if (sc->done != kRun || sc->verify) return;
@@ -723,7 +801,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
     __shared__ __attribute__((aligned(16))) unsigned int ub[12][UR][SW];
     // tile rows 0, 1: r_{j+1} and p_j of the next tile's output cells, held
     // from the tile that loaded them (each thread its own slot)
-    __shared__ f32x4 held[2][2 * TPR];
+    __shared__ VR held[2][2 * TPR];
     const int ncol = (W + TX - 1) / TX, nrow = (H + TY - 1) / TY;
     int ubeg, uend;  // the run of column-major tiles of this workgroup (as k_cgstep_symw)
     {
@@ -749,14 +827,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
     // 0, 1 the next tile's
     const int lrow = (ty + 2) & 3;
     const bool now = ty >= 2;
-    constexpr int NS = 24 * TY;  // side words of the own U rows (row, band, side)
+    constexpr int NS = 24 * SPB * TY;  // side words of the own U rows (row, band, side, word)
     constexpr int SE = (NS + kBlock - 1) / kBlock;
     struct Loads {
-        Raw<R, FMT> u[12];
+        unsigned int u[12][2];  // the thread's R entries of each upper band
         unsigned int sv[SE];
-        f32x4 xr, xa, xp;  // r_j, Ap_j, p_j of the thread's 4 interior window cells
+        VR xr, xa, xp;     // r_j, Ap_j, p_j of the thread's R interior window cells
         float cr, ca, cp;  // ... of one side cell (threads 0 .. 15)
-        f32x4 xo, so;      // x and s of the thread's output cells
+        VR xo, so;         // x and s of the thread's output cells
     };
     // every load and store goes through a range-checked buffer descriptor
     // (out-of-range: loads 0, stores dropped), so every path issues the same
@@ -769,9 +847,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
     // 2^31): the band loads share one offset register, the band is the
     // scalar offset
     const __amdgpu_buffer_rsrc_t bu =
-        __builtin_amdgcn_make_buffer_rsrc(const_cast<unsigned short*>(U + 13 * pitch), 0, (int)(24 * pitch), 0x00020000);
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(Uup), 0, (int)(12 * EB * pitch), 0x00020000);
     const __amdgpu_buffer_rsrc_t be = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<unsigned int*>(uedge), 0, (int)(4LL * ((W + TX - 1) / TX) * H * 24), 0x00020000);
+        const_cast<unsigned int*>(uedge), 0, (int)(4LL * ((W + TX - 1) / TX) * H * 24 * SPB), 0x00020000);
     auto origin = [&](int u, int& x0, int& y0) {
         x0 = (u / nrow) * TX;
         y0 = (u % nrow) * TY;
@@ -794,28 +872,29 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
         const int cx = x0 + R * tx, cy = y0 + ty;
         const bool valid = cx < W && cy < H && !none;
         const size_t ib = (size_t)cy * W + cx;
-        const unsigned uoff = valid ? 2u * (unsigned)ib : 0x80000000u;
+        const unsigned uoff = valid ? (unsigned)EB * (unsigned)ib : 0x80000000u;
 #pragma unroll
         for (int m = 0; m < 12; ++m) {
-            const u32x2 w2 = __builtin_amdgcn_raw_buffer_load_b64(bu, (int)uoff, (int)(2 * m * pitch), kNonTemporal);
-            L.u[m].d[0] = w2.x;
-            L.u[m].d[1] = w2.y;
+            const u32x2 w2 = __builtin_amdgcn_raw_buffer_load_b64(bu, (int)uoff, (int)(EB * m * pitch), kNonTemporal);
+            L.u[m][0] = w2.x;
+            L.u[m][1] = w2.y;
         }
         // side words of the own rows: 96 contiguous words of the edge array
         // (rows of the image only; a row below it feeds no output cell)
 #pragma unroll
         for (int q = 0; q < SE; ++q) {
             const int e = (int)threadIdx.x + q * kBlock;
-            const bool in = e < NS && y0 + e / 24 < H && !none;
-            const unsigned off = in ? 4u * ((unsigned)((x0 / TX) * H + y0) * 24u + (unsigned)e) : 0x80000000u;
+            const bool in = e < NS && y0 + e / (24 * SPB) < H && !none;
+            const unsigned off =
+                in ? 4u * ((unsigned)((x0 / TX) * H + y0) * (24u * SPB) + (unsigned)e) : 0x80000000u;
             L.sv[q] = __builtin_amdgcn_raw_buffer_load_b32(be, (int)off, 0, 0);
         }
-        L.xo = buf_ld4(brx, (long long)ib, !valid || first);
-        L.so = buf_ld4(brs, (long long)ib, !valid);
+        L.xo = ldR(brx, (long long)ib, !valid || first);
+        L.so = ldR(brs, (long long)ib, !valid);
         const long long gi = (long long)(y0 + 2 + lrow) * W + cx;
-        L.xr = buf_ld4(brr, gi, none);
-        L.xa = buf_ld4(bra, gi, first || none);
-        L.xp = buf_ld4(brp, gi, none);
+        L.xr = ldR(brr, gi, none);
+        L.xa = ldR(bra, gi, first || none);
+        L.xp = ldR(brp, gi, none);
         const long long gs = side_gi(x0, y0, 4);
         const bool sk = threadIdx.x >= 16 || none;
         L.cr = buf_ld(brr, gs, sk);
@@ -831,7 +910,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
     constexpr int PE = (2 * SW + kBlock - 1) / kBlock;
     struct Pre {
         unsigned int u[PE][12];
-        f32x4 a, b, c;
+        VR a, b, c;
         float sr, sa, sp;
     };
     auto load_pre = [&](int u, Pre& P) {
@@ -841,18 +920,19 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
         for (int k = 0; k < PE; ++k) {
             const int e = (int)threadIdx.x + k * kBlock;
             const int wy = e / SW, ww = e - wy * SW;
-            const long long gi = (long long)(y0 - 2 + wy) * W + (x0 - 2 + 2 * ww);
+            const long long gi = (long long)(y0 - 2 + wy) * W + (x0 - 2 + EPW * ww);
             const bool in = e < 2 * SW && gi >= 0 && gi < N;
 #pragma unroll
             for (int m = 0; m < 12; ++m) {
-                const unsigned off = in ? 2u * ((unsigned)m * (unsigned)pitch + (unsigned)gi) : 0x80000000u;
+                const unsigned off =
+                    in ? (unsigned)EB * ((unsigned)m * (unsigned)pitch + (unsigned)gi) : 0x80000000u;
                 P.u[k][m] = __builtin_amdgcn_raw_buffer_load_b32(bu, (int)off, 0, 0);
             }
         }
         const long long gi = (long long)(y0 - 2 + lrow) * W + (x0 + R * tx);
-        P.a = buf_ld4(brr, gi);
-        P.b = buf_ld4(bra, gi, first);
-        P.c = buf_ld4(brp, gi);
+        P.a = ldR(brr, gi);
+        P.b = ldR(bra, gi, first);
+        P.c = ldR(brp, gi);
         const long long gs = side_gi(x0, y0, 0);
         const bool sk = threadIdx.x >= 16;
         P.sr = buf_ld(brr, gs, sk);
@@ -915,7 +995,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
             const f32x4 av = reinterpret_cast<const f32x4*>(Apin)[i];
             f32x4 xn, rn;
 #pragma unroll
-            for (int q = 0; q < R; ++q) {
+            for (int q = 0; q < 4; ++q) {
                 xn[q] = fmaf(alpha * sv4[q], pv[q], xv[q]);
                 rn[q] = fmaf(alpha, av[q], rv[q]);
                 const double rd = rn[q];
@@ -963,15 +1043,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
                 }
             }
             {
-                const f32x4 a = P.a, b = P.b, c = P.c;
+                const VR a = P.a, b = P.b, c = P.c;
                 float rn[R], pn[R];
 #pragma unroll
                 for (int q = 0; q < R; ++q) pn[q] = advance(a[q], b[q], c[q], rn[q]);
-                float* d = lds + lrow * ST + 2 + R * tx;
-                *reinterpret_cast<f32x2*>(d) = f32x2{pn[0], pn[1]};
-                *reinterpret_cast<f32x2*>(d + 2) = f32x2{pn[2], pn[3]};
+                put_p(lds + lrow * ST + 2 + R * tx, pn);
                 if (!now) {
-                    held[0][threadIdx.x] = f32x4{rn[0], rn[1], rn[2], rn[3]};
+                    held[0][threadIdx.x] = mkR(rn);
                     held[1][threadIdx.x] = c;
                 }
             }
@@ -988,18 +1066,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
         // own U rows (window rows 2 .. TY + 1) and their side words
         {
             const int wr = utop + 2 + ty < UR ? utop + 2 + ty : utop + 2 + ty - UR;
+            const int w0 = (R * tx + 2) / EPW;  // the word of the thread's first interior cell
 #pragma unroll
             for (int m = 0; m < 12; ++m) {
-                ub[m][wr][1 + 2 * tx] = L.u[m].d[0];
-                ub[m][wr][2 + 2 * tx] = L.u[m].d[1];
+                ub[m][wr][w0] = L.u[m][0];
+                ub[m][wr][w0 + 1] = L.u[m][1];
             }
 #pragma unroll
             for (int q = 0; q < SE; ++q) {
                 const int e = (int)threadIdx.x + q * kBlock;
                 if (e < NS) {
-                    const int row = e / 24, m = (e % 24) >> 1;
+                    // (row, band, side, word): words 0 .. SPB - 1 (left) or SW - SPB .. SW - 1 (right)
+                    const int row = e / (24 * SPB), rem = e % (24 * SPB);
+                    const int m = rem / (2 * SPB), side = (rem / SPB) & 1, wd = rem % SPB;
                     const int wy = utop + 2 + row, ws = wy < UR ? wy : wy - UR;
-                    ub[m][ws][(e & 1) ? SW - 1 : 0] = L.sv[q];
+                    ub[m][ws][side ? SW - SPB + wd : wd] = L.sv[q];
                 }
             }
         }
@@ -1014,9 +1095,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
             float rn[R], pn[R], cur_p[R];
 #pragma unroll
             for (int q = 0; q < R; ++q) pn[q] = advance(L.xr[q], L.xa[q], L.xp[q], rn[q]);
-            float* d = lds + ((xtop + 4 + lrow) & (XR - 1)) * ST + 2 + R * tx;
-            *reinterpret_cast<f32x2*>(d) = f32x2{pn[0], pn[1]};
-            *reinterpret_cast<f32x2*>(d + 2) = f32x2{pn[2], pn[3]};
+            put_p(lds + ((xtop + 4 + lrow) & (XR - 1)) * ST + 2 + R * tx, pn);
             if (now) {
 #pragma unroll
                 for (int q = 0; q < R; ++q) {
@@ -1024,13 +1103,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
                     cur_p[q] = L.xp[q];
                 }
             } else {
-                const f32x4 hr = held[0][threadIdx.x], hp = held[1][threadIdx.x];
+                const VR hr = held[0][threadIdx.x], hp = held[1][threadIdx.x];
 #pragma unroll
                 for (int q = 0; q < R; ++q) {
                     cur_r[q] = hr[q];
                     cur_p[q] = hp[q];
                 }
-                held[0][threadIdx.x] = f32x4{rn[0], rn[1], rn[2], rn[3]};
+                held[0][threadIdx.x] = mkR(rn);
                 held[1][threadIdx.x] = L.xp;
             }
             if (threadIdx.x < 16) {
@@ -1039,11 +1118,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
                     advance(L.cr, L.ca, L.cp, rs);
             }
             {
-                f32x4 xn;
+                float xn[R];
 #pragma unroll
                 for (int q = 0; q < R; ++q) xn[q] = fmaf(alpha * L.so[q], cur_p[q], L.xo[q]);
-                buf_st4(brx, ib, xn, !valid || first);
-                buf_st4(bwr, ib, f32x4{cur_r[0], cur_r[1], cur_r[2], cur_r[3]}, !valid || first);
+                stR(brx, ib, mkR(xn), !valid || first);
+                stR(bwr, ib, mkR(cur_r), !valid || first);
             }
             if (valid) {
 #pragma unroll
@@ -1080,8 +1159,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
                 }
             }
         }
-        buf_st4(bwp, ib, f32x4{pc[0], pc[1], pc[2], pc[3]}, !valid);
-        buf_st4(bwa, ib, f32x4{yv[0], yv[1], yv[2], yv[3]}, !valid);
+        stR(bwp, ib, mkR(pc), !valid);
+        stR(bwa, ib, mkR(yv), !valid);
     }
     store_partials<5>(v, dpf + (size_t)((seq + 1) & 1) * 5 * kMaxGrid);
     if (lead && mode != 2) sc->fh[(seq + 1) & 1] = hout;
@@ -1434,6 +1513,25 @@ __global__ __launch_bounds__(kBlock) void k_scale16(const float* __restrict__ A,
                 h[j / 2] = h0 | (h1 << 16);
             }
             *reinterpret_cast<u32x2*>(A16 + (size_t)k * pitch16 + ib) = u32x2{h[0], h[1]};
+        }
+    }
+}
+
+// The FP32 form of the symmetric scaled operator (the fused FP32 iteration):
+// the 12 upper bands (a_ik s_i) s_{i+o} of As = S A S, band 13 + m at
+// U + m * upitch, 0 where i + o_k leaves the vector.
+__global__ __launch_bounds__(kBlock) void k_scale32(const float* __restrict__ A, size_t pitch, int W, int N,
+                                                    float* __restrict__ U, size_t upitch,
+                                                    const float* __restrict__ svec) {
+#pragma clang fp contract(off)
+    for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long)gridDim.x * kBlock) {
+        const float si = svec[i];
+#pragma unroll
+        for (int m = 0; m < 12; ++m) {
+            const int k = 13 + m;
+            const long n = i + (long)(k / 5 - 2) * W + (k % 5 - 2);  // > i
+            const float sn = n < N ? svec[n] : 0.f;
+            U[(size_t)m * upitch + i] = (A[(size_t)k * pitch + i] * si) * sn;
         }
     }
 }
@@ -1805,8 +1903,8 @@ static gm_status ws_get(gm_stream_t s, long n, int pad, bool need16, Workspace**
 
 // The fused iteration's second residual and product buffers (N floats each)
 // and the edge array of a W x H operator.
-static gm_status ws_fused(Workspace* w, long N, int W) {
-    const long ne = W > 0 ? (long)((W + 255) / 256) * (N / W) * 24 : 0;
+static gm_status ws_fused(Workspace* w, long N, int W, int fmt) {
+    const long ne = fused_edge_words(fmt, W, N);
     if (w->cape < ne) {
         if (w->uedge) (void)hipFree(w->uedge);
         w->uedge = nullptr;
@@ -1892,12 +1990,20 @@ static thread_local bool g_tune_spmv = false;  // the last gm_autotune on this t
 static std::mutex g_tune_mu;
 static std::map<TuneKey, TuneVal> g_tune;
 
+// the fused FP32 iteration's layout: the symmetric scaled operator of a
+// W % 4 == 0 image, 4-row tiles of 128 columns
+static bool fused32_ok(int W, long N) { return sym16_ok(W, N) && N < kFused32MaxN; }
+
 static TuneVal tune_lookup(int fmt, int W, long N) {
     std::lock_guard<std::mutex> g(g_tune_mu);
     auto it = g_tune.find(TuneKey{fmt, W, N});
     if (it != g_tune.end()) return it->second;
     // defaults: the shapes the tuner picks for the 3840x2160 Levin matrix
-    if (fmt == kF32) return shape_ok(kS2R4Y4, fmt, W, N) ? TuneVal{kS2R4Y4, 2, 0, 0} : TuneVal{kS1R4, 4, 0, 0};
+    // FP32: the fused iteration on the symmetric FP32 form where its layout
+    // applies (W % 4 == 0), else the Jacobi step + update
+    if (fmt == kF32)
+        return shape_ok(kS2R4Y4, fmt, W, N) ? TuneVal{kS2R4Y4, 2, 0, 0, fused32_ok(W, N) ? 2 : 0}
+                                            : TuneVal{kS1R4, 4, 0, 0};
     return sym16_ok(W, N) ? TuneVal{kS2R4Y4, 2, 0, 0, 2} : TuneVal{kS1R8, 3, 0, 0, 0};
 }
 
@@ -2025,7 +2131,11 @@ static void launch_scale16(Solver& S) {
     hipLaunchKernelGGL(k_svec, dim3(elementwise_grid(S.dv, S.N, 1)), dim3(kBlock), 0, S.hs, A->a, A->apitch, S.N,
                        w->dinv);
     const int grid = elementwise_grid(S.dv, ((long)S.N + 3) / 4, 1);
-    if (S.fmt == kF16) {
+    if (S.fmt == kF32) {
+        // the FP32 upper bands in the 16-bit operator's buffer (48 of its 50 bytes per row)
+        hipLaunchKernelGGL(k_scale32, dim3(elementwise_grid(S.dv, S.N, 1)), dim3(kBlock), 0, S.hs, A->a, A->apitch,
+                           S.plan.W, S.N, reinterpret_cast<float*>(w->A16), w->pitch16, w->dinv);
+    } else if (S.fmt == kF16) {
         if (S.sym)
             hipLaunchKernelGGL((k_scale16<kF16, true>), dim3(grid), dim3(kBlock), 0, S.hs, A->a, A->apitch,
                                S.plan.W, S.N, w->A16, w->pitch16, w->dinv);
@@ -2042,12 +2152,18 @@ static void launch_scale16(Solver& S) {
     }
 }
 
-// the fused iteration's edge array of the operator just scaled (256-wide columns)
+// the fused iteration's edge array of the operator just scaled (columns of
+// 256 for 16-bit bands, of 128 for FP32)
 static void launch_uedge(Solver& S) {
     const int H = S.N / S.plan.W;
-    const long total = (long)((S.plan.W + 255) / 256) * H * 24;
-    hipLaunchKernelGGL(k_uedge, dim3(elementwise_grid(S.dv, total, 1)), dim3(kBlock), 0, S.hs, S.w->A16,
-                       S.w->pitch16, S.plan.W, H, S.N, 256, S.w->uedge);
+    const long total = fused_edge_words(S.fmt, S.plan.W, S.N);
+    if (S.fmt == kF32)
+        hipLaunchKernelGGL(k_uedge32, dim3(elementwise_grid(S.dv, total, 1)), dim3(kBlock), 0, S.hs,
+                           reinterpret_cast<const float*>(S.w->A16), S.w->pitch16, S.plan.W, H, S.N, 128,
+                           S.w->uedge);
+    else
+        hipLaunchKernelGGL(k_uedge, dim3(elementwise_grid(S.dv, total, 1)), dim3(kBlock), 0, S.hs, S.w->A16,
+                           S.w->pitch16, S.plan.W, H, S.N, 256, S.w->uedge);
 }
 
 template <int TY, int FMT>
@@ -2159,7 +2275,8 @@ static gm_status launch_update(Solver& S) {
 template <int FMT>
 static void launch_fused_t(Solver& S, int mode) {
     Workspace* w = S.w;
-    hipLaunchKernelGGL((k_cgfused_symw<FMT>), dim3(S.g_step), dim3(kBlock), 0, S.hs, w->A16, w->pitch16, S.plan.W,
+    const void* up = FMT == kF32 ? static_cast<const void*>(w->A16) : static_cast<const void*>(w->A16 + 13 * w->pitch16);
+    hipLaunchKernelGGL((k_cgfused_symw<FMT>), dim3(S.g_step), dim3(kBlock), 0, S.hs, up, w->pitch16, S.plan.W,
                        S.N / S.plan.W, S.N, w->dinv, S.x, w->r, w->r2, w->p0, w->p1, w->Ap, w->Ap2, w->uedge,
                        w->sc, w->dpart + 3 * kMaxGrid, S.fseq, mode, S.verify);
     ++S.fseq;
@@ -2168,7 +2285,8 @@ static void launch_fused_t(Solver& S, int mode) {
 // one fused launch (mode as k_cgfused_symw)
 static gm_status launch_fused(Solver& S, int mode) {
     if (S.fmt == kF16) launch_fused_t<kF16>(S, mode);
-    else launch_fused_t<kBF16>(S, mode);
+    else if (S.fmt == kBF16) launch_fused_t<kBF16>(S, mode);
+    else launch_fused_t<kF32>(S, mode);
     GM_LAUNCH_CHECK();
     return GM_OK;
 }
@@ -2355,12 +2473,20 @@ gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, c
         return fail(GM_ERR_UNSUPPORTED, "refine_mode 2 needs CG/PCG");
     if ((o->method == GM_METHOD_PCG || mixed) && S.plan.args.center < 0)
         return fail(GM_ERR_INVALID, "solve: Jacobi scaling needs a band with offset 0");
-    S.pm = mixed ? kScaled : (o->method == GM_METHOD_PCG ? kJacobi : kPlain);
     S.fmt = o->precision == GM_PREC_FP16 ? kF16 : (o->precision == GM_PREC_BF16 ? kBF16 : kF32);
-    S.sym = mixed && S.tiled && sym16_ok(S.plan.W, S.N);
+    const TuneVal tv = tune_lookup(S.fmt, S.plan.W, S.N);
+    S.vec = (reinterpret_cast<uintptr_t>(x) & 15u) == 0;
+    // FP32 Jacobi-PCG on a W % 4 == 0 Levin matrix, when the tuned (or
+    // default) form is the fused one: CG on the symmetric scaled FP32 operator
+    // S A S (upper bands), as the 16-bit forms, with the FP64 residual
+    // replacement through A itself
+    const bool f32fused = !mixed && o->method == GM_METHOD_PCG && S.tiled && S.N > 0 &&
+                          fused32_ok(S.plan.W, S.N) && tv.walk == 2 && tv.shape == kS2R4Y4 && S.vec;
+    S.pm = (mixed || f32fused) ? kScaled : (o->method == GM_METHOD_PCG ? kJacobi : kPlain);
+    S.sym = (mixed && S.tiled && sym16_ok(S.plan.W, S.N)) || f32fused;
     if ((st = dev_info(&S.dv)) != GM_OK) return st;
     if ((st = red_scratch(s, &S.rs)) != GM_OK) return st;
-    if ((st = ws_get(s, S.N > 0 ? S.N : 1, S.plan.maxoff, mixed, &S.w)) != GM_OK) return st;
+    if ((st = ws_get(s, S.N > 0 ? S.N : 1, S.plan.maxoff, mixed || f32fused, &S.w)) != GM_OK) return st;
     if (info) {
         memset(info, 0, sizeof(*info));
         info->true_relres = -1;
@@ -2370,20 +2496,20 @@ gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, c
     S.g_tile8 = grid_for_tiles(S.dv, (S.N + Tile<8>::ROWS - 1) / Tile<8>::ROWS, 3);
     S.g_elem = elementwise_grid(S.dv, S.N, 4);
     S.g_gen = elementwise_grid(S.dv, S.N, 1);
-    S.vec = (reinterpret_cast<uintptr_t>(x) & 15u) == 0;
     S.g_upd = elementwise_grid(S.dv, (S.N + 3) / 4, 2);
     {
-        const TuneVal tv = tune_lookup(S.fmt, S.plan.W, S.N);
         if (S.tiled) set_step_shape(S, tv.shape, tv.per_cu);
         S.walk = tv.walk;
-        // the fused iteration: symmetric 16-bit operator, 4-row column walk,
-        // 16-byte aligned x
-        S.fused = S.sym && tv.walk == 2 && S.shape == kS2R4Y4 && S.N < kFusedMaxN && S.vec &&
+        // the fused iteration: symmetric 16-bit (or FP32) operator, 4-row
+        // column walk, 16-byte aligned x
+        S.fused = S.sym && tv.walk == 2 && S.shape == kS2R4Y4 && S.N < fused_max_n(S.fmt) && S.vec &&
                   o->method != GM_METHOD_GD;
         if (tv.upd_pc > 0) S.g_upd = grid_for_tiles(S.dv, (S.N + 4L * kBlock - 1) / (4L * kBlock), tv.upd_pc);
     }
+    // the scaled FP32 operator exists only in the fused iteration's form
+    if (S.fmt == kF32 && S.pm == kScaled && !S.fused) return fail(GM_ERR_INVALID, "solve: internal: FP32 scaled form");
     Workspace* w = S.w;
-    if (S.fused && (st = ws_fused(w, S.N, S.plan.W)) != GM_OK) return st;
+    if (S.fused && (st = ws_fused(w, S.N, S.plan.W, S.fmt)) != GM_OK) return st;
     if (!S.tiled) {
         // generic kernels read r, p and D^-1 at i + offset up to N + pad: clear
         // whatever a larger earlier solve left beyond N in this workspace
@@ -2416,10 +2542,10 @@ gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, c
     }
 
     const int chunk = o->check_every > 0 ? o->check_every : 16;
-    // FP64 residual replacement period: 16-bit forms and refine mode 2 default
-    // to 50, plain FP32 to none
+    // FP64 residual replacement period: the scaled forms (16-bit, fused FP32)
+    // and refine mode 2 default to 50, the Jacobi FP32 step to none
     const int refine =
-        o->refine_every > 0 ? o->refine_every : (o->refine_every == 0 && (mixed || S.x64) ? 50 : 0);
+        o->refine_every > 0 ? o->refine_every : (o->refine_every == 0 && (S.pm == kScaled || S.x64) ? 50 : 0);
     long enq = 0;  // iterations enqueued
     int polls = 0;
     bool finished = false;
@@ -2459,6 +2585,7 @@ gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, c
                 if ((st = launch_cg_iter(S, restart_next || refine_next)) != GM_OK) return st;
                 ++enq;
                 ++since_restart;
+                bool restarted = false;
                 if (o->restart > 0 && since_restart >= o->restart && enq < o->max_iter) {
                     // cgSolve's periodic restart (main.cu:483-493): r from scratch, p = -z
                     // (the scaled 16-bit system's residual is s (A x - b): the
@@ -2466,8 +2593,12 @@ gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, c
                     if ((st = (S.pm == kScaled || S.x64) ? launch_trueres(S, 2) : launch_resid(S, 0)) != GM_OK)
                         return st;
                     since_restart = 0;
+                    restarted = true;
                 }
-                if (refine > 0 && enq % refine == 0) {
+                // the FP64 residual replacement, unless a restart has just formed
+                // the residual from scratch (a replacement keeps the direction,
+                // and would undo the restart's p = -z)
+                if (refine > 0 && enq % refine == 0 && !restarted) {
                     if ((st = launch_trueres(S, 1)) != GM_OK) return st;
                 }
             }
@@ -2675,7 +2806,8 @@ gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_re
     S.fmt = precision == GM_PREC_FP16 ? kF16 : (precision == GM_PREC_BF16 ? kBF16 : kF32);
     if ((st = dev_info(&S.dv)) != GM_OK) return st;
     if ((st = red_scratch(s, &S.rs)) != GM_OK) return st;
-    if ((st = ws_get(s, S.N, S.plan.maxoff, mixed, &S.w)) != GM_OK) return st;
+    const bool f32fused = !mixed && fused32_ok(S.plan.W, S.N);
+    if ((st = ws_get(s, S.N, S.plan.maxoff, mixed || f32fused, &S.w)) != GM_OK) return st;
     Workspace* w = S.w;
     S.g_elem = elementwise_grid(S.dv, S.N, 4);
     hipLaunchKernelGGL(k_fill, dim3(S.g_elem), dim3(kBlock), 0, S.hs, w->r, S.N, 1.0f);
@@ -2757,7 +2889,7 @@ gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_re
     if (S.sym && best.walk && best.shape == kS2R4Y4 && S.N < kFusedMaxN) {
         // the fused iteration (timing mode: fixed alpha and beta) against the
         // best step + update
-        if ((st = ws_fused(w, S.N, S.plan.W)) != GM_OK) return st;
+        if ((st = ws_fused(w, S.N, S.plan.W, S.fmt)) != GM_OK) return st;
         launch_uedge(S);
         hipLaunchKernelGGL(k_fill, dim3(S.g_elem), dim3(kBlock), 0, S.hs, w->Ap, S.N, 0.25f);
         const int step_pc = best.per_cu;
@@ -2792,6 +2924,47 @@ gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_re
         } else {
             best.per_cu = step_pc;
         }
+    }
+    if (f32fused) {
+        // the fused FP32 iteration on the symmetric scaled form (timing mode)
+        // against the best Jacobi step + update
+        launch_scale16(S);
+        if ((st = ws_fused(w, S.N, S.plan.W, S.fmt)) != GM_OK) return st;
+        launch_uedge(S);
+        hipLaunchKernelGGL(k_fill, dim3(S.g_elem), dim3(kBlock), 0, S.hs, w->Ap, S.N, 0.25f);
+        GM_LAUNCH_CHECK();
+        const TuneVal step_best = best;
+        int fused_pc = 0;
+        S.walk = 2;
+        for (int pc : {2, 3}) {
+            set_step_shape(S, kS2R4Y4, pc);
+            double tot = 0;
+            for (int k = 0; k < reps + 1; ++k) {
+                hipLaunchKernelGGL(k_tune_reset, dim3(1), dim3(64), 0, S.hs, w->sc);
+                GM_HIP(hipEventRecord(e0, S.hs));
+                if ((st = launch_fused(S, 2)) != GM_OK) return st;
+                GM_HIP(hipEventRecord(e1, S.hs));
+                GM_HIP(hipEventSynchronize(e1));
+                float ms = 0;
+                GM_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (k > 0) tot += ms;
+            }
+            const double us = tot * 1e3 / reps;
+            snprintf(line, sizeof(line), "cgfused %d %d %d %d %.2f FP32 symmetric column-walk (step + update %.2f)\n",
+                     (int)kS2R4Y4, 2, 4, pc, us, step_best.us + best_upd);
+            g_tune_report += line;
+            if (fused_pc == 0 || us < fused_us) {
+                fused_us = us;
+                fused_pc = pc;
+            }
+        }
+        if (fused_us < step_best.us + best_upd) {
+            best.shape = kS2R4Y4;
+            best.walk = 2;
+            best.per_cu = fused_pc;
+        }
+        S.walk = best.walk;
+        set_step_shape(S, best.shape, best.per_cu);
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -2849,8 +3022,10 @@ gm_status gm_tune_set_step(int W, int N, int precision, int rows_per_thread, int
         return fail(GM_ERR_INVALID, "gm_tune_set: unsupported shape");
     if (column_walk < 0 || column_walk > 2)
         return fail(GM_ERR_INVALID, "gm_tune_set_step: column_walk must be 0, 1 or 2");
-    if (column_walk != 0 && (fmt == kF32 || !sym16_ok(W, N)))
+    if (column_walk == 1 && (fmt == kF32 || !sym16_ok(W, N)))
         return fail(GM_ERR_INVALID, "gm_tune_set_step: the column walk is a form of the symmetric 16-bit step");
+    if (column_walk == 2 && (fmt == kF32 ? !fused32_ok(W, N) : !sym16_ok(W, N)))
+        return fail(GM_ERR_INVALID, "gm_tune_set_step: the fused iteration needs the symmetric layout (W %% 4 == 0)");
     if (column_walk == 2 && sh != kS2R4Y4)
         return fail(GM_ERR_INVALID, "gm_tune_set_step: the fused iteration walks 4-row tiles");
     std::lock_guard<std::mutex> g(g_tune_mu);

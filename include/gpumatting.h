@@ -195,7 +195,10 @@ typedef struct gm_tune_result_s {
    int step_walk;          /* 16-bit symmetric step: 1 = column-walk form (workgroups walk down
                               columns of tiles and keep the rows shared with the tile above in LDS);
                               2 = column walk with the whole iteration in one launch (the update of
-                              x and r fused into the next step; 4-row tiles) */
+                              x and r fused into the next step; 4-row tiles).  FP32 Jacobi-PCG with
+                              W % 4 == 0: 2 = the same one-launch iteration on the symmetric scaled
+                              FP32 operator S L S (12 upper bands, FP64 residual replacement through
+                              L every 50 iterations), 0 = the Jacobi step + update on L's 25 bands */
    double fused_us;        /* that one-launch iteration's time (0: not timed) */
 } gm_tune_result;
 gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_result* out, gm_stream_t stream);
@@ -208,7 +211,8 @@ const char* gm_autotune_report(void);
 gm_status gm_tune_set(int W, int N, int precision, int rows_per_thread, int tile_rows, int wg_per_cu);
 /* As gm_tune_set, also choosing the symmetric 16-bit step's column-walk form
  * (column_walk = 1; 16-bit with W % 4 == 0 only), or that walk with one
- * launch per CG iteration (column_walk = 2; tile_rows 4 only). */
+ * launch per CG iteration (column_walk = 2; tile_rows 4 and W % 4 == 0 only;
+ * for FP32 the fused iteration on the symmetric scaled FP32 operator). */
 gm_status gm_tune_set_step(int W, int N, int precision, int rows_per_thread, int tile_rows, int wg_per_cu,
                            int column_walk);
 

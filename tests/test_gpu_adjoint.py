@@ -77,7 +77,12 @@ def test_autograd_vs_central_differences_512(gpu):
     image of the differentiation workload against FP64 central differences
     (h = 1e-4) at 256 coordinates: 192 RGB channels and 64 constraint weights
     (tests/golden/fd_grad_512.npz, written by tests/golden/make_fd.py from the
-    oracle).  Relative 2-norm error per group and per-coordinate agreement."""
+    oracle).  Relative 2-norm error per group and per-coordinate agreement.
+    The lam derivative of a constrained pixel, mu_i (t_i - alpha_i) with
+    alpha_i ~ t_i, is a cancelling difference that the FP32 Laplacian's
+    rounding alone moves by ~1e-4 (exact FP64 solves on the FP32 accurate
+    assembly: 1.2e-4 from the central differences; on the FP64 Laplacian:
+    2.6e-6), so its bound is 3e-4 and the solves go to 1e-10."""
     import os
     T = torch
     g = np.load(os.path.join(os.path.dirname(__file__), "golden", "fd_grad_512.npz"))
@@ -87,7 +92,7 @@ def test_autograd_vs_central_differences_512(gpu):
     lam = T.from_numpy(p.lam).cuda().requires_grad_(True)
     tgt = T.from_numpy(p.target).cuda()
     gt = T.from_numpy(p.alpha_gt).cuda()
-    alpha = autograd.matte(im, lam, tgt, 1e-7, 1e-9)
+    alpha = autograd.matte(im, lam, tgt, 1e-7, 1e-10)
     ((alpha - gt) ** 2).sum().backward()
     gim = im.grad.cpu().numpy().astype(np.float64)
     glam = lam.grad.cpu().numpy().astype(np.float64).ravel()
@@ -102,5 +107,5 @@ def test_autograd_vs_central_differences_512(gpu):
         excess = np.abs(got - fd) / (2e-3 * np.abs(fd) + 3e-4 * np.abs(fd).max())
         print(f"\n[fd 512^2] {name}: {fd.size} coordinates, relative 2-norm error {rel:.2e}, "
               f"worst |got - fd| / bound {excess.max():.2f}")
-        assert rel <= 1e-4, (name, rel)
+        assert rel <= (1e-4 if name == "rgb" else 3e-4), (name, rel)
         assert excess.max() <= 1.0, (name, int(np.argmax(excess)), excess.max())

@@ -467,11 +467,12 @@ def test_column_walk_step_matches_tile_parallel(gpu, W, H, ty):
     assert np.abs(outs[1] - outs[0]).max() <= 1e-5 * scale
 
 
-def _fp16_emulation(p):
+def _fp16_emulation(p, store=np.float16):
     """Host FP64 emulation of the scaled 16-bit operator: As = S A S with S =
-    1/sqrt(a_ii), entries rounded to FP16 (RNE) -- for the symmetric layout
-    (W % 4 == 0) the upper entry of each pair serves both -- plus the FP32
-    matrix A itself (as a sparse matrix), S and b."""
+    1/sqrt(a_ii), entries (a_ik s_i) s_j in float, then rounded to `store`
+    (FP16 RNE; float32: the fused FP32 iteration's operator) -- for the
+    symmetric layout (W % 4 == 0) the upper entry of each pair serves both --
+    plus the FP32 matrix A itself (as a sparse matrix), S and b."""
     import scipy.sparse as sp
     W, N = p.W, p.N
     a, b, _ = O.levin_gather_acc_f32(p.image, p.lam, p.target, 1e-7)
@@ -492,7 +493,7 @@ def _fp16_emulation(p):
         else:
             j = np.clip(i + o, 0, N - 1)
             v = (a[k][:N] * s32) * s32[j]
-        v16 = v.astype(np.float16).astype(np.float64)
+        v16 = v.astype(store).astype(np.float64)
         rows.append(i[m]), cols.append(i[m] + o), vals.append(v16[m])
     As = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(N, N))
     As = As + sp.identity(N)
@@ -557,3 +558,78 @@ def test_fp16_restart_matches_emulation(gpu, walk):
         xe = _scaled_cg(As, A, sv, b, k, restart=25)
         assert np.abs(x.numpy() - xe).max() <= 1e-5 * np.abs(xe).max(), k
     call("gm_tune_set_step", p.W, p.N, PREC_FP16, 4, 4, 2, 2)
+
+
+FUSED32_SIZES = [(256, 192), (1000, 37), (1916, 203), (3840, 64), (16640, 8), (512, 3), (8, 64)]
+
+
+@pytest.mark.parametrize("W,H", FUSED32_SIZES)
+def test_fp32_fused_matches_jacobi_step(gpu, W, H):
+    """FP32 Jacobi-PCG as one launch per iteration -- CG on the symmetric
+    scaled FP32 operator S A S (12 upper bands, 128-wide column walk) with the
+    FP64 residual replacement through A -- makes the iterates of the two-kernel
+    Jacobi step + update on A's 25 bands (the same method: x_k of Jacobi-PCG
+    and of CG on S A S coincide in exact arithmetic): fixed-count runs agree to
+    rounding, and runs to 1e-6 reach it in about as many iterations.  Sizes as
+    for the 16-bit fused iteration (W % 4 == 0): ragged tiles, columns with
+    runs of one tile, 130 columns, 3 rows, 8 pixels wide."""
+    p = synth.make_problem(W, H, 4321, 1, 0.08)
+    s = _gpu_system(p)
+    res = {}
+    for walk, pc in ((0, 3), (2, 2), (2, 3)):
+        call("gm_tune_set_step", W, p.N, PREC_FP32, 4, 4, pc, walk)
+        x8, _ = matting.solve(s, method=METHOD_PCG, precision=PREC_FP32, tol=0, max_iter=8, verify=False,
+                              refine_every=-1)
+        x, info = matting.solve(s, method=METHOD_PCG, precision=PREC_FP32, tol=1e-6, max_iter=5000, refine_every=7)
+        res[(walk, pc)] = (x8.numpy(), x.numpy(), info)
+    call("gm_tune_set_step", W, p.N, PREC_FP32, 4, 4, 2, 2)
+    x8r, xr, ir = res[(0, 3)]
+    scale = np.abs(x8r).max()
+    assert scale > 0 and ir.status == 0
+    for key in ((2, 2), (2, 3)):
+        x8, x, info = res[key]
+        print(f"\n[{W}x{H} FP32 fused {key}] {info.iterations} vs {ir.iterations} iterations, "
+              f"{info.replacements} replacements, true relres {info.true_relres:.3e}", end="")
+        assert np.abs(x8 - x8r).max() <= 1e-5 * scale
+        assert info.status == 0 and info.true_relres <= 1e-6 and info.replacements >= 1
+        assert abs(info.iterations - ir.iterations) <= max(3, 0.05 * ir.iterations)
+        assert np.abs(x - xr).max() <= 1e-3
+
+
+@pytest.mark.parametrize("W,H", [(256, 96), (1000, 37), (8, 64)])
+def test_fp32_fused_operator_matches_emulation(gpu, W, H):
+    """The fused FP32 iteration's operator is the host emulation's symmetric
+    scaled FP32 one (upper entry of each pair for both, unit diagonal): k
+    fixed-count iterations without replacement against FP64 CG on it.  A
+    wrong lower index, sign or missing band moves x far more than 1e-5."""
+    p = synth.make_problem(W, H, 77, 1, 0.08)
+    As, A, sv, b = _fp16_emulation(p, np.float32)
+    s = _gpu_system(p)
+    call("gm_tune_set_step", W, p.N, PREC_FP32, 4, 4, 2, 2)
+    for k in (2, 5, 10):
+        x, _ = matting.solve(s, method=METHOD_PCG, precision=PREC_FP32, tol=0, max_iter=k, verify=False,
+                             refine_every=-1)
+        xe = _scaled_cg(As, A, sv, b, k)
+        assert np.abs(x.numpy() - xe).max() <= 1e-5 * np.abs(xe).max(), k
+
+
+def test_fp32_fused_restart_and_iteration_cap(gpu, small):
+    """The fused FP32 form through cgSolve's periodic restart and through the
+    iteration cap: the status, iteration count and final residual of the
+    Jacobi step + update."""
+    p = small[0]
+    s = _gpu_system(p)
+    out = {}
+    for walk, pc in ((0, 3), (2, 2)):
+        call("gm_tune_set_step", p.W, p.N, PREC_FP32, 4, 4, pc, walk)
+        x, info = matting.solve(s, method=METHOD_PCG, precision=PREC_FP32, tol=1e-6, max_iter=5000, restart=25)
+        xc, ic = matting.solve(s, method=METHOD_PCG, precision=PREC_FP32, tol=1e-12, max_iter=37)
+        out[walk] = (x.numpy(), info, xc.numpy(), ic)
+    call("gm_tune_set_step", p.W, p.N, PREC_FP32, 4, 4, 2, 2)
+    (xa, ia, xca, ica), (xb, ib, xcb, icb) = out[0], out[2]
+    assert ia.status == ib.status == 0 and ib.true_relres <= 1e-6
+    assert abs(ia.iterations - ib.iterations) <= 3
+    assert np.abs(xa - xb).max() <= 1e-3
+    assert ica.status == icb.status == 6 and ica.iterations == icb.iterations == 37  # GM_ERR_NOT_CONVERGED
+    assert abs(ica.relres - icb.relres) <= 1e-3 * ica.relres
+    assert np.abs(xca - xcb).max() <= 1e-4
