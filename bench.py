@@ -8,7 +8,11 @@ constraints, eps 1e-7, lambda 100): GPU Laplacian assembly into the 25-band
 matrix, then Jacobi-PCG until the FP64-checked relative residual is <= 1e-6.
 Inputs (image, weights, targets) are resident in HBM before timing starts.
 `value` is the mixed-precision (FP16 diagonals, FP32 accumulate, FP64 residual
-replacement) ms per solve; the FP32 solve is timed the same way beside it.
+replacement) ms per solve; the FP32 solve is timed the same way beside it
+(`fp32_ms`: the tuner's FP32 form, one launch per iteration over the 12
+upper bands of the scaled operator, 84 B per row against the mixed form's 60 B),
+and so is the FP32 form that streams the stored 25 FP32 bands
+(`fp32_25band`: Jacobi step + update kernels, 148 B per row).
 The roofline object is the FP32 banded SpMV (y = A x, 25 diagonals), timed
 with HIP events on its launch stream; `roofline_solve_kernel` is the same
 figure for the kernel that dominates the timed mixed solve, the fused CG
@@ -397,6 +401,32 @@ def main():
         log(f"{name}: {r['ms']:.2f} ms/solve (wall {r['wall_ms']:.2f}), iterations {r['iterations']}, "
             f"true relres {r['true_relres']:.3e}")
 
+    # The FP32 form that stores and streams the drop-in's 25 FP32 bands (Jacobi
+    # step + update kernels, 148 B per row and iteration), beside the tuned
+    # FP32 form above (symmetric scaled operator, 12 FP32 bands, one launch per
+    # iteration, 84 B per row): the tuner's fastest 25-band step shape is
+    # pinned for this run, then the tuner's pick is restored.
+    plain = None
+    if "fp32" in tuned:
+        best = None
+        for ln in lib().gm_autotune_report().decode().splitlines():
+            f = ln.split()
+            if len(f) == 6 and f[0] == "cgstep":
+                if best is None or float(f[5]) < best[3]:
+                    best = (int(f[2]), int(f[3]), int(f[4]), float(f[5]))
+        ft = tuned["fp32"]
+        if best is not None:
+            from gpumatting_amd._lib import call as _call
+            _call("gm_tune_set_step", W, N, PREC_FP32, best[0], best[1], best[2], 0)
+            step(PREC_FP32)
+            plain = timed(PREC_FP32)
+            plain["step_shape"] = {"rows_per_thread": best[0], "tile_rows": best[1], "wg_per_cu": best[2],
+                                   "step_us": best[3]}
+            _call("gm_tune_set_step", W, N, PREC_FP32, ft["rows_per_thread"], ft["tile_rows"], ft["wg_per_cu"],
+                  ft["step_walk"])
+            log(f"fp32, 25 stored bands (step + update): {plain['ms']:.2f} ms/solve, "
+                f"iterations {plain['iterations']}, true relres {plain['true_relres']:.3e}")
+
     # alpha accuracy: against the same system solved to 1e-12 with an FP64
     # master x (outer FP64 residual, refine_mode 1), outside the timed regions
     s = matting.assemble(img, lam, tgt, p.eps, ASSEMBLY_ACCURATE, L=L)
@@ -513,6 +543,25 @@ def main():
                  "frac": round(fused_gbs / HBM_PEAK_GBS, 4), "traffic": ftraffic, "algorithmic_bytes": fused_bytes,
                  "us_per_launch": mt["fused_us"]}
         log(f"fused iteration: {mt['fused_us']:.1f} us/launch, {fused_gbs:.0f} GB/s algorithmic")
+    # the same figure for the FP32 solve's fused iteration (symmetric scaled
+    # FP32 operator: 12 x 4 B bands, r, Ap, p, x, s in; x, r, p, Ap out: 84 B per row)
+    fused32 = None
+    ft = tuned.get("fp32")
+    if ft and ft.get("step_walk") == 2 and ft.get("fused_us", 0) > 0:
+        f32_bytes = 84 * N
+        f32_gbs = f32_bytes / (ft["fused_us"] * 1e-6) / 1e9
+        f32_traffic = None
+        try:
+            with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+                t = json.load(f).get("gm::k_cgfused_symw<0>")
+            if t and t.get("workload", "").startswith(f"{W}x{H}"):
+                f32_traffic = t["bytes_per_launch"]
+        except (OSError, ValueError):
+            pass
+        fused32 = {"bound": "hbm", "kernel": "gm::k_cgfused_symw<0> (fused FP32 CG iteration, symmetric scaled operator)",
+                   "achieved": round(f32_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                   "frac": round(f32_gbs / HBM_PEAK_GBS, 4), "traffic": f32_traffic, "algorithmic_bytes": f32_bytes,
+                   "us_per_launch": ft["fused_us"]}
 
     mixed, fp32 = results["mixed"], results["fp32"]
     out = {
@@ -533,6 +582,11 @@ def main():
                    "width": W, "height": H, "unknowns": N, "tol": 1e-6},
         "fp32_ms": round(fp32["ms"], 3),
         "mixed_speedup_vs_fp32": round(fp32["ms"] / mixed["ms"], 3),
+        "fp32_25band": (None if plain is None else
+                        {"ms": round(plain["ms"], 3), "iterations": plain["iterations"],
+                         "true_relres": plain["true_relres"], "step_shape": plain["step_shape"],
+                         "mixed_speedup": round(plain["ms"] / mixed["ms"], 3),
+                         "form": "Jacobi-PCG streaming the drop-in's 25 FP32 bands (step + update kernels)"}),
         "iterations": {"mixed": mixed["iterations"], "fp32": fp32["iterations"]},
         "true_relres": {"mixed": mixed["true_relres"], "fp32": fp32["true_relres"]},
         "residual_replacements_mixed": mixed["replacements"],
@@ -550,6 +604,7 @@ def main():
                      "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes": spmv_bytes},
         "roofline_solve_kernel": fused,
+        "roofline_fp32_solve_kernel": fused32,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
