@@ -104,11 +104,18 @@ struct BImg {
     size_t pitch;
     const unsigned short* L16;
     const float* Lg;  // FP32 form: the off-diagonal bands 4 per pixel in 6 groups (kb_group32)
+    float* V;         // the systems' vectors: system j's block of kSysVecs vectors of pitch floats at V + j * kSysVecs * pitch
 };
 
 struct BSys {
     float *x, *r[2], *p[2], *Ap[2], *d, *b;
 };
+// A system's block: x, r[0], r[1], p[0], p[1], Ap[0], Ap[1], d, b in that order,
+// pitch floats each (BImg::V).  Images have fewer than 2^25 pixels, so a
+// block spans less than 2^31 bytes and one range-checked buffer descriptor
+// with 32-bit offsets covers it.
+constexpr int kSysVecs = 9;
+enum { kVx = 0, kVr = 1, kVp = 3, kVAp = 5, kVd = 7, kVb = 8 };
 
 // ---------------------------------------------------------------------------
 // slots and folds
@@ -369,53 +376,66 @@ __device__ __forceinline__ unsigned fused_enter(const BImg& I, int im, Scalars* 
     return mask;
 }
 
-// One system's vectors for the step from state `it` in mode `mode`.
-struct SysPtr {
-    const float *rin, *pold, *apin, *d;
-    float *rout, *pnew, *apout, *x;
+// One system's vectors for the step from state `it` in mode `mode`: a
+// range-checked buffer over the system's block (a load out of range reads 0,
+// a store out of range is dropped) and the byte offsets of the vectors in it,
+// all wave-uniform, so a cell's loads need no pointer table and no branch.
+struct SysBuf {
+    __amdgpu_buffer_rsrc_t rs;
+    int rin, rout, apin, apout, pold, pnew, x, d;
 };
-__device__ __forceinline__ SysPtr sys_ptr(const BSys& S, const SysStep& st) {
+__device__ __forceinline__ SysBuf sys_buf(const BImg& I, int j, const SysStep& st) {
     const int a = st.it & 1, b = a ^ 1;
-    SysPtr q;
-    q.rin = S.r[a];
-    q.rout = S.r[b];
-    q.apin = S.Ap[a];
-    q.d = S.d;
-    q.x = S.x;
+    const int P = 4 * (int)I.pitch;  // bytes per vector
+    SysBuf q;
+    q.rs = __builtin_amdgcn_make_buffer_rsrc(I.V + (size_t)j * kSysVecs * I.pitch, 0, kSysVecs * P, 0x00020000);
+    q.x = kVx * P;
+    q.rin = (kVr + a) * P;
+    q.rout = (kVr + b) * P;
+    q.apin = (kVAp + a) * P;
+    q.d = kVd * P;
     if (st.mode == kSysFirst) {  // p_j from p_{j-1} (the other buffer), written where p_j lives
-        q.pold = S.p[b];
-        q.pnew = S.p[a];
-        q.apout = S.Ap[a];
+        q.pold = (kVp + b) * P;
+        q.pnew = (kVp + a) * P;
+        q.apout = (kVAp + a) * P;
     } else {
-        q.pold = S.p[a];
-        q.pnew = S.p[b];
-        q.apout = S.Ap[b];
+        q.pold = (kVp + a) * P;
+        q.pnew = (kVp + b) * P;
+        q.apout = (kVAp + b) * P;
     }
     return q;
 }
+// Byte offset of element g of a vector, or an offset past every block (the
+// access then reads 0 or writes nothing).
+__device__ __forceinline__ unsigned sb_off(bool in, long long g) { return in ? 4u * (unsigned)g : 0x80000000u; }
+__device__ __forceinline__ float sb_ld(const SysBuf& q, unsigned off, int vec) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(q.rs, (int)off, vec, 0));
+}
+__device__ __forceinline__ void sb_st(const SysBuf& q, unsigned off, int vec, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), q.rs, (int)off, vec, 0);
+}
 
 // One staged window cell of one system: r, Ap, p_old, d (and x where the cell
-// is advanced here).
+// is advanced here).  in: the cell is a pixel of the image (else all 0).
 struct CellIn {
     float r, a, p, d, x;
 };
 
-__device__ __forceinline__ CellIn load_cell(const SysPtr& q, const SysStep& st, bool in, bool adv, unsigned o) {
-    CellIn v{0.f, 0.f, 0.f, 0.f, 0.f};
-    if (in) {
-        v.r = at_off(q.rin, o);
-        if (st.mode == kSysNormal) v.a = at_off(q.apin, o);
-        if (st.beta != 0.f || st.mode == kSysNormal) v.p = at_off(q.pold, o);
-        v.d = at_off(q.d, o);
-    }
-    if (adv && st.mode == kSysNormal) v.x = at_off(q.x, o);
+__device__ __forceinline__ CellIn load_cell(const SysBuf& q, const SysStep& st, bool in, bool adv, long long g) {
+    CellIn v;
+    const unsigned o = sb_off(in, g);
+    v.r = sb_ld(q, o, q.rin);
+    v.a = st.mode == kSysNormal ? sb_ld(q, o, q.apin) : 0.f;
+    v.p = (st.beta != 0.f || st.mode == kSysNormal) ? sb_ld(q, o, q.pold) : 0.f;
+    v.d = sb_ld(q, o, q.d);
+    v.x = st.mode == kSysNormal ? sb_ld(q, sb_off(adv, g), q.x) : 0.f;  // adv implies in
     return v;
 }
 
 template <int FMT, int NR, int WXT>
-__global__ __launch_bounds__(kBlock) void kb_fused(const BImg* __restrict__ imgs, const BSys* __restrict__ sys,
-                                                   Scalars* sc, BPart P, double* __restrict__ slots, size_t half,
-                                                   int k, int tune, int verify) {
+__global__ __launch_bounds__(kBlock) void kb_fused(const BImg* __restrict__ imgs, Scalars* sc, BPart P,
+                                                   double* __restrict__ slots, size_t half, int k, int tune,
+                                                   int verify) {
     using Gm = BGeom<WXT>;
     constexpr int TX = Gm::TX, TY = Gm::TY, WX = Gm::WX, WY = Gm::WY;
     __shared__ float wp[NR][WY * WX];  // p_{j+1} of the window cells
@@ -459,23 +479,21 @@ __global__ __launch_bounds__(kBlock) void kb_fused(const BImg* __restrict__ imgs
         const long long ng = (long long)(y0 - 2 + nwy) * I.W + (x0 - 2 + nwx);
         const bool nin = ng >= 0 && ng < I.N;
         const bool nadv = bplan::advances(nwy, nwx, x0, y0, TX, I.W, I.H);
-        const unsigned no = 4u * (unsigned)(nin ? ng : 0);
 #pragma unroll
         for (int j = 0; j < NR; ++j) {
             if (!((mask >> j) & 1u)) continue;
             const SysStep st = uniform_step(cst[j]);
-            const SysPtr q = sys_ptr(sys[I.sys0 + j], st);
-            cin[j] = load_cell(q, st, nin, nadv, no);
+            cin[j] = load_cell(sys_buf(I, j, st), st, nin, nadv, ng);
         }
         __syncthreads();  // the previous tile is done with the window
 #pragma unroll
         for (int j = 0; j < NR; ++j) {
             if (!((mask >> j) & 1u)) continue;
             const SysStep st = uniform_step(cst[j]);
-            const SysPtr q = sys_ptr(sys[I.sys0 + j], st);
+            const SysBuf q = sys_buf(I, j, st);
             // r_{j+1}, z_{j+1}, p_{j+1} of a cell into the window; x and r
-            // advanced where this tile advances the cell
-            auto step_cell = [&](const CellIn& v, bool in, bool adv, unsigned o, int li) {
+            // advanced where this tile advances the cell (adv implies in)
+            auto step_cell = [&](const CellIn& v, bool in, bool adv, long long g, int li) {
                 const float rn = st.mode == kSysNormal ? fmaf(st.alpha, v.a, v.r) : v.r;
                 float zn = inv_diag(v.d) * rn;
                 float pn = st.beta != 0.f ? fmaf(st.beta, v.p, -zn) : -zn;
@@ -483,10 +501,11 @@ __global__ __launch_bounds__(kBlock) void kb_fused(const BImg* __restrict__ imgs
                 if (adv) {
                     acc[j][1] += (double)rn * (double)zn;
                     acc[j][2] += (double)rn * (double)rn;
-                    if (st.mode == kSysNormal) {
-                        store_off(q.x, o, fmaf(st.alpha, v.p, v.x));
-                        store_off(q.rout, o, rn);
-                    }
+                }
+                if (st.mode == kSysNormal) {
+                    const unsigned o = sb_off(adv, g);
+                    sb_st(q, o, q.x, fmaf(st.alpha, v.p, v.x));
+                    sb_st(q, o, q.rout, rn);
                 }
                 wp[j][li] = pn;
                 wzd[j][li] = f32x2{zn, in ? v.d : 0.f};
@@ -497,12 +516,11 @@ __global__ __launch_bounds__(kBlock) void kb_fused(const BImg* __restrict__ imgs
                 const long long g = (long long)(y0 - 2 + wy) * I.W + (x0 - 2 + wx);
                 const bool in = g >= 0 && g < I.N;
                 const bool adv = bplan::advances(wy, wx, x0, y0, TX, I.W, I.H);
-                const unsigned o = 4u * (unsigned)(in ? g : 0);
-                step_cell(load_cell(q, st, in, adv, o), in, adv, o, ring_row(top + wy, WY) * WX + wx);
+                step_cell(load_cell(q, st, in, adv, g), in, adv, g, ring_row(top + wy, WY) * WX + wx);
             };
             if (!carry)
                 for (int e = (int)threadIdx.x; e < Gm::KEEP; e += kBlock) cell_now(e / WX, e % WX);
-            step_cell(cin[j], nin, nadv, no, ring_row(top + nwy, WY) * WX + nwx);
+            step_cell(cin[j], nin, nadv, ng, ring_row(top + nwy, WY) * WX + nwx);
         }
         __syncthreads();
         wx0 = x0;
@@ -518,7 +536,7 @@ __global__ __launch_bounds__(kBlock) void kb_fused(const BImg* __restrict__ imgs
 #pragma unroll
         for (int j = 0; j < NR; ++j) {
             if (!((mask >> j) & 1u)) continue;
-            const SysPtr q = sys_ptr(sys[I.sys0 + j], uniform_step(cst[j]));
+            const SysBuf q = sys_buf(I, j, uniform_step(cst[j]));
             const float* __restrict__ w = wp[j];
             float yv = 0.f;
 #pragma unroll
@@ -530,8 +548,8 @@ __global__ __launch_bounds__(kBlock) void kb_fused(const BImg* __restrict__ imgs
             const float pc = w[rb[2] + 2];
             const f32x2 zd = wzd[j][rb[2] + 2];
             yv = fmaf(zd.y, pc, yv);
-            store_off(q.pnew, oi, pc);
-            store_off(q.apout, oi, yv);
+            sb_st(q, oi, q.pnew, pc);
+            sb_st(q, oi, q.apout, yv);
             acc[j][0] += (double)(pc * yv);
             acc[j][3] += (double)yv * (double)zd.x;
             acc[j][4] += (double)yv * ((double)yv * (double)inv_diag(zd.y));
@@ -894,7 +912,7 @@ __global__ void kb_init(Scalars* sc, int nsys, int maxit) {
 // launch tables: kb_fused<FMT, NR, TX> for the batch's format, right-hand
 // sides rounded up to a power of two, and tile width
 
-typedef void (*FusedFn)(const BImg*, const BSys*, Scalars*, BPart, double*, size_t, int, int, int);
+typedef void (*FusedFn)(const BImg*, Scalars*, BPart, double*, size_t, int, int, int);
 constexpr int kTileW[3] = {60, 124, 252};  // tile widths (windows 64, 128, 256 wide)
 constexpr int kNumTileW = 3;
 
@@ -1091,10 +1109,12 @@ gm_status gm_batch_create(gm_batch** out, int nimages, const int* widths, const 
         return fail(GM_ERR_UNSUPPORTED, "gm_batch_create: batched forms are FP32, FP16 and BF16");
     for (int i = 0; i < nimages; ++i) {
         const int W = widths[i], H = heights[i];
-        // below 2^28 pixels: the grouped band loads address 16 bytes per pixel
-        // with a 32-bit byte offset
-        if (W < 1 || H < 1 || (long long)W * H >= (1LL << 28))
-            return fail(GM_ERR_INVALID, "gm_batch_create: image %d is %dx%d (1 to 2^28-1 pixels)", i, W, H);
+        // below 2^25 pixels: a system's block of 9 vectors spans less than
+        // 2^31 bytes (one range-checked buffer descriptor per system in the
+        // fused iteration), and the grouped band loads' 32-bit byte offsets
+        // (16 bytes per pixel) fit
+        if (W < 1 || H < 1 || (long long)W * H >= (1LL << 25))
+            return fail(GM_ERR_INVALID, "gm_batch_create: image %d is %dx%d (1 to 2^25-1 pixels)", i, W, H);
     }
     gm_batch* b = new gm_batch();
     b->nimg = nimages;
@@ -1122,7 +1142,7 @@ gm_status gm_batch_create(gm_batch** out, int nimages, const int* widths, const 
         lgoff[i] = Lgn;
         if (precision == GM_PREC_FP32) Lgn += 24 * pitch;
         voff[i] = vfloats;
-        vfloats += (size_t)nrhs * 9 * pitch;
+        vfloats += (size_t)nrhs * kSysVecs * pitch;
         b->himg.push_back(I);
     }
     hipError_t e = hipSuccess;
@@ -1151,8 +1171,9 @@ gm_status gm_batch_create(gm_batch** out, int nimages, const int* widths, const 
         I.L = b->Lmem + loff[i];
         I.L16 = L16n ? b->L16mem + l16off[i] : nullptr;
         I.Lg = Lgn ? b->Lgmem + lgoff[i] : nullptr;
+        I.V = b->vmem + voff[i];
         for (int j = 0; j < nrhs; ++j) {
-            float* base = b->vmem + voff[i] + (size_t)j * 9 * I.pitch;
+            float* base = I.V + (size_t)j * kSysVecs * I.pitch;
             BSys S;
             S.x = base;
             S.r[0] = base + 1 * I.pitch;
@@ -1257,8 +1278,8 @@ static double* slot3(gm_batch* b) { return b->dslot + 2 * b->fhalf; }
 static gm_status launch_fused(gm_batch* b, int k, int tune, int verify, hipStream_t hs) {
     FusedFn f = fused_fn(b->precision, b->nr, b->tile_w);
     if (!f) return fail(GM_ERR_UNSUPPORTED, "gm_batch: no %d-wide tiles for %d right-hand sides", b->tile_w, b->nr);
-    hipLaunchKernelGGL(f, dim3(b->pf.plan.G), dim3(kBlock), 0, hs, b->dimg, b->dsys, b->dsc, b->pf.dev(), fslots(b),
-                       b->fhalf, k, tune, verify);
+    hipLaunchKernelGGL(f, dim3(b->pf.plan.G), dim3(kBlock), 0, hs, b->dimg, b->dsc, b->pf.dev(), fslots(b), b->fhalf,
+                       k, tune, verify);
     GM_LAUNCH_CHECK();
     return GM_OK;
 }

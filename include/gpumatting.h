@@ -284,7 +284,7 @@ typedef struct gm_batch_s gm_batch;
 /* Allocate a batch: nimages images (widths[i] x heights[i]) with nrhs <= 8
  * constraint sets each; precision GM_PREC_FP32, GM_PREC_FP16 or GM_PREC_BF16
  * (16-bit off-diagonal bands, FP32 diagonals and vectors, FP64 residual
- * replacement).  Each image has 1 to 2^28 - 1 pixels (GM_ERR_INVALID
+ * replacement).  Each image has 1 to 2^25 - 1 pixels (GM_ERR_INVALID
  * otherwise, before any device call). */
 gm_status gm_batch_create(gm_batch** batch, int nimages, const int* widths, const int* heights, int nrhs,
                           int precision);

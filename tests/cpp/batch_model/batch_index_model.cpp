@@ -8,7 +8,7 @@
 //   * every LDS ring cell a pixel's product reads holds that pixel's
 //     neighbour (each stored cell is tagged with the window cell it holds);
 //   * every vector index is inside its image, every 16-byte band offset fits
-//     32 bits;
+//     32 bits, and a system's block of vectors stays below 2^31 bytes;
 //   * per launch, each pixel of each image is advanced (x, r) exactly once by
 //     the fused step and is an output cell exactly once; the true residual
 //     and the update-only chunks cover each pixel exactly once.
@@ -27,6 +27,7 @@
 using namespace gm::bplan;
 
 constexpr int kBlock = 256, kKeep = 4, kRTX = 128, kChunk = 4 * kBlock, kNQ = 5, kMaxRhs = 8, kMaxGrid = 4096;
+constexpr int kSysVecs = 9;  // vectors per system block (x, r0, r1, p0, p1, Ap0, Ap1, d, b)
 
 static long g_checks = 0, g_fail = 0;
 #define REQ(c, ...)                                     \
@@ -111,6 +112,16 @@ static void walk_tiles(const std::vector<int>& W, const std::vector<int>& H, con
                 const long long g = (long long)(y0 - 2 + wy) * Wi + (x0 - 2 + wx);
                 const bool in = g >= 0 && g < N;
                 if (in) REQ(4ull * (unsigned long long)g < (1ull << 32), "byte offset of %lld", g);
+                // kb_fused addresses a system's vectors as byte offsets into its
+                // block of kSysVecs vectors of `pitch` floats: the vector's
+                // offset plus the element's must stay inside the block, which
+                // must stay below 2^31 bytes (the sentinel offset 2^31 reads 0)
+                if (in) {
+                    const unsigned long long pitch = ((unsigned long long)N + 63) / 64 * 64;
+                    REQ(4ull * kSysVecs * pitch < (1ull << 31), "system block of %llu bytes", 4ull * kSysVecs * pitch);
+                    REQ(4ull * ((kSysVecs - 1) * pitch + (unsigned long long)g) < 4ull * kSysVecs * pitch,
+                        "element %lld beyond its vector", g);
+                }
                 if (fused && advances(wy, wx, x0, y0, TX, Wi, Hi)) {
                     REQ(in, "advanced cell %lld outside image %d", g, im);
                     if (in) adv[im][g] += 1;

@@ -58,9 +58,9 @@ def test_version_and_error_paths_without_gpu():
     x = np.zeros(6, np.float32)
     # jacobi_host with too little padding is rejected on the host
     assert L.gm_jacobi_host(x.ctypes.data + 4, C.byref(h), x.ctypes.data, 2, 0, C.c_float(0.5)) == 1
-    # batched images of 2^28 pixels or more are rejected before any device call
+    # batched images of 2^25 pixels or more are rejected before any device call
     out = C.c_void_p()
-    for w, h_ in ((16384, 16384), (1 << 20, 1 << 10), (0, 5)):
+    for w, h_ in ((16384, 16384), (1 << 20, 1 << 10), (8192, 4097), (0, 5)):
         W = np.array([64, w], np.int32)
         H = np.array([64, h_], np.int32)
         assert L.gm_batch_create(C.byref(out), 2, W.ctypes.data, H.ctypes.data, 4, 0) == 1
