@@ -5,8 +5,10 @@
 // A work list concatenates the items (tiles or chunks) of all images; a
 // launch splits it over G ranges, one per workgroup, contiguous and in list
 // order.  Workgroup b takes range range_of(b, G): the XCD of b (b % 8) takes
-// a contiguous eighth of the ranges, so neighbouring ranges -- which share
-// window rows at their ends -- run on one XCD and meet in its L2.  A range
+// blocks of kXcdBlock consecutive ranges, dealt to the XCDs in turn, so
+// neighbouring ranges -- which share window rows at their ends -- mostly run
+// on one XCD and meet in its L2, while an image that spans many ranges is
+// spread over every XCD (late in a solve only a few images still step).  A range
 // that touches several images keeps one slot of partial sums per image:
 // slot (r * K + im - wimg[r]), K = the most images one range touches.
 #pragma once
@@ -27,7 +29,18 @@ namespace bplan {
 
 constexpr int kXcd = 8;
 
-GM_HD inline int range_of(int b, int G) { return (G % kXcd) ? b : (b % kXcd) * (G / kXcd) + b / kXcd; }
+constexpr int kXcdBlock = 8;
+// A bijection of [0, G) when G % kXcd == 0 (else the identity): XCD x's l-th
+// workgroup takes range (l / B * kXcd + x) * B + l % B of the first
+// kXcd * B * (M / B) ranges (M = G / kXcd, B = kXcdBlock), and the XCDs
+// share the M % B ranges left at the end in contiguous runs.
+GM_HD inline int range_of(int b, int G) {
+    if (G % kXcd) return b;
+    const int M = G / kXcd, x = b % kXcd, l = b / kXcd, full = M / kXcdBlock * kXcdBlock;
+    if (l < full) return (l / kXcdBlock * kXcd + x) * kXcdBlock + l % kXcdBlock;
+    const int rem = M - full;
+    return kXcd * full + x * rem + (l - full);
+}
 GM_HD inline int part_begin(int r, int T, int G) { return (int)(((long long)r * T) / G); }
 GM_HD inline long long slot_index(int r, int K, int im, int wimg_r, int stride, int off) {
     return ((long long)r * K + (im - wimg_r)) * stride + off;
