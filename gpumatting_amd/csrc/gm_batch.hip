@@ -67,6 +67,7 @@ using bplan::ring_row;
 using bplan::slot_index;
 
 constexpr int kMaxRhs = 8;
+constexpr int kBatchMaxGrid = 16384;  // workgroups of a batched launch, at most
 constexpr int kNQ = 5;                           // sums per system of a fused launch: <p,Ap>, <r,z>, ||r||^2, <Ap,z>, <Ap,D^-1 Ap>
 constexpr int kRTX = 128, kRTY = kBlock / kRTX;  // true-residual tiles: 128 x 2
 constexpr int kChunk = 4 * kBlock;               // update-only chunks: 1024 pixels, 4 per thread
@@ -994,7 +995,7 @@ struct gm_batch_s {
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     int graph_iters = 0, graph_verify = -1;
     hipEvent_t ev = nullptr;
-    int per_cu = 16;   // workgroups per CU of the fused launch (gm_batch_set_grid, gm_batch_autotune)
+    int per_cu = 32;   // workgroups per CU of the fused launch (gm_batch_set_grid, gm_batch_autotune)
     int tile_w = 124;  // fused tile width (gm_batch_set_tiles, gm_batch_autotune)
     double tuned_us = 0;
 };
@@ -1033,12 +1034,14 @@ static void batch_free(gm_batch* b) {
 // Workgroups of a batched launch: per_cu per CU (more than fit at once: a
 // workgroup whose range lies in converged images ends at once and the
 // hardware starts the next, which balances the uneven work), at most one per
-// item, a multiple of the XCD count, at most kMaxGrid.
+// item, a multiple of the XCD count, at most kBatchMaxGrid (the batched
+// kernels' slot arrays are sized by the grid they run on: 8192 workgroups, 32
+// per CU, shorten the ranges the last images to converge are split into).
 static int batch_grid(const DevInfo& d, long items, int per_cu) {
     long g = std::min((long)d.cus * per_cu, items);
     g = std::max(g, 1L);
     g = (g + kNumXcd - 1) / kNumXcd * kNumXcd;
-    return (int)std::min<long>(g, kMaxGrid);
+    return (int)std::min<long>(g, kBatchMaxGrid);
 }
 
 static gm_status upload_part(HostPart& p) {

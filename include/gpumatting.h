@@ -306,10 +306,11 @@ gm_status gm_batch_solve(gm_batch* batch, const gm_solve_opts* opts, gm_stream_t
 gm_status gm_batch_alpha(gm_batch* batch, int image, int rhs, float** alpha);
 /* Per-system result of the last gm_batch_solve. */
 gm_status gm_batch_info(gm_batch* batch, int image, int rhs, gm_solve_info* info);
-/* Workgroups per CU of the batched launches, 1..64 (default 16: more than fit
- * at once, so workgroups whose ranges lie in converged images end early and
- * the hardware balances the rest; 4 per CU, one resident wave, is 10 % slower
- * on uneven batches). */
+/* Workgroups per CU of the batched launches, 1..64, at most 16384 in all
+ * (default 32: more than fit at once, so workgroups whose ranges lie in
+ * converged images end early and the hardware balances the rest, and the
+ * images that converge last are split into short ranges; 4 per CU, one
+ * resident wave, is 10 % slower on uneven batches). */
 gm_status gm_batch_set_grid(gm_batch* batch, int per_cu);
 /* Tile width of the fused iteration: 60 (60 x 4 tiles), 124 (124 x 2, the
  * default) or 252 (252 x 1; batches of up to 4 right-hand sides). */

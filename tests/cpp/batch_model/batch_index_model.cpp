@@ -26,7 +26,7 @@
 
 using namespace gm::bplan;
 
-constexpr int kBlock = 256, kKeep = 4, kRTX = 128, kChunk = 4 * kBlock, kNQ = 5, kMaxRhs = 8, kMaxGrid = 4096;
+constexpr int kBlock = 256, kKeep = 4, kRTX = 128, kChunk = 4 * kBlock, kNQ = 5, kMaxRhs = 8, kMaxGrid = 16384;
 constexpr int kSysVecs = 9;  // vectors per system block (x, r0, r1, p0, p1, Ap0, Ap1, d, b)
 
 static long g_checks = 0, g_fail = 0;

@@ -44,10 +44,10 @@ def test_edge_sizes(model, nrhs):
 
 def test_batched_workload_images(model):
     """The largest, smallest, widest and tallest images of the workload and two
-    more, in one batch, at 1 and 16 workgroups per CU (one range spanning
-    several images, and the default grid)."""
+    more, in one batch, at 1, 16 and 32 workgroups per CU (one range spanning
+    several images, a tuner candidate, and the default grid)."""
     plan = synth.config_batched_plan()
     n = [w * h for w, h, _ in plan]
     asp = [w / h for w, h, _ in plan]
     pick = sorted({n.index(max(n)), n.index(min(n)), asp.index(max(asp)), asp.index(min(asp)), 0, 33})
-    _run(model, 4, [1, 16], [plan[i][:2] for i in pick])
+    _run(model, 4, [1, 16, 32], [plan[i][:2] for i in pick])
