@@ -2649,9 +2649,11 @@ gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, c
         }
     }
     if (o->method != GM_METHOD_GD && (st = launch_fold(S)) != GM_OK) return st;
+    // the final state through the pinned slot, ordered on the solve's stream
+    // (a synchronous copy into pageable memory idled the GPU ~0.2 ms per solve)
+    GM_HIP(hipMemcpyAsync(&w->hsc[0], w->sc, sizeof(Scalars), hipMemcpyDeviceToHost, S.hs));
     GM_HIP(hipStreamSynchronize(S.hs));
-    Scalars fin;
-    GM_HIP(hipMemcpy(&fin, w->sc, sizeof(Scalars), hipMemcpyDeviceToHost));
+    Scalars fin = w->hsc[0];
     const double bb = fin.bb > 0 ? fin.bb : 1.0;
     gm_status result = status_of_done(fin.done);
     if (fin.done == kRun) result = (o->tol > 0) ? GM_ERR_NOT_CONVERGED : GM_OK;
@@ -2669,8 +2671,9 @@ gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, c
         // which x is not updated again: its g0 is already that of the final x.
         if (!(S.verify && fin.done == kConverged && fin.g0 >= 0)) {
             if ((st = launch_trueres(S, 0)) != GM_OK) return st;
+            GM_HIP(hipMemcpyAsync(&w->hsc[0], w->sc, sizeof(Scalars), hipMemcpyDeviceToHost, S.hs));
             GM_HIP(hipStreamSynchronize(S.hs));
-            GM_HIP(hipMemcpy(&fin, w->sc, sizeof(Scalars), hipMemcpyDeviceToHost));
+            fin = w->hsc[0];
         }
         info->true_relres = sqrt(fin.g0 / bb);
     }
