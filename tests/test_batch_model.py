@@ -51,3 +51,19 @@ def test_batched_workload_images(model):
     asp = [w / h for w, h, _ in plan]
     pick = sorted({n.index(max(n)), n.index(min(n)), asp.index(max(asp)), asp.index(min(asp)), 0, 33})
     _run(model, 4, [1, 16, 32], [plan[i][:2] for i in pick])
+
+
+def test_range_map_is_a_bijection(tmp_path):
+    """range_of (workgroup -> range of a batched launch) is a bijection for
+    every grid size up to 20000, gives each XCD an equal share, and keeps each
+    block of consecutive ranges on one XCD."""
+    if shutil.which("g++") is None:
+        pytest.skip("no g++")
+    src = os.path.join(HERE, "cpp", "batch_model", "range_map_check.cpp")
+    exe = tmp_path / "range_map_check"
+    b = subprocess.run(["g++", "-O2", "-std=c++17", "-o", str(exe), src], capture_output=True, text=True, timeout=300)
+    assert b.returncode == 0, b.stderr
+    r = subprocess.run([str(exe), "20000"], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert " 0 failed" in r.stdout, r.stdout
+
