@@ -10,7 +10,8 @@ Inputs (image, weights, targets) are resident in HBM before timing starts.
 `value` is the mixed-precision (FP16 diagonals, FP32 accumulate, FP64 residual
 replacement) ms per solve; the FP32 solve is timed the same way beside it
 (`fp32_ms`: the tuner's FP32 form, one launch per iteration over the 12
-upper bands of the scaled operator, 84 B per row against the mixed form's 60 B),
+upper bands of the scaled operator, 82 B per row and iteration against the
+mixed form's 58 B),
 and so is the FP32 form that streams the stored 25 FP32 bands
 (`fp32_25band`: Jacobi step + update kernels, 148 B per row).
 The roofline object is the FP32 banded SpMV (y = A x, 25 diagonals), timed
@@ -404,7 +405,7 @@ def main():
     # The FP32 form that stores and streams the drop-in's 25 FP32 bands (Jacobi
     # step + update kernels, 148 B per row and iteration), beside the tuned
     # FP32 form above (symmetric scaled operator, 12 FP32 bands, one launch per
-    # iteration, 84 B per row): the tuner's fastest 25-band step shape is
+    # iteration, 82 B per row): the tuner's fastest 25-band step shape is
     # pinned for this run, then the tuner's pick is restored.
     plain = None
     if "fp32" in tuned:
@@ -523,12 +524,14 @@ def main():
 
     # the dominant kernel of the timed mixed solve: the fused CG iteration
     # (about 86 % of the solve), timed by the autotuner with HIP events on its
-    # stream in this process; 60 B per row (12 x 2 B bands, r, Ap, p, x, s in;
-    # x, r, p, Ap out)
+    # stream in this process.  A solve alternates two launches: one defers its
+    # update of x (12 x 2 B bands, r, Ap, p, s in; r, p, Ap out: 52 B per row),
+    # the next makes it and its own (also x and the deferred p in, x out:
+    # 64 B per row); per launch 58 B per row over the mean of the two times
     fused = None
     mt = tuned.get("mixed")
     if mt and mt.get("fused_us", 0) > 0:
-        fused_bytes = 60 * N
+        fused_bytes = 58 * N
         fused_gbs = fused_bytes / (mt["fused_us"] * 1e-6) / 1e9
         ftraffic = None
         try:
@@ -541,14 +544,18 @@ def main():
         fused = {"bound": "hbm", "kernel": "gm::k_cgfused_symw<1> (fused mixed CG iteration, FP16 bands)",
                  "achieved": round(fused_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                  "frac": round(fused_gbs / HBM_PEAK_GBS, 4), "traffic": ftraffic, "algorithmic_bytes": fused_bytes,
-                 "us_per_launch": mt["fused_us"]}
-        log(f"fused iteration: {mt['fused_us']:.1f} us/launch, {fused_gbs:.0f} GB/s algorithmic")
+                 "us_per_launch": mt["fused_us"],
+                 "launches": {"x_updated": {"us": mt["fused_update_us"], "algorithmic_bytes": 64 * N},
+                              "x_deferred": {"us": mt["fused_defer_us"], "algorithmic_bytes": 52 * N}}}
+        log(f"fused iteration: {mt['fused_us']:.1f} us/launch ({mt['fused_update_us']:.1f} updating x, "
+            f"{mt['fused_defer_us']:.1f} deferring it), {fused_gbs:.0f} GB/s algorithmic")
     # the same figure for the FP32 solve's fused iteration (symmetric scaled
-    # FP32 operator: 12 x 4 B bands, r, Ap, p, x, s in; x, r, p, Ap out: 84 B per row)
+    # FP32 operator, 12 x 4 B bands: 76 B per row deferring x, 88 B making both
+    # updates, 82 B per launch)
     fused32 = None
     ft = tuned.get("fp32")
     if ft and ft.get("step_walk") == 2 and ft.get("fused_us", 0) > 0:
-        f32_bytes = 84 * N
+        f32_bytes = 82 * N
         f32_gbs = f32_bytes / (ft["fused_us"] * 1e-6) / 1e9
         f32_traffic = None
         try:
@@ -561,7 +568,9 @@ def main():
         fused32 = {"bound": "hbm", "kernel": "gm::k_cgfused_symw<0> (fused FP32 CG iteration, symmetric scaled operator)",
                    "achieved": round(f32_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                    "frac": round(f32_gbs / HBM_PEAK_GBS, 4), "traffic": f32_traffic, "algorithmic_bytes": f32_bytes,
-                   "us_per_launch": ft["fused_us"]}
+                   "us_per_launch": ft["fused_us"],
+                   "launches": {"x_updated": {"us": ft["fused_update_us"], "algorithmic_bytes": 88 * N},
+                                "x_deferred": {"us": ft["fused_defer_us"], "algorithmic_bytes": 76 * N}}}
 
     mixed, fp32 = results["mixed"], results["fp32"]
     out = {

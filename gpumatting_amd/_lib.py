@@ -57,7 +57,7 @@ class TuneResult(C.Structure):
                 ("step_us", C.c_double), ("spmv_rows_per_thread", C.c_int), ("spmv_tile_rows", C.c_int),
                 ("spmv_wg_per_cu", C.c_int), ("spmv_us", C.c_double), ("update_wg_per_cu", C.c_int),
                 ("update_us", C.c_double), ("spmv_direct", C.c_int), ("step_walk", C.c_int),
-                ("fused_us", C.c_double)]
+                ("fused_us", C.c_double), ("fused_update_us", C.c_double), ("fused_defer_us", C.c_double)]
 
 
 class SolveInfo(C.Structure):

@@ -116,9 +116,11 @@ enum { kHandRun = 0, kHandStop = 1, kHandIdle = 2 };
 struct FHand {
     double rr_prev;  // <r,r> of the residual before the one the state holds
     float beta;      // beta of the newest direction
-    int iters;       // iterations completed (updates of x)
+    int iters;       // iterations completed (updates of r)
     int npart;       // workgroup partial sums left
-    int pad_;
+    int xpend;       // 1: the launch deferred its update of x (x lags r by one iteration);
+                     //    the next launch applies xa s p_prev first
+    float xa;        // alpha of that deferred update
 };
 
 // The two CG kernels have no grid-wide reduction of their own: each

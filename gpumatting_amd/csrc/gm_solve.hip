@@ -683,10 +683,23 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
 // cells never have to be read twice.
 // mode 0: continue from the previous launch (slot seq & 1); 1: first step
 // after a residual from scratch (state from the Scalars, no update: the step
-// j of the current residual); 2: timing (as 0 with alpha = beta = 0.5, no
-// sums folded, no stop test); 3: the update of iteration j only, when the
-// residual is replaced next (its step would be redone from the new residual):
-// x and r_{j+1} of every cell, <r,r> and ||r/s||^2 of r_{j+1}.  The loads of tile u + 1 are in flight while
+// j of the current residual); 2: timing (as 0 with alpha = beta = 0.5 and an
+// update of x deferred by the launch before, no sums folded, no stop test);
+// 3: the update of iteration j only, when the residual is replaced next (its
+// step would be redone from the new residual): x and r_{j+1} of every cell,
+// <r,r> and ||r/s||^2 of r_{j+1}; 4: as 0, but the update of x is deferred to
+// the next launch; 5: timing of mode 4.
+// Deferring x: a launch in mode 4 neither reads nor writes x (52 instead of
+// 60 B per row with 16-bit bands; s is still read for ||r/s||^2, so every
+// launch runs the convergence test).  The launch after it finds alpha_{j-1}
+// in its hand-off slot and p_{j-1} in the buffer it writes p_{j+1} to (each
+// cell is read by the thread that overwrites it, before it does), and makes
+// both updates, x + alpha_{j-1} s p_{j-1} + alpha_j s p_j, with the roundings
+// of two launches (64 B per row): x is bit-identical to the undeferred
+// sequence.  A stop that launch decides first brings x up to date.  The host
+// never defers the last launch before a fold, a replacement or the iteration
+// cap.
+// The loads of tile u + 1 are in flight while
 // tile u is computed; two workgroups per CU (the tuner's choice: a third, or a
 // second tile of loads in flight, measured slower at 4K).
 // The side words of the symmetric 16-bit operator's column walk, gathered
@@ -741,7 +754,11 @@ inline long fused_edge_words(int fmt, int W, long N) {
 }
 // Uup: band 13 of the scaled operator (16-bit: bands at k * pitch of the
 // 25-band array, so Uup = A16 + 13 pitch; FP32: the 12 upper bands at m * pitch).
-template <int FMT>
+// XM: kXPlain (x updated, none pending), kXDefer (modes 4, 5) or kXFlush (the
+// launch before deferred: modes 0 and 3 after a mode 4, and mode 2), so that
+// each form carries only its own loads.
+enum { kXPlain = 0, kXDefer = 1, kXFlush = 2 };
+template <int FMT, int XM>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void k_cgfused_symw(
     const void* __restrict__ Uup, size_t pitch, int W, int H, int N, const float* __restrict__ svec,
     float* __restrict__ x, float* r0, float* r1, float* p0, float* p1, float* Ap0, float* Ap1,
@@ -774,21 +791,28 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
     static_assert(TPR == 64 && XR == 2 * TY, "one wave per tile row; the x ring holds two tiles of rows");
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     if (sc->done != kRun || sc->verify) return;
-    const bool first = mode == 1;
+    const bool first = mode == 1, timing = mode == 2 || mode == 5;
     FHand hin;
     if (first) {
         hin.rr_prev = sc->rz_prev;
         hin.beta = sc->beta;
         hin.iters = sc->iters;
         hin.npart = 0;
-    } else if (mode == 2) {
+        hin.xpend = 0;
+        hin.xa = 0.f;
+    } else if (timing) {
         hin.rr_prev = 1.0;
         hin.beta = 0.5f;
         hin.iters = 0;
         hin.npart = 0;
+        hin.xpend = XM == kXFlush;
+        hin.xa = 0.5f;
     } else {
         hin = sc->fh[seq & 1];
     }
+    // this launch defers its update of x; the launch before did (the host
+    // alternates them, never two deferring launches in a row)
+    constexpr bool defer = XM == kXDefer, xpend = XM == kXFlush;
     const int j = hin.iters;
     const int k = first ? j : j + 1;  // the step this launch makes
     const float* rin = (j & 1) ? r1 : r0;
@@ -835,6 +859,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
         VR xr, xa, xp;     // r_j, Ap_j, p_j of the thread's R interior window cells
         float cr, ca, cp;  // ... of one side cell (threads 0 .. 15)
         VR xo, so;         // x and s of the thread's output cells
+        VR xq;             // p_{j-1} of the output cells (kXFlush)
     };
     // every load and store goes through a range-checked buffer descriptor
     // (out-of-range: loads 0, stores dropped), so every path issues the same
@@ -889,8 +914,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
                 in ? 4u * ((unsigned)((x0 / TX) * H + y0) * (24u * SPB) + (unsigned)e) : 0x80000000u;
             L.sv[q] = __builtin_amdgcn_raw_buffer_load_b32(be, (int)off, 0, 0);
         }
-        L.xo = ldR(brx, (long long)ib, !valid || first);
+        if constexpr (!defer) L.xo = ldR(brx, (long long)ib, !valid || first);
         L.so = ldR(brs, (long long)ib, !valid);
+        if constexpr (xpend) L.xq = ldR(bwp, (long long)ib, !valid);
         const long long gi = (long long)(y0 + 2 + lrow) * W + cx;
         L.xr = ldR(brr, gi, none);
         L.xa = ldR(bra, gi, first || none);
@@ -946,8 +972,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
         load_pre(ubeg, P);
     }
     float alpha = 0.f, beta = hin.beta;
-    FHand hout{hin.rr_prev, hin.beta, j, (int)gridDim.x, 0};
-    if (mode == 2) {
+    FHand hout{hin.rr_prev, hin.beta, j, (int)gridDim.x, 0, 0.f};
+    if (timing) {
         alpha = 0.5f;
     } else if (!first) {
         double t[5];
@@ -965,6 +991,19 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
             stop = kBreakdown;
         }
         if (stop != kRun || ver) {
+            if constexpr (xpend) {
+                // the update deferred by the launch before (p_{j-1} is in the
+                // buffer this launch would have written), so that x matches r_j
+                const int n4 = N >> 2;
+                for (int i = blockIdx.x * kBlock + (int)threadIdx.x; i < n4; i += gridDim.x * kBlock) {
+                    const f32x4 sv4 = reinterpret_cast<const f32x4*>(svec)[i];
+                    const f32x4 qv = reinterpret_cast<const f32x4*>(pnew)[i];
+                    f32x4 xv = reinterpret_cast<const f32x4*>(x)[i];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) xv[q] = fmaf(hin.xa * sv4[q], qv[q], xv[q]);
+                    reinterpret_cast<f32x4*>(x)[i] = xv;
+                }
+            }
             if (lead) {
                 sc->rz = rr;
                 sc->rz_prev = hin.rr_prev;
@@ -985,14 +1024,25 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
         hout.beta = beta;
         hout.iters = j + 1;
     }
+    if constexpr (defer) {
+        hout.xpend = 1;
+        hout.xa = alpha;
+    }
     if (mode == 3) {
-        // own cells only, float4 groups over the whole vector
+        // own cells only, float4 groups over the whole vector (and the update
+        // of x deferred by the launch before, if any)
         double w[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         const int n4 = N >> 2;
         for (int i = blockIdx.x * kBlock + (int)threadIdx.x; i < n4; i += gridDim.x * kBlock) {
-            const f32x4 xv = reinterpret_cast<const f32x4*>(x)[i], sv4 = reinterpret_cast<const f32x4*>(svec)[i];
+            f32x4 xv = reinterpret_cast<const f32x4*>(x)[i];
+            const f32x4 sv4 = reinterpret_cast<const f32x4*>(svec)[i];
             const f32x4 pv = reinterpret_cast<const f32x4*>(pold)[i], rv = reinterpret_cast<const f32x4*>(rin)[i];
             const f32x4 av = reinterpret_cast<const f32x4*>(Apin)[i];
+            if constexpr (xpend) {
+                const f32x4 qv = reinterpret_cast<const f32x4*>(pnew)[i];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) xv[q] = fmaf(hin.xa * sv4[q], qv[q], xv[q]);
+            }
             f32x4 xn, rn;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
@@ -1118,10 +1168,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
                     advance(L.cr, L.ca, L.cp, rs);
             }
             {
-                float xn[R];
+                // x_{j+1} = x_j + alpha_j s p_j, after x_j = x_{j-1} + alpha_{j-1} s p_{j-1}
+                // where the launch before deferred that
+                if constexpr (!defer) {
+                    float xn[R];
 #pragma unroll
-                for (int q = 0; q < R; ++q) xn[q] = fmaf(alpha * L.so[q], cur_p[q], L.xo[q]);
-                stR(brx, ib, mkR(xn), !valid || first);
+                    for (int q = 0; q < R; ++q) {
+                        float xb = L.xo[q];
+                        if constexpr (xpend) xb = fmaf(hin.xa * L.so[q], L.xq[q], xb);
+                        xn[q] = fmaf(alpha * L.so[q], cur_p[q], xb);
+                    }
+                    stR(brx, ib, mkR(xn), !valid || first);
+                }
                 stR(bwr, ib, mkR(cur_r), !valid || first);
             }
             if (valid) {
@@ -1163,7 +1221,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
         stR(bwa, ib, mkR(yv), !valid);
     }
     store_partials<5>(v, dpf + (size_t)((seq + 1) & 1) * 5 * kMaxGrid);
-    if (lead && mode != 2) sc->fh[(seq + 1) & 1] = hout;
+    if (lead && !timing) sc->fh[(seq + 1) & 1] = hout;
 }
 
 // Publish the fused iteration's live state (slot seq & 1) into the Scalars
@@ -1178,11 +1236,15 @@ __global__ __launch_bounds__(kBlock) void k_fused_fold(Scalars* sc, const double
     if (threadIdx.x != 0) return;
     sc->rz = t[1];
     sc->rz_prev = h.rr_prev;
-    sc->rr = t[4];
     sc->beta = h.beta;
     sc->pAp = t[0];
     sc->iters = h.iters;
     sc->pend = 0;
+    sc->rr = t[4];
+    // x lags r after a launch that deferred its update (the host folds only
+    // after one that did not): no stop here, the next launch decides it and
+    // brings x up to date
+    if (h.xpend) return;
     if (!isfinite(t[0]) || !isfinite(t[1]) || !isfinite(t[4])) sc->done = kNonFinite;
     else if (sc->tol2 >= 0 && t[4] <= sc->tol2) {
         if (verify) sc->verify = 1;
@@ -2031,6 +2093,7 @@ struct Solver {
     bool fused;         // one launch per iteration (k_cgfused_symw)
     int fseq;           // fused launches so far (hand-off slot parity)
     int fstate;         // 0: the Scalars hold the state (next launch starts a direction); 1: the fused slots do
+    int fdefer;         // the last fused launch deferred its update of x (mode 4)
 };
 
 static gm_status launch_resid(Solver& S, int start) {
@@ -2276,9 +2339,17 @@ template <int FMT>
 static void launch_fused_t(Solver& S, int mode) {
     Workspace* w = S.w;
     const void* up = FMT == kF32 ? static_cast<const void*>(w->A16) : static_cast<const void*>(w->A16 + 13 * w->pitch16);
-    hipLaunchKernelGGL((k_cgfused_symw<FMT>), dim3(S.g_step), dim3(kBlock), 0, S.hs, up, w->pitch16, S.plan.W,
-                       S.N / S.plan.W, S.N, w->dinv, S.x, w->r, w->r2, w->p0, w->p1, w->Ap, w->Ap2, w->uedge,
-                       w->sc, w->dpart + 3 * kMaxGrid, S.fseq, mode, S.verify);
+    // the form: deferring (modes 4, 5), making a deferred update too (mode 2,
+    // and modes 0 and 3 after a deferring launch), or neither
+    const int xm = (mode == 4 || mode == 5) ? kXDefer
+                   : (mode == 2 || ((mode == 0 || mode == 3) && S.fdefer)) ? kXFlush
+                                                                          : kXPlain;
+    auto k = xm == kXDefer ? k_cgfused_symw<FMT, kXDefer>
+             : xm == kXFlush ? k_cgfused_symw<FMT, kXFlush>
+                             : k_cgfused_symw<FMT, kXPlain>;
+    hipLaunchKernelGGL(k, dim3(S.g_step), dim3(kBlock), 0, S.hs, up, w->pitch16, S.plan.W, S.N / S.plan.W, S.N,
+                       w->dinv, S.x, w->r, w->r2, w->p0, w->p1, w->Ap, w->Ap2, w->uedge, w->sc,
+                       w->dpart + 3 * kMaxGrid, S.fseq, mode, S.verify);
     ++S.fseq;
 }
 
@@ -2301,18 +2372,25 @@ static gm_status fused_sync(Solver& S) {
     return GM_OK;
 }
 
-// replace_next: a residual from scratch follows this iteration
-static gm_status launch_cg_iter(Solver& S, bool replace_next = false) {
+// replace_next: a residual from scratch follows this iteration.  may_defer:
+// the next launch is another iteration (no fold, no replacement, below the
+// iteration cap), which may then make this one's update of x.
+static gm_status launch_cg_iter(Solver& S, bool replace_next = false, bool may_defer = false) {
     gm_status st;
     if (S.fused) {
         // a residual from scratch first takes its step (the first direction
         // or the one kept across a replacement), then one launch per
-        // iteration; the iteration before a replacement makes only its update
+        // iteration; the iteration before a replacement makes only its
+        // update; every other launch defers its update of x to the next
         if (!S.fstate) {
             if ((st = launch_fused(S, 1)) != GM_OK) return st;
             S.fstate = 1;
+            S.fdefer = 0;
         }
-        return launch_fused(S, replace_next ? 3 : 0);
+        const int mode = replace_next ? 3 : (may_defer && !S.fdefer) ? 4 : 0;
+        st = launch_fused(S, mode);  // (the form after a deferring launch: S.fdefer)
+        S.fdefer = mode == 4;
+        return st;
     }
     if ((st = launch_step(S)) != GM_OK) return st;
     return launch_update(S);
@@ -2582,7 +2660,8 @@ gm_status solve_impl(float* x, const BandedMatrix* A, const float* b, int pad, c
             for (int j = 0; j < this_chunk && enq < o->max_iter; ++j) {
                 const bool restart_next = o->restart > 0 && since_restart + 1 >= o->restart && enq + 1 < o->max_iter;
                 const bool refine_next = refine > 0 && (enq + 1) % refine == 0;
-                if ((st = launch_cg_iter(S, restart_next || refine_next)) != GM_OK) return st;
+                const bool may_defer = j + 1 < this_chunk && enq + 1 < o->max_iter;
+                if ((st = launch_cg_iter(S, restart_next || refine_next, may_defer)) != GM_OK) return st;
                 ++enq;
                 ++since_restart;
                 bool restarted = false;
@@ -2888,7 +2967,26 @@ gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_re
             best.upd_pc = pc;
         }
     }
-    double fused_us = 0;
+    // the fused iteration in timing mode (fixed alpha and beta): mode 2 makes
+    // its own and a deferred update of x, mode 5 defers its own; a solve
+    // alternates the two, so their mean is the time per iteration
+    double fused_us = 0, fused_upd_us = 0, fused_def_us = 0;
+    auto time_fused = [&](int mode, double& us) -> gm_status {
+        double tot = 0;
+        for (int k = 0; k < reps + 1; ++k) {
+            hipLaunchKernelGGL(k_tune_reset, dim3(1), dim3(64), 0, S.hs, w->sc);
+            GM_HIP(hipEventRecord(e0, S.hs));
+            gm_status r = launch_fused(S, mode);
+            if (r != GM_OK) return r;
+            GM_HIP(hipEventRecord(e1, S.hs));
+            GM_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            GM_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (k > 0) tot += ms;
+        }
+        us = tot * 1e3 / reps;
+        return GM_OK;
+    };
     if (S.sym && best.walk && best.shape == kS2R4Y4 && S.N < kFusedMaxN) {
         // the fused iteration (timing mode: fixed alpha and beta) against the
         // best step + update
@@ -2900,23 +2998,17 @@ gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_re
         S.walk = 2;
         for (int pc : {2, 3}) {
             set_step_shape(S, kS2R4Y4, pc);
-            double tot = 0;
-            for (int k = 0; k < reps + 1; ++k) {
-                hipLaunchKernelGGL(k_tune_reset, dim3(1), dim3(64), 0, S.hs, w->sc);
-                GM_HIP(hipEventRecord(e0, S.hs));
-                if ((st = launch_fused(S, 2)) != GM_OK) return st;
-                GM_HIP(hipEventRecord(e1, S.hs));
-                GM_HIP(hipEventSynchronize(e1));
-                float ms = 0;
-                GM_HIP(hipEventElapsedTime(&ms, e0, e1));
-                if (k > 0) tot += ms;
-            }
-            const double us = tot * 1e3 / reps;
-            snprintf(line, sizeof(line), "cgfused %d %d %d %d %.2f column-walk (step + update %.2f)\n",
-                     (int)kS2R4Y4, 4, 4, pc, us, best.us + best_upd);
+            double uf = 0, ud = 0;
+            if ((st = time_fused(2, uf)) != GM_OK || (st = time_fused(5, ud)) != GM_OK) return st;
+            const double us = 0.5 * (uf + ud);
+            snprintf(line, sizeof(line),
+                     "cgfused %d %d %d %d %.2f column-walk, mean of x updated %.2f / deferred %.2f (step + update %.2f)\n",
+                     (int)kS2R4Y4, 4, 4, pc, us, uf, ud, best.us + best_upd);
             g_tune_report += line;
             if (fused_pc == 0 || us < fused_us) {
                 fused_us = us;
+                fused_upd_us = uf;
+                fused_def_us = ud;
                 fused_pc = pc;
             }
         }
@@ -2941,23 +3033,18 @@ gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_re
         S.walk = 2;
         for (int pc : {2, 3}) {
             set_step_shape(S, kS2R4Y4, pc);
-            double tot = 0;
-            for (int k = 0; k < reps + 1; ++k) {
-                hipLaunchKernelGGL(k_tune_reset, dim3(1), dim3(64), 0, S.hs, w->sc);
-                GM_HIP(hipEventRecord(e0, S.hs));
-                if ((st = launch_fused(S, 2)) != GM_OK) return st;
-                GM_HIP(hipEventRecord(e1, S.hs));
-                GM_HIP(hipEventSynchronize(e1));
-                float ms = 0;
-                GM_HIP(hipEventElapsedTime(&ms, e0, e1));
-                if (k > 0) tot += ms;
-            }
-            const double us = tot * 1e3 / reps;
-            snprintf(line, sizeof(line), "cgfused %d %d %d %d %.2f FP32 symmetric column-walk (step + update %.2f)\n",
-                     (int)kS2R4Y4, 2, 4, pc, us, step_best.us + best_upd);
+            double uf = 0, ud = 0;
+            if ((st = time_fused(2, uf)) != GM_OK || (st = time_fused(5, ud)) != GM_OK) return st;
+            const double us = 0.5 * (uf + ud);
+            snprintf(line, sizeof(line),
+                     "cgfused %d %d %d %d %.2f FP32 symmetric column-walk, mean of x updated %.2f / deferred %.2f "
+                     "(step + update %.2f)\n",
+                     (int)kS2R4Y4, 2, 4, pc, us, uf, ud, step_best.us + best_upd);
             g_tune_report += line;
             if (fused_pc == 0 || us < fused_us) {
                 fused_us = us;
+                fused_upd_us = uf;
+                fused_def_us = ud;
                 fused_pc = pc;
             }
         }
@@ -2984,6 +3071,8 @@ gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_re
         out->step_us = best.us;
         out->step_walk = best.walk;
         out->fused_us = fused_us;
+        out->fused_update_us = fused_upd_us;
+        out->fused_defer_us = fused_def_us;
         out->spmv_rows_per_thread = 0;
         out->spmv_tile_rows = 0;
         out->spmv_direct = 0;

@@ -199,7 +199,11 @@ typedef struct gm_tune_result_s {
                               W % 4 == 0: 2 = the same one-launch iteration on the symmetric scaled
                               FP32 operator S L S (12 upper bands, FP64 residual replacement through
                               L every 50 iterations), 0 = the Jacobi step + update on L's 25 bands */
-   double fused_us;        /* that one-launch iteration's time (0: not timed) */
+   double fused_us;        /* that one-launch iteration's time (0: not timed): the mean of the
+                              two launches a solve alternates, */
+   double fused_update_us; /* ... one that updates x (its own update and the one deferred by
+                              the launch before: 64 B per row with 16-bit bands) */
+   double fused_defer_us;  /* ... and one that defers its update of x (52 B per row) */
 } gm_tune_result;
 gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_result* out, gm_stream_t stream);
 /* Text table of every candidate timed by the last gm_autotune on this thread. */

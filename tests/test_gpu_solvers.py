@@ -388,10 +388,12 @@ def test_every_tile_shape_same_iteration(gpu, small, prec, shapes, width):
 def test_fused_iteration_matches_step_and_update(gpu, W, H):
     """One launch per CG iteration (the update of x and r fused into the next
     step, beta from the expanded <r,r> recurrence, r and Ap alternating between
-    two buffers) makes the iterates of the two-kernel form: fixed-count runs
-    agree to rounding, and runs to a tolerance take the same iterations and
-    replacements (every 7 iterations here, so that replacements land on odd
-    and even iteration counts) and reach it.  Sizes: ragged right and bottom
+    two buffers, every other launch deferring its update of x to the next)
+    makes the iterates of the two-kernel form: fixed-count runs agree to
+    rounding (8 iterations in one chunk; 7 in chunks of 3, so that chunks end
+    after deferring and updating launches alike), and runs to a tolerance take
+    the same iterations and replacements (every 7 iterations here, so that
+    replacements land on odd and even iteration counts) and reach it.  Sizes: ragged right and bottom
     tiles, 15 columns with runs of one tile, 65 columns (more than the
     workgroups per XCD: runs that cross into the next column), an image of 3
     rows (one partial tile per column) and one 8 pixels wide."""
@@ -402,17 +404,20 @@ def test_fused_iteration_matches_step_and_update(gpu, W, H):
         call("gm_tune_set_step", W, p.N, PREC_FP16, 4, 4, pc, walk)
         x8, _ = matting.solve(s, method=METHOD_PCG, precision=PREC_FP16, tol=0, max_iter=8, verify=False,
                               refine_every=-1)
+        x7, _ = matting.solve(s, method=METHOD_PCG, precision=PREC_FP16, tol=0, max_iter=7, verify=False,
+                              refine_every=-1, check_every=3)
         x, info = matting.solve(s, method=METHOD_PCG, precision=PREC_FP16, tol=1e-6, max_iter=5000, refine_every=7)
-        res[(walk, pc)] = (x8.numpy(), x.numpy(), info)
+        res[(walk, pc)] = (x8.numpy(), x7.numpy(), x.numpy(), info)
     call("gm_tune_set_step", W, p.N, PREC_FP16, 4, 4, 2, 2)
-    x8r, xr, ir = res[(1, 3)]
+    x8r, x7r, xr, ir = res[(1, 3)]
     scale = np.abs(x8r).max()
     assert scale > 0
     for key in ((2, 2), (2, 3)):
-        x8, x, info = res[key]
+        x8, x7, x, info = res[key]
         print(f"\n[{W}x{H} walk {key}] {info.iterations} vs {ir.iterations} iterations, "
               f"{info.replacements} vs {ir.replacements} replacements, true relres {info.true_relres:.3e}", end="")
         assert np.abs(x8 - x8r).max() <= 1e-5 * scale
+        assert np.abs(x7 - x7r).max() <= 1e-5 * scale
         assert info.status == 0 and info.true_relres <= 1e-6
         assert abs(info.iterations - ir.iterations) <= 2
         assert np.abs(x - xr).max() <= 1e-3
