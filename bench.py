@@ -536,12 +536,12 @@ def main():
         ftraffic = None
         try:
             with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
-                t = json.load(f).get("gm::k_cgfused_symw<1>")
+                t = json.load(f).get("gm::k_cgfused_symw<1, *>")
             if t and t.get("workload", "").startswith(f"{W}x{H}"):
                 ftraffic = t["bytes_per_launch"]
         except (OSError, ValueError):
             pass
-        fused = {"bound": "hbm", "kernel": "gm::k_cgfused_symw<1> (fused mixed CG iteration, FP16 bands)",
+        fused = {"bound": "hbm", "kernel": "gm::k_cgfused_symw<1, *> (fused mixed CG iteration, FP16 bands; <1, 1> deferring x, <1, 2> updating it)",
                  "achieved": round(fused_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                  "frac": round(fused_gbs / HBM_PEAK_GBS, 4), "traffic": ftraffic, "algorithmic_bytes": fused_bytes,
                  "us_per_launch": mt["fused_us"],
@@ -560,12 +560,12 @@ def main():
         f32_traffic = None
         try:
             with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
-                t = json.load(f).get("gm::k_cgfused_symw<0>")
+                t = json.load(f).get("gm::k_cgfused_symw<0, *>")
             if t and t.get("workload", "").startswith(f"{W}x{H}"):
                 f32_traffic = t["bytes_per_launch"]
         except (OSError, ValueError):
             pass
-        fused32 = {"bound": "hbm", "kernel": "gm::k_cgfused_symw<0> (fused FP32 CG iteration, symmetric scaled operator)",
+        fused32 = {"bound": "hbm", "kernel": "gm::k_cgfused_symw<0, *> (fused FP32 CG iteration, symmetric scaled operator; <0, 1> deferring x, <0, 2> updating it)",
                    "achieved": round(f32_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                    "frac": round(f32_gbs / HBM_PEAK_GBS, 4), "traffic": f32_traffic, "algorithmic_bytes": f32_bytes,
                    "us_per_launch": ft["fused_us"],
