@@ -575,7 +575,9 @@ def test_fp32_fused_matches_jacobi_step(gpu, W, H):
     FP64 residual replacement through A -- makes the iterates of the two-kernel
     Jacobi step + update on A's 25 bands (the same method: x_k of Jacobi-PCG
     and of CG on S A S coincide in exact arithmetic): fixed-count runs agree to
-    rounding, and runs to 1e-6 reach it in about as many iterations.  Sizes as
+    rounding (8 iterations in one chunk, 7 in chunks of 3: every other launch
+    defers its update of x to the next, and chunks end after both kinds), and
+    runs to 1e-6 reach it in about as many iterations.  Sizes as
     for the 16-bit fused iteration (W % 4 == 0): ragged tiles, columns with
     runs of one tile, 130 columns, 3 rows, 8 pixels wide."""
     p = synth.make_problem(W, H, 4321, 1, 0.08)
@@ -585,17 +587,20 @@ def test_fp32_fused_matches_jacobi_step(gpu, W, H):
         call("gm_tune_set_step", W, p.N, PREC_FP32, 4, 4, pc, walk)
         x8, _ = matting.solve(s, method=METHOD_PCG, precision=PREC_FP32, tol=0, max_iter=8, verify=False,
                               refine_every=-1)
+        x7, _ = matting.solve(s, method=METHOD_PCG, precision=PREC_FP32, tol=0, max_iter=7, verify=False,
+                              refine_every=-1, check_every=3)
         x, info = matting.solve(s, method=METHOD_PCG, precision=PREC_FP32, tol=1e-6, max_iter=5000, refine_every=7)
-        res[(walk, pc)] = (x8.numpy(), x.numpy(), info)
+        res[(walk, pc)] = (x8.numpy(), x7.numpy(), x.numpy(), info)
     call("gm_tune_set_step", W, p.N, PREC_FP32, 4, 4, 2, 2)
-    x8r, xr, ir = res[(0, 3)]
+    x8r, x7r, xr, ir = res[(0, 3)]
     scale = np.abs(x8r).max()
     assert scale > 0 and ir.status == 0
     for key in ((2, 2), (2, 3)):
-        x8, x, info = res[key]
+        x8, x7, x, info = res[key]
         print(f"\n[{W}x{H} FP32 fused {key}] {info.iterations} vs {ir.iterations} iterations, "
               f"{info.replacements} replacements, true relres {info.true_relres:.3e}", end="")
         assert np.abs(x8 - x8r).max() <= 1e-5 * scale
+        assert np.abs(x7 - x7r).max() <= 1e-5 * scale
         assert info.status == 0 and info.true_relres <= 1e-6 and info.replacements >= 1
         assert abs(info.iterations - ir.iterations) <= max(3, 0.05 * ir.iterations)
         assert np.abs(x - xr).max() <= 1e-3
