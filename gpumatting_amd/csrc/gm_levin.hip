@@ -18,15 +18,17 @@
 // channel -- to L[i, j].  The operations (and their FMAs) are fixed, so the
 // result is deterministic and restated exactly by the CPU oracle.
 //
-// Workgroup tile: 64 x 4 pixels.  LDS holds the 68 x 8 pixel halo and the
-// statistics (mean, FP64 inverse covariance) of the 66 x 6 windows touching
-// the tile; each window's statistics are computed once per workgroup.
-// Output per pixel: 25 diagonals (100 B), b, 1/L_ii -> HBM-bound on the writes.
+// Workgroup tile: 32 x 8 pixels (a wave's 64 pixels are 2 rows of 32, so
+// each of its band stores writes two whole 128-byte lines).  LDS holds the
+// 36 x 12 pixel halo and the statistics (mean, FP64 inverse covariance) of the
+// 34 x 10 windows touching the tile; each window's statistics are computed
+// once per workgroup.  ACCURATE keeps 30 KB of LDS and 92 VGPRs: 5 workgroups
+// per CU.  Output per pixel: 25 diagonals (100 B), b, 1/L_ii.
 #include "gm_internal.h"
 
 namespace gm {
 
-constexpr int kTX = 64, kTY = 4;              // pixels per workgroup
+constexpr int kTX = 32, kTY = 8;              // pixels per workgroup
 constexpr int kWX = kTX + 2, kWY = kTY + 2;   // windows touching the tile
 constexpr int kPX = kTX + 4, kPY = kTY + 4;   // pixels those windows read
 constexpr int kNW = kWX * kWY;
@@ -45,9 +47,9 @@ __global__ __launch_bounds__(kBlock) void k_levin(const float* __restrict__ im, 
     // per-window statistics, structure of arrays:
     //   REFERENCE: ws[0..5] = adjugate d11 d12 d13 d22 d23 d33, ws[6] = det,
     //              ws[7..9] = float mean (held exactly as double)
-    //   ACCURATE : ws[0..5] = -inverse/9: S'11 S'12 S'13 S'22 S'23 S'33, ws[7..9] = mean
-    __shared__ double ws[10][kNW];
-    __shared__ float wm[6][kNW];  // ACCURATE: mean = head wm[0..2] + tail wm[3..5]
+    //   ACCURATE : ws[0..5] = -inverse/9: S'11 S'12 S'13 S'22 S'23 S'33
+    __shared__ double ws[MODE == GM_ASSEMBLY_ACCURATE ? 6 : 10][kNW];
+    __shared__ float wm[MODE == GM_ASSEMBLY_ACCURATE ? 6 : 1][kNW];  // ACCURATE: mean = head wm[0..2] + tail wm[3..5]
     __shared__ unsigned char wvalid[kNW];
 
     const int x0 = blockIdx.x * kTX, y0 = blockIdx.y * kTY;
@@ -140,7 +142,6 @@ __global__ __launch_bounds__(kBlock) void k_levin(const float* __restrict__ im, 
             const double q = -1.0 / (9.0 * det);
             ws[0][e] = d11 * q; ws[1][e] = d12 * q; ws[2][e] = d13 * q;
             ws[3][e] = d22 * q; ws[4][e] = d23 * q; ws[5][e] = d33 * q;
-            ws[7][e] = m0; ws[8][e] = m1; ws[9][e] = m2;
             const float h0 = (float)m0, h1 = (float)m1, h2 = (float)m2;
             wm[0][e] = h0; wm[1][e] = h1; wm[2][e] = h2;
             wm[3][e] = (float)(m0 - (double)h0);
@@ -197,7 +198,10 @@ __global__ __launch_bounds__(kBlock) void k_levin(const float* __restrict__ im, 
             const int e = (ty + n / 3) * kWX + (tx + n % 3);
             vc[n] = v0[n] = v1[n] = v2[n] = h0[n] = h1[n] = h2[n] = 0.f;
             if (!wvalid[e]) continue;
-            const double r0 = xd0 - ws[7][e], r1 = xd1 - ws[8][e], r2 = xd2 - ws[9][e];
+            // x_i - mu in double, mu as its float head and tail
+            const double r0 = (xd0 - (double)wm[0][e]) - (double)wm[3][e];
+            const double r1 = (xd1 - (double)wm[1][e]) - (double)wm[4][e];
+            const double r2 = (xd2 - (double)wm[2][e]) - (double)wm[5][e];
             const double a00 = ws[0][e], a01 = ws[1][e], a02 = ws[2][e];
             const double a11 = ws[3][e], a12 = ws[4][e], a22 = ws[5][e];
             v0[n] = (float)__builtin_fma(a02, r2, __builtin_fma(a01, r1, a00 * r0));
@@ -208,8 +212,10 @@ __global__ __launch_bounds__(kBlock) void k_levin(const float* __restrict__ im, 
             h0[n] = wm[0][e]; h1[n] = wm[1][e]; h2[n] = wm[2][e];
         }
         // member j = i + (dx, dy) lies in window n iff |dx - cx_n| <= 1 and
-        // |dy - cy_n| <= 1; L[i, j] sums its windows in raster order.  The
-        // diagonal is set from the row sum below.
+        // |dy - cy_n| <= 1; L[i, j] sums its windows in raster order and is
+        // stored as soon as it is complete (the stores then overlap the rest
+        // of the arithmetic, and the entry's register is free).  The diagonal
+        // is set from the row sum below.
 #pragma unroll
         for (int dy = -2; dy <= 2; ++dy)
 #pragma unroll
@@ -225,6 +231,7 @@ __global__ __launch_bounds__(kBlock) void k_levin(const float* __restrict__ im, 
                         v2[n], p2 - h2[n], __builtin_fmaf(v1[n], p1 - h1[n], __builtin_fmaf(v0[n], p0 - h0[n], vc[n])));
                     acc[12 + dx + 5 * dy] += t;
                 }
+                A[(size_t)(12 + dx + 5 * dy) * apitch + (size_t)gy * (size_t)W + (size_t)gx] = acc[12 + dx + 5 * dy];
             }
     }
 
@@ -256,8 +263,12 @@ __global__ __launch_bounds__(kBlock) void k_levin(const float* __restrict__ im, 
         acc[12] += l;
         bi = l * tg;
     }
+    if constexpr (MODE == GM_ASSEMBLY_REFERENCE) {
 #pragma unroll
-    for (int k = 0; k < 25; ++k) A[(size_t)k * apitch + i] = acc[k];
+        for (int k = 0; k < 25; ++k) A[(size_t)k * apitch + i] = acc[k];
+    } else {
+        A[(size_t)12 * apitch + i] = acc[12];  // (the off-diagonal entries are stored)
+    }
     if (bvec) bvec[i] = bi;
     if (dinv) dinv[i] = acc[12] != 0.f ? 1.f / acc[12] : 0.f;
 }

@@ -257,14 +257,15 @@ void orc_levin_gather_f32(float* a, size_t apitch, float* b, float* dinv, const 
  * colours and of their products (float products are exact in double), the
  * 1/9 scalings as multiplications by the double 1/9, the covariance as
  * fma(-mu_a, mu_b, s_ab/9) (+ eps/9 on the diagonal), FMA-contracted adjugate
- * and determinant, and S' = -inverse/9 = adjugate * (-1 / (9 det)).  Per pixel
- * i and window w (raster order): r = x_i - mu (double), v = S' r and
- * c = -1/9 + v.r (double, FMA chains), rounded to float once; every other
- * member j = i + (dx, dy) adds c + v.(x_j - x_i) (float FMAs, x_j - x_i in
- * float) to L[i, j], windows in raster order.  The diagonal is -sum of the
- * off-diagonal entries in double, then + lam_i.  Every fused operation is an
- * explicit fma()/fmaf() (the file is built without contraction), so this is
- * the GPU kernel's arithmetic operation for operation. */
+ * and determinant, and S' = -inverse/9 = adjugate * (-1 / (9 det)); the mean
+ * mu as mh + ml (float head and tail).  Per pixel i and window w (raster
+ * order): r = (x_i - mh) - ml (double), v = float(S' r) (double FMA chains)
+ * and c = -1/9 - v.ml (float FMAs); every other member j = i + (dx, dy) adds
+ * c + v.(x_j - mh) (float FMAs, x_j - mh in float) to L[i, j], windows in
+ * raster order.  The diagonal is -sum of the off-diagonal entries in double,
+ * then + lam_i.  Every fused operation is an explicit fma()/fmaf() (the file
+ * is built without contraction), so this is the GPU kernel's arithmetic
+ * operation for operation. */
 typedef struct {
     double m[3];
     double S[6]; /* S'11 S'12 S'13 S'22 S'23 S'33 (= -inverse / 9) */
@@ -332,8 +333,10 @@ void orc_levin_gather_acc_f32(float* a, size_t apitch, float* b, float* dinv, co
                     if (cy < 1 || cy > H - 2 || cx < 1 || cx > W - 2) continue;
                     const orc_wacc_t* w = &wst[(size_t)(cy - py + 1) * W + cx];
                     const double* S = w->S;
-                    double r0 = (double)pi[0] - w->m[0], r1 = (double)pi[1] - w->m[1],
-                           r2 = (double)pi[2] - w->m[2];
+                    /* x_i - mu in double, mu as its float head and tail */
+                    double r0 = ((double)pi[0] - (double)w->mh[0]) - (double)w->ml[0];
+                    double r1 = ((double)pi[1] - (double)w->mh[1]) - (double)w->ml[1];
+                    double r2 = ((double)pi[2] - (double)w->mh[2]) - (double)w->ml[2];
                     double u0 = fma(S[2], r2, fma(S[1], r1, S[0] * r0));
                     double u1 = fma(S[4], r2, fma(S[3], r1, S[1] * r0));
                     double u2 = fma(S[5], r2, fma(S[4], r1, S[2] * r0));
