@@ -472,8 +472,10 @@ def main():
     spmv_ms = e0.elapsed_time(e1) / args.spmv_reps
     ft = tuned.get("fp32")
     sp_r, sp_ty, sp_wg, sp_d = ((ft["spmv_rows_per_thread"], ft["spmv_tile_rows"], ft["spmv_wg_per_cu"],
-                                 ft["spmv_direct"]) if ft else (4, 4, 8, 0))
-    if sp_d:
+                                 ft["spmv_direct"]) if ft else (12, 4, 6, 2))
+    if sp_d == 2:
+        spmv_name = f"gm::k_spmv_seg<{sp_r // 4}, 0>"
+    elif sp_d:
         spmv_name = f"gm::k_spmv_direct<{sp_r}, {sp_ty}, 0>"
     else:
         spmv_name = (f"gm::k_spmv<gm::Geom2D<{sp_r}, {sp_ty}>, 0, 0>" if sp_ty > 0

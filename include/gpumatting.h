@@ -191,7 +191,9 @@ typedef struct gm_tune_result_s {
    double spmv_us;
    int update_wg_per_cu;   /* grid of the CG update kernel */
    double update_us;
-   int spmv_direct;        /* 1: the plain product's LDS-free form (x read through the caches) */
+   int spmv_direct;        /* 1: the plain product's LDS-free form (x read through the caches);
+                              2: its long-run form (4 rows per tile, each lane rows_per_thread / 4
+                              runs of 4 pixels 256 pixels apart) */
    int step_walk;          /* 16-bit symmetric step: 1 = column-walk form (workgroups walk down
                               columns of tiles and keep the rows shared with the tile above in LDS);
                               2 = column walk with the whole iteration in one launch (the update of
@@ -222,9 +224,11 @@ gm_status gm_tune_set_step(int W, int N, int precision, int rows_per_thread, int
 
 /* Pin the plain product's (gm_bm_ax) form for FP32 Levin matrices of this
  * shape without timing: the staged forms (rows_per_thread, tile_rows) as for
- * gm_tune_set's FP32 shapes, or direct = 1 with (4,4) (4,8) (2,8) (2,16) for
+ * gm_tune_set's FP32 shapes, direct = 1 with (4,4) (4,8) (2,8) (2,16) for
  * the LDS-free form (x 4*rows_per_thread-byte aligned, else a staged form
- * runs).  No reference counterpart. */
+ * runs), or direct = 2 with (8,4) (12,4) (16,4) for the long-run form
+ * (W % 4 == 0; out, x and z 16-byte aligned, else a staged form runs).  No
+ * reference counterpart. */
 gm_status gm_spmv_tune_set(int W, int N, int rows_per_thread, int tile_rows, int direct, int wg_per_cu);
 
 /* FP64-accumulated ||b - A x|| / ||b|| (synchronises; writes *relres). */
