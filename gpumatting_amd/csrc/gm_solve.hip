@@ -1520,11 +1520,28 @@ __global__ __launch_bounds__(kBlock) void k_trueres_fin(Scalars* sc, const doubl
 // Mixed-precision operator: As = S A S with S = diag(1/sqrt(a_ii)) (unit
 // diagonal, band 12 not stored), rounded to FP16 or BF16, in two streaming
 // passes.  k_svec: s (1 where a_ii <= 0).  k_scale16: each thread 4
-// consecutive rows, every band's 16-byte chunk and the 4 neighbour scales of
+// consecutive rows, every band's 16-byte chunk and the neighbour scales of
 // each band issued before any conversion; the neighbour scales (s at i + o,
 // 0 outside the vector, where the band entry is 0 too) come from L2, since s
 // was just written.  (a_ik s_i) s_{i+o} in that order.
-// SYM: only the upper bands 13..24 (the symmetric step kernel's form).
+// SYM: only the upper bands 13..24 (the symmetric step kernel's form; W % 4
+// == 0): the neighbours of rows ib .. ib + 3 in window row dy = 0, 1, 2 are
+// cells ib + dy W - 2 .. ib + dy W + 5, read as three aligned 16-byte groups
+// per window row through a range-checked buffer (a group outside the vector
+// reads 0: N % 4 == 0, so a group is all inside or all outside).
+template <bool SYM>
+struct ScaleWin {
+    f32x4 g[SYM ? 3 : 1][SYM ? 3 : 1];
+};
+// the 3 x 3 aligned groups of s around rows ib .. ib + 3 (window rows dy = 0..2)
+__device__ __forceinline__ void load_swin(__amdgpu_buffer_rsrc_t bs, int W, int ib, ScaleWin<true>& sw) {
+#pragma unroll
+    for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) sw.g[dy][c] = buf_ld4(bs, (long long)ib + (long long)dy * W - 4 + 4 * c);
+}
+// s at row ib + j + dy W + dx (j + dx + 4 in 0 .. 11)
+__device__ __forceinline__ float swin_at(const ScaleWin<true>& sw, int dy, int c) { return sw.g[dy][c >> 2][c & 3]; }
 __global__ __launch_bounds__(kBlock) void k_svec(const float* __restrict__ A, size_t pitch, int N,
                                                  float* __restrict__ svec) {
     for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long)gridDim.x * kBlock) {
@@ -1544,11 +1561,17 @@ __global__ __launch_bounds__(kBlock) void k_scale16(const float* __restrict__ A,
 #pragma clang fp contract(off)
     constexpr int K0 = SYM ? 13 : 0, NB = 25 - K0 - (SYM ? 0 : 1);
     const long nq = ((long)N + 3) / 4;
+    const __amdgpu_buffer_rsrc_t bs = buf_rsrc(svec, N);
     for (long q = (long)blockIdx.x * kBlock + threadIdx.x; q < nq; q += (long)gridDim.x * kBlock) {
         const int ib = (int)(4 * q);
         float si[4], a[NB][4], sn[NB][4];
+        ScaleWin<SYM> sw;
+        if constexpr (SYM) load_swin(bs, W, ib, sw);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) si[j] = ib + j < N ? svec[ib + j] : 0.f;
+        for (int j = 0; j < 4; ++j) {
+            if constexpr (SYM) si[j] = swin_at(sw, 0, j + 4);
+            else si[j] = ib + j < N ? svec[ib + j] : 0.f;
+        }
 #pragma unroll
         for (int m = 0; m < NB; ++m) {
             const int k = K0 + m + (!SYM && K0 + m >= 12 ? 1 : 0);
@@ -1558,8 +1581,12 @@ __global__ __launch_bounds__(kBlock) void k_scale16(const float* __restrict__ A,
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 a[m][j] = v[j];
-                const long n = (long)ib + j + o;
-                sn[m][j] = (n >= 0 && n < N) ? svec[n] : 0.f;
+                if constexpr (SYM) {
+                    sn[m][j] = swin_at(sw, k / 5 - 2, j + k % 5 - 2 + 4);
+                } else {
+                    const long n = (long)ib + j + o;
+                    sn[m][j] = (n >= 0 && n < N) ? svec[n] : 0.f;
+                }
             }
         }
 #pragma unroll
@@ -1582,18 +1609,28 @@ __global__ __launch_bounds__(kBlock) void k_scale16(const float* __restrict__ A,
 // The FP32 form of the symmetric scaled operator (the fused FP32 iteration):
 // the 12 upper bands (a_ik s_i) s_{i+o} of As = S A S, band 13 + m at
 // U + m * upitch, 0 where i + o_k leaves the vector.
+// Each thread 4 consecutive rows (W % 4 == 0, N % 4 == 0), the neighbour
+// scales as in k_scale16<*, true>.
 __global__ __launch_bounds__(kBlock) void k_scale32(const float* __restrict__ A, size_t pitch, int W, int N,
                                                     float* __restrict__ U, size_t upitch,
                                                     const float* __restrict__ svec) {
 #pragma clang fp contract(off)
-    for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < N; i += (long)gridDim.x * kBlock) {
-        const float si = svec[i];
+    const __amdgpu_buffer_rsrc_t bs = buf_rsrc(svec, N);
+    const long nq = (long)N / 4;
+    for (long q = (long)blockIdx.x * kBlock + threadIdx.x; q < nq; q += (long)gridDim.x * kBlock) {
+        const int ib = (int)(4 * q);
+        ScaleWin<true> sw;
+        load_swin(bs, W, ib, sw);
+        f32x4 a[12];
+#pragma unroll
+        for (int m = 0; m < 12; ++m) a[m] = *reinterpret_cast<const f32x4*>(A + (size_t)(13 + m) * pitch + ib);
 #pragma unroll
         for (int m = 0; m < 12; ++m) {
             const int k = 13 + m;
-            const long n = i + (long)(k / 5 - 2) * W + (k % 5 - 2);  // > i
-            const float sn = n < N ? svec[n] : 0.f;
-            U[(size_t)m * upitch + i] = (A[(size_t)k * pitch + i] * si) * sn;
+            f32x4 u;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) u[j] = (a[m][j] * swin_at(sw, 0, j + 4)) * swin_at(sw, k / 5 - 2, j + k % 5 + 2);
+            *reinterpret_cast<f32x4*>(U + (size_t)m * upitch + ib) = u;
         }
     }
 }
@@ -2196,7 +2233,7 @@ static void launch_scale16(Solver& S) {
     const int grid = elementwise_grid(S.dv, ((long)S.N + 3) / 4, 1);
     if (S.fmt == kF32) {
         // the FP32 upper bands in the 16-bit operator's buffer (48 of its 50 bytes per row)
-        hipLaunchKernelGGL(k_scale32, dim3(elementwise_grid(S.dv, S.N, 1)), dim3(kBlock), 0, S.hs, A->a, A->apitch,
+        hipLaunchKernelGGL(k_scale32, dim3(grid), dim3(kBlock), 0, S.hs, A->a, A->apitch,
                            S.plan.W, S.N, reinterpret_cast<float*>(w->A16), w->pitch16, w->dinv);
     } else if (S.fmt == kF16) {
         if (S.sym)
