@@ -290,9 +290,9 @@ __device__ __forceinline__ float band_entry(const unsigned int* w, int q) {
 // thread's R + 4 x cells (columns R tx .. R tx + R + 3) in window row ty + q;
 // urow[d]: LDS row of U window row ty + 2 - d.  Sums in band order, so every
 // staging scheme gives bit-identical results.
-template <int TY, int FMT>
+template <int TY, int FMT, int URR = SymGeom<TY, FMT>::UR>
 __device__ __forceinline__ void sym_tile(const float* __restrict__ lds, const int (&xrow)[5],
-                                         const unsigned int (*ub)[SymGeom<TY, FMT>::UR][SymGeom<TY, FMT>::SW],
+                                         const unsigned int (*ub)[URR][SymGeom<TY, FMT>::SW],
                                          const int (&urow)[3], int tx, float (&y)[SymGeom<TY, FMT>::R],
                                          float (&pc)[SymGeom<TY, FMT>::R]) {
     using SG = SymGeom<TY, FMT>;
@@ -765,8 +765,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
     const unsigned int* __restrict__ uedge, Scalars* sc, double* __restrict__ dpf, int seq, int mode, int verify) {
     constexpr int TY = 4;
     using SG = SymGeom<TY, FMT>;
-    constexpr int R = SG::R, TPR = SG::TPR, TX = SG::TX, ST = SG::ST, UR = SG::UR, SW = SG::SW, XR = SG::XR;
+    constexpr int R = SG::R, TPR = SG::TPR, TX = SG::TX, ST = SG::ST, SW = SG::SW, XR = SG::XR;
+    // U ring of two tiles' rows plus the 2 above: the own rows of tile u + 1
+    // arrive by direct-to-LDS loads while tile u's product reads its window
+    constexpr int UR = 2 * TY + 2;
     constexpr int EPW = SG::EPW, EB = FMT == kF32 ? 4 : 2;  // band entries per word, bytes per entry
+    static_assert(SW >= 2 * 64 + 2, "a staged U row holds the 128 words of two direct loads");
     constexpr int SPB = 2 / EPW;                             // side words per band and side
     using VR = std::conditional_t<R == 4, f32x4, f32x2>;
     auto ldR = [](__amdgpu_buffer_rsrc_t rs, long long i, bool skip = false) -> VR {
@@ -854,7 +858,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
     constexpr int NS = 24 * SPB * TY;  // side words of the own U rows (row, band, side, word)
     constexpr int SE = (NS + kBlock - 1) / kBlock;
     struct Loads {
-        unsigned int u[12][2];  // the thread's R entries of each upper band
         unsigned int sv[SE];
         VR xr, xa, xp;     // r_j, Ap_j, p_j of the thread's R interior window cells
         float cr, ca, cp;  // ... of one side cell (threads 0 .. 15)
@@ -891,19 +894,39 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
     };
     // the loads of tile u (none: the same instructions, all out of range, so
     // that every pass of the loop issues as many)
+    // The own U rows of tile u straight into the ring (window row 2 + ty;
+    // ut: the ring row of the tile's window row 0): two 4-byte loads per lane
+    // and band, lane l taking interior words l and 64 + l of the row's 128
+    // (entries x0 + EPW l ..; the interior starts at word 2 / EPW, after the
+    // left side words), out of range -> 0.  None: the same loads, all out of
+    // range (their zeros land in rows no tile reads).
+    auto issue_u = [&](int u, bool none, int ut) {
+        int x0, y0;
+        origin(u, x0, y0);
+        const int cy = y0 + ty;
+        {
+            const int wr = ut + 2 + ty < UR ? ut + 2 + ty : ut + 2 + ty - UR;
+            const int lane = (int)threadIdx.x & 63;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int xw = x0 + EPW * (64 * h + lane);
+                const unsigned off =
+                    (xw < W && cy < H && !none) ? 4u * (unsigned)(((size_t)cy * W + xw) / EPW) : 0x80000000u;
+#pragma unroll
+                for (int m = 0; m < 12; ++m)
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                        bu, (__attribute__((address_space(3))) void*)&ub[m][wr][2 / EPW + 64 * h], 4, (int)off,
+                        (int)(EB * m * pitch), 0, kNonTemporal);
+            }
+        }
+    };
+    // the rest of tile u's loads, into registers
     auto issue = [&](int u, Loads& L, bool none) {
         int x0, y0;
         origin(u, x0, y0);
         const int cx = x0 + R * tx, cy = y0 + ty;
         const bool valid = cx < W && cy < H && !none;
         const size_t ib = (size_t)cy * W + cx;
-        const unsigned uoff = valid ? (unsigned)EB * (unsigned)ib : 0x80000000u;
-#pragma unroll
-        for (int m = 0; m < 12; ++m) {
-            const u32x2 w2 = __builtin_amdgcn_raw_buffer_load_b64(bu, (int)uoff, (int)(EB * m * pitch), kNonTemporal);
-            L.u[m][0] = w2.x;
-            L.u[m][1] = w2.y;
-        }
         // side words of the own rows: 96 contiguous words of the edge array
         // (rows of the image only; a row below it feeds no output cell)
 #pragma unroll
@@ -968,6 +991,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
     Loads L;
     Pre P;
     if (ubeg < uend && mode != 3) {
+        issue_u(ubeg, false, 0);
         issue(ubeg, L, false);
         load_pre(ubeg, P);
     }
@@ -1077,19 +1101,44 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
         origin(u, x0, y0);
         const bool start = u == ubeg || y0 == 0;
         __syncthreads();  // the previous tile is done with lds and ub
+        // the U ring advances by a tile's rows on every tile (its own rows were
+        // loaded there during the tile before); the x ring restarts at a run's
+        // or column's first tile
+        if (u != ubeg) utop = utop + TY < UR ? utop + TY : utop + TY - UR;
+        // the side words of the own U rows (window rows 2 .. TY + 1; their
+        // interior words are already in the ring)
+        {
+#pragma unroll
+            for (int q = 0; q < SE; ++q) {
+                const int e = (int)threadIdx.x + q * kBlock;
+                if (e < NS) {
+                    // (row, band, side, word): words 0 .. SPB - 1 (left) or SW - SPB .. SW - 1 (right)
+                    const int row = e / (24 * SPB), rem = e % (24 * SPB);
+                    const int m = rem / (2 * SPB), side = (rem / SPB) & 1, wd = rem % SPB;
+                    const int wy = utop + 2 + row, ws = wy < UR ? wy : wy - UR;
+                    ub[m][ws][side ? SW - SPB + wd : wd] = L.sv[q];
+                }
+            }
+        }
+        // the next tile's own U rows, once this tile's side words are written:
+        // ring rows this tile does not read, and which the tile before read
+        // only up to the barrier passed above (the ring accesses after them --
+        // a run's first rows of a new column, the barrier before the product
+        // -- wait for them)
+        issue_u(u + 1, u + 1 >= uend, utop + TY < UR ? utop + TY : utop + TY - UR);
         if (start) {
             // the rows shared with the tile above: U window rows 0, 1 and x
             // window rows 0 .. 3 (as the tile above would have loaded them)
             xtop = 0;
-            utop = 0;
             if (u != ubeg) load_pre(u, P);  // a run that enters the next column
 #pragma unroll
             for (int k = 0; k < PE; ++k) {
                 const int e = (int)threadIdx.x + k * kBlock;
                 if (e < 2 * SW) {
                     const int wy = e / SW, ww = e - wy * SW;
+                    const int ws = utop + wy < UR ? utop + wy : utop + wy - UR;
 #pragma unroll
-                    for (int m = 0; m < 12; ++m) ub[m][wy][ww] = P.u[k][m];
+                    for (int m = 0; m < 12; ++m) ub[m][ws][ww] = P.u[k][m];
                 }
             }
             {
@@ -1111,28 +1160,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
         } else {
             // window rows TY.. of the tile above are rows 0.. of this one
             xtop = (xtop + TY) & (XR - 1);
-            utop = utop + TY < UR ? utop + TY : utop + TY - UR;
-        }
-        // own U rows (window rows 2 .. TY + 1) and their side words
-        {
-            const int wr = utop + 2 + ty < UR ? utop + 2 + ty : utop + 2 + ty - UR;
-            const int w0 = (R * tx + 2) / EPW;  // the word of the thread's first interior cell
-#pragma unroll
-            for (int m = 0; m < 12; ++m) {
-                ub[m][wr][w0] = L.u[m][0];
-                ub[m][wr][w0 + 1] = L.u[m][1];
-            }
-#pragma unroll
-            for (int q = 0; q < SE; ++q) {
-                const int e = (int)threadIdx.x + q * kBlock;
-                if (e < NS) {
-                    // (row, band, side, word): words 0 .. SPB - 1 (left) or SW - SPB .. SW - 1 (right)
-                    const int row = e / (24 * SPB), rem = e % (24 * SPB);
-                    const int m = rem / (2 * SPB), side = (rem / SPB) & 1, wd = rem % SPB;
-                    const int wy = utop + 2 + row, ws = wy < UR ? wy : wy - UR;
-                    ub[m][ws][side ? SW - SPB + wd : wd] = L.sv[q];
-                }
-            }
         }
         // x window rows 4 .. 7: p_{j+1}.  The update of the output cells (x,
         // r_{j+1} and their squared norms) is done here, so that only r_{j+1}
@@ -1206,7 +1233,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
                 const int lr = utop + ty + 2 - d;
                 urow[d] = lr < UR ? lr : lr - UR;
             }
-            sym_tile<TY, FMT>(lds, xrow, ub, urow, tx, yv, pc);
+            sym_tile<TY, FMT, UR>(lds, xrow, ub, urow, tx, yv, pc);
 #pragma unroll
             for (int q = 0; q < R; ++q) {
                 if (ib + q < N) {
