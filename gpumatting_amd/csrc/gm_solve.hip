@@ -1531,6 +1531,147 @@ __global__ __launch_bounds__(kBlock) void k_trueres2d(const float* __restrict__ 
     store_partials<3>(v, partials);
 }
 
+// k_trueres2d on long row runs (as the banded SpMV k_spmv_seg): tiles of
+// 256 G x 4 pixels, one wave per tile row, each lane G runs of 4 pixels 1 KB
+// apart, the bands in five passes of five (window row dy, then dx), each
+// pass's loads issued before its FP64 FMAs.  Every row's sum is the same FMA
+// chain in band order k = 0..24 as k_trueres2d, and so are the residual and
+// its roundings; only the partition of the partial sums differs.  The x
+// window (XT: float, or the FP64 master) is staged in LDS with window column c
+// at LDS column c.  W % 4 == 0, 16-byte aligned vectors, 25 band planes below
+// 2^31 bytes.
+template <int G, int PM, class XT>
+__global__ __launch_bounds__(kBlock) void k_trueres_seg(const float* __restrict__ A, size_t pitch, int W, int H,
+                                                        int N, const XT* __restrict__ x, const float* __restrict__ b,
+                                                        const float* __restrict__ dinv, float* __restrict__ r,
+                                                        float* __restrict__ r_odd, int op, int xzero, Scalars* sc,
+                                                        double* __restrict__ partials) {
+    if (op != 0 && sc->done != kRun) return;
+    if (r_odd && (sc->iters & 1)) r = r_odd;
+    const bool zero = xzero && sc->xzero;
+    constexpr int TX = 256 * G, TY = 4, ST = TX + 4, C4 = TX / 4;
+    constexpr int IE = (TY + 4) * C4 / kBlock;  // interior 4-cell groups staged per thread
+    static_assert((TY + 4) * C4 % kBlock == 0, "whole staging rounds");
+    __shared__ __attribute__((aligned(16))) XT lds[(TY + 4) * ST];
+    const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
+    const int ncol = (W + TX - 1) / TX, nrow = (H + TY - 1) / TY;
+    const TileRange tr = xcd_tiles(ncol * nrow);
+    const __amdgpu_buffer_rsrc_t bx = __builtin_amdgcn_make_buffer_rsrc(const_cast<XT*>(x), 0,
+                                                                        (int)(sizeof(XT) * (size_t)N), 0x00020000);
+    const __amdgpu_buffer_rsrc_t bb = buf_rsrc(b, N), bd = buf_rsrc(dinv, PM != kPlain ? N : 0),
+                                 br = buf_rsrc(r, N);
+    const __amdgpu_buffer_rsrc_t ba =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(A), 0, (int)(25 * 4 * pitch), 0x00020000);
+    // 4 consecutive x cells from element i (i % 4 == 0; out of range: 0)
+    auto ldx4 = [&](long long i, XT (&v)[4]) {
+        const unsigned off = (unsigned)i * (unsigned)sizeof(XT);
+        if constexpr (sizeof(XT) == 4) {
+            const f32x4 t = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(bx, (int)off, 0, 0));
+#pragma unroll
+            for (int q = 0; q < 4; ++q) v[q] = t[q];
+        } else {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const u32x4 t = __builtin_amdgcn_raw_buffer_load_b128(bx, (int)(off + 16u * h), 0, 0);
+                v[2 * h] = __builtin_bit_cast(double, u32x2{t.x, t.y});
+                v[2 * h + 1] = __builtin_bit_cast(double, u32x2{t.z, t.w});
+            }
+        }
+    };
+    auto ldx1 = [&](long long i) -> XT {
+        const unsigned off = (unsigned)i * (unsigned)sizeof(XT);
+        if constexpr (sizeof(XT) == 4) return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(bx, (int)off, 0, 0));
+        else return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(bx, (int)off, 0, 0));
+    };
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int t = tr.begin; t < tr.end; t += tr.step) {
+        const int x0 = (t % ncol) * TX, y0 = (t / ncol) * TY, y = y0 + w;
+        unsigned off[G];  // byte offset of each run in a band plane (out of range: invalid)
+        bool val[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int xc = x0 + 256 * g + 4 * lane;
+            val[g] = xc < W && y < H;
+            off[g] = val[g] ? 4u * (unsigned)(y * W + xc) : 0x80000000u;
+        }
+        f32x4 a[5][G];
+        auto issue = [&](int dy) {
+#pragma unroll
+            for (int dx = 0; dx < 5; ++dx)
+#pragma unroll
+                for (int g = 0; g < G; ++g)
+                    a[dx][g] = __builtin_bit_cast(
+                        f32x4, __builtin_amdgcn_raw_buffer_load_b128(ba, (int)(zero ? 0x80000000u : off[g]),
+                                                                     (int)(4 * (5 * dy + dx) * pitch), kNonTemporal));
+        };
+        __syncthreads();  // the previous tile is done with lds
+        if (!zero) {
+            XT xi[IE][4];
+#pragma unroll
+            for (int q = 0; q < IE; ++q) {
+                const int e = (int)threadIdx.x + q * kBlock, ly = e / C4, c4 = e - ly * C4;
+                ldx4((long long)(y0 - 2 + ly) * W + x0 + 4 * c4, xi[q]);
+            }
+            XT xh = XT(0);
+            const int hly = (int)threadIdx.x >> 2, hc = (int)threadIdx.x & 3, hlx = hc < 2 ? hc : TX + hc;
+            if (threadIdx.x < 4 * (TY + 4)) xh = ldx1((long long)(y0 - 2 + hly) * W + x0 - 2 + hlx);
+            issue(0);
+#pragma unroll
+            for (int q = 0; q < IE; ++q) {
+                const int e = (int)threadIdx.x + q * kBlock, ly = e / C4, c4 = e - ly * C4;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) lds[ly * ST + 2 + 4 * c4 + c] = xi[q][c];
+            }
+            if (threadIdx.x < 4 * (TY + 4)) lds[hly * ST + hlx] = xh;
+        }
+        __syncthreads();
+        double acc[G][4];
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[g][j] = 0.0;
+        if (!zero) {
+#pragma unroll
+            for (int dy = 0; dy < 5; ++dy) {
+                if (dy > 0) issue(dy);
+#pragma unroll
+                for (int g = 0; g < G; ++g) {
+                    const XT* sx = lds + (w + dy) * ST + 256 * g + 4 * lane;
+#pragma unroll
+                    for (int dx = 0; dx < 5; ++dx)
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            acc[g][j] = fma((double)a[dx][g][j], (double)sx[j + dx], acc[g][j]);
+                }
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const unsigned o = val[g] ? off[g] : 0x80000000u;
+            const f32x4 bv = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(bb, (int)o, 0, 0));
+            f32x4 dv = {0.f, 0.f, 0.f, 0.f}, rv = {0.f, 0.f, 0.f, 0.f};
+            if (PM != kPlain) dv = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(bd, (int)o, 0, 0));
+            if (op == 1) rv = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(br, (int)o, 0, 0));
+            f32x4 out;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const double tj = acc[g][j] - (double)bv[j];
+                float rj = (float)tj;
+                if (PM == kScaled) rj = (float)(tj * (double)dv[j]);
+                out[j] = rj;
+                if (val[g]) {
+                    v[1] += tj * tj;
+                    if (PM == kJacobi) v[0] += (double)(rj * (dv[j] * rj));
+                    else v[0] += (double)(rj * rj);
+                    if (op == 1) v[2] += (double)(rj - rv[j]) * (double)(rj - rv[j]);
+                }
+            }
+            if (op != 0) __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, out), br, (int)o, 0, 0);
+        }
+    }
+    store_partials<3>(v, partials);
+}
+
 // The reduction of k_trueres2d (one workgroup): g0, and for op 1 / 2 the
 // replacement and stopping test of k_trueres25's finaliser.
 __global__ __launch_bounds__(kBlock) void k_trueres_fin(Scalars* sc, const double* __restrict__ partials, int npart,
@@ -2186,7 +2327,32 @@ static gm_status launch_resid(Solver& S, int start) {
 template <class XT>
 static void launch_trueres_t(Solver& S, const XT* x, int op, int xzero) {
     using G = Geom2D<4, 4>;
-    if (S.tiled && shape_ok(kS2R4Y4, kF32, S.plan.W, S.N)) {
+    auto al16 = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; };
+    const bool seg = S.tiled && S.plan.W >= 4 && S.plan.W % 4 == 0 && S.N % S.plan.W == 0 &&
+                     (double)S.A->apitch * 100.0 < 2147483648.0 && al16(x) && al16(S.b) && al16(S.w->dinv) &&
+                     al16(S.w->r) && (!S.fused || al16(S.w->r2));
+    if (seg) {
+        // 768 x 4 tiles, 6 workgroups per CU (at 4K: 166.6 us, against 168.2 at 3 per CU and 183.3 us for
+        // k_trueres2d's 256 x 4 tiles)
+        constexpr int GR = 3;
+        const int H = S.N / S.plan.W, ntiles = ((S.plan.W + 256 * GR - 1) / (256 * GR)) * ((H + 3) / 4);
+        const int grid = grid_for_tiles(S.dv, ntiles, 6);
+        float* r_odd = S.fused ? S.w->r2 : nullptr;
+        if (S.pm == kJacobi)
+            hipLaunchKernelGGL((k_trueres_seg<GR, kJacobi, XT>), dim3(grid), dim3(kBlock), 0, S.hs, S.A->a,
+                               S.A->apitch, S.plan.W, H, S.N, x, S.b, S.w->dinv, S.w->r, r_odd, op, xzero, S.w->sc,
+                               S.rs.partials);
+        else if (S.pm == kScaled)
+            hipLaunchKernelGGL((k_trueres_seg<GR, kScaled, XT>), dim3(grid), dim3(kBlock), 0, S.hs, S.A->a,
+                               S.A->apitch, S.plan.W, H, S.N, x, S.b, S.w->dinv, S.w->r, r_odd, op, xzero, S.w->sc,
+                               S.rs.partials);
+        else
+            hipLaunchKernelGGL((k_trueres_seg<GR, kPlain, XT>), dim3(grid), dim3(kBlock), 0, S.hs, S.A->a,
+                               S.A->apitch, S.plan.W, H, S.N, x, S.b, S.w->dinv, S.w->r, r_odd, op, xzero, S.w->sc,
+                               S.rs.partials);
+        hipLaunchKernelGGL(k_trueres_fin, dim3(1), dim3(kBlock), 0, S.hs, S.w->sc, S.rs.partials, grid, op,
+                           S.pm == kScaled ? 1 : 0);
+    } else if (S.tiled && shape_ok(kS2R4Y4, kF32, S.plan.W, S.N)) {
         // 3 workgroups per CU (the FP32 step's pick), partial sums folded by k_trueres_fin
         const int grid = grid_for_tiles(S.dv, shape_tiles(kS2R4Y4, S.plan.W, S.N), 3);
         float* r_odd = S.fused ? S.w->r2 : nullptr;
