@@ -3217,9 +3217,9 @@ gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_re
         us = tot * 1e3 / reps;
         return GM_OK;
     };
-    if (S.sym && best.walk && best.shape == kS2R4Y4 && S.N < kFusedMaxN) {
-        // the fused iteration (timing mode: fixed alpha and beta) against the
-        // best step + update
+    if (S.sym && S.N < kFusedMaxN) {
+        // the fused iteration (timing mode: fixed alpha and beta; 4-row tiles,
+        // column walk) against the best step + update, whichever step form won
         if ((st = ws_fused(w, S.N, S.plan.W, S.fmt)) != GM_OK) return st;
         launch_uedge(S);
         hipLaunchKernelGGL(k_fill, dim3(S.g_elem), dim3(kBlock), 0, S.hs, w->Ap, S.N, 0.25f);
@@ -3242,13 +3242,15 @@ gm_status gm_autotune(const BandedMatrix* L, int precision, int reps, gm_tune_re
                 fused_pc = pc;
             }
         }
-        S.walk = best.walk;
         if (fused_us < best.us + best_upd) {
+            best.shape = kS2R4Y4;
             best.walk = 2;
             best.per_cu = fused_pc;
         } else {
             best.per_cu = step_pc;
         }
+        S.walk = best.walk;
+        set_step_shape(S, best.shape, best.per_cu);
     }
     if (f32fused) {
         // the fused FP32 iteration on the symmetric scaled form (timing mode)
